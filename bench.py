@@ -1,0 +1,174 @@
+"""Benchmark: nets/s (complete fields + canonical paths, bit-exact) on the
+512x512x4 board with 256 two-pin nets, plus p50/p99 per-batch solve latency.
+
+One step = one batch of 256 nets solved on a board resident in HBM: complete
+u64 (dist, hops) label fields over the full board for every net, canonical
+path reconstruction, and the host re-validation of every path.  Inputs (board
+planes) are uploaded once before the timed region.
+
+Output: one JSON line (see the keys below).  ``roofline`` uses the
+algorithmic bytes of the relaxation kernel: 19 B per cell visit (8 B label
+read + 8 B label write + 2 B history + 1 B mask) times the device counter of
+cell visits, over the kernel's average duration from HIP events on the
+engine's stream.  ``cpu_baseline`` times the CPU oracle (a C restatement of
+mr-cpu, not the Rust reference) on a bounded sample of the same nets.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "metal-route_amd"))
+
+METRIC = "nets/sec (fields+paths, bit-exact) at 512²×4L; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_VISIT = 19
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, help="synthetic config (2 = 512x512x4, 256 nets)")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--mode", choices=["persistent", "plain"], default="persistent")
+    ap.add_argument("--tiles", default="32,32,256,32", help="tw,th,threads,max_local")
+    ap.add_argument("--cpu-sample", type=int, default=16, help="nets in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get(f"{kernel}@{workload}")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(s, sample, got_paths):
+    """Time the CPU oracle on `sample` nets of the same workload (fields + paths)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mrcpu
+
+    g = s.grid
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost, board=g.board)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    spec = mrcpu.Spec(og, oe, additive=s.additive)
+    nets = [mrcpu.Net(int(a), int(b)) for a, b in zip(s.src[:sample], s.dst[:sample])]
+    threads = min(os.cpu_count() or 1, 16, sample)
+    t0 = time.perf_counter()
+    dist, hops = mrcpu.fields_batch(spec, nets, [[n.src] for n in nets], threads=threads)
+    paths = [mrcpu.reconstruct(spec, dist[i], hops[i], n.dst) for i, n in enumerate(nets)]
+    dt = time.perf_counter() - t0
+    exact = all((p is None and gp is None) or (gp is not None and p == gp[0]) for p, gp in zip(paths, got_paths))
+    return {
+        "value": round(sample / dt, 3),
+        "unit": "nets/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample} nets of the same board: complete lexicographic (dist,hops) Dijkstra fields "
+                  f"+ canonical reconstruction, {threads} threads, {dt:.2f} s",
+        "paths_bit_exact_vs_gpu": bool(exact),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+
+    import mrfield
+
+    tw, th, threads, ml = (int(v) for v in a.tiles.split(","))
+    eng = mrfield.Engine(0)
+    eng.set_tiles(tw, th, threads, ml)
+    eng.set_mode(mrfield.MODE_PERSISTENT if a.mode == "persistent" else mrfield.MODE_PLAIN)
+    s = mrfield.synth_config(a.config, seed=a.seed)
+    g = s.grid
+    board = mrfield.Board(eng, g, s.edges, s.pricing())
+    nets = s.nets()
+    wins = s.windows
+
+    lat, relax_ms, path_ms, visits, cells_visited = [], [], [], [], []
+    got = None
+    for i in range(a.warmup + a.steps):
+        if i == a.warmup:
+            t_start = time.perf_counter()
+        t0 = time.perf_counter()
+        got = board.solve(nets, windows=wins, retry_full=wins is not None)
+        t1 = time.perf_counter()
+        if i >= a.warmup:
+            st = eng.stats()
+            lat.append((t1 - t0) * 1e3)
+            relax_ms.append(st["relax_ms"])
+            path_ms.append(st["path_ms"])
+            visits.append(st["tile_visits"])
+            cells_visited.append(st["cells_visited"])
+    wall = time.perf_counter() - t_start
+    n_nets = len(nets)
+    value = n_nets * a.steps / wall
+    routed = sum(1 for p in got if p is not None)
+    kern_ms = float(np.mean(relax_ms))
+    cells = float(np.mean(cells_visited))
+    achieved = cells * BYTES_PER_CELL_VISIT / (kern_ms * 1e-3) / 1e9
+    workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets"
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "nets/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{g.w}x{g.h}x{g.layers} board, {n_nets} two-pin nets, complete fields + canonical "
+                        f"paths per step (synthetic config {a.config}, seed {a.seed})",
+            "form": a.mode,
+            "tiles": a.tiles,
+        },
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "fields_per_s": round(value, 3),
+        "routed": routed,
+        "relax_kernel_ms": round(kern_ms, 4),
+        "path_ms": round(float(np.mean(path_ms)), 4),
+        "tile_visits": int(np.mean(visits)),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic("relax_persistent" if a.mode == "persistent" else "relax_round", workload),
+            "kernel": "relax_persistent" if a.mode == "persistent" else "relax_round",
+            "bytes_per_cell_visit": BYTES_PER_CELL_VISIT,
+            "cell_visits_per_launch": int(cells),
+        },
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(s, min(a.cpu_sample, n_nets), got)
+    board.close()
+    eng.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

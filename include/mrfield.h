@@ -1,0 +1,253 @@
+/*
+ * mrfield.h -- C ABI of the MI355X batched shortest-path field engine
+ * (libmrfield.so, built from metal-route_amd/csrc).
+ *
+ * The ABI is the drop-in boundary for metalroute's isolated-net provider seam
+ * and the two secondary seams of its Metal engine.  Paths below are relative to
+ * the reference repository root.
+ *
+ *   mrf_route_isolated_batch  replaces IsolatedRouteProvider::route_isolated_batch
+ *                             (crates/mr-cpu/src/negotiated.rs:554-560) as
+ *                             implemented by MetalIsolatedProvider
+ *                             (crates/mr-cli/src/lib.rs:612-625) over
+ *                             metal_isolated_paths_with (:549-609): window pass,
+ *                             then full-board retry of unreachable nets.
+ *   mrf_route_isolated_pass   replaces mr_metal::metal_route_isolated_batch /
+ *                             _ragged (crates/mr-metal/src/lib.rs:414, :695):
+ *                             one pass, no retry.
+ *   mrf_sweep_fields          replaces mr_metal::metal_sweep_fields
+ *                             (crates/mr-metal/src/lib.rs:172-192).
+ *   mrf_router_route          replaces MetalRouter::route (Router trait,
+ *                             crates/mr-metal/src/lib.rs:1143-1235).
+ *   mrf_fields                complete multi-source canonical fields with
+ *                             edge prices and negotiation pricing (no single
+ *                             reference function: the field form of
+ *                             route_legal_once / route_negotiated,
+ *                             crates/mr-cpu/src/negotiated.rs:2746-3052).
+ *   mrf_board_* / mrf_solve   device-resident boards for repeated solves and
+ *                             the serving loop (no reference counterpart; the
+ *                             reference re-uploads per call, mr-metal gpu.rs:880-888).
+ *
+ * Conventions
+ *   * Cells use mr-core's mapping idx3(x, y, l) = (l*h + y)*w + x
+ *     (crates/mr-core/src/lib.rs:90-116); UINT32_MAX is OBSTACLE / unreachable.
+ *   * Every operation is all-or-error: on a non-zero return nothing in an
+ *     output is valid (crates/mr-cpu/src/negotiated.rs:539-553).
+ *   * Error codes mirror RouterError (crates/mr-core/src/lib.rs:699-708).
+ *   * Board masks and via masks are supported (the reference's Metal engine
+ *     rejected them, crates/mr-metal/src/lib.rs:51-59, :428-432).
+ *   * A context serialises its GPU lane; a call that finds the lane busy returns
+ *     MRF_BACKEND_UNAVAILABLE at once (crates/mr-cli/src/lib.rs:499, :620-622).
+ */
+#ifndef MRFIELD_H
+#define MRFIELD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRF_ABI_VERSION 1
+
+/* RouterError (crates/mr-core/src/lib.rs:699-708) */
+enum {
+    MRF_OK = 0,
+    MRF_BACKEND_UNAVAILABLE = 1,
+    MRF_INVALID_ENDPOINT = 2,
+    MRF_MALFORMED_GRID = 3
+};
+
+/* mr-core Grid::BOARD_* bits (crates/mr-core/src/lib.rs:411-416) */
+enum {
+    MRF_BOARD_TRACE_NODE = 1 << 0,
+    MRF_BOARD_VIA_NODE = 1 << 1,
+    MRF_BOARD_EDGE_NEG_Y = 1 << 2,
+    MRF_BOARD_EDGE_NEG_X = 1 << 3,
+    MRF_BOARD_EDGE_POS_X = 1 << 4,
+    MRF_BOARD_EDGE_POS_Y = 1 << 5
+};
+
+typedef struct mrf_ctx mrf_ctx;
+typedef struct mrf_board mrf_board;
+
+/* Dims (crates/mr-core/src/lib.rs:40-60) */
+typedef struct {
+    uint32_t w, h, layers;
+} mrf_dims;
+
+/* Grid (crates/mr-core/src/lib.rs:394-408).  A zero length means the empty
+ * Vec (no mask).  Lengths are checked like Grid::is_well_formed (:508-512). */
+typedef struct {
+    const uint32_t *cost;
+    size_t n_cost;
+    const uint8_t *via_forbidden;
+    size_t n_via_forbidden;
+    const uint8_t *board_constraint;
+    size_t n_board_constraint;
+} mrf_grid;
+
+/* NetEndpoints (crates/mr-core/src/lib.rs:520-537) without the name. */
+typedef struct {
+    uint32_t src, dst;
+    const uint32_t *pads;
+    size_t n_pads;
+    const uint32_t *via_pads;
+    size_t n_via_pads;
+} mrf_net;
+
+/* IsolatedRouteWindow / MetalWindow, inclusive (negotiated.rs:495-519). */
+typedef struct {
+    uint32_t x0, y0, x1, y1;
+} mrf_window;
+
+/* Edge prices of IsolatedRouteRequest (negotiated.rs:530-537): x[w-1], y[h-1],
+ * via[layers-1] with via_allowed[k] == 0 meaning None. */
+typedef struct {
+    const uint32_t *x;
+    size_t nx;
+    const uint32_t *y;
+    size_t ny;
+    const uint32_t *via;
+    const uint8_t *via_allowed;
+    size_t nvia;
+} mrf_edges;
+
+/* Negotiation pricing (negotiated.rs:2777-2797): entering v adds
+ * additive[v] + halo_penalty * [halo_owner[v] is neither -1 nor the net's id].
+ * All fields may be empty for the alone-path price. */
+typedef struct {
+    const uint32_t *additive;
+    size_t n_additive;
+    const int32_t *halo_owner;
+    size_t n_halo_owner;
+    uint32_t halo_penalty;
+    const int32_t *net_ids; /* per net own id, or NULL (all -1) */
+} mrf_pricing;
+
+typedef struct {
+    int code;
+    uint32_t net; /* offending net index for MRF_INVALID_ENDPOINT */
+    char message[256];
+} mrf_error;
+
+/* Library-owned path batch; release with mrf_free_paths. */
+typedef struct {
+    size_t n;
+    uint8_t *routed;       /* n: 0 = None (unreachable) */
+    uint32_t *search_cost; /* n: fixed-point search cost (MetalIsolatedRoute::search_cost) */
+    uint64_t *offsets;     /* n+1: path i is cells[offsets[i] .. offsets[i+1]) */
+    uint32_t *cells;       /* src ..= dst */
+} mrf_paths;
+
+/* Solve statistics of the last call on a context. */
+typedef struct {
+    double relax_ms;       /* device time of the relaxation (HIP events) */
+    double path_ms;        /* device time of the path reconstruction */
+    double total_ms;       /* host wall time of the call */
+    uint64_t tile_visits;  /* (field, tile) loads from HBM */
+    uint64_t local_rounds; /* in-LDS relaxation rounds */
+    uint64_t cells_visited;
+    uint64_t field_cells;
+    uint32_t host_rounds;  /* plain form: launches; persistent form: 1 */
+} mrf_stats;
+
+/* Solve form of a context. */
+enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1 };
+
+int mrf_abi_version(void);
+
+/* Context: one HIP stream on one device, reusable device workspace. */
+int mrf_open(int device, mrf_ctx **out, mrf_error *err);
+void mrf_close(mrf_ctx *ctx);
+/* Tile shape (tw x th cells, all layers), workgroup size, in-LDS round cap. */
+int mrf_set_tiles(mrf_ctx *ctx, uint32_t tw, uint32_t th, uint32_t threads, uint32_t max_local,
+                  mrf_error *err);
+int mrf_set_mode(mrf_ctx *ctx, int mode);
+int mrf_last_stats(mrf_ctx *ctx, mrf_stats *out);
+
+/* IsolatedRouteProvider::route_isolated_batch + MetalIsolatedProvider adapter
+ * (negotiated.rs:554-560; mr-cli lib.rs:549-625): solve each net inside its
+ * window, then only the unreachable nets on the full board. */
+int mrf_route_isolated_batch(mrf_ctx *ctx, mrf_dims dims, const mrf_grid *grid, const mrf_net *nets,
+                             size_t n_nets, const mrf_window *windows, const mrf_edges *edges,
+                             mrf_paths *out, mrf_error *err);
+
+/* metal_route_isolated_batch (mr-metal lib.rs:414-558): one windowed pass;
+ * routed[i] == 0 means unreachable inside windows[i]. */
+int mrf_route_isolated_pass(mrf_ctx *ctx, mrf_dims dims, const mrf_grid *grid, const mrf_net *nets,
+                            size_t n_nets, const mrf_window *windows, const mrf_edges *edges,
+                            mrf_paths *out, mrf_error *err);
+
+/* metal_sweep_fields (mr-metal lib.rs:172-192): planar single-source fields
+ * with additive enter cost (bfs_distance_field, mr-cpu sweep.rs:143-148);
+ * dist_out holds n_sources * dims.len() u32, in source order. */
+int mrf_sweep_fields(mrf_ctx *ctx, mrf_dims dims, const mrf_grid *grid, const uint32_t *sources,
+                     size_t n_sources, uint32_t *dist_out, mrf_error *err);
+
+/* MetalRouter::route / LeeRouter::route (mr-metal lib.rs:1143-1235,
+ * mr-cpu lee.rs:68-110): per-net canonical paths with cost = sum of enter costs
+ * excluding the source.  routed[i] == 0 lists the net in BoardRoute::unrouted. */
+int mrf_router_route(mrf_ctx *ctx, mrf_dims dims, const mrf_grid *grid, const mrf_net *nets,
+                     size_t n_nets, mrf_paths *out, mrf_error *err);
+
+/* Complete canonical fields: field i is seeded by sources[i][0..n_sources[i])
+ * at label (0, 0) inside windows[i] (NULL = full board).  Sources and dsts[i]
+ * (NULL or UINT32_MAX = none) stay passable like the alone search's endpoints.
+ * dist_out (and hops_out unless NULL) receive n_fields * dims.len() u32 each. */
+int mrf_fields(mrf_ctx *ctx, mrf_dims dims, const mrf_grid *grid, const mrf_edges *edges,
+               const mrf_pricing *pricing, const uint32_t *const *sources, const size_t *n_sources,
+               const uint32_t *dsts, size_t n_fields, const mrf_window *windows, uint32_t *dist_out,
+               uint32_t *hops_out, mrf_error *err);
+
+void mrf_free_paths(mrf_paths *paths);
+
+/* ---- device-resident boards (serving and repeated solves) ---------------- */
+int mrf_board_create(mrf_ctx *ctx, mrf_dims dims, const mrf_grid *grid, const mrf_edges *edges,
+                     const mrf_pricing *pricing, mrf_board **out, mrf_error *err);
+void mrf_board_destroy(mrf_board *board);
+size_t mrf_board_bytes(const mrf_board *board);
+/* Solve nets on a resident board: one pass in `windows` (NULL = full board),
+ * then, when retry_full != 0, the unreachable nets on the full board.
+ * Multi-pin nets: pass pins as sources via mrf_solve_fields. */
+int mrf_solve(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets,
+              const mrf_window *windows, int retry_full, mrf_paths *out, mrf_error *err);
+/* Fields on a resident board; labels stay on the device until the next call,
+ * read them with mrf_read_field. */
+int mrf_solve_fields(mrf_ctx *ctx, mrf_board *board, const uint32_t *const *sources,
+                     const size_t *n_sources, const uint32_t *dsts, const int32_t *net_ids,
+                     size_t n_fields, const mrf_window *windows, mrf_error *err);
+int mrf_read_field(mrf_ctx *ctx, mrf_board *board, size_t field, uint32_t *dist_out,
+                   uint32_t *hops_out, mrf_error *err);
+
+/* ---- synthetic workloads --------------------------------------------------- */
+/* Seeded generators for the benchmark configurations (documented in
+ * metal-route_amd/csrc/synth.cpp).  Output arrays are library-owned; free with
+ * mrf_free_synth. */
+typedef struct {
+    mrf_dims dims;
+    uint32_t *cost;
+    uint8_t *board_constraint; /* NULL when the config has no outline */
+    uint32_t *additive;        /* NULL when the config has no history plane */
+    int32_t *halo_owner;       /* NULL unless the config prices foreign-pin halos */
+    uint32_t halo_penalty;
+    uint32_t *x_edges, *y_edges, *via_edges;
+    uint8_t *via_allowed;
+    size_t n_nets;
+    uint32_t *src, *dst;       /* two-pin nets (dst = UINT32_MAX for multi-pin) */
+    uint64_t *pin_off;         /* multi-pin nets: pins[pin_off[i]..pin_off[i+1]) */
+    uint32_t *pins;
+    mrf_window *windows;       /* per-net search windows, or NULL (full board) */
+} mrf_synth;
+
+int mrf_synth_config(int config, uint64_t seed, uint32_t scale_nets, mrf_synth *out, mrf_error *err);
+/* The config-2 board generator at a given side, layer count and net count
+ * (serving requests). */
+int mrf_synth_board(uint32_t side, uint32_t layers, uint32_t nets, uint64_t seed, mrf_synth *out);
+void mrf_free_synth(mrf_synth *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,807 @@
+// C ABI of the field engine (include/mrfield.h): validation, the provider
+// adapter's window -> full-board retry, chunking by the label budget, and the
+// fail-closed host re-validation of every returned path.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <unordered_set>
+
+#include "../../include/mrfield.h"
+#include "engine.h"
+
+using mrf::Error;
+using mrf::kMax;
+
+struct mrf_ctx {
+  mrf::Ctx c;
+};
+
+struct mrf_board {
+  mrf_ctx* ctx = nullptr;
+  mrf::DevBoard b;
+  mrf_dims dims{};
+  // Host copies needed for request validation and path checks.
+  std::vector<uint32_t> cost;
+  std::vector<uint8_t> vf, board;
+  std::vector<uint32_t> xe, ye, ve, add;
+  std::vector<uint8_t> va;
+  std::vector<int32_t> halo;
+  uint32_t halo_pen = 0;
+};
+
+namespace {
+
+void set_err(mrf_error* err, int code, const char* msg, uint32_t net = 0) {
+  if (!err) return;
+  err->code = code;
+  err->net = net;
+  snprintf(err->message, sizeof(err->message), "%s", msg);
+}
+
+int fail(mrf_error* err, const Error& e) {
+  set_err(err, e.code(), e.what(), e.net());
+  return e.code();
+}
+
+size_t dims_len(mrf_dims d) { return (size_t)d.w * d.h * d.layers; }
+
+// Host view of a grid + prices, shared by validation and path checks.
+struct HostGrid {
+  mrf_dims d{};
+  const uint32_t* cost = nullptr;
+  const uint8_t* vf = nullptr;
+  const uint8_t* board = nullptr;
+  const uint32_t* xe = nullptr;
+  const uint32_t* ye = nullptr;
+  const uint32_t* ve = nullptr;
+  const uint8_t* va = nullptr;
+  const uint32_t* add = nullptr;
+  const int32_t* halo = nullptr;
+  uint32_t halo_pen = 0;
+  bool obst(uint32_t c) const { return cost[c] == kMax; }
+  bool trace(uint32_t c) const { return board && (board[c] & MRF_BOARD_TRACE_NODE); }
+  bool vianode(uint32_t c) const { return board && (board[c] & MRF_BOARD_VIA_NODE); }
+  bool viaforb(uint32_t c) const { return vf && vf[c]; }
+  // mr-core is_board_planar_step_forbidden (lib.rs:466-496) for neighbours.
+  bool planar_forbidden(uint32_t u, uint32_t v) const {
+    if (!board) return false;
+    uint32_t plane = d.w * d.h;
+    uint32_t ur = u % plane, vr = v % plane;
+    uint32_t ux = ur % d.w, uy = ur / d.w, vx = vr % d.w, vy = vr / d.w;
+    uint8_t sb, db;
+    if (ux == vx && uy == vy + 1) {
+      sb = MRF_BOARD_EDGE_NEG_Y;
+      db = MRF_BOARD_EDGE_POS_Y;
+    } else if (uy == vy && ux == vx + 1) {
+      sb = MRF_BOARD_EDGE_NEG_X;
+      db = MRF_BOARD_EDGE_POS_X;
+    } else if (uy == vy && ux + 1 == vx) {
+      sb = MRF_BOARD_EDGE_POS_X;
+      db = MRF_BOARD_EDGE_NEG_X;
+    } else {
+      sb = MRF_BOARD_EDGE_POS_Y;
+      db = MRF_BOARD_EDGE_NEG_Y;
+    }
+    return (board[u] & sb) || (board[v] & db);
+  }
+};
+
+bool sorted_has(const std::vector<uint32_t>& a, uint32_t v) { return std::binary_search(a.begin(), a.end(), v); }
+
+std::vector<uint32_t> sorted_unique(const uint32_t* p, size_t n) {
+  std::vector<uint32_t> v(p, p + n);
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  return v;
+}
+
+void check_grid(mrf_dims d, const mrf_grid* g) {
+  size_t n = dims_len(d);
+  if (!g || !g->cost || g->n_cost != n || (g->n_via_forbidden != 0 && g->n_via_forbidden != n) ||
+      (g->n_board_constraint != 0 && g->n_board_constraint != n))
+    throw Error(MRF_MALFORMED_GRID, "grid is malformed: cost length does not match dims");
+  if (d.w == 0 || d.h == 0 || d.layers == 0) throw Error(MRF_MALFORMED_GRID, "grid has no cells");
+  if (n > 0xFFFFFFFEull) throw Error(MRF_BACKEND_UNAVAILABLE, "grid exceeds 32-bit cell indexing");
+  if (d.layers > 16) throw Error(MRF_BACKEND_UNAVAILABLE, "at most 16 layers are supported");
+}
+
+void check_edges(mrf_dims d, const mrf_edges* e) {
+  if (!e || e->nx != (size_t)(d.w - 1) || e->ny != (size_t)(d.h - 1) || e->nvia != (size_t)(d.layers - 1) ||
+      (d.w > 1 && !e->x) || (d.h > 1 && !e->y) || (d.layers > 1 && (!e->via || !e->via_allowed)))
+    throw Error(MRF_BACKEND_UNAVAILABLE, "isolated batch shape does not match grid/nets");
+}
+
+void check_window(mrf_dims d, const mrf_window& w, size_t i) {
+  if (!(w.x0 <= w.x1 && w.y0 <= w.y1 && w.x1 < d.w && w.y1 < d.h))
+    throw Error(MRF_BACKEND_UNAVAILABLE, "net " + std::to_string(i) + " has an invalid isolated-route window");
+}
+
+// Endpoint contract of provider_alone_paths / metal_route_isolated_batch
+// (negotiated.rs:1396-1407, mr-metal lib.rs:446-466).
+void check_net(const HostGrid& g, const mrf_net& n, size_t i) {
+  size_t len = dims_len(g.d);
+  auto bad = [&]() { throw Error(MRF_INVALID_ENDPOINT, "net " + std::to_string(i) + ": endpoint out of bounds or on an obstacle", (uint32_t)i); };
+  if ((n.n_pads && !n.pads) || (n.n_via_pads && !n.via_pads)) bad();
+  for (size_t k = 0; k < n.n_pads; ++k)
+    if (n.pads[k] >= len) bad();
+  std::vector<uint32_t> pads = sorted_unique(n.pads, n.n_pads);
+  for (size_t k = 0; k < n.n_via_pads; ++k)
+    if (n.via_pads[k] >= len || !sorted_has(pads, n.via_pads[k])) bad();
+  for (uint32_t c : {n.src, n.dst}) {
+    if (c >= len || g.trace(c) || (g.obst(c) && !sorted_has(pads, c))) bad();
+  }
+}
+
+struct NetPlan {
+  std::vector<uint32_t> pads, vpads;
+};
+
+uint32_t step_price(uint32_t e, uint32_t w, uint64_t a) {
+  uint64_t p = (uint64_t)e * w + a;
+  return p < 0xFFFFFFFEull ? (uint32_t)p : 0xFFFFFFFEu;
+}
+
+// Fail-closed re-validation of a device path: provider_path_is_valid
+// (negotiated.rs:1284-1331) plus the ragged result checks of
+// mr-metal lib.rs:902-974 (window, recomputed cost == search cost).
+bool path_ok(const HostGrid& g, const mrf_net& n, const NetPlan& p, const mrf_window& w, int32_t own_id,
+             const uint32_t* path, size_t len, uint32_t search_cost, bool lee) {
+  size_t total = dims_len(g.d);
+  if (len == 0 || len > total || path[0] != n.src || path[len - 1] != n.dst) return false;
+  uint32_t plane = g.d.w * g.d.h;
+  std::unordered_set<uint32_t> seen;
+  seen.reserve(len * 2);
+  uint64_t cost = 0;
+  for (size_t i = 0; i < len; ++i) {
+    uint32_t c = path[i];
+    if (c >= total || g.trace(c) || (g.obst(c) && !sorted_has(p.pads, c)) || !seen.insert(c).second) return false;
+    uint32_t r = c % plane, x = r % g.d.w, y = r / g.d.w;
+    if (x < w.x0 || x > w.x1 || y < w.y0 || y > w.y1) return false;
+    if (i == 0) continue;
+    uint32_t a = path[i - 1];
+    uint32_t al = a / plane, ar = a % plane, ax = ar % g.d.w, ay = ar / g.d.w, cl = c / plane;
+    uint32_t e;
+    if (al == cl) {
+      bool adj = (ax == x && (ay + 1 == y || y + 1 == ay)) || (ay == y && (ax + 1 == x || x + 1 == ax));
+      if (!adj || g.planar_forbidden(a, c)) return false;
+      e = lee ? 1u : (ax != x ? g.xe[std::min(ax, x)] : g.ye[std::min(ay, y)]);
+    } else {
+      uint32_t lo = std::min(al, cl), hi = std::max(al, cl);
+      if (lee || ar != r || hi != lo + 1 || !g.va[lo] || g.vianode(a) || g.vianode(c)) return false;
+      if ((g.viaforb(a) && !sorted_has(p.vpads, a)) || (g.viaforb(c) && !sorted_has(p.vpads, c))) return false;
+      e = g.ve[lo];
+    }
+    uint32_t wgt = (g.obst(c) && sorted_has(p.pads, c)) ? 1u : g.cost[c];
+    uint64_t add = 0;
+    if (g.add) add += g.add[c];
+    if (g.halo && g.halo[c] != -1 && g.halo[c] != own_id) add += g.halo_pen;
+    cost += step_price(e, wgt, add);
+    if (cost >= kMax) return false;
+  }
+  return cost == search_cost;
+}
+
+struct Batch {
+  std::vector<uint8_t> routed;
+  std::vector<uint32_t> cost;
+  std::vector<std::vector<uint32_t>> paths;
+};
+
+// Solve one pass of nets (each inside its window) on a resident board, chunked
+// by the label budget, re-validating every path on the host.
+void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mrf_net* nets,
+                const std::vector<size_t>& idx, const std::vector<mrf_window>& wins, bool lee,
+                const int32_t* net_ids, Batch& out) {
+  size_t n = idx.size();
+  out.routed.assign(n, 0);
+  out.cost.assign(n, kMax);
+  out.paths.assign(n, {});
+  std::vector<NetPlan> plans(n);
+  for (size_t k = 0; k < n; ++k) {
+    const mrf_net& nt = nets[idx[k]];
+    plans[k].pads = sorted_unique(nt.pads, nt.n_pads);
+    plans[k].vpads = sorted_unique(nt.via_pads, nt.n_via_pads);
+  }
+  size_t budget_words = std::max<size_t>(c.label_budget / 8, 1);
+  size_t k0 = 0;
+  while (k0 < n) {
+    size_t words = 0, k1 = k0;
+    while (k1 < n) {
+      const mrf_window& w = wins[k1];
+      size_t fw = mrf::field_label_words(c.shape, w.x1 - w.x0 + 1, w.y1 - w.y0 + 1, g.d.layers);
+      if (k1 > k0 && words + fw > budget_words) break;
+      words += fw;
+      ++k1;
+    }
+    std::vector<mrf::FieldReq> reqs(k1 - k0);
+    for (size_t k = k0; k < k1; ++k) {
+      const mrf_net& nt = nets[idx[k]];
+      mrf::FieldReq& q = reqs[k - k0];
+      q.x0 = wins[k].x0;
+      q.y0 = wins[k].y0;
+      q.x1 = wins[k].x1;
+      q.y1 = wins[k].y1;
+      q.sources = {nt.src};
+      q.dst = nt.dst;
+      q.pads = plans[k].pads;
+      q.via_pads = plans[k].vpads;
+      q.own_id = net_ids ? net_ids[idx[k]] : -1;
+      if (!lee) {
+        q.forced = {nt.src, nt.dst};
+        q.flags = mrf::F_FORCE;
+      }
+    }
+    mrf::SolveStats st;
+    mrf::solve_fields(c, b, reqs, &st);
+    std::vector<uint8_t> routed;
+    std::vector<uint32_t> cost, cells;
+    std::vector<uint64_t> off;
+    mrf::reconstruct_paths(c, b, reqs.size(), routed, cost, off, cells, &st);
+    if (c.inject == 1) throw Error(MRF_BACKEND_UNAVAILABLE, "injected backend failure");
+    if (c.inject == 2 && !cells.empty()) cells[cells.size() / 2] ^= 1u;
+    for (size_t k = k0; k < k1; ++k) {
+      size_t j = k - k0;
+      out.routed[k] = routed[j];
+      if (!routed[j]) continue;
+      const uint32_t* p = cells.data() + off[j];
+      size_t len = off[j + 1] - off[j];
+      int32_t own = net_ids ? net_ids[idx[k]] : -1;
+      if (!path_ok(g, nets[idx[k]], plans[k], wins[k], own, p, len, cost[j], lee))
+        throw Error(MRF_BACKEND_UNAVAILABLE, "device path failed host re-validation");
+      out.cost[k] = cost[j];
+      out.paths[k].assign(p, p + len);
+    }
+    c.last.rounds += 0;
+    k0 = k1;
+  }
+}
+
+void export_paths(const Batch& b, mrf_paths* out) {
+  size_t n = b.routed.size();
+  size_t total = 0;
+  for (auto& p : b.paths) total += p.size();
+  out->n = n;
+  out->routed = (uint8_t*)malloc(std::max<size_t>(n, 1));
+  out->search_cost = (uint32_t*)malloc(std::max<size_t>(n, 1) * 4);
+  out->offsets = (uint64_t*)malloc((n + 1) * 8);
+  out->cells = (uint32_t*)malloc(std::max<size_t>(total, 1) * 4);
+  if (!out->routed || !out->search_cost || !out->offsets || !out->cells) throw std::bad_alloc();
+  out->offsets[0] = 0;
+  for (size_t i = 0; i < n; ++i) {
+    out->routed[i] = b.routed[i];
+    out->search_cost[i] = b.cost[i];
+    memcpy(out->cells + out->offsets[i], b.paths[i].data(), b.paths[i].size() * 4);
+    out->offsets[i + 1] = out->offsets[i] + b.paths[i].size();
+  }
+}
+
+HostGrid host_grid(mrf_dims d, const mrf_grid* g, const mrf_edges* e) {
+  HostGrid h;
+  h.d = d;
+  h.cost = g->cost;
+  h.vf = g->n_via_forbidden ? g->via_forbidden : nullptr;
+  h.board = g->n_board_constraint ? g->board_constraint : nullptr;
+  if (e) {
+    h.xe = e->x;
+    h.ye = e->y;
+    h.ve = e->via;
+    h.va = e->via_allowed;
+  }
+  return h;
+}
+
+struct Lane {
+  std::unique_lock<std::mutex> lk;
+  explicit Lane(mrf::Ctx& c) : lk(c.lane, std::try_to_lock) {
+    if (!lk.owns_lock()) throw Error(MRF_BACKEND_UNAVAILABLE, "field engine lane is busy");
+    if (hipSetDevice(c.device) != hipSuccess) throw Error(MRF_BACKEND_UNAVAILABLE, "hipSetDevice failed");
+  }
+};
+
+// Unit planar prices for the Lee / planar-field entries (no vias).
+struct UnitEdges {
+  std::vector<uint32_t> x, y, v;
+  std::vector<uint8_t> a;
+  mrf_edges e{};
+  explicit UnitEdges(mrf_dims d)
+      : x(std::max<uint32_t>(d.w, 1), 1u), y(std::max<uint32_t>(d.h, 1), 1u), v(std::max<uint32_t>(d.layers, 1), 0u),
+        a(std::max<uint32_t>(d.layers, 1), 0u) {
+    e.x = x.data();
+    e.nx = d.w - 1;
+    e.y = y.data();
+    e.ny = d.h - 1;
+    e.via = v.data();
+    e.via_allowed = a.data();
+    e.nvia = d.layers - 1;
+  }
+};
+
+void upload(mrf::Ctx& c, mrf::DevBoard& b, mrf_dims d, const mrf_grid* g, const mrf_edges* e,
+            const mrf_pricing* pr) {
+  const uint32_t* add = nullptr;
+  const int32_t* halo = nullptr;
+  uint32_t pen = 0;
+  size_t n = dims_len(d);
+  if (pr) {
+    if (pr->n_additive) {
+      if (pr->n_additive != n || !pr->additive) throw Error(MRF_MALFORMED_GRID, "additive plane length does not match dims");
+      add = pr->additive;
+    }
+    if (pr->n_halo_owner) {
+      if (pr->n_halo_owner != n || !pr->halo_owner) throw Error(MRF_MALFORMED_GRID, "halo plane length does not match dims");
+      halo = pr->halo_owner;
+      pen = pr->halo_penalty;
+    }
+  }
+  mrf::board_upload(c, b, d.w, d.h, d.layers, g->cost, g->n_via_forbidden ? g->via_forbidden : nullptr,
+                    g->n_board_constraint ? g->board_constraint : nullptr, e->x, e->y, e->via, e->via_allowed,
+                    add, halo, pen);
+}
+
+int isolated(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* nets, size_t n_nets,
+             const mrf_window* windows, const mrf_edges* edges, mrf_paths* out, mrf_error* err, bool retry) {
+  if (out) memset(out, 0, sizeof(*out));
+  if (!ctx || !out) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, "null context or output");
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+  mrf::DevBoard b;
+  try {
+    Lane lane(ctx->c);
+    check_grid(dims, grid);
+    check_edges(dims, edges);
+    if (n_nets && (!nets || !windows)) throw Error(MRF_BACKEND_UNAVAILABLE, "isolated request windows are not net-aligned");
+    HostGrid g = host_grid(dims, grid, edges);
+    for (size_t i = 0; i < n_nets; ++i) check_window(dims, windows[i], i);
+    for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
+    Batch all;
+    all.routed.assign(n_nets, 0);
+    all.cost.assign(n_nets, kMax);
+    all.paths.assign(n_nets, {});
+    if (n_nets) {
+      upload(ctx->c, b, dims, grid, edges, nullptr);
+      std::vector<size_t> idx(n_nets);
+      for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
+      std::vector<mrf_window> wins(windows, windows + n_nets);
+      Batch first;
+      solve_nets(ctx->c, b, g, nets, idx, wins, false, nullptr, first);
+      std::vector<size_t> retry_idx;
+      for (size_t i = 0; i < n_nets; ++i) {
+        if (first.routed[i]) {
+          all.routed[i] = 1;
+          all.cost[i] = first.cost[i];
+          all.paths[i] = std::move(first.paths[i]);
+        } else {
+          retry_idx.push_back(i);
+        }
+      }
+      if (retry && !retry_idx.empty()) {
+        mrf_window full{0, 0, dims.w - 1, dims.h - 1};
+        std::vector<mrf_window> fw(retry_idx.size(), full);
+        Batch second;
+        solve_nets(ctx->c, b, g, nets, retry_idx, fw, false, nullptr, second);
+        for (size_t k = 0; k < retry_idx.size(); ++k) {
+          size_t i = retry_idx[k];
+          all.routed[i] = second.routed[k];
+          all.cost[i] = second.cost[k];
+          all.paths[i] = std::move(second.paths[k]);
+        }
+      }
+    }
+    mrf::board_free(b);
+    export_paths(all, out);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf::board_free(b);
+    mrf_free_paths(out);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf::board_free(b);
+    mrf_free_paths(out);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mrf_abi_version(void) { return MRF_ABI_VERSION; }
+
+int mrf_open(int device, mrf_ctx** out, mrf_error* err) {
+  if (!out) return MRF_BACKEND_UNAVAILABLE;
+  *out = nullptr;
+  mrf_ctx* c = new (std::nothrow) mrf_ctx();
+  if (!c) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, "out of host memory");
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+  try {
+    mrf::ctx_init(c->c, device);
+    const char* inj = getenv("MRF_INJECT_FAULT");
+    if (inj) c->c.inject = atoi(inj);
+  } catch (const Error& e) {
+    mrf::ctx_free(c->c);
+    delete c;
+    return fail(err, e);
+  }
+  *out = c;
+  if (err) set_err(err, MRF_OK, "ok");
+  return MRF_OK;
+}
+
+void mrf_close(mrf_ctx* ctx) {
+  if (!ctx) return;
+  mrf::ctx_free(ctx->c);
+  delete ctx;
+}
+
+int mrf_set_tiles(mrf_ctx* ctx, uint32_t tw, uint32_t th, uint32_t threads, uint32_t max_local, mrf_error* err) {
+  if (!ctx || tw < 4 || tw > 128 || th < 1 || th > 128 || threads < 64 || threads > 1024 || threads % 64 ||
+      max_local < 1) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, "invalid tile shape");
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+  std::lock_guard<std::mutex> g(ctx->c.lane);
+  ctx->c.shape.tw = tw;
+  ctx->c.shape.th = th;
+  ctx->c.shape.threads = threads;
+  ctx->c.shape.max_local = max_local;
+  return MRF_OK;
+}
+
+int mrf_set_mode(mrf_ctx* ctx, int mode) {
+  if (!ctx || (mode != MRF_MODE_PLAIN && mode != MRF_MODE_PERSISTENT)) return MRF_BACKEND_UNAVAILABLE;
+  std::lock_guard<std::mutex> g(ctx->c.lane);
+  ctx->c.mode = mode;
+  return MRF_OK;
+}
+
+int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
+  if (!ctx || !out) return MRF_BACKEND_UNAVAILABLE;
+  const mrf::SolveStats& s = ctx->c.last;
+  out->relax_ms = s.relax_ms;
+  out->path_ms = s.path_ms;
+  out->total_ms = s.total_ms;
+  out->tile_visits = s.tile_visits;
+  out->local_rounds = s.local_rounds;
+  out->cells_visited = s.cells_visited;
+  out->field_cells = s.field_cells;
+  out->host_rounds = s.rounds;
+  return MRF_OK;
+}
+
+int mrf_route_isolated_batch(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* nets, size_t n_nets,
+                             const mrf_window* windows, const mrf_edges* edges, mrf_paths* out, mrf_error* err) {
+  return isolated(ctx, dims, grid, nets, n_nets, windows, edges, out, err, true);
+}
+
+int mrf_route_isolated_pass(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* nets, size_t n_nets,
+                            const mrf_window* windows, const mrf_edges* edges, mrf_paths* out, mrf_error* err) {
+  return isolated(ctx, dims, grid, nets, n_nets, windows, edges, out, err, false);
+}
+
+int mrf_router_route(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* nets, size_t n_nets,
+                     mrf_paths* out, mrf_error* err) {
+  if (out) memset(out, 0, sizeof(*out));
+  if (!ctx || !out) return MRF_BACKEND_UNAVAILABLE;
+  mrf::DevBoard b;
+  try {
+    Lane lane(ctx->c);
+    check_grid(dims, grid);
+    UnitEdges ue(dims);
+    HostGrid g = host_grid(dims, grid, &ue.e);
+    if (n_nets && !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "null nets");
+    // LeeRouter::route validation order (lee.rs:73-97): pads, then endpoints.
+    for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
+    Batch res;
+    if (n_nets) {
+      upload(ctx->c, b, dims, grid, &ue.e, nullptr);
+      std::vector<size_t> idx(n_nets);
+      for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
+      std::vector<mrf_window> wins(n_nets, mrf_window{0, 0, dims.w - 1, dims.h - 1});
+      solve_nets(ctx->c, b, g, nets, idx, wins, true, nullptr, res);
+    }
+    mrf::board_free(b);
+    export_paths(res, out);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf::board_free(b);
+    mrf_free_paths(out);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf::board_free(b);
+    mrf_free_paths(out);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+static int fields_impl(mrf::Ctx& c, const mrf::DevBoard& b, mrf_dims dims, const uint32_t* const* sources,
+                       const size_t* n_sources, const uint32_t* dsts, const int32_t* net_ids, size_t n_fields,
+                       const mrf_window* windows, bool force, bool lee, uint32_t* dist_out, uint32_t* hops_out) {
+  size_t n = dims_len(dims);
+  size_t budget_words = std::max<size_t>(c.label_budget / 8, 1);
+  size_t k0 = 0;
+  while (k0 < n_fields) {
+    size_t words = 0, k1 = k0;
+    while (k1 < n_fields) {
+      mrf_window w = windows ? windows[k1] : mrf_window{0, 0, dims.w - 1, dims.h - 1};
+      size_t fw = mrf::field_label_words(c.shape, w.x1 - w.x0 + 1, w.y1 - w.y0 + 1, dims.layers);
+      if (k1 > k0 && words + fw > budget_words) break;
+      words += fw;
+      ++k1;
+    }
+    std::vector<mrf::FieldReq> reqs(k1 - k0);
+    for (size_t k = k0; k < k1; ++k) {
+      mrf::FieldReq& q = reqs[k - k0];
+      mrf_window w = windows ? windows[k] : mrf_window{0, 0, dims.w - 1, dims.h - 1};
+      q.x0 = w.x0;
+      q.y0 = w.y0;
+      q.x1 = w.x1;
+      q.y1 = w.y1;
+      for (size_t j = 0; j < n_sources[k]; ++j)
+        if (sources[k][j] < n) q.sources.push_back(sources[k][j]);
+      q.dst = (dsts && dsts[k] < n) ? dsts[k] : kMax;
+      q.own_id = net_ids ? net_ids[k] : -1;
+      if (force && !lee) {
+        q.forced = q.sources;
+        if (q.dst != kMax) q.forced.push_back(q.dst);
+        q.flags = mrf::F_FORCE;
+      }
+    }
+    mrf::SolveStats st;
+    mrf::solve_fields(c, b, reqs, &st);
+    if (dist_out)
+      for (size_t k = k0; k < k1; ++k)
+        mrf::read_field(c, b, k - k0, dist_out + k * n, hops_out ? hops_out + k * n : nullptr);
+    k0 = k1;
+  }
+  return MRF_OK;
+}
+
+int mrf_sweep_fields(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const uint32_t* sources, size_t n_sources,
+                     uint32_t* dist_out, mrf_error* err) {
+  if (!ctx) return MRF_BACKEND_UNAVAILABLE;
+  mrf::DevBoard b;
+  try {
+    Lane lane(ctx->c);
+    check_grid(dims, grid);
+    if (n_sources && (!sources || !dist_out)) throw Error(MRF_BACKEND_UNAVAILABLE, "null sources or output");
+    UnitEdges ue(dims);
+    if (n_sources) {
+      upload(ctx->c, b, dims, grid, &ue.e, nullptr);
+      std::vector<const uint32_t*> sp(n_sources);
+      std::vector<size_t> ns(n_sources, 1);
+      for (size_t i = 0; i < n_sources; ++i) sp[i] = sources + i;
+      fields_impl(ctx->c, b, dims, sp.data(), ns.data(), nullptr, nullptr, n_sources, nullptr, false, true, dist_out,
+                  nullptr);
+    }
+    mrf::board_free(b);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf::board_free(b);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf::board_free(b);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+int mrf_fields(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_edges* edges, const mrf_pricing* pricing,
+               const uint32_t* const* sources, const size_t* n_sources, const uint32_t* dsts, size_t n_fields,
+               const mrf_window* windows, uint32_t* dist_out, uint32_t* hops_out, mrf_error* err) {
+  if (!ctx) return MRF_BACKEND_UNAVAILABLE;
+  mrf::DevBoard b;
+  try {
+    Lane lane(ctx->c);
+    check_grid(dims, grid);
+    check_edges(dims, edges);
+    if (n_fields && (!sources || !n_sources || !dist_out)) throw Error(MRF_BACKEND_UNAVAILABLE, "null sources or output");
+    for (size_t i = 0; windows && i < n_fields; ++i) check_window(dims, windows[i], i);
+    if (n_fields) {
+      upload(ctx->c, b, dims, grid, edges, pricing);
+      fields_impl(ctx->c, b, dims, sources, n_sources, dsts, pricing ? pricing->net_ids : nullptr, n_fields, windows,
+                  true, false, dist_out, hops_out);
+    }
+    mrf::board_free(b);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf::board_free(b);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf::board_free(b);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+void mrf_free_paths(mrf_paths* p) {
+  if (!p) return;
+  free(p->routed);
+  free(p->search_cost);
+  free(p->offsets);
+  free(p->cells);
+  memset(p, 0, sizeof(*p));
+}
+
+// ---- resident boards ----------------------------------------------------------
+
+int mrf_board_create(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_edges* edges,
+                     const mrf_pricing* pricing, mrf_board** out, mrf_error* err) {
+  if (!ctx || !out) return MRF_BACKEND_UNAVAILABLE;
+  *out = nullptr;
+  mrf_board* bd = new (std::nothrow) mrf_board();
+  if (!bd) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(ctx->c);
+    check_grid(dims, grid);
+    check_edges(dims, edges);
+    size_t n = dims_len(dims);
+    bd->ctx = ctx;
+    bd->dims = dims;
+    bd->cost.assign(grid->cost, grid->cost + n);
+    if (grid->n_via_forbidden) bd->vf.assign(grid->via_forbidden, grid->via_forbidden + n);
+    if (grid->n_board_constraint) bd->board.assign(grid->board_constraint, grid->board_constraint + n);
+    bd->xe.assign(edges->x, edges->x + edges->nx);
+    bd->ye.assign(edges->y, edges->y + edges->ny);
+    bd->ve.assign(edges->via, edges->via + edges->nvia);
+    bd->va.assign(edges->via_allowed, edges->via_allowed + edges->nvia);
+    bd->xe.push_back(0);
+    bd->ye.push_back(0);
+    bd->ve.push_back(0);
+    bd->va.push_back(0);
+    if (pricing && pricing->n_additive) bd->add.assign(pricing->additive, pricing->additive + pricing->n_additive);
+    if (pricing && pricing->n_halo_owner) {
+      bd->halo.assign(pricing->halo_owner, pricing->halo_owner + pricing->n_halo_owner);
+      bd->halo_pen = pricing->halo_penalty;
+    }
+    upload(ctx->c, bd->b, dims, grid, edges, pricing);
+  } catch (const Error& e) {
+    mrf::board_free(bd->b);
+    delete bd;
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf::board_free(bd->b);
+    delete bd;
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+  *out = bd;
+  if (err) set_err(err, MRF_OK, "ok");
+  return MRF_OK;
+}
+
+void mrf_board_destroy(mrf_board* bd) {
+  if (!bd) return;
+  if (bd->ctx) {
+    std::lock_guard<std::mutex> g(bd->ctx->c.lane);
+    (void)hipSetDevice(bd->ctx->c.device);
+    mrf::board_free(bd->b);
+  }
+  delete bd;
+}
+
+size_t mrf_board_bytes(const mrf_board* bd) { return bd ? bd->b.bytes : 0; }
+
+static HostGrid board_host(const mrf_board* bd) {
+  HostGrid g;
+  g.d = bd->dims;
+  g.cost = bd->cost.data();
+  g.vf = bd->vf.empty() ? nullptr : bd->vf.data();
+  g.board = bd->board.empty() ? nullptr : bd->board.data();
+  g.xe = bd->xe.data();
+  g.ye = bd->ye.data();
+  g.ve = bd->ve.data();
+  g.va = bd->va.data();
+  g.add = bd->add.empty() ? nullptr : bd->add.data();
+  g.halo = bd->halo.empty() ? nullptr : bd->halo.data();
+  g.halo_pen = bd->halo_pen;
+  return g;
+}
+
+int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
+              int retry_full, mrf_paths* out, mrf_error* err) {
+  if (out) memset(out, 0, sizeof(*out));
+  if (!ctx || !bd || !out || bd->ctx != ctx) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(ctx->c);
+    HostGrid g = board_host(bd);
+    mrf_dims dims = bd->dims;
+    if (n_nets && !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "null nets");
+    for (size_t i = 0; windows && i < n_nets; ++i) check_window(dims, windows[i], i);
+    for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
+    mrf_window full{0, 0, dims.w - 1, dims.h - 1};
+    std::vector<size_t> idx(n_nets);
+    for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
+    std::vector<mrf_window> wins(n_nets, full);
+    if (windows) wins.assign(windows, windows + n_nets);
+    Batch all;
+    solve_nets(ctx->c, bd->b, g, nets, idx, wins, false, nullptr, all);
+    mrf::SolveStats first = ctx->c.last;
+    if (retry_full && windows) {
+      std::vector<size_t> retry_idx;
+      for (size_t i = 0; i < n_nets; ++i)
+        if (!all.routed[i]) retry_idx.push_back(i);
+      if (!retry_idx.empty()) {
+        std::vector<mrf_window> fw(retry_idx.size(), full);
+        Batch second;
+        solve_nets(ctx->c, bd->b, g, nets, retry_idx, fw, false, nullptr, second);
+        for (size_t k = 0; k < retry_idx.size(); ++k) {
+          size_t i = retry_idx[k];
+          all.routed[i] = second.routed[k];
+          all.cost[i] = second.cost[k];
+          all.paths[i] = std::move(second.paths[k]);
+        }
+        mrf::SolveStats& s = ctx->c.last;
+        s.relax_ms += first.relax_ms;
+        s.path_ms += first.path_ms;
+        s.tile_visits += first.tile_visits;
+        s.local_rounds += first.local_rounds;
+        s.cells_visited += first.cells_visited;
+        s.field_cells += first.field_cells;
+        s.rounds += first.rounds;
+      }
+    }
+    export_paths(all, out);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf_free_paths(out);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf_free_paths(out);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+int mrf_solve_fields(mrf_ctx* ctx, mrf_board* bd, const uint32_t* const* sources, const size_t* n_sources,
+                     const uint32_t* dsts, const int32_t* net_ids, size_t n_fields, const mrf_window* windows,
+                     mrf_error* err) {
+  if (!ctx || !bd || bd->ctx != ctx) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(ctx->c);
+    mrf_dims dims = bd->dims;
+    if (n_fields && (!sources || !n_sources)) throw Error(MRF_BACKEND_UNAVAILABLE, "null sources");
+    for (size_t i = 0; windows && i < n_fields; ++i) check_window(dims, windows[i], i);
+    size_t total = 0;
+    for (size_t k = 0; k < n_fields; ++k) {
+      mrf_window w = windows ? windows[k] : mrf_window{0, 0, dims.w - 1, dims.h - 1};
+      total += mrf::field_label_words(ctx->c.shape, w.x1 - w.x0 + 1, w.y1 - w.y0 + 1, dims.layers);
+    }
+    if (total * 8 > ctx->c.label_budget)
+      throw Error(MRF_BACKEND_UNAVAILABLE, "resident fields exceed the label budget; split the request");
+    fields_impl(ctx->c, bd->b, dims, sources, n_sources, dsts, net_ids, n_fields, windows, true, false, nullptr,
+                nullptr);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+int mrf_read_field(mrf_ctx* ctx, mrf_board* bd, size_t field, uint32_t* dist_out, uint32_t* hops_out,
+                   mrf_error* err) {
+  if (!ctx || !bd || !dist_out) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(ctx->c);
+    mrf::read_field(ctx->c, bd->b, field, dist_out, hops_out);
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Internal engine interface shared by the kernels (engine.hip), the C ABI
+// (capi.cpp) and the serving loop (serve.cpp).  Host-only C++; no torch types.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace mrf {
+
+constexpr uint32_t kMax = 0xFFFFFFFFu;  // mr-core OBSTACLE / unreachable sentinel
+
+// Board mask bits (one byte per cell, built once per resident board).
+enum : uint8_t {
+  M_TRACE = 0x01,  // BOARD_TRACE_NODE: never passable (mr-core lib.rs:445-452)
+  M_OBST = 0x02,   // cost == OBSTACLE
+  M_NY = 0x04,     // planar edge to y-1 open (is_board_planar_step_forbidden false)
+  M_NX = 0x08,     // planar edge to x-1 open
+  M_PX = 0x10,     // planar edge to x+1 open
+  M_PY = 0x20,     // planar edge to y+1 open
+  M_DN = 0x40,     // via to layer-1 legal without per-net exemptions
+  M_UP = 0x80,     // via to layer+1 legal without per-net exemptions
+};
+// Raw via bits kept for the per-net via-exemption path.
+enum : uint8_t { R_VIANODE = 0x01, R_VF = 0x02 };
+
+// Per-field flags.
+enum : uint32_t {
+  F_FORCE = 0x1,  // sources and dst stay passable unless trace-node (negotiated.rs:2946-2955)
+};
+
+struct BoardView {
+  uint32_t w, h, layers, plane;
+  const uint32_t* cost;     // enter weight; kMax = obstacle
+  const uint32_t* add;      // additive price plane or nullptr
+  const int32_t* halo;      // halo owner plane (-1 free) or nullptr
+  uint32_t halo_pen;
+  const uint8_t* bmask;     // M_* bits
+  const uint8_t* raw;       // R_* bits
+  const uint32_t* xe;       // planar x gap prices [w-1] (+1 pad)
+  const uint32_t* ye;       // planar y gap prices [h-1] (+1 pad)
+  const uint32_t* ve;       // via prices [layers-1] (+1 pad)
+};
+
+// One complete field: a window (all layers), its sources, overrides.
+struct FieldDesc {
+  uint64_t lab_off;    // offset of the field's first tile block in the label buffer (u64 units)
+  uint32_t x0, y0;     // window origin (board coords)
+  uint32_t ww, wh;     // window size
+  uint32_t tx, ty;     // tiles along x and y
+  uint32_t item0;      // first work item (tile) of the field
+  uint32_t src_off, n_src;    // sources in the cell pool
+  uint32_t forced_off, n_forced;  // forced-passable cells in the cell pool
+  uint32_t pad_off, n_pad;    // sorted own pads
+  uint32_t vpad_off, n_vpad;  // sorted via-exempt pads
+  uint32_t dst;        // target cell or kMax
+  int32_t own_id;      // halo owner id of this net
+  uint32_t flags;
+  uint32_t _pad;
+};
+
+struct TileShape {
+  uint32_t tw = 32, th = 32;  // tile width / height in cells (all layers per tile)
+  uint32_t threads = 256;     // workgroup size
+  uint32_t max_local = 32;    // in-LDS relaxation rounds per tile visit
+};
+
+struct DevBoard {
+  uint32_t w = 0, h = 0, layers = 0;
+  uint32_t* cost = nullptr;
+  uint32_t* add = nullptr;
+  int32_t* halo = nullptr;
+  uint32_t halo_pen = 0;
+  uint8_t* bmask = nullptr;
+  uint8_t* raw = nullptr;
+  uint32_t* xe = nullptr;
+  uint32_t* ye = nullptr;
+  uint32_t* ve = nullptr;
+  std::vector<uint8_t> via_allowed;  // host copy
+  size_t bytes = 0;
+  BoardView view() const;
+};
+
+// Host-side description of one field to solve.
+struct FieldReq {
+  uint32_t x0, y0, x1, y1;  // inclusive window
+  std::vector<uint32_t> sources;
+  std::vector<uint32_t> forced;
+  std::vector<uint32_t> pads;      // sorted unique
+  std::vector<uint32_t> via_pads;  // sorted unique
+  uint32_t dst = kMax;
+  int32_t own_id = -1;
+  uint32_t flags = 0;
+};
+
+struct SolveStats {
+  double relax_ms = 0, path_ms = 0, total_ms = 0;
+  uint64_t tile_visits = 0;    // device counter of tile loads (HBM sweep visits)
+  uint64_t local_rounds = 0;   // in-LDS relaxation rounds
+  uint64_t cells_visited = 0;  // tile_visits x tile cells
+  uint32_t rounds = 0;         // host rounds (plain form) or 1 (persistent)
+  uint64_t field_cells = 0;    // cells in all fields (window x layers)
+};
+
+enum SolveMode : int { MODE_PLAIN = 0, MODE_PERSISTENT = 1 };
+
+class Error : public std::exception {
+ public:
+  Error(int code, std::string msg, uint32_t net = 0) : code_(code), net_(net), msg_(std::move(msg)) {}
+  const char* what() const noexcept override { return msg_.c_str(); }
+  int code() const { return code_; }
+  uint32_t net() const { return net_; }
+
+ private:
+  int code_;
+  uint32_t net_;
+  std::string msg_;
+};
+
+// Device workspace for one batch of fields (grow-only buffers).
+struct Workspace {
+  uint64_t* labels = nullptr;
+  size_t labels_cap = 0;
+  uint32_t* state = nullptr;  // per-item state bits
+  size_t state_cap = 0;
+  uint8_t* act = nullptr;     // 2 x items activation flags
+  size_t act_cap = 0;
+  uint32_t* queue = nullptr;  // persistent work queue
+  size_t queue_cap = 0;
+  FieldDesc* fields = nullptr;
+  size_t fields_cap = 0;
+  uint32_t* item_field = nullptr;
+  size_t item_field_cap = 0;
+  uint32_t* pool = nullptr;  // cell pool (sources, forced, pads, seed items)
+  size_t pool_cap = 0;
+  uint32_t* counters = nullptr;  // device counters
+  uint32_t* plen = nullptr;      // per-field path length
+  size_t plen_cap = 0;
+  uint64_t* poff = nullptr;      // per-field path offsets
+  size_t poff_cap = 0;
+  uint32_t* pcost = nullptr;     // per-field search cost
+  size_t pcost_cap = 0;
+  uint32_t* pcells = nullptr;    // path cells
+  size_t pcells_cap = 0;
+  uint32_t* fout = nullptr;      // extracted field output staging
+  size_t fout_cap = 0;
+  uint32_t* pinned = nullptr;    // host-pinned counter readback
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex lane;  // single GPU lane (mr-cli lib.rs:499 METAL_ISOLATED_MUTEX)
+  Workspace ws;
+  TileShape shape;
+  int mode = MODE_PERSISTENT;
+  size_t label_budget = size_t(48) << 30;  // bytes of labels per chunk
+  int cus = 256;
+  SolveStats last;
+  std::vector<FieldDesc> layout;  // fields of the last solve (for paths / reads)
+  // Fault injection for fail-closed tests (0 none, 1 backend error, 2 corrupt path).
+  int inject = 0;
+};
+
+// ---- engine.hip -------------------------------------------------------------
+void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
+                  const uint32_t* cost, const uint8_t* via_forbidden, const uint8_t* board,
+                  const uint32_t* xe, const uint32_t* ye, const uint32_t* ve,
+                  const uint8_t* via_allowed, const uint32_t* add, const int32_t* halo,
+                  uint32_t halo_pen);
+void board_free(DevBoard& b);
+
+// Solve complete fields for `reqs` on a resident board.  Labels stay on the
+// device in the workspace until the next solve.
+void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, SolveStats* st);
+// After solve_fields: reconstruct canonical paths for fields with a dst.
+// routed[i]=0 when the dst is unreachable.
+void reconstruct_paths(Ctx& c, const DevBoard& b, size_t n_fields, std::vector<uint8_t>& routed,
+                       std::vector<uint32_t>& cost, std::vector<uint64_t>& off,
+                       std::vector<uint32_t>& cells, SolveStats* st);
+// After solve_fields: copy field i's dist (and hops) out, full board layout
+// (cells outside the window are kMax).
+void read_field(Ctx& c, const DevBoard& b, size_t field, uint32_t* dist, uint32_t* hops);
+// Device pointer variant used by tests of the persistent kernel: copy all
+// fields as window-local row-major (dist) into a host buffer.
+void ctx_init(Ctx& c, int device);
+void ctx_free(Ctx& c);
+size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t layers);
+
+}  // namespace mrf

@@ -1,0 +1,487 @@
+"""Python host binding of libmrfield.so (the MI355X field engine's C ABI).
+
+Mirrors the reference's provider seam for tests and benchmarks:
+``Engine.route_isolated_batch`` is IsolatedRouteProvider::route_isolated_batch
+(crates/mr-cpu/src/negotiated.rs:554-560) through MetalIsolatedProvider
+(crates/mr-cli/src/lib.rs:612-625); ``route_isolated_pass`` is
+metal_route_isolated_batch (crates/mr-metal/src/lib.rs:414); ``sweep_fields``
+is metal_sweep_fields (:172); ``router_route`` is MetalRouter::route (:1143).
+
+There is no CPU fallback: every call runs the HIP kernels, and loading fails
+loudly when the library or a HIP device is missing.  Errors raise
+``MrfError`` carrying the RouterError code (crates/mr-core/src/lib.rs:699-708).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmrfield.so")
+
+MAX = 0xFFFFFFFF
+OK, BACKEND_UNAVAILABLE, INVALID_ENDPOINT, MALFORMED_GRID = 0, 1, 2, 3
+MODE_PLAIN, MODE_PERSISTENT = 0, 1
+
+U32P = C.POINTER(C.c_uint32)
+U8P = C.POINTER(C.c_uint8)
+I32P = C.POINTER(C.c_int32)
+U64P = C.POINTER(C.c_uint64)
+
+
+class Dims(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("layers", C.c_uint32)]
+
+
+class GridS(C.Structure):
+    _fields_ = [("cost", U32P), ("n_cost", C.c_size_t), ("via_forbidden", U8P), ("n_via_forbidden", C.c_size_t),
+                ("board_constraint", U8P), ("n_board_constraint", C.c_size_t)]
+
+
+class NetS(C.Structure):
+    _fields_ = [("src", C.c_uint32), ("dst", C.c_uint32), ("pads", U32P), ("n_pads", C.c_size_t),
+                ("via_pads", U32P), ("n_via_pads", C.c_size_t)]
+
+
+class WindowS(C.Structure):
+    _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("x1", C.c_uint32), ("y1", C.c_uint32)]
+
+
+class EdgesS(C.Structure):
+    _fields_ = [("x", U32P), ("nx", C.c_size_t), ("y", U32P), ("ny", C.c_size_t), ("via", U32P),
+                ("via_allowed", U8P), ("nvia", C.c_size_t)]
+
+
+class PricingS(C.Structure):
+    _fields_ = [("additive", U32P), ("n_additive", C.c_size_t), ("halo_owner", I32P), ("n_halo_owner", C.c_size_t),
+                ("halo_penalty", C.c_uint32), ("net_ids", I32P)]
+
+
+class ErrorS(C.Structure):
+    _fields_ = [("code", C.c_int), ("net", C.c_uint32), ("message", C.c_char * 256)]
+
+
+class PathsS(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("routed", U8P), ("search_cost", U32P), ("offsets", U64P), ("cells", U32P)]
+
+
+class StatsS(C.Structure):
+    _fields_ = [("relax_ms", C.c_double), ("path_ms", C.c_double), ("total_ms", C.c_double),
+                ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
+                ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32)]
+
+
+class SynthS(C.Structure):
+    _fields_ = [("dims", Dims), ("cost", U32P), ("board_constraint", U8P), ("additive", U32P),
+                ("halo_owner", I32P), ("halo_penalty", C.c_uint32), ("x_edges", U32P), ("y_edges", U32P),
+                ("via_edges", U32P), ("via_allowed", U8P), ("n_nets", C.c_size_t), ("src", U32P), ("dst", U32P),
+                ("pin_off", U64P), ("pins", U32P), ("windows", C.POINTER(WindowS))]
+
+
+class MrfError(RuntimeError):
+    def __init__(self, code: int, message: str, net: int = 0):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.net = net
+
+
+def build() -> None:
+    """Compile libmrfield.so for gfx950 (hipcc, in-tree)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MrfError(BACKEND_UNAVAILABLE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    E = C.POINTER(ErrorS)
+    sig = {
+        "mrf_abi_version": (C.c_int, []),
+        "mrf_open": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), E]),
+        "mrf_close": (None, [C.c_void_p]),
+        "mrf_set_tiles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, E]),
+        "mrf_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
+        "mrf_last_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsS)]),
+        "mrf_route_isolated_batch": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(NetS), C.c_size_t,
+                                               C.POINTER(WindowS), C.POINTER(EdgesS), C.POINTER(PathsS), E]),
+        "mrf_route_isolated_pass": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(NetS), C.c_size_t,
+                                              C.POINTER(WindowS), C.POINTER(EdgesS), C.POINTER(PathsS), E]),
+        "mrf_sweep_fields": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), U32P, C.c_size_t, U32P, E]),
+        "mrf_router_route": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(NetS), C.c_size_t,
+                                       C.POINTER(PathsS), E]),
+        "mrf_fields": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(EdgesS), C.POINTER(PricingS),
+                                 C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, C.c_size_t, C.POINTER(WindowS), U32P,
+                                 U32P, E]),
+        "mrf_free_paths": (None, [C.POINTER(PathsS)]),
+        "mrf_board_create": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(EdgesS), C.POINTER(PricingS),
+                                       C.POINTER(C.c_void_p), E]),
+        "mrf_board_destroy": (None, [C.c_void_p]),
+        "mrf_board_bytes": (C.c_size_t, [C.c_void_p]),
+        "mrf_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t, C.POINTER(WindowS), C.c_int,
+                                C.POINTER(PathsS), E]),
+        "mrf_solve_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, I32P,
+                                       C.c_size_t, C.POINTER(WindowS), E]),
+        "mrf_read_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, U32P, U32P, E]),
+        "mrf_synth_config": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(SynthS), E]),
+        "mrf_synth_board": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(SynthS)]),
+        "mrf_free_synth": (None, [C.POINTER(SynthS)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> List[str]:
+    return [n for n in dir(lib()) if n.startswith("mrf_")]
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _p32(a):
+    return a.ctypes.data_as(U32P) if a is not None else None
+
+
+def _p8(a):
+    return a.ctypes.data_as(U8P) if a is not None else None
+
+
+def _check(rc: int, err: ErrorS):
+    if rc != OK:
+        raise MrfError(rc, err.message.decode(errors="replace"), err.net)
+
+
+@dataclass
+class Grid:
+    """mr-core Grid (crates/mr-core/src/lib.rs:394-408)."""
+
+    w: int
+    h: int
+    layers: int
+    cost: np.ndarray
+    via_forbidden: Optional[np.ndarray] = None
+    board: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.cost = _u32(self.cost)
+        if self.via_forbidden is not None:
+            self.via_forbidden = np.ascontiguousarray(np.asarray(self.via_forbidden, dtype=np.uint8))
+        if self.board is not None:
+            self.board = np.ascontiguousarray(np.asarray(self.board, dtype=np.uint8))
+
+    @property
+    def n(self):
+        return self.w * self.h * self.layers
+
+    def dims(self):
+        return Dims(self.w, self.h, self.layers)
+
+    def cstruct(self):
+        return GridS(_p32(self.cost), len(self.cost),
+                     _p8(self.via_forbidden), 0 if self.via_forbidden is None else len(self.via_forbidden),
+                     _p8(self.board), 0 if self.board is None else len(self.board))
+
+
+@dataclass
+class Edges:
+    """Rounded request prices (negotiated.rs:530-537); None via = forbidden step."""
+
+    x: Sequence[int]
+    y: Sequence[int]
+    via: Sequence[Optional[int]]
+
+    def cstruct(self):
+        self._x = _u32(list(self.x) + [0])
+        self._y = _u32(list(self.y) + [0])
+        self._v = _u32([0 if v is None else v for v in self.via] + [0])
+        self._a = np.ascontiguousarray(np.array([0 if v is None else 1 for v in self.via] + [0], dtype=np.uint8))
+        return EdgesS(_p32(self._x), len(self.x), _p32(self._y), len(self.y), _p32(self._v), _p8(self._a),
+                      len(self.via))
+
+
+@dataclass
+class Net:
+    src: int
+    dst: int
+    pads: Sequence[int] = ()
+    via_pads: Sequence[int] = ()
+
+
+@dataclass
+class Pricing:
+    additive: Optional[np.ndarray] = None
+    halo_owner: Optional[np.ndarray] = None
+    halo_penalty: int = 0
+    net_ids: Optional[Sequence[int]] = None
+
+    def cstruct(self):
+        self._a = _u32(self.additive) if self.additive is not None else None
+        self._h = (np.ascontiguousarray(np.asarray(self.halo_owner, dtype=np.int32))
+                   if self.halo_owner is not None else None)
+        self._i = (np.ascontiguousarray(np.asarray(self.net_ids, dtype=np.int32))
+                   if self.net_ids is not None else None)
+        return PricingS(_p32(self._a), 0 if self._a is None else len(self._a),
+                        self._h.ctypes.data_as(I32P) if self._h is not None else None,
+                        0 if self._h is None else len(self._h), self.halo_penalty,
+                        self._i.ctypes.data_as(I32P) if self._i is not None else None)
+
+
+def _nets(nets: Sequence[Net]):
+    keep = []
+    arr = (NetS * max(len(nets), 1))()
+    for i, n in enumerate(nets):
+        p = _u32(list(n.pads) + [0])
+        v = _u32(list(n.via_pads) + [0])
+        keep += [p, v]
+        arr[i] = NetS(n.src, n.dst, _p32(p), len(n.pads), _p32(v), len(n.via_pads))
+    return arr, keep
+
+
+def _windows(wins, n):
+    if wins is None:
+        return None
+    return (WindowS * max(n, 1))(*[WindowS(*w) for w in wins])
+
+
+def _take_paths(p: PathsS) -> List[Optional[Tuple[List[int], int]]]:
+    out = []
+    try:
+        n = p.n
+        if n:
+            off = np.ctypeslib.as_array(p.offsets, shape=(n + 1,)).copy()
+            total = int(off[-1])
+            cells = np.ctypeslib.as_array(p.cells, shape=(max(total, 1),))[:total].copy()
+            routed = np.ctypeslib.as_array(p.routed, shape=(n,)).copy()
+            cost = np.ctypeslib.as_array(p.search_cost, shape=(n,)).copy()
+            for i in range(n):
+                if routed[i]:
+                    out.append(([int(c) for c in cells[off[i]:off[i + 1]]], int(cost[i])))
+                else:
+                    out.append(None)
+    finally:
+        lib().mrf_free_paths(C.byref(p))
+    return out
+
+
+class Engine:
+    """One context (HIP stream + workspace) on one device."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        err = ErrorS()
+        _check(lib().mrf_open(device, C.byref(self._h), C.byref(err)), err)
+
+    def close(self):
+        if self._h:
+            lib().mrf_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tiles(self, tw=32, th=32, threads=256, max_local=32):
+        err = ErrorS()
+        _check(lib().mrf_set_tiles(self._h, tw, th, threads, max_local, C.byref(err)), err)
+
+    def set_mode(self, mode: int):
+        if lib().mrf_set_mode(self._h, mode) != OK:
+            raise MrfError(BACKEND_UNAVAILABLE, "bad mode")
+
+    def stats(self) -> dict:
+        s = StatsS()
+        lib().mrf_last_stats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in StatsS._fields_}
+
+    # -- provider seam -------------------------------------------------------
+    def _isolated(self, fn, grid: Grid, nets, windows, edges: Edges):
+        arr, keep = _nets(nets)
+        g = grid.cstruct()
+        e = edges.cstruct()
+        w = _windows(windows, len(nets))
+        out = PathsS()
+        err = ErrorS()
+        rc = fn(self._h, grid.dims(), C.byref(g), arr, len(nets), w, C.byref(e), C.byref(out), C.byref(err))
+        _check(rc, err)
+        return _take_paths(out)
+
+    def route_isolated_batch(self, grid, nets, windows, edges):
+        return self._isolated(lib().mrf_route_isolated_batch, grid, nets, windows, edges)
+
+    def route_isolated_pass(self, grid, nets, windows, edges):
+        return self._isolated(lib().mrf_route_isolated_pass, grid, nets, windows, edges)
+
+    def router_route(self, grid: Grid, nets):
+        arr, keep = _nets(nets)
+        g = grid.cstruct()
+        out = PathsS()
+        err = ErrorS()
+        _check(lib().mrf_router_route(self._h, grid.dims(), C.byref(g), arr, len(nets), C.byref(out),
+                                      C.byref(err)), err)
+        return _take_paths(out)
+
+    def sweep_fields(self, grid: Grid, sources: Sequence[int]) -> np.ndarray:
+        s = _u32(list(sources) + [0])
+        out = np.zeros((max(len(sources), 1), grid.n), dtype=np.uint32)
+        g = grid.cstruct()
+        err = ErrorS()
+        _check(lib().mrf_sweep_fields(self._h, grid.dims(), C.byref(g), _p32(s), len(sources), _p32(out),
+                                      C.byref(err)), err)
+        return out[: len(sources)]
+
+    def fields(self, grid: Grid, edges: Edges, sources: Sequence[Sequence[int]], dsts=None, windows=None,
+               pricing: Optional[Pricing] = None, want_hops=True):
+        n = len(sources)
+        srcs = [_u32(list(s) + [0]) for s in sources]
+        sp = (U32P * max(n, 1))(*[_p32(s) for s in srcs])
+        ns = (C.c_size_t * max(n, 1))(*[len(s) for s in sources])
+        d = _u32(list(dsts) + [0]) if dsts is not None else None
+        dist = np.zeros((max(n, 1), grid.n), dtype=np.uint32)
+        hops = np.zeros((max(n, 1), grid.n), dtype=np.uint32) if want_hops else None
+        g = grid.cstruct()
+        e = edges.cstruct()
+        pr = pricing.cstruct() if pricing is not None else None
+        err = ErrorS()
+        _check(lib().mrf_fields(self._h, grid.dims(), C.byref(g), C.byref(e), C.byref(pr) if pr else None, sp, ns,
+                                _p32(d), n, _windows(windows, n), _p32(dist), _p32(hops), C.byref(err)), err)
+        return dist[:n], (hops[:n] if hops is not None else None)
+
+
+class Board:
+    """A board whose planes stay resident in HBM across solves."""
+
+    def __init__(self, eng: Engine, grid: Grid, edges: Edges, pricing: Optional[Pricing] = None):
+        self.eng = eng
+        self.grid = grid
+        self._h = C.c_void_p()
+        g = grid.cstruct()
+        e = edges.cstruct()
+        self._pr = pricing.cstruct() if pricing is not None else None
+        err = ErrorS()
+        _check(lib().mrf_board_create(eng._h, grid.dims(), C.byref(g), C.byref(e),
+                                      C.byref(self._pr) if self._pr else None, C.byref(self._h), C.byref(err)), err)
+
+    def close(self):
+        if self._h:
+            lib().mrf_board_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bytes(self) -> int:
+        return int(lib().mrf_board_bytes(self._h))
+
+    def solve(self, nets, windows=None, retry_full=True):
+        arr, keep = _nets(nets)
+        out = PathsS()
+        err = ErrorS()
+        _check(lib().mrf_solve(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)),
+                               1 if retry_full else 0, C.byref(out), C.byref(err)), err)
+        return _take_paths(out)
+
+    def solve_fields(self, sources, dsts=None, net_ids=None, windows=None):
+        n = len(sources)
+        srcs = [_u32(list(s) + [0]) for s in sources]
+        sp = (U32P * max(n, 1))(*[_p32(s) for s in srcs])
+        ns = (C.c_size_t * max(n, 1))(*[len(s) for s in sources])
+        d = _u32(list(dsts) + [0]) if dsts is not None else None
+        ids = np.ascontiguousarray(np.asarray(list(net_ids) + [0], dtype=np.int32)) if net_ids is not None else None
+        err = ErrorS()
+        _check(lib().mrf_solve_fields(self.eng._h, self._h, sp, ns, _p32(d),
+                                      ids.ctypes.data_as(I32P) if ids is not None else None, n,
+                                      _windows(windows, n), C.byref(err)), err)
+
+    def read_field(self, i: int, want_hops=True):
+        dist = np.zeros(self.grid.n, dtype=np.uint32)
+        hops = np.zeros(self.grid.n, dtype=np.uint32) if want_hops else None
+        err = ErrorS()
+        _check(lib().mrf_read_field(self.eng._h, self._h, i, _p32(dist), _p32(hops), C.byref(err)), err)
+        return dist, hops
+
+
+@dataclass
+class Synth:
+    grid: Grid
+    edges: Edges
+    additive: Optional[np.ndarray]
+    halo_owner: Optional[np.ndarray]
+    halo_penalty: int
+    src: np.ndarray
+    dst: np.ndarray
+    pins: Optional[List[List[int]]]
+    windows: Optional[List[Tuple[int, int, int, int]]]
+
+    def nets(self) -> List[Net]:
+        return [Net(int(s), int(d)) for s, d in zip(self.src, self.dst)]
+
+    def pricing(self, net_ids=None) -> Optional[Pricing]:
+        if self.additive is None and self.halo_owner is None:
+            return None
+        return Pricing(self.additive, self.halo_owner, self.halo_penalty, net_ids)
+
+
+def _from_synth(s: SynthS) -> Synth:
+    try:
+        d = s.dims
+        n = d.w * d.h * d.layers
+        arr = lambda p, k, dt=None: np.ctypeslib.as_array(p, shape=(k,)).copy()
+        cost = arr(s.cost, n)
+        board = arr(s.board_constraint, n) if s.board_constraint else None
+        add = arr(s.additive, n) if s.additive else None
+        halo = arr(s.halo_owner, n) if s.halo_owner else None
+        ex = [int(v) for v in arr(s.x_edges, max(d.w, 1))[: d.w - 1]]
+        ey = [int(v) for v in arr(s.y_edges, max(d.h, 1))[: d.h - 1]]
+        ve = arr(s.via_edges, max(d.layers, 1))[: d.layers - 1]
+        va = arr(s.via_allowed, max(d.layers, 1))[: d.layers - 1]
+        via = [int(v) if a else None for v, a in zip(ve, va)]
+        nn = s.n_nets
+        src = arr(s.src, max(nn, 1))[:nn]
+        dst = arr(s.dst, max(nn, 1))[:nn]
+        pins = None
+        if s.pin_off:
+            off = arr(s.pin_off, nn + 1)
+            allp = arr(s.pins, max(int(off[-1]), 1))
+            pins = [[int(c) for c in allp[off[i]:off[i + 1]]] for i in range(nn)]
+        wins = None
+        if s.windows:
+            wins = [(s.windows[i].x0, s.windows[i].y0, s.windows[i].x1, s.windows[i].y1) for i in range(nn)]
+        return Synth(Grid(d.w, d.h, d.layers, cost, None, board), Edges(ex, ey, via), add, halo,
+                     int(s.halo_penalty), src, dst, pins, wins)
+    finally:
+        lib().mrf_free_synth(C.byref(s))
+
+
+def synth_config(config: int, seed: int = 1, scale_nets: int = 0) -> Synth:
+    s = SynthS()
+    err = ErrorS()
+    _check(lib().mrf_synth_config(config, seed, scale_nets, C.byref(s), C.byref(err)), err)
+    return _from_synth(s)
+
+
+def synth_board(side: int, layers: int, nets: int, seed: int) -> Synth:
+    s = SynthS()
+    rc = lib().mrf_synth_board(side, layers, nets, seed, C.byref(s))
+    if rc != OK:
+        raise MrfError(rc, "synth_board failed")
+    return _from_synth(s)
