@@ -43,6 +43,7 @@ struct BoardView {
   const uint32_t* xe;       // planar x gap prices [w-1] (+1 pad)
   const uint32_t* ye;       // planar y gap prices [h-1] (+1 pad)
   const uint32_t* ve;       // via prices [layers-1] (+1 pad)
+  uint8_t va[16];           // via_allowed per layer pair
 };
 
 // One complete field: a window (all layers), its sources, overrides.
@@ -59,7 +60,7 @@ struct FieldDesc {
   uint32_t dst;        // target cell or kMax
   int32_t own_id;      // halo owner id of this net
   uint32_t flags;
-  uint32_t _pad;
+  uint32_t board;      // index into the launch's board table
 };
 
 struct TileShape {
@@ -94,6 +95,7 @@ struct FieldReq {
   uint32_t dst = kMax;
   int32_t own_id = -1;
   uint32_t flags = 0;
+  uint32_t board = 0;  // index into the boards passed to solve_fields
 };
 
 struct SolveStats {
@@ -148,6 +150,8 @@ struct Workspace {
   uint32_t* fout = nullptr;      // extracted field output staging
   size_t fout_cap = 0;
   uint32_t* pinned = nullptr;    // host-pinned counter readback
+  BoardView* boards = nullptr;   // device board table of the current batch
+  size_t boards_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
@@ -162,6 +166,7 @@ struct Ctx {
   int cus = 256;
   SolveStats last;
   std::vector<FieldDesc> layout;  // fields of the last solve (for paths / reads)
+  std::vector<BoardView> views;   // boards of the last solve
   // Fault injection for fail-closed tests (0 none, 1 backend error, 2 corrupt path).
   int inject = 0;
 };
@@ -174,17 +179,28 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
                   uint32_t halo_pen);
 void board_free(DevBoard& b);
 
-// Solve complete fields for `reqs` on a resident board.  Labels stay on the
-// device in the workspace until the next solve.
-void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, SolveStats* st);
+// Solve complete fields for `reqs` (FieldReq::board indexes `boards`) in one
+// launch.  Labels stay on the device in the workspace until the next solve.
+void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std::vector<FieldReq>& reqs,
+                  SolveStats* st);
+inline void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, SolveStats* st) {
+  solve_fields(c, std::vector<const DevBoard*>{&b}, reqs, st);
+}
 // After solve_fields: reconstruct canonical paths for fields with a dst.
 // routed[i]=0 when the dst is unreachable.
-void reconstruct_paths(Ctx& c, const DevBoard& b, size_t n_fields, std::vector<uint8_t>& routed,
-                       std::vector<uint32_t>& cost, std::vector<uint64_t>& off,
-                       std::vector<uint32_t>& cells, SolveStats* st);
-// After solve_fields: copy field i's dist (and hops) out, full board layout
-// (cells outside the window are kMax).
-void read_field(Ctx& c, const DevBoard& b, size_t field, uint32_t* dist, uint32_t* hops);
+void reconstruct_paths(Ctx& c, size_t n_fields, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
+                       std::vector<uint64_t>& off, std::vector<uint32_t>& cells, SolveStats* st);
+inline void reconstruct_paths(Ctx& c, const DevBoard&, size_t n, std::vector<uint8_t>& routed,
+                              std::vector<uint32_t>& cost, std::vector<uint64_t>& off,
+                              std::vector<uint32_t>& cells, SolveStats* st) {
+  reconstruct_paths(c, n, routed, cost, off, cells, st);
+}
+// After solve_fields: copy field i's dist (and hops) out in its board's full
+// layout (cells outside the window are kMax).
+void read_field(Ctx& c, size_t field, uint32_t* dist, uint32_t* hops);
+inline void read_field(Ctx& c, const DevBoard&, size_t field, uint32_t* dist, uint32_t* hops) {
+  read_field(c, field, dist, hops);
+}
 // Device pointer variant used by tests of the persistent kernel: copy all
 // fields as window-local row-major (dist) into a host buffer.
 void ctx_init(Ctx& c, int device);

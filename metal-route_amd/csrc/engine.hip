@@ -91,6 +91,7 @@ BoardView DevBoard::view() const {
   v.xe = xe;
   v.ye = ye;
   v.ve = ve;
+  for (int i = 0; i < 16; ++i) v.va[i] = via_allowed.size() > (size_t)i ? via_allowed[i] : 0;
   return v;
 }
 
@@ -148,8 +149,8 @@ __device__ __forceinline__ uint32_t add_price(const BoardView& b, const FieldDes
 
 // Via bits of global cell g for field F, including per-net via exemptions
 // (negotiated.rs:3010-3043).
-__device__ uint8_t via_bits(const BoardView& b, const FieldDesc& F, const uint32_t* pool,
-                            const uint8_t* va, uint32_t g) {
+__device__ uint8_t via_bits(const BoardView& b, const FieldDesc& F, const uint32_t* pool, uint32_t g) {
+  const uint8_t* va = b.va;
   uint32_t l = g / b.plane;
   const uint32_t* vp = pool + F.vpad_off;
   uint8_t r = b.raw[g];
@@ -175,8 +176,7 @@ struct CellInfo {
   uint32_t w, a;
 };
 
-__device__ CellInfo cell_info(const BoardView& b, const FieldDesc& F, const uint32_t* pool,
-                              const uint8_t* va, uint32_t g) {
+__device__ CellInfo cell_info(const BoardView& b, const FieldDesc& F, const uint32_t* pool, uint32_t g) {
   CellInfo ci;
   uint8_t bm = b.bmask[g];
   uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
@@ -188,15 +188,15 @@ __device__ CellInfo cell_info(const BoardView& b, const FieldDesc& F, const uint
       if (pool[F.forced_off + k] == g) pass = true;
   }
   if (pass) m |= K_PASS;
-  if (F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(b, F, pool, va, g));
+  if (F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(b, F, pool, g));
   ci.mask = m;
   ci.w = (obst && pad) ? 1u : b.cost[g];
-  ci.a = add_price(b, F, g);
+  ci.a = (b.add || b.halo) ? add_price(b, F, g) : 0u;
   return ci;
 }
 
 struct RelaxArgs {
-  BoardView b;
+  const BoardView* boards;
   const FieldDesc* fields;
   const uint32_t* item_field;
   const uint32_t* pool;
@@ -211,7 +211,6 @@ struct RelaxArgs {
   uint32_t use_add;
   uint32_t qcap;
   uint32_t spin_limit;
-  uint8_t va[16];  // via_allowed per layer pair (layers <= 16)
 };
 
 // LDS carve-up for one tile (all sizes in bytes, 16-byte aligned sections).
@@ -285,16 +284,17 @@ enum : uint32_t { T_LEFT = 1, T_RIGHT = 2, T_TOP = 4, T_BOTTOM = 8, T_ANY = 16, 
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
 __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
-  const BoardView& B = A.b;
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = A.item_field[item];
   const FieldDesc F = A.fields[f];
+  const BoardView B = A.boards[F.board];
   const uint32_t t = item - F.item0;
   const uint32_t txi = t % F.tx, tyi = t / F.tx;
   const uint32_t tw = A.tw, th = A.th, L = B.layers;
   const uint32_t ncell = L * th * tw, plane_t = th * tw;
   const uint32_t wx0 = txi * tw, wy0 = tyi * th;
-  const bool use_add = A.use_add != 0;
+  const bool use_add = A.use_add != 0;  // LDS holds an additive plane in this launch
+  const bool has_add = B.add != nullptr || B.halo != nullptr;
   Lds s = carve(smem, tw, th, L, use_add);
   const uint32_t rs = s.rs, rh = s.rh;
   const uint64_t blockbase = F.lab_off + (uint64_t)t * ncell;
@@ -331,7 +331,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       m = (uint8_t)((bm >> 1) & 0x7E);
       if (!(bm & (M_TRACE | M_OBST))) m |= K_PASS;
       wgt = B.cost[g];
-      if (use_add) ap = add_price(B, F, g);
+      if (has_add) ap = add_price(B, F, g);
       if (touched)
         lab = __hip_atomic_load(&A.labels[blockbase + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -425,7 +425,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
           if (ll < 0 || ll >= (int)L) continue;
           uint32_t g = vp[k] + (uint32_t)((int)plane * dl);
           uint32_t c = ((uint32_t)ll * th + y) * tw + x;
-          uint8_t vb = via_bits(B, F, A.pool, A.va, g);
+          uint8_t vb = via_bits(B, F, A.pool, g);
           s.m[c] = (uint8_t)((s.m[c] & ~(K_DN | K_UP)) | vb);
         }
       }
@@ -781,10 +781,11 @@ __global__ void set_u8(uint8_t* p, const uint32_t* idx, uint32_t n) {
 // ---------------------------------------------------------------------------
 
 struct FieldRead {
+  const BoardView* boards;
   const FieldDesc* fields;
   const uint64_t* labels;
   const uint32_t* state;
-  uint32_t tw, th, layers;
+  uint32_t tw, th;
 };
 
 // Label of global cell g in field F (kInf outside the window or untouched tile).
@@ -796,14 +797,15 @@ __device__ uint64_t field_label(const FieldRead& R, const BoardView& B, const Fi
   uint32_t txi = wx / R.tw, tyi = wy / R.th, t = tyi * F.tx + txi;
   if (!(R.state[F.item0 + t] & S_TOUCHED)) return kInf;
   uint32_t x = wx - txi * R.tw, y = wy - tyi * R.th;
-  uint64_t ncell = (uint64_t)R.layers * R.th * R.tw;
+  uint64_t ncell = (uint64_t)B.layers * R.th * R.tw;
   return R.labels[F.lab_off + t * ncell + (l * R.th + y) * R.tw + x];
 }
 
-__global__ void path_lengths(FieldRead R, BoardView B, uint32_t n, uint32_t* plen, uint32_t* pcost) {
+__global__ void path_lengths(FieldRead R, uint32_t n, uint32_t* plen, uint32_t* pcost) {
   uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
   const FieldDesc F = R.fields[f];
+  const BoardView B = R.boards[F.board];
   uint32_t len = 0, cost = kMax;
   if (F.dst != kMax) {
     uint64_t lab = field_label(R, B, F, F.dst);
@@ -818,14 +820,12 @@ __global__ void path_lengths(FieldRead R, BoardView B, uint32_t n, uint32_t* ple
 
 struct WalkArgs {
   FieldRead R;
-  BoardView B;
   const uint32_t* pool;
   const uint64_t* poff;
   const uint32_t* plen;
   uint32_t* cells;
   uint32_t* counters;
   uint32_t n;
-  uint8_t va[16];
 };
 
 // One wavefront per field: lanes 0..5 test the six candidate predecessors in
@@ -835,7 +835,7 @@ __global__ void walk_paths(WalkArgs W) {
   uint32_t lane = threadIdx.x;
   if (f >= W.n || W.plen[f] == 0) return;
   const FieldDesc F = W.R.fields[f];
-  const BoardView& B = W.B;
+  const BoardView B = W.R.boards[F.board];
   uint32_t* out = W.cells + W.poff[f];
   uint32_t cur = F.dst;
   uint64_t lab = field_label(W.R, B, F, cur);
@@ -843,7 +843,7 @@ __global__ void walk_paths(WalkArgs W) {
     uint32_t hops = (uint32_t)lab;
     if (lane == 0) out[hops] = cur;
     if (hops == 0) return;
-    CellInfo ci = cell_info(B, F, W.pool, W.va, cur);
+    CellInfo ci = cell_info(B, F, W.pool, cur);
     uint32_t l = cur / B.plane, r = cur - l * B.plane, y = r / B.w, x = r - y * B.w;
     bool ok = false;
     uint32_t p = 0, e = 0;
@@ -877,9 +877,10 @@ __global__ void walk_paths(WalkArgs W) {
 }
 
 // Tile-major labels of one field -> window-major u32 dist (and hops).
-__global__ void extract_field(FieldRead R, BoardView B, uint32_t f, uint32_t* dist, uint32_t* hops) {
+__global__ void extract_field(FieldRead R, uint32_t f, uint32_t* dist, uint32_t* hops) {
   const FieldDesc F = R.fields[f];
-  size_t n = (size_t)F.ww * F.wh * R.layers;
+  const BoardView B = R.boards[F.board];
+  size_t n = (size_t)F.ww * F.wh * B.layers;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     uint32_t l = (uint32_t)(i / ((size_t)F.ww * F.wh));
     uint32_t r = (uint32_t)(i - (size_t)l * F.ww * F.wh);
@@ -1035,10 +1036,10 @@ size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t 
   return tx * ty * (size_t)layers * s.th * s.tw;
 }
 
-static RelaxArgs make_args(Ctx& c, const DevBoard& b, uint32_t n_items, bool use_add) {
+static RelaxArgs make_args(Ctx& c, uint32_t n_items, bool use_add) {
   RelaxArgs A;
   memset(&A, 0, sizeof(A));
-  A.b = b.view();
+  A.boards = c.ws.boards;
   A.fields = c.ws.fields;
   A.item_field = c.ws.item_field;
   A.pool = c.ws.pool;
@@ -1055,7 +1056,6 @@ static RelaxArgs make_args(Ctx& c, const DevBoard& b, uint32_t n_items, bool use
   A.use_add = use_add ? 1 : 0;
   A.qcap = n_items;
   A.spin_limit = 1u << 26;
-  for (int i = 0; i < 16; ++i) A.va[i] = b.via_allowed.size() > (size_t)i ? b.via_allowed[i] : 0;
   return A;
 }
 
@@ -1070,16 +1070,23 @@ static void launch_fill8(hipStream_t s, uint8_t* p, uint8_t v, size_t n) {
   hipLaunchKernelGGL(fill_u8, dim3(blocks), dim3(256), 0, s, p, v, n);
 }
 
-void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, SolveStats* st) {
+void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std::vector<FieldReq>& reqs,
+                  SolveStats* st) {
   HIP_OK(hipSetDevice(c.device));
   auto t0 = std::chrono::steady_clock::now();
   const TileShape& S = c.shape;
-  const uint32_t L = b.layers;
-  if ((size_t)L * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
+  uint32_t Lmax = 1;
+  bool use_add = false;
+  c.views.clear();
+  for (const DevBoard* b : boards) {
+    Lmax = std::max(Lmax, b->layers);
+    use_add = use_add || b->add != nullptr || b->halo != nullptr;
+    c.views.push_back(b->view());
+  }
+  if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  const bool use_add = b.add != nullptr || b.halo != nullptr;
-  size_t lds = lds_bytes(S.tw, S.th, L, use_add);
-  if (lds > 160 * 1024) throw Error(1, "tile shape exceeds 160 KiB of LDS");
+  size_t lds = lds_bytes(S.tw, S.th, Lmax, use_add);
+  if (lds > 160 * 1024 - 64) throw Error(1, "tile shape exceeds 160 KiB of LDS");
 
   std::vector<FieldDesc>& layout = c.layout;
   layout.clear();
@@ -1088,6 +1095,9 @@ void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, 
   uint32_t item = 0;
   for (size_t i = 0; i < reqs.size(); ++i) {
     const FieldReq& q = reqs[i];
+    if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
+    const DevBoard& b = *boards[q.board];
+    const uint32_t L = b.layers;
     FieldDesc F;
     memset(&F, 0, sizeof(F));
     F.x0 = q.x0;
@@ -1110,6 +1120,7 @@ void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, 
     F.dst = q.dst;
     F.own_id = q.own_id;
     F.flags = q.flags;
+    F.board = q.board;
     uint32_t nt = F.tx * F.ty;
     for (uint32_t k = 0; k < nt; ++k) item_field.push_back((uint32_t)i);
     // seed tiles: tiles holding a source inside the window
@@ -1138,7 +1149,9 @@ void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, 
   grow(W.fields, W.fields_cap, std::max<size_t>(reqs.size(), 1));
   grow(W.item_field, W.item_field_cap, std::max<size_t>(item, 1));
   grow(W.pool, W.pool_cap, std::max<size_t>(pool.size() + seeds.size(), 1));
+  grow(W.boards, W.boards_cap, std::max<size_t>(c.views.size(), 1));
   hipStream_t s = c.stream;
+  HIP_OK(hipMemcpyAsync(W.boards, c.views.data(), c.views.size() * sizeof(BoardView), hipMemcpyHostToDevice, s));
   if (!layout.empty())
     HIP_OK(hipMemcpyAsync(W.fields, layout.data(), layout.size() * sizeof(FieldDesc),
                           hipMemcpyHostToDevice, s));
@@ -1152,10 +1165,10 @@ void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, 
   launch_fill32(s, W.state, 0u, item);
   launch_fill32(s, W.counters, 0u, C_NCOUNTERS);
 
-  RelaxArgs A = make_args(c, b, item, use_add);
+  RelaxArgs A = make_args(c, item, use_add);
   SolveStats local;
   local.field_cells = field_cells;
-  HIP_OK(hipEventRecord(W.ev0, s));
+  bool timed = false;
   if (item && !seeds.empty()) {
     if (c.mode == MODE_PLAIN) {
       launch_fill8(s, W.act, 0, (size_t)item * 2);
@@ -1164,6 +1177,8 @@ void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, 
       uint32_t rounds = 0;
       uint32_t* pinned = W.pinned;
       uint32_t grid = ((item + 7) / 8) * 8;  // xcd_item needs a multiple of 8 workgroups
+      HIP_OK(hipEventRecord(W.ev0, s));
+      timed = true;
       for (;;) {
         launch_fill32(s, W.counters + C_CHANGED, 0u, 1);
         hipLaunchKernelGGL(relax_round, dim3(grid), dim3(S.threads), lds, s, A);
@@ -1184,39 +1199,41 @@ void solve_fields(Ctx& c, const DevBoard& b, const std::vector<FieldReq>& reqs, 
       HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relax_persistent, S.threads, lds));
       if (occ < 1) throw Error(1, "persistent kernel does not fit on a CU");
       uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)occ * c.cus, item);
+      HIP_OK(hipEventRecord(W.ev0, s));
+      timed = true;
       hipLaunchKernelGGL(relax_persistent, dim3(grid), dim3(S.threads), lds, s, A);
       HIP_OK(hipGetLastError());
       local.rounds = 1;
     }
   }
-  HIP_OK(hipEventRecord(W.ev1, s));
-  uint32_t cnt[C_NCOUNTERS];
-  HIP_OK(hipMemcpyAsync(cnt, W.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  if (timed) HIP_OK(hipEventRecord(W.ev1, s));
+  uint32_t* cnt = W.pinned;
+  HIP_OK(hipMemcpyAsync(cnt, W.counters, C_NCOUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   if (cnt[C_ERROR]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR]));
   float ms = 0;
-  HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+  if (timed) HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   local.relax_ms = ms;
   local.tile_visits = cnt[C_VISITS];
   local.local_rounds = cnt[C_ROUNDS];
-  local.cells_visited = (uint64_t)cnt[C_VISITS] * L * S.th * S.tw;
+  local.cells_visited = (uint64_t)cnt[C_VISITS] * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (st) *st = local;
   c.last = local;
 }
 
-static FieldRead make_read(Ctx& c, const DevBoard& b) {
+static FieldRead make_read(Ctx& c) {
   FieldRead R;
+  R.boards = c.ws.boards;
   R.fields = c.ws.fields;
   R.labels = c.ws.labels;
   R.state = c.ws.state;
   R.tw = c.shape.tw;
   R.th = c.shape.th;
-  R.layers = b.layers;
   return R;
 }
 
-void reconstruct_paths(Ctx& c, const DevBoard& b, size_t n, std::vector<uint8_t>& routed,
+void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
                        std::vector<uint32_t>& cost, std::vector<uint64_t>& off, std::vector<uint32_t>& cells,
                        SolveStats* st) {
   HIP_OK(hipSetDevice(c.device));
@@ -1230,10 +1247,9 @@ void reconstruct_paths(Ctx& c, const DevBoard& b, size_t n, std::vector<uint8_t>
   grow(W.plen, W.plen_cap, n);
   grow(W.pcost, W.pcost_cap, n);
   grow(W.poff, W.poff_cap, n + 1);
-  FieldRead R = make_read(c, b);
-  BoardView B = b.view();
+  FieldRead R = make_read(c);
   HIP_OK(hipEventRecord(W.ev1, s));
-  hipLaunchKernelGGL(path_lengths, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, R, B, (uint32_t)n, W.plen,
+  hipLaunchKernelGGL(path_lengths, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, R, (uint32_t)n, W.plen,
                      W.pcost);
   std::vector<uint32_t> plen(n);
   HIP_OK(hipMemcpyAsync(plen.data(), W.plen, n * 4, hipMemcpyDeviceToHost, s));
@@ -1251,14 +1267,12 @@ void reconstruct_paths(Ctx& c, const DevBoard& b, size_t n, std::vector<uint8_t>
     WalkArgs WA;
     memset(&WA, 0, sizeof(WA));
     WA.R = R;
-    WA.B = B;
     WA.pool = W.pool;
     WA.poff = W.poff;
     WA.plen = W.plen;
     WA.cells = W.pcells;
     WA.counters = W.counters;
     WA.n = (uint32_t)n;
-    for (int i = 0; i < 16; ++i) WA.va[i] = b.via_allowed.size() > (size_t)i ? b.via_allowed[i] : 0;
     launch_fill32(s, W.counters + C_ERROR, 0u, 1);
     hipLaunchKernelGGL(walk_paths, dim3((uint32_t)n), dim3(64), 0, s, WA);
     HIP_OK(hipGetLastError());
@@ -1275,16 +1289,17 @@ void reconstruct_paths(Ctx& c, const DevBoard& b, size_t n, std::vector<uint8_t>
   }
 }
 
-void read_field(Ctx& c, const DevBoard& b, size_t f, uint32_t* dist, uint32_t* hops) {
+void read_field(Ctx& c, size_t f, uint32_t* dist, uint32_t* hops) {
   HIP_OK(hipSetDevice(c.device));
   if (f >= c.layout.size()) throw Error(1, "field index out of range");
   const FieldDesc& F = c.layout[f];
+  const BoardView& b = c.views[F.board];
   size_t n = (size_t)F.ww * F.wh * b.layers;
   Workspace& W = c.ws;
   grow(W.fout, W.fout_cap, 2 * n);
-  FieldRead R = make_read(c, b);
+  FieldRead R = make_read(c);
   uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(extract_field, dim3(blocks), dim3(256), 0, c.stream, R, b.view(), (uint32_t)f, W.fout,
+  hipLaunchKernelGGL(extract_field, dim3(blocks), dim3(256), 0, c.stream, R, (uint32_t)f, W.fout,
                      hops ? W.fout + n : nullptr);
   std::vector<uint32_t> tmp(hops ? 2 * n : n);
   HIP_OK(hipMemcpyAsync(tmp.data(), W.fout, tmp.size() * 4, hipMemcpyDeviceToHost, c.stream));
