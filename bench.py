@@ -36,11 +36,27 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, help="synthetic config (2 = 512x512x4, 256 nets)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["persistent", "plain"], default="persistent")
-    ap.add_argument("--tiles", default="32,32,256,32", help="tw,th,threads,max_local")
+    ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
+    ap.add_argument("--tiles", default="16,16,128,32", help="tw,th,threads,max_local")
     ap.add_argument("--cpu-sample", type=int, default=16, help="nets in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (rocprof counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib: int) -> None:
+    """A plain HBM copy of known size: under rocprofv3 --pmc its FETCH_SIZE /
+    WRITE_SIZE give the factor between the counters and real bytes."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "checksum": int(dst[-1].item())}))
 
 
 def load_traffic(kernel: str, workload: str):
@@ -83,6 +99,9 @@ def cpu_baseline(s, sample, got_paths):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": a.gpus}))
@@ -94,14 +113,14 @@ def main():
     tw, th, threads, ml = (int(v) for v in a.tiles.split(","))
     eng = mrfield.Engine(0)
     eng.set_tiles(tw, th, threads, ml)
-    eng.set_mode(mrfield.MODE_PERSISTENT if a.mode == "persistent" else mrfield.MODE_PLAIN)
+    eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)
     s = mrfield.synth_config(a.config, seed=a.seed)
     g = s.grid
     board = mrfield.Board(eng, g, s.edges, s.pricing())
     nets = s.nets()
     wins = s.windows
 
-    lat, relax_ms, path_ms, visits, cells_visited = [], [], [], [], []
+    lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
@@ -116,6 +135,8 @@ def main():
             path_ms.append(st["path_ms"])
             visits.append(st["tile_visits"])
             cells_visited.append(st["cells_visited"])
+            rounds_l.append(st["local_rounds"])
+            host_rounds.append(st["host_rounds"])
     wall = time.perf_counter() - t_start
     n_nets = len(nets)
     value = n_nets * a.steps / wall
@@ -150,14 +171,16 @@ def main():
         "relax_kernel_ms": round(kern_ms, 4),
         "path_ms": round(float(np.mean(path_ms)), 4),
         "tile_visits": int(np.mean(visits)),
+        "local_rounds_per_visit": round(float(np.mean(rounds_l)) / max(float(np.mean(visits)), 1), 3),
+        "host_rounds": int(np.mean(host_rounds)),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic("relax_persistent" if a.mode == "persistent" else "relax_round", workload),
-            "kernel": "relax_persistent" if a.mode == "persistent" else "relax_round",
+            "traffic": load_traffic("relax_list" if a.mode == "graph" else "relax_round", workload),
+            "kernel": "relax_list" if a.mode == "graph" else "relax_round",
             "bytes_per_cell_visit": BYTES_PER_CELL_VISIT,
             "cell_visits_per_launch": int(cells),
         },

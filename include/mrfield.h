@@ -150,18 +150,26 @@ typedef struct {
     uint64_t local_rounds; /* in-LDS relaxation rounds */
     uint64_t cells_visited;
     uint64_t field_cells;
-    uint32_t host_rounds;  /* plain form: launches; persistent form: 1 */
+    uint32_t host_rounds;  /* launches of the round kernel (+1 for the persistent kernel) */
+    double persistent_ms;  /* persistent form: the work-queue kernel alone */
 } mrf_stats;
 
-/* Solve form of a context. */
-enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1 };
+/* Solve form of a context.
+ *   PLAIN       one launch per relaxation round; the host reads the number of
+ *               activated tiles after each round.
+ *   PERSISTENT  resident grid with a device barrier between rounds: disabled in
+ *               this build, mrf_solve* return MRF_BACKEND_UNAVAILABLE for it.
+ *   GRAPH       (default) grid-stride launches over a device work list, rounds
+ *               captured in a hipGraph and replayed; one host read per replay. */
+enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1, MRF_MODE_GRAPH = 2 };
 
 int mrf_abi_version(void);
 
 /* Context: one HIP stream on one device, reusable device workspace. */
 int mrf_open(int device, mrf_ctx **out, mrf_error *err);
 void mrf_close(mrf_ctx *ctx);
-/* Tile shape (tw x th cells, all layers), workgroup size, in-LDS round cap. */
+/* Tile shape (tw x th cells, all layers), workgroup size (64..256), in-LDS
+ * round cap.  Instantiated shapes: 32x32, 32x16, 16x16, 16x8, 8x4, 4x1. */
 int mrf_set_tiles(mrf_ctx *ctx, uint32_t tw, uint32_t th, uint32_t threads, uint32_t max_local,
                   mrf_error *err);
 int mrf_set_mode(mrf_ctx *ctx, int mode);

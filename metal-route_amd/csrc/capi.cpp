@@ -441,9 +441,9 @@ void mrf_close(mrf_ctx* ctx) {
 }
 
 int mrf_set_tiles(mrf_ctx* ctx, uint32_t tw, uint32_t th, uint32_t threads, uint32_t max_local, mrf_error* err) {
-  if (!ctx || tw < 4 || tw > 128 || th < 1 || th > 128 || threads < 64 || threads > 1024 || threads % 64 ||
+  if (!ctx || !mrf::tile_shape_supported(tw, th) || threads < 64 || threads > 256 || threads % 64 ||
       max_local < 1) {
-    set_err(err, MRF_BACKEND_UNAVAILABLE, "invalid tile shape");
+    set_err(err, MRF_BACKEND_UNAVAILABLE, "unsupported tile shape");
     return MRF_BACKEND_UNAVAILABLE;
   }
   std::lock_guard<std::mutex> g(ctx->c.lane);
@@ -455,7 +455,8 @@ int mrf_set_tiles(mrf_ctx* ctx, uint32_t tw, uint32_t th, uint32_t threads, uint
 }
 
 int mrf_set_mode(mrf_ctx* ctx, int mode) {
-  if (!ctx || (mode != MRF_MODE_PLAIN && mode != MRF_MODE_PERSISTENT)) return MRF_BACKEND_UNAVAILABLE;
+  if (!ctx || (mode != MRF_MODE_PLAIN && mode != MRF_MODE_PERSISTENT && mode != MRF_MODE_GRAPH))
+    return MRF_BACKEND_UNAVAILABLE;
   std::lock_guard<std::mutex> g(ctx->c.lane);
   ctx->c.mode = mode;
   return MRF_OK;
@@ -472,6 +473,7 @@ int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   out->cells_visited = s.cells_visited;
   out->field_cells = s.field_cells;
   out->host_rounds = s.rounds;
+  out->persistent_ms = s.persistent_ms;
   return MRF_OK;
 }
 
@@ -749,6 +751,7 @@ int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, c
         s.cells_visited += first.cells_visited;
         s.field_cells += first.field_cells;
         s.rounds += first.rounds;
+        s.persistent_ms += first.persistent_ms;
       }
     }
     export_paths(all, out);

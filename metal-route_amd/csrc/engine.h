@@ -105,9 +105,18 @@ struct SolveStats {
   uint64_t cells_visited = 0;  // tile_visits x tile cells
   uint32_t rounds = 0;         // host rounds (plain form) or 1 (persistent)
   uint64_t field_cells = 0;    // cells in all fields (window x layers)
+  double persistent_ms = 0;    // persistent kernel alone (persistent form)
+  uint32_t replays = 0;        // hipGraph replays (graph form)
 };
 
-enum SolveMode : int { MODE_PLAIN = 0, MODE_PERSISTENT = 1 };
+// MODE_PLAIN: one launch per round, host reads the activation count each round.
+// MODE_PERSISTENT: resident-grid kernel with a device grid barrier between
+//   rounds.  Disabled: it did not return on MI355X; requesting
+//   it raises an error.
+// MODE_GRAPH: grid-stride kernel over a device work list per round, rounds
+//   separated by kernel boundaries, captured in a hipGraph and replayed; the
+//   host reads the activation count once per replay.
+enum SolveMode : int { MODE_PLAIN = 0, MODE_PERSISTENT = 1, MODE_GRAPH = 2 };
 
 class Error : public std::exception {
  public:
@@ -128,8 +137,10 @@ struct Workspace {
   size_t labels_cap = 0;
   uint32_t* state = nullptr;  // per-item state bits
   size_t state_cap = 0;
-  uint8_t* act = nullptr;     // 2 x items activation flags
+  uint32_t* act = nullptr;    // 2 x items activation flags
   size_t act_cap = 0;
+  uint32_t* list = nullptr;   // 2 x items active-item lists (round form)
+  size_t list_cap = 0;
   uint32_t* queue = nullptr;  // persistent work queue
   size_t queue_cap = 0;
   FieldDesc* fields = nullptr;
@@ -155,15 +166,23 @@ struct Workspace {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint8_t> key;  // launch arguments the graph was captured with
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex lane;  // single GPU lane (mr-cli lib.rs:499 METAL_ISOLATED_MUTEX)
   Workspace ws;
+  GraphCache graph;
   TileShape shape;
-  int mode = MODE_PERSISTENT;
+  int mode = MODE_GRAPH;
+  uint32_t graph_rounds = 8;  // rounds captured per hipGraph replay
   size_t label_budget = size_t(48) << 30;  // bytes of labels per chunk
   int cus = 256;
+  uint64_t spin_ticks = 400000000ull;  // bound of device spin waits (wall-clock ticks)
   SolveStats last;
   std::vector<FieldDesc> layout;  // fields of the last solve (for paths / reads)
   std::vector<BoardView> views;   // boards of the last solve
@@ -206,5 +225,6 @@ inline void read_field(Ctx& c, const DevBoard&, size_t field, uint32_t* dist, ui
 void ctx_init(Ctx& c, int device);
 void ctx_free(Ctx& c);
 size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t layers);
+bool tile_shape_supported(uint32_t tw, uint32_t th);
 
 }  // namespace mrf

@@ -73,7 +73,23 @@ enum : int {
   C_QHEAD = 4,
   C_QTAIL = 5,
   C_PENDING = 6,
-  C_NCOUNTERS = 8
+  C_NEXTN = 7,
+  C_BAR_ARRIVE = 8,
+  C_BAR_GEN = 9,
+  C_TAKEN0 = 10,  // 10..12
+  C_NEXT0 = 13,   // 13..15
+  C_PROUNDS = 16,
+  C_MISC = 17,
+  C_GROUND = 18,  // graph form: current round index
+  C_CURN = 19,    // graph form: length of the current round's list
+  C_NCOUNTERS = 20
+};
+// Each device counter sits on its own 128-byte line: polled words and the
+// work-list atomics must not share a line, or every pop and push queues
+// behind the pollers.
+constexpr uint32_t kCtrStride = 32;
+constexpr uint32_t kCtrWords = C_NCOUNTERS * kCtrStride;
+enum : int { C_PAD_ = 0
 };
 
 BoardView DevBoard::view() const {
@@ -113,6 +129,15 @@ __device__ __forceinline__ uint64_t extend(uint64_t prev, uint32_t s) {
   uint64_t cand = prev + ((((uint64_t)s) << 32) | 1ull);
   return (cand > prev && cand < kLabLimit) ? cand : kInf;
 }
+
+// Workgroup-uniform read of an LDS word.  Loop exits and branches around
+// __syncthreads() must be uniform for the compiler, not only at run time: a
+// value loaded from LDS lives in a vector register, and a loop whose exit
+// depends on it may be restructured so that lanes leave it at different
+// points (lanes 1..63 re-running the barrier while lane 0 is parked).
+// readfirstlane turns the value into a scalar so every such branch stays a
+// scalar branch.
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ bool sorted_has(const uint32_t* a, uint32_t n, uint32_t v) {
   uint32_t lo = 0, hi = n;
@@ -202,8 +227,10 @@ struct RelaxArgs {
   const uint32_t* pool;
   uint64_t* labels;
   uint32_t* state;
-  uint8_t* act_cur;
-  uint8_t* act_next;
+  uint32_t* act_cur;
+  uint32_t* act_next;
+  const uint32_t* list_cur;
+  uint32_t* list_next;
   uint32_t* counters;
   uint32_t* queue;
   uint32_t n_items;
@@ -211,26 +238,27 @@ struct RelaxArgs {
   uint32_t use_add;
   uint32_t qcap;
   uint32_t spin_limit;
+  uint64_t spin_ticks;  // real-time bound of any spin wait (wall_clock64 ticks)
 };
 
 // LDS carve-up for one tile (all sizes in bytes, 16-byte aligned sections).
 struct Lds {
   uint32_t* d;   // [L][th+2][rs] dist incl. halo ring
   uint32_t* h;   // [L][th+2][rs] hops
-  uint32_t* w;   // [L][th][tw] enter weight
-  uint32_t* a;   // [L][th][tw] additive price
-  uint8_t* m;    // [L][th][tw] mask
+  uint32_t* w;   // [L][th][tw+1] enter weight (odd row stride: no bank conflicts)
+  uint32_t* a;   // [L][th][tw+1] additive price
+  uint8_t* m;    // [L][th][tw+1] mask
   uint32_t* xg;  // [tw+1] x gap prices entering column i from the left
   uint32_t* yg;  // [th+1]
   uint32_t* vg;  // [16]
   uint32_t* misc;
-  uint32_t rs, rh;
+  uint32_t rs, rh, ps;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 __host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L, bool use_add) {
-  size_t rs = tw + 3, rh = th + 2, ncell = (size_t)L * th * tw, nlab = (size_t)L * rh * rs;
+  size_t rs = tw + 3, rh = th + 2, ncell = (size_t)L * th * (tw + 1), nlab = (size_t)L * rh * rs;
   size_t s = 0;
   s += align16(nlab * 4) * 2;
   s += align16(ncell * 4);
@@ -244,7 +272,8 @@ __device__ inline Lds carve(char* base, uint32_t tw, uint32_t th, uint32_t L, bo
   Lds s;
   s.rs = tw + 3;
   s.rh = th + 2;
-  size_t ncell = (size_t)L * th * tw, nlab = (size_t)L * s.rh * s.rs;
+  s.ps = tw + 1;
+  size_t ncell = (size_t)L * th * s.ps, nlab = (size_t)L * s.rh * s.rs;
   size_t o = 0;
   s.d = (uint32_t*)(base + o);
   o += align16(nlab * 4);
@@ -278,25 +307,195 @@ __device__ __forceinline__ void st_lab(const Lds& s, uint32_t i, uint64_t v) {
   s.h[i] = (uint32_t)v;
 }
 
+// Step price of entering LDS cell c over an edge priced e (kMax when the
+// edge or the cell is closed: `bits` must all be set in the cell mask).
+__device__ __forceinline__ uint32_t gated_step(const Lds& s, uint32_t c, uint8_t bits, uint32_t e, bool use_add) {
+  uint8_t m = s.m[c];
+  return ((m & bits) == bits) ? step_price(e, s.w[c], use_add ? s.a[c] : 0u) : kMax;
+}
+
+// Sweep one line held in registers: forward (index increasing) then backward.
+// lab[0] and lab[N+1] are the halo ends; sf[i] / sb[i] are the prices of
+// entering cell i from i-1 / i+1.  Returns a bit mask of improved cells.
+template <int N>
+__device__ __forceinline__ uint64_t sweep_line(uint64_t (&lab)[N + 2], const uint32_t (&sf)[N],
+                                               const uint32_t (&sb)[N]) {
+  uint64_t chg = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    uint64_t c = extend(lab[i], sf[i]);
+    if (c < lab[i + 1]) {
+      lab[i + 1] = c;
+      chg |= 1ull << i;
+    }
+  }
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    uint64_t c = extend(lab[i + 2], sb[i]);
+    if (c < lab[i + 1]) {
+      lab[i + 1] = c;
+      chg |= 1ull << i;
+    }
+  }
+  return chg;
+}
+
+// In-LDS relaxation of one tile to a local fixpoint: rows, columns and via
+// columns are each swept forward and backward with the whole line held in
+// registers (one lane per line).  Returns true when the round cap was hit.
+template <int TW, int TH, int LM>
+__device__ bool relax_local(const Lds& s, uint32_t L, bool use_add, uint32_t max_local, uint32_t& rounds) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  const uint32_t rs = s.rs, rh = s.rh;
+  constexpr uint32_t PS = TW + 1;  // padded interior row stride
+  constexpr uint32_t plane_p = TH * PS;
+  for (rounds = 0; rounds < max_local; ++rounds) {
+    bool ch = false;
+    // rows (lane per (layer, y)); forward then backward, each with its own
+    // step array so only one is live at a time
+    for (uint32_t line = tid; line < L * TH; line += nt) {
+      uint32_t l = line / TH, y = line - l * TH;
+      uint32_t lb = (l * rh + y + 1) * rs, cb = (l * TH + y) * PS;
+      uint64_t lab[TW + 2];
+      uint32_t st[TW];
+      uint64_t chg = 0;
+#pragma unroll
+      for (int i = 0; i < TW + 2; ++i) lab[i] = ld_lab(s, lb + i);
+#pragma unroll
+      for (int x = 0; x < TW; ++x) st[x] = gated_step(s, cb + x, K_PASS | K_NX, s.xg[x], use_add);
+#pragma unroll
+      for (int x = 0; x < TW; ++x) {
+        uint64_t c = extend(lab[x], st[x]);
+        if (c < lab[x + 1]) {
+          lab[x + 1] = c;
+          chg |= 1ull << x;
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < TW; ++x) st[x] = gated_step(s, cb + x, K_PASS | K_PX, s.xg[x + 1], use_add);
+#pragma unroll
+      for (int x = TW - 1; x >= 0; --x) {
+        uint64_t c = extend(lab[x + 2], st[x]);
+        if (c < lab[x + 1]) {
+          lab[x + 1] = c;
+          chg |= 1ull << x;
+        }
+      }
+      if (chg) {
+        ch = true;
+#pragma unroll
+        for (int x = 0; x < TW; ++x)
+          if ((chg >> x) & 1) {
+            st_lab(s, lb + x + 1, lab[x + 1]);
+            s.m[cb + x] |= K_CHG;
+          }
+      }
+    }
+    __syncthreads();
+    // columns (lane per (layer, x))
+    for (uint32_t line = tid; line < L * TW; line += nt) {
+      uint32_t l = line / TW, x = line - l * TW;
+      uint32_t lb = l * rh * rs + x + 1, cb = l * plane_p + x;
+      uint64_t lab[TH + 2];
+      uint32_t st[TH];
+      uint64_t chg = 0;
+#pragma unroll
+      for (int i = 0; i < TH + 2; ++i) lab[i] = ld_lab(s, lb + i * rs);
+#pragma unroll
+      for (int y = 0; y < TH; ++y) st[y] = gated_step(s, cb + y * PS, K_PASS | K_NY, s.yg[y], use_add);
+#pragma unroll
+      for (int y = 0; y < TH; ++y) {
+        uint64_t c = extend(lab[y], st[y]);
+        if (c < lab[y + 1]) {
+          lab[y + 1] = c;
+          chg |= 1ull << y;
+        }
+      }
+#pragma unroll
+      for (int y = 0; y < TH; ++y) st[y] = gated_step(s, cb + y * PS, K_PASS | K_PY, s.yg[y + 1], use_add);
+#pragma unroll
+      for (int y = TH - 1; y >= 0; --y) {
+        uint64_t c = extend(lab[y + 2], st[y]);
+        if (c < lab[y + 1]) {
+          lab[y + 1] = c;
+          chg |= 1ull << y;
+        }
+      }
+      if (chg) {
+        ch = true;
+#pragma unroll
+        for (int y = 0; y < TH; ++y)
+          if ((chg >> y) & 1) {
+            st_lab(s, lb + (y + 1) * rs, lab[y + 1]);
+            s.m[cb + y * PS] |= K_CHG;
+          }
+      }
+    }
+    __syncthreads();
+    // via columns (lane per (y, x)); at most LM layers
+    if (L > 1) {
+      for (uint32_t col = tid; col < TW * TH; col += nt) {
+        uint32_t y = col / TW, x = col - y * TW;
+        uint32_t lb = (y + 1) * rs + x + 1, cb = y * PS + x;
+        uint64_t lab[LM + 2];
+        uint32_t sf[LM], sb[LM];
+#pragma unroll
+        for (int l = 0; l < LM + 2; ++l) lab[l] = kInf;
+#pragma unroll
+        for (int l = 0; l < LM; ++l) {
+          sf[l] = kMax;
+          sb[l] = kMax;
+          if ((uint32_t)l < L) {
+            uint32_t c = l * plane_p + cb;
+            lab[l + 1] = ld_lab(s, l * rh * rs + lb);
+            if (l > 0) sf[l] = gated_step(s, c, K_PASS | K_DN, s.vg[l - 1], use_add);
+            if ((uint32_t)l + 1 < L) sb[l] = gated_step(s, c, K_PASS | K_UP, s.vg[l], use_add);
+          }
+        }
+        uint64_t chg = sweep_line<LM>(lab, sf, sb);
+        if (chg) {
+          ch = true;
+#pragma unroll
+          for (int l = 0; l < LM; ++l)
+            if ((chg >> l) & 1) {
+              st_lab(s, l * rh * rs + lb, lab[l + 1]);
+              s.m[l * plane_p + cb] |= K_CHG;
+            }
+        }
+      }
+    }
+    if (ch) s.misc[2] = rounds + 1;
+    __syncthreads();
+    if (uniform_u32(s.misc[2]) != rounds + 1) {
+      ++rounds;
+      return false;
+    }
+  }
+  return true;
+}
+
 // Result flags of one tile visit.
 enum : uint32_t { T_LEFT = 1, T_RIGHT = 2, T_TOP = 4, T_BOTTOM = 8, T_ANY = 16, T_CAPPED = 32 };
 
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
+template <int TW, int TH, int LM>
 __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = A.item_field[item];
   const FieldDesc F = A.fields[f];
-  const BoardView B = A.boards[F.board];
+  const BoardView& B = A.boards[F.board];
   const uint32_t t = item - F.item0;
   const uint32_t txi = t % F.tx, tyi = t / F.tx;
-  const uint32_t tw = A.tw, th = A.th, L = B.layers;
+  constexpr uint32_t tw = TW, th = TH;
+  const uint32_t L = B.layers;
   const uint32_t ncell = L * th * tw, plane_t = th * tw;
   const uint32_t wx0 = txi * tw, wy0 = tyi * th;
   const bool use_add = A.use_add != 0;  // LDS holds an additive plane in this launch
   const bool has_add = B.add != nullptr || B.halo != nullptr;
   Lds s = carve(smem, tw, th, L, use_add);
   const uint32_t rs = s.rs, rh = s.rh;
+  constexpr uint32_t PS = TW + 1;  // padded interior row stride (mask / weight / additive)
   const uint64_t blockbase = F.lab_off + (uint64_t)t * ncell;
 
   if (tid == 0) {
@@ -315,7 +514,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
   }
   for (uint32_t i = tid; i < 16; i += nt) s.vg[i] = (i + 1 < L) ? B.ve[i] : 0u;
   __syncthreads();
-  const bool touched = (s.misc[0] & S_TOUCHED) != 0;
+  const bool touched = (uniform_u32(s.misc[0]) & S_TOUCHED) != 0;
 
   // interior cells
   for (uint32_t c = tid; c < ncell; c += nt) {
@@ -335,9 +534,10 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       if (touched)
         lab = __hip_atomic_load(&A.labels[blockbase + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    s.m[c] = m;
-    s.w[c] = wgt;
-    if (use_add) s.a[c] = ap;
+    uint32_t pc = (l * th + y) * PS + x;
+    s.m[pc] = m;
+    s.w[pc] = wgt;
+    if (use_add) s.a[pc] = ap;
     st_lab(s, li, lab);
   }
   // halo strips (neighbour tiles of the same field)
@@ -393,7 +593,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       uint32_t row0 = (l * B.h + F.y0 + wy) * B.w + F.x0 + wx0;
       uint32_t lim = min(tw, F.ww - wx0);
       for (uint32_t k = lower_bound(pads, F.n_pad, row0); k < F.n_pad && pads[k] < row0 + lim; ++k) {
-        uint32_t x = pads[k] - row0, c = (l * th + y) * tw + x;
+        uint32_t x = pads[k] - row0, c = (l * th + y) * PS + x;
         uint8_t bm = B.bmask[pads[k]];
         if (!(bm & M_TRACE)) s.m[c] |= K_PASS;
         if (bm & M_OBST) s.w[c] = 1u;
@@ -407,7 +607,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       if (gx < F.x0 + wx0 || gy < F.y0 + wy0) continue;
       uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0;
       if (x >= tw || y >= th || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
-      if (!(B.bmask[g] & M_TRACE)) s.m[(l * th + y) * tw + x] |= K_PASS;
+      if (!(B.bmask[g] & M_TRACE)) s.m[(l * th + y) * PS + x] |= K_PASS;
     }
   }
   __syncthreads();
@@ -424,7 +624,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
           int ll = (int)l + dl;
           if (ll < 0 || ll >= (int)L) continue;
           uint32_t g = vp[k] + (uint32_t)((int)plane * dl);
-          uint32_t c = ((uint32_t)ll * th + y) * tw + x;
+          uint32_t c = ((uint32_t)ll * th + y) * PS + x;
           uint8_t vb = via_bits(B, F, A.pool, g);
           s.m[c] = (uint8_t)((s.m[c] & ~(K_DN | K_UP)) | vb);
         }
@@ -440,7 +640,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       if (gx < F.x0 + wx0 || gy < F.y0 + wy0) continue;
       uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0;
       if (x >= tw || y >= th || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
-      uint32_t c = (l * th + y) * tw + x;
+      uint32_t c = (l * th + y) * PS + x;
       if (s.m[c] & K_PASS) {
         st_lab(s, (l * rh + y + 1) * rs + x + 1, 0ull);
         s.m[c] |= K_CHG;
@@ -451,147 +651,12 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
 
   // ---- in-LDS relaxation to a local fixpoint ----------------------------------
   uint32_t rounds = 0;
-  bool capped = true;
-  for (; rounds < A.max_local; ++rounds) {
-    bool ch = false;
-    // rows: left->right then right->left
-    for (uint32_t line = tid; line < L * th; line += nt) {
-      uint32_t l = line / th, y = line - l * th;
-      uint32_t lb = (l * rh + y + 1) * rs, cb = (l * th + y) * tw;
-      uint64_t prev = ld_lab(s, lb);
-      for (uint32_t x = 0; x < tw; ++x) {
-        uint32_t c = cb + x;
-        uint8_t m = s.m[c];
-        uint64_t cur = ld_lab(s, lb + x + 1);
-        uint32_t st = ((m & (K_PASS | K_NX)) == (K_PASS | K_NX))
-                          ? step_price(s.xg[x], s.w[c], use_add ? s.a[c] : 0u)
-                          : kMax;
-        uint64_t cand = extend(prev, st);
-        if (cand < cur) {
-          cur = cand;
-          st_lab(s, lb + x + 1, cur);
-          s.m[c] = m | K_CHG;
-          ch = true;
-        }
-        prev = cur;
-      }
-      prev = ld_lab(s, lb + tw + 1);
-      for (int x = (int)tw - 1; x >= 0; --x) {
-        uint32_t c = cb + x;
-        uint8_t m = s.m[c];
-        uint64_t cur = ld_lab(s, lb + x + 1);
-        uint32_t st = ((m & (K_PASS | K_PX)) == (K_PASS | K_PX))
-                          ? step_price(s.xg[x + 1], s.w[c], use_add ? s.a[c] : 0u)
-                          : kMax;
-        uint64_t cand = extend(prev, st);
-        if (cand < cur) {
-          cur = cand;
-          st_lab(s, lb + x + 1, cur);
-          s.m[c] = m | K_CHG;
-          ch = true;
-        }
-        prev = cur;
-      }
-    }
-    __syncthreads();
-    // columns: top->bottom then bottom->top
-    for (uint32_t line = tid; line < L * tw; line += nt) {
-      uint32_t l = line / tw, x = line - l * tw;
-      uint32_t lb = l * rh * rs + x + 1, cb = l * th * tw + x;
-      uint64_t prev = ld_lab(s, lb);
-      for (uint32_t y = 0; y < th; ++y) {
-        uint32_t c = cb + y * tw;
-        uint8_t m = s.m[c];
-        uint32_t li = lb + (y + 1) * rs;
-        uint64_t cur = ld_lab(s, li);
-        uint32_t st = ((m & (K_PASS | K_NY)) == (K_PASS | K_NY))
-                          ? step_price(s.yg[y], s.w[c], use_add ? s.a[c] : 0u)
-                          : kMax;
-        uint64_t cand = extend(prev, st);
-        if (cand < cur) {
-          cur = cand;
-          st_lab(s, li, cur);
-          s.m[c] = m | K_CHG;
-          ch = true;
-        }
-        prev = cur;
-      }
-      prev = ld_lab(s, lb + (th + 1) * rs);
-      for (int y = (int)th - 1; y >= 0; --y) {
-        uint32_t c = cb + y * tw;
-        uint8_t m = s.m[c];
-        uint32_t li = lb + (y + 1) * rs;
-        uint64_t cur = ld_lab(s, li);
-        uint32_t st = ((m & (K_PASS | K_PY)) == (K_PASS | K_PY))
-                          ? step_price(s.yg[y + 1], s.w[c], use_add ? s.a[c] : 0u)
-                          : kMax;
-        uint64_t cand = extend(prev, st);
-        if (cand < cur) {
-          cur = cand;
-          st_lab(s, li, cur);
-          s.m[c] = m | K_CHG;
-          ch = true;
-        }
-        prev = cur;
-      }
-    }
-    __syncthreads();
-    // vias: per (x, y) column, up then down
-    if (L > 1) {
-      for (uint32_t col = tid; col < plane_t; col += nt) {
-        uint32_t y = col / tw, x = col - y * tw;
-        uint32_t lb = (y + 1) * rs + x + 1;
-        uint64_t prev = ld_lab(s, lb);
-        for (uint32_t l = 1; l < L; ++l) {
-          uint32_t c = l * plane_t + col;
-          uint8_t m = s.m[c];
-          uint32_t li = l * rh * rs + lb;
-          uint64_t cur = ld_lab(s, li);
-          uint32_t st = ((m & (K_PASS | K_DN)) == (K_PASS | K_DN))
-                            ? step_price(s.vg[l - 1], s.w[c], use_add ? s.a[c] : 0u)
-                            : kMax;
-          uint64_t cand = extend(prev, st);
-          if (cand < cur) {
-            cur = cand;
-            st_lab(s, li, cur);
-            s.m[c] = m | K_CHG;
-            ch = true;
-          }
-          prev = cur;
-        }
-        for (int l = (int)L - 2; l >= 0; --l) {
-          uint32_t c = (uint32_t)l * plane_t + col;
-          uint8_t m = s.m[c];
-          uint32_t li = (uint32_t)l * rh * rs + lb;
-          uint64_t cur = ld_lab(s, li);
-          uint32_t st = ((m & (K_PASS | K_UP)) == (K_PASS | K_UP))
-                            ? step_price(s.vg[l], s.w[c], use_add ? s.a[c] : 0u)
-                            : kMax;
-          uint64_t cand = extend(prev, st);
-          if (cand < cur) {
-            cur = cand;
-            st_lab(s, li, cur);
-            s.m[c] = m | K_CHG;
-            ch = true;
-          }
-          prev = cur;
-        }
-      }
-    }
-    if (ch) s.misc[2] = rounds + 1;
-    __syncthreads();
-    if (s.misc[2] != rounds + 1) {
-      capped = false;
-      ++rounds;
-      break;
-    }
-  }
-
+  bool capped = relax_local<TW, TH, LM>(s, L, use_add, A.max_local, rounds);
   // ---- write back -----------------------------------------------------------
   uint32_t flags = 0;
   for (uint32_t c = tid; c < ncell; c += nt) {
-    uint8_t m = s.m[c];
     uint32_t l = c / plane_t, r = c - l * plane_t, y = r / tw, x = r - y * tw;
+    uint8_t m = s.m[(l * th + y) * PS + x];
     if (!touched || (m & K_CHG)) {
       uint64_t v = ld_lab(s, (l * rh + y + 1) * rs + x + 1);
       __hip_atomic_store(&A.labels[blockbase + c], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -605,14 +670,12 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
     }
   }
   if (flags) atomicOr(&s.misc[1], flags);
-  __threadfence();
-  __syncthreads();
-  uint32_t res = s.misc[1] | (capped ? T_CAPPED : 0u);
+  __syncthreads();  // also orders this workgroup's label stores before the flag / queue updates
+  uint32_t res = uniform_u32(s.misc[1]) | (capped ? T_CAPPED : 0u);
   if (tid == 0) {
-    if (!touched)
-      __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    atomicAdd(&A.counters[C_VISITS], 1u);
-    atomicAdd(&A.counters[C_ROUNDS], rounds);
+    if (!touched) __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
+    atomicAdd(&A.counters[(C_ROUNDS) * kCtrStride], rounds);
   }
   __syncthreads();
   return res;
@@ -626,137 +689,183 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
   return xcd * per + k;
 }
 
-__global__ void relax_round(RelaxArgs A) {
+// Activate `nb` for the next round once (flag + append to the next list).
+__device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb) {
+  if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[(C_NEXTN) * kCtrStride], 1u)] = nb;
+}
+
+template <int TW, int TH, int LM>
+__global__ void __launch_bounds__(256, 2) relax_round(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
-  uint32_t item = xcd_item(blockIdx.x, gridDim.x);  // gridDim.x is a multiple of 8
-  if (item >= A.n_items) return;
-  if (!A.act_cur[item]) return;
-  __syncthreads();
-  uint32_t res = process_tile(A, item, smem);
+  uint32_t item = A.list_cur[blockIdx.x];
+  uint32_t res = process_tile<TW, TH, LM>(A, item, smem);
   if (threadIdx.x == 0) {
     A.act_cur[item] = 0;
     const FieldDesc F = A.fields[A.item_field[item]];
     uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-    if ((res & T_LEFT) && txi > 0) A.act_next[item - 1] = 1;
-    if ((res & T_RIGHT) && txi + 1 < F.tx) A.act_next[item + 1] = 1;
-    if ((res & T_TOP) && tyi > 0) A.act_next[item - F.tx] = 1;
-    if ((res & T_BOTTOM) && tyi + 1 < F.ty) A.act_next[item + F.tx] = 1;
-    if (res & T_CAPPED) A.act_next[item] = 1;
-    if (res & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[C_CHANGED], 1u);
+    if ((res & T_LEFT) && txi > 0) activate(A, item - 1);
+    if ((res & T_RIGHT) && txi + 1 < F.tx) activate(A, item + 1);
+    if ((res & T_TOP) && tyi > 0) activate(A, item - F.tx);
+    if ((res & T_BOTTOM) && tyi + 1 < F.ty) activate(A, item + F.tx);
+    if (res & T_CAPPED) activate(A, item);
+    if (res & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
   }
 }
 
 // ---------------------------------------------------------------------------
-// persistent work-queue form
+// persistent form: device-side rounds over a work list
 // ---------------------------------------------------------------------------
 //
-// state bits: PEND = needs (re)processing, ACTIVE = owned by a workgroup.
-// push: old = or(PEND); enqueue iff PEND was clear and ACTIVE clear.
-// pop:  xor(PEND|ACTIVE) (PEND 1->0, ACTIVE 0->1) -- exclusive ownership.
-// done: CAS(ACTIVE -> idle); if PEND was set meanwhile, clear it and re-run.
-// pending counts enqueued-but-unfinished items; all workgroups exit when it
-// reaches zero.  Every spin is bounded by spin_limit (error flag + exit).
+// All workgroups stay resident.  A round pops items from the current active
+// list with one atomicAdd per item (a device work queue), processes them and
+// appends activated neighbours to the next list (deduplicated by flags).  A
+// grid barrier with agent-scope fences separates rounds, so every round reads
+// labels written by earlier rounds exactly as the host-driven form does, and
+// the loop ends when a round activates nothing.  Per-round counters are
+// triple-buffered so a slow workgroup can still read round r's counters while
+// round r+1 runs.  Every wait is bounded by spin_limit (error flag + exit).
 
-__device__ void q_push(const RelaxArgs& A, uint32_t item) {
-  uint32_t old = __hip_atomic_fetch_or(&A.state[item], S_PEND, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  if ((old & (S_PEND | S_ACTIVE)) == 0) {
-    __hip_atomic_fetch_add(&A.counters[C_PENDING], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t slot = __hip_atomic_fetch_add(&A.counters[C_QTAIL], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t* q = &A.queue[slot % A.qcap];
-    uint32_t spins = 0;
-    while (__hip_atomic_load(q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != kMax) {
-      if (++spins > A.spin_limit) {
-        atomicOr(&A.counters[C_ERROR], 2u);
-        return;
+__device__ bool grid_barrier(const RelaxArgs& A, uint32_t nblocks, uint32_t& my_gen) {
+  __syncthreads();
+  bool ok = true;
+  if (threadIdx.x == 0) {
+    __threadfence();  // release this workgroup's label stores
+    uint32_t g = my_gen;
+    if (atomicAdd(&A.counters[(C_BAR_ARRIVE) * kCtrStride], 1u) == nblocks - 1) {
+      atomicExch(&A.counters[(C_BAR_ARRIVE) * kCtrStride], 0u);
+      __hip_atomic_fetch_add(&A.counters[(C_BAR_GEN) * kCtrStride], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      // bounded by the real-time clock, with exponential backoff
+      uint64_t t0 = wall_clock64();
+      uint32_t naps = 1;
+      while (__hip_atomic_load(&A.counters[(C_BAR_GEN) * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (wall_clock64() - t0 > A.spin_ticks) {
+          atomicOr(&A.counters[(C_ERROR) * kCtrStride], 8u);
+          ok = false;
+          break;
+        }
+        for (uint32_t i = 0; i < naps; ++i) __builtin_amdgcn_s_sleep(8);
+        naps = naps < 16 ? naps * 2 : 16;
       }
-      __builtin_amdgcn_s_sleep(1);
     }
-    __hip_atomic_store(q, item, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();  // acquire labels written before the barrier
+    my_gen = g + 1;
   }
+  __shared__ uint32_t s_ok;
+  if (threadIdx.x == 0) s_ok = ok ? 1u : 0u;
+  __syncthreads();
+  return uniform_u32(s_ok) != 0;
 }
 
-__global__ void relax_persistent(RelaxArgs A) {
+template <int TW, int TH, int LM>
+__global__ void __launch_bounds__(256, 2) relax_persistent(RelaxArgs A, uint32_t n_first) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ uint32_t s_item;
-  uint32_t idle = 0;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      uint32_t got = kMax;
-      uint32_t h = __hip_atomic_load(&A.counters[C_QHEAD], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t tl = __hip_atomic_load(&A.counters[C_QTAIL], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (h < tl &&
-          __hip_atomic_compare_exchange_strong(&A.counters[C_QHEAD], &h, h + 1, __ATOMIC_ACQ_REL,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        uint32_t* q = &A.queue[h % A.qcap];
-        uint32_t spins = 0;
-        for (;;) {
-          got = __hip_atomic_exchange(q, kMax, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-          if (got != kMax) break;
-          if (++spins > A.spin_limit) {
-            atomicOr(&A.counters[C_ERROR], 4u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        if (got != kMax)
-          __hip_atomic_fetch_xor(&A.state[got], S_PEND | S_ACTIVE, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      } else if (__hip_atomic_load(&A.counters[C_PENDING], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 ||
-                 __hip_atomic_load(&A.counters[C_ERROR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-        got = kMax - 1;  // drain: nothing pending anywhere
-      } else if (++idle > A.spin_limit) {
-        atomicOr(&A.counters[C_ERROR], 8u);
-        got = kMax - 1;
-      }
-      s_item = got;
-    }
-    __syncthreads();
-    uint32_t item = s_item;
-    __syncthreads();
-    if (item == kMax - 1) break;
-    if (item == kMax) {
-      if (threadIdx.x == 0) __builtin_amdgcn_s_sleep(2);
-      continue;
-    }
-    idle = 0;
-    __threadfence();  // acquire: see labels published before the push
+  __shared__ uint32_t s_k, s_n;
+  RelaxArgs R = A;
+  uint32_t my_gen = 0;
+  uint32_t n_cur = n_first;
+  for (uint32_t round = 0;; ++round) {
+    uint32_t* taken = &A.counters[(C_TAKEN0 + round % 3) * kCtrStride];
+    uint32_t* nextn = &A.counters[(C_NEXT0 + round % 3) * kCtrStride];
+    R.counters = A.counters;
+    // pop / process this round's list
     for (;;) {
-      uint32_t res = process_tile(A, item, smem);
+      if (threadIdx.x == 0) s_k = atomicAdd(taken, 1u);
+      __syncthreads();
+      uint32_t k = uniform_u32(s_k);
+      __syncthreads();
+      if (k >= n_cur) break;
+      uint32_t item = uniform_u32(R.list_cur[k]);
+      uint32_t res = process_tile<TW, TH, LM>(R, item, smem);
       if (threadIdx.x == 0) {
-        const FieldDesc F = A.fields[A.item_field[item]];
+        R.act_cur[item] = 0;
+        const FieldDesc F = R.fields[R.item_field[item]];
         uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-        if ((res & T_LEFT) && txi > 0) q_push(A, item - 1);
-        if ((res & T_RIGHT) && txi + 1 < F.tx) q_push(A, item + 1);
-        if ((res & T_TOP) && tyi > 0) q_push(A, item - F.tx);
-        if ((res & T_BOTTOM) && tyi + 1 < F.ty) q_push(A, item + F.tx);
-        uint32_t again = (res & T_CAPPED) ? 1u : 0u;
-        if (!again) {
-          uint32_t cur = __hip_atomic_load(&A.state[item], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-          uint32_t want = cur & ~(S_ACTIVE | S_PEND);
-          if (cur & S_PEND) {
-            __hip_atomic_fetch_and(&A.state[item], ~S_PEND, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            again = 1;
-          } else if (!__hip_atomic_compare_exchange_strong(&A.state[item], &cur, want, __ATOMIC_ACQ_REL,
-                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            // PEND raced in: clear it and run again while still owning the item
-            __hip_atomic_fetch_and(&A.state[item], ~S_PEND, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            again = 1;
-          }
-        }
-        if (!again) __hip_atomic_fetch_sub(&A.counters[C_PENDING], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        s_item = again;
+        auto act = [&](uint32_t nb) {
+          if (atomicExch(&R.act_next[nb], 1u) == 0u) R.list_next[atomicAdd(nextn, 1u)] = nb;
+        };
+        if ((res & T_LEFT) && txi > 0) act(item - 1);
+        if ((res & T_RIGHT) && txi + 1 < F.tx) act(item + 1);
+        if ((res & T_TOP) && tyi > 0) act(item - F.tx);
+        if ((res & T_BOTTOM) && tyi + 1 < F.ty) act(item + F.tx);
+        if (res & T_CAPPED) act(item);
       }
-      __syncthreads();
-      uint32_t again = s_item;
-      __syncthreads();
-      if (!again) break;
-      __threadfence();
     }
+    // the counters of round + 1 were last used in round - 2: clear them now
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      atomicExch(&A.counters[(C_TAKEN0 + (round + 1) % 3) * kCtrStride], 0u);
+      atomicExch(&A.counters[(C_NEXT0 + (round + 1) % 3) * kCtrStride], 0u);
+    }
+    if (!grid_barrier(A, gridDim.x, my_gen)) return;
+    if (threadIdx.x == 0)
+      s_n = __hip_atomic_load(nextn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    n_cur = uniform_u32(s_n);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.counters[(C_PROUNDS) * kCtrStride], 1u);
+    if (n_cur == 0 ||
+        uniform_u32(__hip_atomic_load(&A.counters[(C_ERROR) * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+      return;
+    uint32_t* ta = R.act_cur;
+    R.act_cur = R.act_next;
+    R.act_next = ta;
+    const uint32_t* tl = R.list_cur;
+    R.list_cur = R.list_next;
+    R.list_next = const_cast<uint32_t*>(tl);
   }
 }
 
-__global__ void seed_queue(RelaxArgs A, const uint32_t* items, uint32_t n) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) q_push(A, items[i]);
+// ---------------------------------------------------------------------------
+// graph form: one grid-stride launch per round, rounds replayed from a hipGraph
+// ---------------------------------------------------------------------------
+//
+// A launch reads the round index and list length that the previous
+// advance_round wrote, pops items from the round's list with one atomicAdd
+// per item (the device work queue) and appends activated neighbours to the
+// other list.  Kernel boundaries separate rounds, so every round reads
+// coherent labels; no workgroup ever waits on another.  A launch whose list is
+// empty returns at once, so a captured sequence can overshoot the fixpoint.
+
+template <int TW, int TH, int LM>
+__global__ void __launch_bounds__(256, 2) relax_list(RelaxArgs E, RelaxArgs O) {
+  extern __shared__ __align__(16) char smem[];
+  const uint32_t r =
+      uniform_u32(__hip_atomic_load(&E.counters[C_GROUND * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const uint32_t n_cur =
+      uniform_u32(__hip_atomic_load(&E.counters[C_CURN * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  // E holds the even-round list/flag pointers, O the same pointers swapped;
+  // both stay kernel arguments (no private copy).
+  const RelaxArgs& R = (r & 1) ? O : E;
+  // Grid-stride loop: the index is a workgroup-uniform scalar, so the loop
+  // (which contains __syncthreads) stays convergent.  A pop loop of the form
+  // `if (tid == 0) k = atomicAdd(..); __syncthreads(); ..` is restructured by
+  // the compiler so lanes other than 0 re-run the body without a new pop.
+  for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
+    uint32_t item = uniform_u32(R.list_cur[k]);
+    uint32_t res = process_tile<TW, TH, LM>(R, item, smem);
+    if (threadIdx.x == 0) {
+      R.act_cur[item] = 0;
+      const FieldDesc F = R.fields[R.item_field[item]];
+      uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
+      if ((res & T_LEFT) && txi > 0) activate(R, item - 1);
+      if ((res & T_RIGHT) && txi + 1 < F.tx) activate(R, item + 1);
+      if ((res & T_TOP) && tyi > 0) activate(R, item - F.tx);
+      if ((res & T_BOTTOM) && tyi + 1 < F.ty) activate(R, item + F.tx);
+      if (res & T_CAPPED) activate(R, item);
+    }
+    __syncthreads();
+  }
+}
+
+// Single-thread round step between two relax_list launches.
+__global__ void advance_round(uint32_t* ctr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t n = ctr[C_NEXTN * kCtrStride];
+  if (ctr[C_CURN * kCtrStride]) ctr[C_PROUNDS * kCtrStride] += 1;
+  ctr[C_CURN * kCtrStride] = n;
+  ctr[C_NEXTN * kCtrStride] = 0;
+  ctr[C_TAKEN0 * kCtrStride] = 0;
+  ctr[C_GROUND * kCtrStride] += 1;
 }
 
 __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -771,9 +880,22 @@ __global__ void fill_u8(uint8_t* p, uint8_t v, size_t n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ void set_u8(uint8_t* p, const uint32_t* idx, uint32_t n) {
+// Activate items idx[0..n) for the first round: flags and list.
+__global__ void seed_list(uint32_t* act, uint32_t* list, const uint32_t* idx, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[idx[i]] = 1;
+  if (i < n) {
+    act[idx[i]] = 1u;
+    list[i] = idx[i];
+  }
+}
+
+// Activate every item: flags and the identity list.
+__global__ void all_list(uint32_t* act, uint32_t* list, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    act[i] = 1u;
+    list[i] = i;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -865,7 +987,7 @@ __global__ void walk_paths(WalkArgs W) {
     }
     uint64_t mask = __ballot(ok);
     if (mask == 0) {
-      if (lane == 0) atomicOr(&W.counters[C_ERROR], 16u);
+      if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 16u);
       return;
     }
     uint32_t win = __ffsll((long long)mask) - 1;
@@ -873,7 +995,7 @@ __global__ void walk_paths(WalkArgs W) {
     uint32_t lo = __shfl((uint32_t)lab, win), hi = __shfl((uint32_t)(lab >> 32), win);
     lab = ((uint64_t)hi << 32) | lo;
   }
-  if (lane == 0) atomicOr(&W.counters[C_ERROR], 32u);
+  if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 32u);
 }
 
 // Tile-major labels of one field -> window-major u32 dist (and hops).
@@ -914,6 +1036,55 @@ static T* dev_copy(const T* h, size_t n, hipStream_t s) {
   return d;
 }
 
+// Instantiated tile shapes (width x height) and layer bounds.
+struct KernelSet {
+  uint32_t tw, th, lm;
+  const void* round;
+  const void* persistent;
+  const void* list;
+};
+
+template <int TW, int TH, int LM>
+static KernelSet make_set() {
+  return KernelSet{TW, TH, LM, (const void*)relax_round<TW, TH, LM>, (const void*)relax_persistent<TW, TH, LM>,
+                   (const void*)relax_list<TW, TH, LM>};
+}
+
+template <int TW, int TH>
+static void add_shape(std::vector<KernelSet>& v) {
+  v.push_back(make_set<TW, TH, 2>());
+  v.push_back(make_set<TW, TH, 4>());
+  v.push_back(make_set<TW, TH, 8>());
+  v.push_back(make_set<TW, TH, 16>());
+}
+
+static const std::vector<KernelSet>& kernel_sets() {
+  static const std::vector<KernelSet> v = [] {
+    std::vector<KernelSet> k;
+    add_shape<32, 32>(k);
+    add_shape<32, 16>(k);
+    add_shape<16, 16>(k);
+    add_shape<16, 8>(k);
+    add_shape<8, 4>(k);
+    add_shape<4, 1>(k);
+    return k;
+  }();
+  return v;
+}
+
+bool tile_shape_supported(uint32_t tw, uint32_t th) {
+  for (const KernelSet& k : kernel_sets())
+    if (k.tw == tw && k.th == th) return true;
+  return false;
+}
+
+static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers) {
+  uint32_t lm = layers <= 2 ? 2 : layers <= 4 ? 4 : layers <= 8 ? 8 : 16;
+  for (const KernelSet& k : kernel_sets())
+    if (k.tw == tw && k.th == th && k.lm == lm) return k;
+  throw Error(1, "tile shape " + std::to_string(tw) + "x" + std::to_string(th) + " is not instantiated");
+}
+
 void ctx_init(Ctx& c, int device) {
   int n = 0;
   HIP_OK(hipGetDeviceCount(&n));
@@ -924,26 +1095,34 @@ void ctx_init(Ctx& c, int device) {
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, device));
   c.cus = prop.multiProcessorCount;
+  int rate_khz = 0;
+  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
+    rate_khz = 100000;
+  c.spin_ticks = (uint64_t)rate_khz * 1000ull * 4ull;  // 4 s
   HIP_OK(hipEventCreate(&c.ws.ev0));
   HIP_OK(hipEventCreate(&c.ws.ev1));
   HIP_OK(hipEventCreate(&c.ws.ev2));
   size_t total = prop.totalGlobalMem;
   c.label_budget = std::min<size_t>(c.label_budget, total / 2);
-  HIP_OK(hipMalloc((void**)&c.ws.counters, C_NCOUNTERS * sizeof(uint32_t)));
-  HIP_OK(hipHostMalloc((void**)&c.ws.pinned, C_NCOUNTERS * sizeof(uint32_t), hipHostMallocDefault));
-  HIP_OK(hipFuncSetAttribute((const void*)relax_round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  HIP_OK(hipFuncSetAttribute((const void*)relax_persistent, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             160 * 1024 - 64));
+  HIP_OK(hipMalloc((void**)&c.ws.counters, kCtrWords * sizeof(uint32_t)));
+  HIP_OK(hipHostMalloc((void**)&c.ws.pinned, kCtrWords * sizeof(uint32_t), hipHostMallocDefault));
+  for (const KernelSet& k : kernel_sets()) {
+    HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    HIP_OK(hipFuncSetAttribute(k.persistent, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+  }
 }
 
 void ctx_free(Ctx& c) {
   Workspace& w = c.ws;
-  void* ptrs[] = {w.labels, w.state, w.act, w.queue, w.fields, w.item_field, w.pool, w.counters,
-                  w.plen, w.poff, w.pcost, w.pcells, w.fout};
+  void* ptrs[] = {w.labels, w.state, w.act, w.list, w.queue, w.fields, w.item_field, w.pool, w.counters,
+                  w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards};
   (void)hipSetDevice(c.device);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (w.pinned) (void)hipHostFree(w.pinned);
+  if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
+  c.graph = GraphCache();
   if (w.ev0) (void)hipEventDestroy(w.ev0);
   if (w.ev1) (void)hipEventDestroy(w.ev1);
   if (w.ev2) (void)hipEventDestroy(w.ev2);
@@ -1047,6 +1226,8 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items, bool use_add) {
   A.state = c.ws.state;
   A.act_cur = c.ws.act;
   A.act_next = c.ws.act + n_items;
+  A.list_cur = c.ws.list;
+  A.list_next = c.ws.list + n_items;
   A.counters = c.ws.counters;
   A.queue = c.ws.queue;
   A.n_items = n_items;
@@ -1056,6 +1237,7 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items, bool use_add) {
   A.use_add = use_add ? 1 : 0;
   A.qcap = n_items;
   A.spin_limit = 1u << 26;
+  A.spin_ticks = c.spin_ticks;
   return A;
 }
 
@@ -1068,6 +1250,36 @@ static void launch_fill8(hipStream_t s, uint8_t* p, uint8_t v, size_t n) {
   if (!n) return;
   uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(fill_u8, dim3(blocks), dim3(256), 0, s, p, v, n);
+}
+
+// Capture (or reuse) `graph_rounds` rounds of relax_list + advance_round.
+static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_t lds, const RelaxArgs& A) {
+  GraphCache& g = c.graph;
+  std::vector<uint8_t> key(sizeof(RelaxArgs) + 32);
+  memcpy(key.data(), &A, sizeof(RelaxArgs));
+  uint64_t extra[4] = {(uint64_t)(uintptr_t)K.list, grid, lds, (uint64_t)c.shape.threads * 1000 + c.graph_rounds};
+  memcpy(key.data() + sizeof(RelaxArgs), extra, sizeof(extra));
+  if (g.exec && g.key == key) return g.exec;
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t s = c.stream;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  RelaxArgs e = A, o = A;
+  o.act_cur = A.act_next;
+  o.act_next = A.act_cur;
+  o.list_cur = A.list_next;
+  o.list_next = const_cast<uint32_t*>(A.list_cur);
+  void* args[] = {&e, &o};
+  for (uint32_t r = 0; r < c.graph_rounds; ++r) {
+    HIP_OK(hipLaunchKernel(K.list, dim3(grid), dim3(c.shape.threads), args, lds, s));
+    hipLaunchKernelGGL(advance_round, dim3(1), dim3(64), 0, s, c.ws.counters);
+  }
+  HIP_OK(hipStreamEndCapture(s, &graph));
+  HIP_OK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(graph));
+  g.key = key;
+  return g.exec;
 }
 
 void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std::vector<FieldReq>& reqs,
@@ -1146,6 +1358,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   grow(W.state, W.state_cap, std::max<size_t>(item, 1));
   grow(W.queue, W.queue_cap, std::max<size_t>(item, 1));
   grow(W.act, W.act_cap, std::max<size_t>((size_t)item * 2, 2));
+  grow(W.list, W.list_cap, std::max<size_t>((size_t)item * 2, 2));
   grow(W.fields, W.fields_cap, std::max<size_t>(reqs.size(), 1));
   grow(W.item_field, W.item_field_cap, std::max<size_t>(item, 1));
   grow(W.pool, W.pool_cap, std::max<size_t>(pool.size() + seeds.size(), 1));
@@ -1163,60 +1376,82 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     HIP_OK(hipMemcpyAsync(W.pool, pool_all.data(), pool_all.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   const uint32_t* d_seeds = W.pool + pool.size();
   launch_fill32(s, W.state, 0u, item);
-  launch_fill32(s, W.counters, 0u, C_NCOUNTERS);
+  launch_fill32(s, W.counters, 0u, kCtrWords);
 
   RelaxArgs A = make_args(c, item, use_add);
   SolveStats local;
   local.field_cells = field_cells;
   bool timed = false;
   if (item && !seeds.empty()) {
-    if (c.mode == MODE_PLAIN) {
-      launch_fill8(s, W.act, 0, (size_t)item * 2);
-      hipLaunchKernelGGL(set_u8, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, d_seeds,
-                         (uint32_t)seeds.size());
-      uint32_t rounds = 0;
-      uint32_t* pinned = W.pinned;
-      uint32_t grid = ((item + 7) / 8) * 8;  // xcd_item needs a multiple of 8 workgroups
-      HIP_OK(hipEventRecord(W.ev0, s));
-      timed = true;
-      for (;;) {
-        launch_fill32(s, W.counters + C_CHANGED, 0u, 1);
-        hipLaunchKernelGGL(relax_round, dim3(grid), dim3(S.threads), lds, s, A);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(pinned, W.counters, C_NCOUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    const KernelSet& K = pick_kernels(S.tw, S.th, Lmax);
+    uint32_t* pinned = W.pinned;
+    void* args[] = {&A};
+    // Rounds of relax_round over the active-item list until no item is
+    // activated.  Every round starts at a kernel boundary, so it reads
+    // coherent labels; an empty next list certifies the fixpoint.
+    auto plain_rounds = [&](uint32_t& rounds, uint32_t n_active) {
+      while (n_active) {
+        launch_fill32(s, W.counters + C_NEXTN * kCtrStride, 0u, 1);
+        HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
+        HIP_OK(hipMemcpyAsync(pinned, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         ++rounds;
-        if (pinned[C_CHANGED] == 0) break;
+        if (pinned[C_ERROR * kCtrStride]) break;
+        n_active = pinned[C_NEXTN * kCtrStride];
         std::swap(A.act_cur, A.act_next);
+        uint32_t* lc = const_cast<uint32_t*>(A.list_cur);
+        A.list_cur = A.list_next;
+        A.list_next = lc;
         if (rounds > 1000000) throw Error(1, "relaxation did not converge");
       }
-      local.rounds = rounds;
-    } else {
-      launch_fill32(s, W.queue, kMax, item);
-      hipLaunchKernelGGL(seed_queue, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, A, d_seeds,
-                         (uint32_t)seeds.size());
-      int occ = 0;
-      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, relax_persistent, S.threads, lds));
-      if (occ < 1) throw Error(1, "persistent kernel does not fit on a CU");
-      uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)occ * c.cus, item);
+    };
+    uint32_t rounds = 0;
+    launch_fill32(s, W.act, 0u, (size_t)item * 2);
+    if (c.mode == MODE_PLAIN) {
+      hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
+                         d_seeds, (uint32_t)seeds.size());
       HIP_OK(hipEventRecord(W.ev0, s));
       timed = true;
-      hipLaunchKernelGGL(relax_persistent, dim3(grid), dim3(S.threads), lds, s, A);
-      HIP_OK(hipGetLastError());
-      local.rounds = 1;
+      plain_rounds(rounds, (uint32_t)seeds.size());
+    } else if (c.mode == MODE_GRAPH) {
+      hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
+                         d_seeds, (uint32_t)seeds.size());
+      launch_fill32(s, W.counters + C_CURN * kCtrStride, (uint32_t)seeds.size(), 1);
+      int occ = 0;
+      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.list, S.threads, lds));
+      if (occ < 1) throw Error(1, "relaxation kernel does not fit on a CU");
+      uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)occ * c.cus, item));
+      hipGraphExec_t exec = graph_for(c, K, grid, lds, A);
+      HIP_OK(hipEventRecord(W.ev0, s));
+      timed = true;
+      uint32_t replays = 0;
+      for (;;) {
+        HIP_OK(hipGraphLaunch(exec, s));
+        HIP_OK(hipMemcpyAsync(pinned, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        ++replays;
+        if (pinned[C_ERROR * kCtrStride] || pinned[C_CURN * kCtrStride] == 0) break;
+        if (replays > 1000000) throw Error(1, "relaxation did not converge");
+      }
+      rounds = pinned[C_PROUNDS * kCtrStride];
+      local.replays = replays;
+    } else {
+      throw Error(1, "the persistent grid-barrier form is disabled on this build (it did not return on MI355X); "
+                     "use the plain or graph form");
     }
+    local.rounds = rounds;
   }
   if (timed) HIP_OK(hipEventRecord(W.ev1, s));
   uint32_t* cnt = W.pinned;
-  HIP_OK(hipMemcpyAsync(cnt, W.counters, C_NCOUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(cnt, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
-  if (cnt[C_ERROR]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR]));
+  if (cnt[C_ERROR * kCtrStride]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR * kCtrStride]));
   float ms = 0;
   if (timed) HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   local.relax_ms = ms;
-  local.tile_visits = cnt[C_VISITS];
-  local.local_rounds = cnt[C_ROUNDS];
-  local.cells_visited = (uint64_t)cnt[C_VISITS] * Lmax * S.th * S.tw;
+  local.tile_visits = cnt[C_VISITS * kCtrStride];
+  local.local_rounds = cnt[C_ROUNDS * kCtrStride];
+  local.cells_visited = (uint64_t)cnt[C_VISITS * kCtrStride] * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (st) *st = local;
   c.last = local;
@@ -1273,12 +1508,12 @@ void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
     WA.cells = W.pcells;
     WA.counters = W.counters;
     WA.n = (uint32_t)n;
-    launch_fill32(s, W.counters + C_ERROR, 0u, 1);
+    launch_fill32(s, W.counters + C_ERROR * kCtrStride, 0u, 1);
     hipLaunchKernelGGL(walk_paths, dim3((uint32_t)n), dim3(64), 0, s, WA);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(cells.data(), W.pcells, total * 4, hipMemcpyDeviceToHost, s));
     uint32_t err = 0;
-    HIP_OK(hipMemcpyAsync(&err, W.counters + C_ERROR, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&err, W.counters + C_ERROR * kCtrStride, 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipEventRecord(W.ev2, s));
     HIP_OK(hipStreamSynchronize(s));
     if (err) throw Error(1, "device path reconstruction failed (flags " + std::to_string(err) + ")");

@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "libmrfield.so")
 
 MAX = 0xFFFFFFFF
 OK, BACKEND_UNAVAILABLE, INVALID_ENDPOINT, MALFORMED_GRID = 0, 1, 2, 3
-MODE_PLAIN, MODE_PERSISTENT = 0, 1
+MODE_PLAIN, MODE_PERSISTENT, MODE_GRAPH = 0, 1, 2
 
 U32P = C.POINTER(C.c_uint32)
 U8P = C.POINTER(C.c_uint8)
@@ -73,7 +73,7 @@ class PathsS(C.Structure):
 class StatsS(C.Structure):
     _fields_ = [("relax_ms", C.c_double), ("path_ms", C.c_double), ("total_ms", C.c_double),
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
-                ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32)]
+                ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double)]
 
 
 class SynthS(C.Structure):

@@ -17,7 +17,12 @@ from cases import (MAX, grid_from_case, isolated_stress_case, load_vectors, weig
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(32, 32, 256, 32), (8, 4, 64, 8), (16, 8, 128, 3), (4, 1, 64, 2)]
-MODES = [mrfield.MODE_PLAIN, mrfield.MODE_PERSISTENT]
+# The persistent grid-barrier form did not return on MI355X (twice, no fault);
+# it is switched off in the library and its cases stay skipped until a cause
+# is shown.  The graph form replaces it.
+PERSISTENT_SKIP = pytest.param(mrfield.MODE_PERSISTENT, marks=pytest.mark.skip(
+    reason="persistent grid-barrier form disabled: did not return on MI355X; cause not yet shown"))
+MODES = [mrfield.MODE_PLAIN, PERSISTENT_SKIP, mrfield.MODE_GRAPH]
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +50,7 @@ VEC = load_vectors()
 
 @pytest.mark.parametrize("mode", MODES)
 def test_reference_vectors(eng, mode):
-    configure(eng, (4, 2, 64, 4), mode)
+    configure(eng, (4, 1, 64, 4), mode)
     for case in VEC["cases"]:
         g, src, dst = grid_from_case(case)
         pads = case.get("pads", [])
@@ -249,11 +254,31 @@ def test_config1_complete_fields_and_paths(eng, mode):
     assert n_routed > 32
 
 
-def test_plain_and_persistent_agree_and_are_deterministic(eng):
+@pytest.mark.skip(reason="persistent grid-barrier form disabled: did not return on MI355X; cause not yet shown")
+def test_persistent_form_agrees_with_plain(eng):
+    s = mrfield.synth_config(1, seed=3, scale_nets=24)
+    g = s.grid
+    configure(eng, (16, 16, 256, 8), mrfield.MODE_PLAIN)
+    ref = eng.fields(g, s.edges, [[int(a)] for a in s.src])
+    configure(eng, (16, 16, 256, 8), mrfield.MODE_PERSISTENT)
+    got = eng.fields(g, s.edges, [[int(a)] for a in s.src])
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+
+
+def test_persistent_form_is_refused_cleanly(eng):
+    configure(eng, (16, 16, 128, 8), mrfield.MODE_PERSISTENT)
+    s = mrfield.synth_config(1, seed=3, scale_nets=2)
+    with pytest.raises(mrfield.MrfError) as e:
+        eng.fields(s.grid, s.edges, [[int(a)] for a in s.src])
+    assert e.value.code == mrfield.BACKEND_UNAVAILABLE
+    configure(eng, (16, 16, 128, 8), mrfield.MODE_GRAPH)
+
+
+def test_plain_and_graph_agree_and_are_deterministic(eng):
     s = mrfield.synth_config(1, seed=3, scale_nets=24)
     g = s.grid
     out = []
-    for mode in MODES:
+    for mode in (mrfield.MODE_PLAIN, mrfield.MODE_GRAPH):
         for _ in range(2):
             configure(eng, (16, 16, 256, 8), mode)
             out.append(eng.fields(g, s.edges, [[int(a)] for a in s.src]))
@@ -263,7 +288,7 @@ def test_plain_and_persistent_agree_and_are_deterministic(eng):
 
 def test_alone_equals_mixed(eng):
     """A net's result is the same solved alone and inside a larger batch."""
-    configure(eng, (16, 8, 128, 8), mrfield.MODE_PERSISTENT)
+    configure(eng, (16, 8, 128, 8), mrfield.MODE_GRAPH)
     s = mrfield.synth_config(1, seed=5, scale_nets=40)
     g = s.grid
     full = [(0, 0, g.w - 1, g.h - 1)]
@@ -274,7 +299,7 @@ def test_alone_equals_mixed(eng):
 
 
 def test_fail_closed_contracts(eng):
-    configure(eng, (32, 32, 256, 32), mrfield.MODE_PERSISTENT)
+    configure(eng, (32, 32, 256, 32), mrfield.MODE_GRAPH)
     g = mrfield.Grid(3, 2, 1, np.ones(6, dtype=np.uint32))
     e = mrfield.Edges([16, 16], [16], [])
     bad = mrfield.Grid(3, 2, 1, np.ones(5, dtype=np.uint32))
