@@ -6,12 +6,17 @@ u64 (dist, hops) label fields over the full board for every net, canonical
 path reconstruction, and the host re-validation of every path.  Inputs (board
 planes) are uploaded once before the timed region.
 
-Output: one JSON line (see the keys below).  ``roofline`` uses the
-algorithmic bytes of the relaxation kernel: 19 B per cell visit (8 B label
-read + 8 B label write + 2 B history + 1 B mask) times the device counter of
-cell visits, over the kernel's average duration from HIP events on the
-engine's stream.  ``cpu_baseline`` times the CPU oracle (a C restatement of
-mr-cpu, not the Rust reference) on a bounded sample of the same nets.
+Output: one JSON line (see the keys below).  ``roofline`` is per launch of
+the relaxation kernel (relax_round in the plain form): algorithmic bytes =
+19 B per cell visit (8 B label read + 8 B label write + 2 B history + 1 B
+mask) times the cell visits of one launch (device counter of tile loads x
+tile cells, averaged over the step's launches), divided by the kernel's
+average launch duration from HIP events recorded around every launch on the
+engine's stream.  ``traffic`` is the measured HBM bytes per launch from a
+committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated
+against a copy of known size).  ``cpu_baseline`` times the CPU oracle (a C
+restatement of mr-cpu, not the Rust reference) on a bounded sample of the
+same nets.
 """
 from __future__ import annotations
 
@@ -37,7 +42,7 @@ def parse():
     ap.add_argument("--config", type=int, default=2, help="synthetic config (2 = 512x512x4, 256 nets)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
-    ap.add_argument("--tiles", default="16,16,128,32", help="tw,th,threads,max_local")
+    ap.add_argument("--tiles", default="16,16,256,32", help="tw,th,threads,max_local")
     ap.add_argument("--cpu-sample", type=int, default=16, help="nets in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -85,7 +90,10 @@ def cpu_baseline(s, sample, got_paths):
     dist, hops = mrcpu.fields_batch(spec, nets, [[n.src] for n in nets], threads=threads)
     paths = [mrcpu.reconstruct(spec, dist[i], hops[i], n.dst) for i, n in enumerate(nets)]
     dt = time.perf_counter() - t0
-    exact = all((p is None and gp is None) or (gp is not None and p == gp[0]) for p, gp in zip(paths, got_paths))
+    exact = True
+    for i, p in enumerate(paths):
+        gp = got_paths.path(i)
+        exact = exact and ((p is None and gp is None) or (gp is not None and p == gp[0]))
     return {
         "value": round(sample / dt, 3),
         "unit": "nets/s",
@@ -121,12 +129,13 @@ def main():
     wins = s.windows
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
+    kern_ms, launches = [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
             t_start = time.perf_counter()
         t0 = time.perf_counter()
-        got = board.solve(nets, windows=wins, retry_full=wins is not None)
+        got = board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
         t1 = time.perf_counter()
         if i >= a.warmup:
             st = eng.stats()
@@ -137,14 +146,18 @@ def main():
             cells_visited.append(st["cells_visited"])
             rounds_l.append(st["local_rounds"])
             host_rounds.append(st["host_rounds"])
+            kern_ms.append(st["relax_kernel_ms"])
+            launches.append(st["relax_launches"])
     wall = time.perf_counter() - t_start
     n_nets = len(nets)
     value = n_nets * a.steps / wall
-    routed = sum(1 for p in got if p is not None)
-    kern_ms = float(np.mean(relax_ms))
-    cells = float(np.mean(cells_visited))
-    achieved = cells * BYTES_PER_CELL_VISIT / (kern_ms * 1e-3) / 1e9
-    workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets"
+    routed = int(np.count_nonzero(got.routed))
+    # per launch of the relaxation kernel: algorithmic bytes / average duration
+    n_launch = float(np.mean(launches))
+    cells_per_launch = float(np.mean(cells_visited)) / max(n_launch, 1.0)
+    avg_launch_ms = float(np.mean(kern_ms)) / max(n_launch, 1.0)
+    achieved = cells_per_launch * BYTES_PER_CELL_VISIT / (avg_launch_ms * 1e-3) / 1e9
+    workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -168,7 +181,10 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "fields_per_s": round(value, 3),
         "routed": routed,
-        "relax_kernel_ms": round(kern_ms, 4),
+        "relax_phase_ms": round(float(np.mean(relax_ms)), 4),
+        "relax_kernel_ms": round(float(np.mean(kern_ms)), 4),
+        "relax_launches": int(n_launch),
+        "relax_avg_launch_ms": round(avg_launch_ms, 5),
         "path_ms": round(float(np.mean(path_ms)), 4),
         "tile_visits": int(np.mean(visits)),
         "local_rounds_per_visit": round(float(np.mean(rounds_l)) / max(float(np.mean(visits)), 1), 3),
@@ -182,7 +198,7 @@ def main():
             "traffic": load_traffic("relax_list" if a.mode == "graph" else "relax_round", workload),
             "kernel": "relax_list" if a.mode == "graph" else "relax_round",
             "bytes_per_cell_visit": BYTES_PER_CELL_VISIT,
-            "cell_visits_per_launch": int(cells),
+            "cell_visits_per_launch": int(cells_per_launch),
         },
         "cpu_baseline": None,
     }

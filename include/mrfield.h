@@ -152,6 +152,9 @@ typedef struct {
     uint64_t field_cells;
     uint32_t host_rounds;  /* launches of the round kernel (+1 for the persistent kernel) */
     double persistent_ms;  /* persistent form: the work-queue kernel alone */
+  double relax_kernel_ms;  /* relaxation kernels alone: sum of per-launch HIP event times (plain
+                              form; the graph form reports its replay time) */
+  uint32_t relax_launches; /* relaxation kernel launches */
 } mrf_stats;
 
 /* Solve form of a context.

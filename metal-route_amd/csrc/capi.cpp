@@ -6,7 +6,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
-#include <unordered_set>
+#include <thread>
+#include <vector>
 
 #include "../../include/mrfield.h"
 #include "engine.h"
@@ -151,12 +152,15 @@ bool path_ok(const HostGrid& g, const mrf_net& n, const NetPlan& p, const mrf_wi
   size_t total = dims_len(g.d);
   if (len == 0 || len > total || path[0] != n.src || path[len - 1] != n.dst) return false;
   uint32_t plane = g.d.w * g.d.h;
-  std::unordered_set<uint32_t> seen;
-  seen.reserve(len * 2);
+  // simple path: no cell twice
+  thread_local std::vector<uint32_t> sorted;
+  sorted.assign(path, path + len);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return false;
   uint64_t cost = 0;
   for (size_t i = 0; i < len; ++i) {
     uint32_t c = path[i];
-    if (c >= total || g.trace(c) || (g.obst(c) && !sorted_has(p.pads, c)) || !seen.insert(c).second) return false;
+    if (c >= total || g.trace(c) || (g.obst(c) && !sorted_has(p.pads, c))) return false;
     uint32_t r = c % plane, x = r % g.d.w, y = r / g.d.w;
     if (x < w.x0 || x > w.x1 || y < w.y0 || y > w.y1) return false;
     if (i == 0) continue;
@@ -241,19 +245,38 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
     mrf::reconstruct_paths(c, b, reqs.size(), routed, cost, off, cells, &st);
     if (c.inject == 1) throw Error(MRF_BACKEND_UNAVAILABLE, "injected backend failure");
     if (c.inject == 2 && !cells.empty()) cells[cells.size() / 2] ^= 1u;
-    for (size_t k = k0; k < k1; ++k) {
-      size_t j = k - k0;
-      out.routed[k] = routed[j];
-      if (!routed[j]) continue;
-      const uint32_t* p = cells.data() + off[j];
-      size_t len = off[j + 1] - off[j];
-      int32_t own = net_ids ? net_ids[idx[k]] : -1;
-      if (!path_ok(g, nets[idx[k]], plans[k], wins[k], own, p, len, cost[j], lee))
-        throw Error(MRF_BACKEND_UNAVAILABLE, "device path failed host re-validation");
-      out.cost[k] = cost[j];
-      out.paths[k].assign(p, p + len);
+    // Re-validate on up to 8 host threads; any failure fails the whole call.
+    std::vector<uint8_t> bad(k1 - k0, 0);
+    auto check = [&](size_t a, size_t b) {
+      for (size_t k = a; k < b; ++k) {
+        size_t j = k - k0;
+        out.routed[k] = routed[j];
+        if (!routed[j]) continue;
+        const uint32_t* p = cells.data() + off[j];
+        size_t len = off[j + 1] - off[j];
+        int32_t own = net_ids ? net_ids[idx[k]] : -1;
+        if (!path_ok(g, nets[idx[k]], plans[k], wins[k], own, p, len, cost[j], lee)) {
+          bad[j] = 1;
+          continue;
+        }
+        out.cost[k] = cost[j];
+        out.paths[k].assign(p, p + len);
+      }
+    };
+    size_t nthreads = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), (k1 - k0 + 31) / 32});
+    if (nthreads <= 1) {
+      check(k0, k1);
+    } else {
+      std::vector<std::thread> pool;
+      size_t per = (k1 - k0 + nthreads - 1) / nthreads;
+      for (size_t t = 0; t < nthreads; ++t) {
+        size_t a = k0 + t * per, b = std::min(k1, a + per);
+        if (a < b) pool.emplace_back(check, a, b);
+      }
+      for (auto& th : pool) th.join();
     }
-    c.last.rounds += 0;
+    for (uint8_t v : bad)
+      if (v) throw Error(MRF_BACKEND_UNAVAILABLE, "device path failed host re-validation");
     k0 = k1;
   }
 }
@@ -294,9 +317,11 @@ HostGrid host_grid(mrf_dims d, const mrf_grid* g, const mrf_edges* e) {
 
 struct Lane {
   std::unique_lock<std::mutex> lk;
-  explicit Lane(mrf::Ctx& c) : lk(c.lane, std::try_to_lock) {
+  // `solve`: the call runs relaxations; its statistics replace the last call's.
+  explicit Lane(mrf::Ctx& c, bool solve = true) : lk(c.lane, std::try_to_lock) {
     if (!lk.owns_lock()) throw Error(MRF_BACKEND_UNAVAILABLE, "field engine lane is busy");
     if (hipSetDevice(c.device) != hipSuccess) throw Error(MRF_BACKEND_UNAVAILABLE, "hipSetDevice failed");
+    if (solve) c.last = mrf::SolveStats{};
   }
 };
 
@@ -474,6 +499,8 @@ int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   out->field_cells = s.field_cells;
   out->host_rounds = s.rounds;
   out->persistent_ms = s.persistent_ms;
+  out->relax_kernel_ms = s.kernel_ms;
+  out->relax_launches = s.launches;
   return MRF_OK;
 }
 
@@ -799,7 +826,7 @@ int mrf_read_field(mrf_ctx* ctx, mrf_board* bd, size_t field, uint32_t* dist_out
                    mrf_error* err) {
   if (!ctx || !bd || !dist_out) return MRF_BACKEND_UNAVAILABLE;
   try {
-    Lane lane(ctx->c);
+    Lane lane(ctx->c, false);
     mrf::read_field(ctx->c, bd->b, field, dist_out, hops_out);
     return MRF_OK;
   } catch (const Error& e) {

@@ -43,7 +43,8 @@ struct BoardView {
   const uint32_t* xe;       // planar x gap prices [w-1] (+1 pad)
   const uint32_t* ye;       // planar y gap prices [h-1] (+1 pad)
   const uint32_t* ve;       // via prices [layers-1] (+1 pad)
-  uint8_t va[16];           // via_allowed per layer pair
+  uint32_t va;              // bit k: vias allowed between layers k and k+1
+  uint32_t pad_;
 };
 
 // One complete field: a window (all layers), its sources, overrides.
@@ -107,6 +108,22 @@ struct SolveStats {
   uint64_t field_cells = 0;    // cells in all fields (window x layers)
   double persistent_ms = 0;    // persistent kernel alone (persistent form)
   uint32_t replays = 0;        // hipGraph replays (graph form)
+  double kernel_ms = 0;        // relaxation kernels alone: sum of per-launch event times (plain form)
+  uint32_t launches = 0;       // relaxation kernel launches
+  void add(const SolveStats& o) {
+    relax_ms += o.relax_ms;
+    path_ms += o.path_ms;
+    total_ms += o.total_ms;
+    tile_visits += o.tile_visits;
+    local_rounds += o.local_rounds;
+    cells_visited += o.cells_visited;
+    rounds += o.rounds;
+    field_cells += o.field_cells;
+    persistent_ms += o.persistent_ms;
+    replays += o.replays;
+    kernel_ms += o.kernel_ms;
+    launches += o.launches;
+  }
 };
 
 // MODE_PLAIN: one launch per round, host reads the activation count each round.
@@ -163,7 +180,7 @@ struct Workspace {
   uint32_t* pinned = nullptr;    // host-pinned counter readback
   BoardView* boards = nullptr;   // device board table of the current batch
   size_t boards_cap = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ek0 = nullptr, ek1 = nullptr;
 };
 
 struct GraphCache {

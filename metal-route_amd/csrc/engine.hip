@@ -14,20 +14,31 @@
 //     bmask u8 (trace/obstacle + 6 directed-edge bits), raw u8 (via bits).
 //   * labels: one u64 per cell, (dist << 32) | hops, so the lexicographic
 //     minimum is a plain unsigned minimum.  Stored tile-major per field
-//     ([tile][layer][y][x], 256-byte aligned blocks) so no cache line is shared
-//     by two tiles.
+//     ([tile][layer][y][x]), each tile block followed by copies of the tile's
+//     first and last columns so the side neighbours read their halo from two
+//     contiguous runs.
 //   * per-tile state word (touched / pending / active bits).
 //
+// Tile visit (process_tile): one workgroup loads a TW x TH x layers tile of one
+// field into LDS with a one-cell halo ring from the neighbour tiles, computes
+// the six step prices of entering every cell once (kMax = closed), then
+// sweeps rows, columns and via columns (one line per lane, the whole line in
+// registers, forward then backward) until a local fixpoint, writes back the
+// improved labels and activates the neighbours whose shared border changed.
+//
 // Kernels
-//   relax_round       one visit of every active (field, tile) item; host loop
-//                     until no item changes (the plain per-batch-launch form).
-//   relax_persistent  device work queue over (field, tile) items with a
-//                     pending/active state machine; no host round trips.
+//   relax_round       one visit of every item on the round's active list;
+//                     host loop until no item is activated (plain form).
+//   relax_list        the same visits as a grid-stride loop over a device
+//                     work list; rounds captured in a hipGraph with
+//                     advance_round between them (graph form).
+//   relax_persistent  resident grid with a device barrier between rounds;
+//                     disabled on the host side.
 //   path_lengths / walk_paths
 //                     canonical reconstruction: from dst, the lowest-CellIdx
 //                     predecessor p with label(p) + (step(p, c), 1) == label(c)
 //                     (mr-metal/src/lib.rs:304-396 order; masks re-checked).
-//   extract_dist      tile-major labels -> window-major u32 dist / hops.
+//   extract_field     tile-major labels -> window-major u32 dist / hops.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -107,7 +118,10 @@ BoardView DevBoard::view() const {
   v.xe = xe;
   v.ye = ye;
   v.ve = ve;
-  for (int i = 0; i < 16; ++i) v.va[i] = via_allowed.size() > (size_t)i ? via_allowed[i] : 0;
+  v.va = 0;
+  v.pad_ = 0;
+  for (int i = 0; i < 16; ++i)
+    if (via_allowed.size() > (size_t)i && via_allowed[i]) v.va |= 1u << i;
   return v;
 }
 
@@ -138,6 +152,28 @@ __device__ __forceinline__ uint64_t extend(uint64_t prev, uint32_t s) {
 // readfirstlane turns the value into a scalar so every such branch stays a
 // scalar branch.
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Workgroup-uniform copy of a descriptor read from memory: every dword goes
+// through readfirstlane, so the fields (and the plane pointers in them) live in
+// scalar registers for the whole tile visit instead of being reloaded.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor size must be a whole number of dwords");
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
+// The same pointer in the global address space: loads through it are
+// global_load (no flat address check, no LDS counter in the wait).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
 
 __device__ __forceinline__ bool sorted_has(const uint32_t* a, uint32_t n, uint32_t v) {
   uint32_t lo = 0, hi = n;
@@ -175,19 +211,18 @@ __device__ __forceinline__ uint32_t add_price(const BoardView& b, const FieldDes
 // Via bits of global cell g for field F, including per-net via exemptions
 // (negotiated.rs:3010-3043).
 __device__ uint8_t via_bits(const BoardView& b, const FieldDesc& F, const uint32_t* pool, uint32_t g) {
-  const uint8_t* va = b.va;
   uint32_t l = g / b.plane;
   const uint32_t* vp = pool + F.vpad_off;
   uint8_t r = b.raw[g];
   bool self_ok = !(r & R_VIANODE) && (!(r & R_VF) || sorted_has(vp, F.n_vpad, g));
   uint8_t bits = 0;
   if (!self_ok) return 0;
-  if (l > 0 && va[l - 1]) {
+  if (l > 0 && ((b.va >> (l - 1)) & 1u)) {
     uint32_t q = g - b.plane;
     uint8_t rq = b.raw[q];
     if (!(rq & R_VIANODE) && (!(rq & R_VF) || sorted_has(vp, F.n_vpad, q))) bits |= K_DN;
   }
-  if (l + 1 < b.layers && va[l]) {
+  if (l + 1 < b.layers && ((b.va >> l) & 1u)) {
     uint32_t q = g + b.plane;
     uint8_t rq = b.raw[q];
     if (!(rq & R_VIANODE) && (!(rq & R_VF) || sorted_has(vp, F.n_vpad, q))) bits |= K_UP;
@@ -235,240 +270,231 @@ struct RelaxArgs {
   uint32_t* queue;
   uint32_t n_items;
   uint32_t tw, th, max_local;
-  uint32_t use_add;
   uint32_t qcap;
   uint32_t spin_limit;
   uint64_t spin_ticks;  // real-time bound of any spin wait (wall_clock64 ticks)
 };
 
-// LDS carve-up for one tile (all sizes in bytes, 16-byte aligned sections).
+// Words of one tile block in the label buffer: the L x TH x TW labels
+// ([layer][y][x]) followed by copies of the tile's first and last columns
+// ([layer][y] each), which the left and right neighbours read as their halo
+// without touching one cache line per cell.
+__host__ __device__ inline uint64_t tile_block_words(uint32_t L, uint32_t tw, uint32_t th) {
+  return (uint64_t)L * th * tw + 2ull * L * th;
+}
+
+// LDS carve-up for one tile of TW x TH cells on L layers.
+//   lab   u64 labels [L][TH+2][RS] with a one-cell halo ring; RS = TW+3 is odd
+//         so lanes that each own a row hit distinct banks.
+//   sxl / sxr / syt / syb / svd / svu
+//         u32 step prices [L][TH][PS] for entering each cell from x-1, x+1,
+//         y-1, y+1, layer-1, layer+1 (kMax = closed).  Computed once per
+//         visit while the tile loads; the relaxation rounds only read them.
+//         PS = TW+1 is odd.
+//   rchg / cchg / vchg
+//         per-line change masks (row lines, column lines, via columns): bit i
+//         set when cell i of the line improved during this visit.
 struct Lds {
-  uint32_t* d;   // [L][th+2][rs] dist incl. halo ring
-  uint32_t* h;   // [L][th+2][rs] hops
-  uint32_t* w;   // [L][th][tw+1] enter weight (odd row stride: no bank conflicts)
-  uint32_t* a;   // [L][th][tw+1] additive price
-  uint8_t* m;    // [L][th][tw+1] mask
-  uint32_t* xg;  // [tw+1] x gap prices entering column i from the left
-  uint32_t* yg;  // [th+1]
+  uint64_t* lab;
+  uint32_t* sxl;
+  uint32_t* sxr;
+  uint32_t* syt;
+  uint32_t* syb;
+  uint32_t* svd;
+  uint32_t* svu;
+  uint32_t* rchg;
+  uint32_t* cchg;
+  uint32_t* vchg;
+  uint32_t* xg;  // [TW+1] x gap prices entering column i from the left
+  uint32_t* yg;  // [TH+1]
   uint32_t* vg;  // [16]
   uint32_t* misc;
-  uint32_t rs, rh, ps;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L, bool use_add) {
-  size_t rs = tw + 3, rh = th + 2, ncell = (size_t)L * th * (tw + 1), nlab = (size_t)L * rh * rs;
+__host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L) {
+  const size_t rs = tw + 3, rh = th + 2, ps = tw + 1;
+  const size_t ncell = (size_t)L * th * ps, nlab = (size_t)L * rh * rs;
+  const size_t nstep = L > 1 ? 6 : 4;
   size_t s = 0;
-  s += align16(nlab * 4) * 2;
-  s += align16(ncell * 4);
-  if (use_add) s += align16(ncell * 4);
-  s += align16(ncell);
+  s += align16(nlab * 8);
+  s += nstep * align16(ncell * 4);
+  s += align16((size_t)L * th * 4) + align16((size_t)L * tw * 4) + align16((size_t)th * tw * 4);
   s += align16((tw + 1) * 4) + align16((th + 1) * 4) + align16(16 * 4) + 64;
   return s;
 }
 
-__device__ inline Lds carve(char* base, uint32_t tw, uint32_t th, uint32_t L, bool use_add) {
+template <int TW, int TH>
+__device__ inline Lds carve(char* base, uint32_t L) {
+  constexpr size_t RS = TW + 3, RH = TH + 2, PS = TW + 1;
+  const size_t ncell = (size_t)L * TH * PS, nlab = (size_t)L * RH * RS;
   Lds s;
-  s.rs = tw + 3;
-  s.rh = th + 2;
-  s.ps = tw + 1;
-  size_t ncell = (size_t)L * th * s.ps, nlab = (size_t)L * s.rh * s.rs;
   size_t o = 0;
-  s.d = (uint32_t*)(base + o);
-  o += align16(nlab * 4);
-  s.h = (uint32_t*)(base + o);
-  o += align16(nlab * 4);
-  s.w = (uint32_t*)(base + o);
-  o += align16(ncell * 4);
-  if (use_add) {
-    s.a = (uint32_t*)(base + o);
-    o += align16(ncell * 4);
+  auto take = [&](size_t bytes) {
+    char* p = base + o;
+    o += align16(bytes);
+    return p;
+  };
+  s.lab = (uint64_t*)take(nlab * 8);
+  s.sxl = (uint32_t*)take(ncell * 4);
+  s.sxr = (uint32_t*)take(ncell * 4);
+  s.syt = (uint32_t*)take(ncell * 4);
+  s.syb = (uint32_t*)take(ncell * 4);
+  if (L > 1) {
+    s.svd = (uint32_t*)take(ncell * 4);
+    s.svu = (uint32_t*)take(ncell * 4);
   } else {
-    s.a = nullptr;
+    s.svd = s.svu = nullptr;
   }
-  s.m = (uint8_t*)(base + o);
-  o += align16(ncell);
-  s.xg = (uint32_t*)(base + o);
-  o += align16((tw + 1) * 4);
-  s.yg = (uint32_t*)(base + o);
-  o += align16((th + 1) * 4);
-  s.vg = (uint32_t*)(base + o);
-  o += align16(16 * 4);
-  s.misc = (uint32_t*)(base + o);
+  s.rchg = (uint32_t*)take((size_t)L * TH * 4);
+  s.cchg = (uint32_t*)take((size_t)L * TW * 4);
+  s.vchg = (uint32_t*)take((size_t)TH * TW * 4);
+  s.xg = (uint32_t*)take((TW + 1) * 4);
+  s.yg = (uint32_t*)take((TH + 1) * 4);
+  s.vg = (uint32_t*)take(16 * 4);
+  s.misc = (uint32_t*)take(64);
   return s;
 }
 
-__device__ __forceinline__ uint64_t ld_lab(const Lds& s, uint32_t i) {
-  return ((uint64_t)s.d[i] << 32) | s.h[i];
-}
-__device__ __forceinline__ void st_lab(const Lds& s, uint32_t i, uint64_t v) {
-  s.d[i] = (uint32_t)(v >> 32);
-  s.h[i] = (uint32_t)v;
+// In-LDS label of a cell no path reaches: (MAX, 0).  Every label held in LDS
+// is at most this value, and a candidate whose search cost saturates to MAX
+// is (MAX, hops >= 1), never below it, so the saturating relaxation step
+// below needs no separate reject test.
+constexpr uint64_t kUnreached = kLabLimit;
+
+// Relax `cur` through an edge of price s from label `from`: the candidate is
+// (min(dist + s, MAX), hops + 1).  A sum that reaches MAX is the candidate
+// astar_buf rejects (dijkstra.rs:303-313); saturated, it cannot win.
+__device__ __forceinline__ void relax_to(uint64_t from, uint32_t s, uint64_t& cur, uint32_t& chg, uint32_t bit) {
+  const uint32_t hi = __builtin_elementwise_add_sat((uint32_t)(from >> 32), s);
+  const uint32_t lo = (uint32_t)from + 1u;
+  const uint64_t cand = ((uint64_t)hi << 32) | lo;
+  const bool better = cand < cur;
+  cur = better ? cand : cur;
+  chg = better ? (chg | bit) : chg;
 }
 
-// Step price of entering LDS cell c over an edge priced e (kMax when the
-// edge or the cell is closed: `bits` must all be set in the cell mask).
-__device__ __forceinline__ uint32_t gated_step(const Lds& s, uint32_t c, uint8_t bits, uint32_t e, bool use_add) {
-  uint8_t m = s.m[c];
-  return ((m & bits) == bits) ? step_price(e, s.w[c], use_add ? s.a[c] : 0u) : kMax;
+// Gated step price of entering a cell: `open` is the cell's passability and
+// the directed-edge bit of the move.
+__device__ __forceinline__ uint32_t gate(bool open, uint32_t e, uint32_t w, uint32_t a) {
+  return open ? step_price(e, w, a) : kMax;
 }
 
-// Sweep one line held in registers: forward (index increasing) then backward.
-// lab[0] and lab[N+1] are the halo ends; sf[i] / sb[i] are the prices of
-// entering cell i from i-1 / i+1.  Returns a bit mask of improved cells.
-template <int N>
-__device__ __forceinline__ uint64_t sweep_line(uint64_t (&lab)[N + 2], const uint32_t (&sf)[N],
-                                               const uint32_t (&sb)[N]) {
-  uint64_t chg = 0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    uint64_t c = extend(lab[i], sf[i]);
-    if (c < lab[i + 1]) {
-      lab[i + 1] = c;
-      chg |= 1ull << i;
-    }
-  }
-#pragma unroll
-  for (int i = N - 1; i >= 0; --i) {
-    uint64_t c = extend(lab[i + 2], sb[i]);
-    if (c < lab[i + 1]) {
-      lab[i + 1] = c;
-      chg |= 1ull << i;
-    }
-  }
-  return chg;
+// Per-phase change flags.  Phase k writes slot k & 1 with stamp k + 1; the
+// slot is next written in phase k + 2, after the barrier that ends phase
+// k + 1, so every thread reads phase k's flag before anyone overwrites it.
+__device__ __forceinline__ bool phase_changed(const Lds& s, bool ch, uint32_t k) {
+  if (ch) s.misc[8 + (k & 1)] = k + 1;
+  __syncthreads();
+  return uniform_u32(s.misc[8 + (k & 1)]) == k + 1;
 }
 
-// In-LDS relaxation of one tile to a local fixpoint: rows, columns and via
-// columns are each swept forward and backward with the whole line held in
-// registers (one lane per line).  Returns true when the round cap was hit.
+// In-LDS relaxation of one tile to a local fixpoint.  Rows, columns and via
+// columns are swept in turn, forward then backward, one line per lane with
+// the whole line in registers; a forward + backward sweep leaves a line at its
+// own fixpoint, so the tile is at a fixpoint once every other phase after a
+// sweep has changed nothing.  Returns true when the round cap was hit.
 template <int TW, int TH, int LM>
-__device__ bool relax_local(const Lds& s, uint32_t L, bool use_add, uint32_t max_local, uint32_t& rounds) {
+__device__ bool relax_local(const Lds& s, uint32_t L, uint32_t max_local, uint32_t& rounds) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
-  const uint32_t rs = s.rs, rh = s.rh;
-  constexpr uint32_t PS = TW + 1;  // padded interior row stride
-  constexpr uint32_t plane_p = TH * PS;
+  constexpr uint32_t RS = TW + 3, RH = TH + 2, PS = TW + 1;
+  constexpr uint32_t LAB_PLANE = RH * RS, STEP_PLANE = TH * PS;
+  const uint32_t nphase = L > 1 ? 3u : 2u;
+  uint32_t k = 0, quiet = 0;
   for (rounds = 0; rounds < max_local; ++rounds) {
+    // rows: lane per (layer, y)
     bool ch = false;
-    // rows (lane per (layer, y)); forward then backward, each with its own
-    // step array so only one is live at a time
     for (uint32_t line = tid; line < L * TH; line += nt) {
-      uint32_t l = line / TH, y = line - l * TH;
-      uint32_t lb = (l * rh + y + 1) * rs, cb = (l * TH + y) * PS;
-      uint64_t lab[TW + 2];
+      const uint32_t l = line / TH, y = line - l * TH;
+      uint64_t* lp = s.lab + l * LAB_PLANE + (y + 1) * RS;
+      const uint32_t* sp = s.sxl + l * STEP_PLANE + y * PS;
+      uint64_t v[TW + 2];
       uint32_t st[TW];
-      uint64_t chg = 0;
+      uint32_t chg = 0;
 #pragma unroll
-      for (int i = 0; i < TW + 2; ++i) lab[i] = ld_lab(s, lb + i);
+      for (int i = 0; i < TW + 2; ++i) v[i] = lp[i];
 #pragma unroll
-      for (int x = 0; x < TW; ++x) st[x] = gated_step(s, cb + x, K_PASS | K_NX, s.xg[x], use_add);
+      for (int x = 0; x < TW; ++x) st[x] = sp[x];
 #pragma unroll
-      for (int x = 0; x < TW; ++x) {
-        uint64_t c = extend(lab[x], st[x]);
-        if (c < lab[x + 1]) {
-          lab[x + 1] = c;
-          chg |= 1ull << x;
-        }
-      }
+      for (int x = 0; x < TW; ++x) relax_to(v[x], st[x], v[x + 1], chg, 1u << x);
+      sp = s.sxr + l * STEP_PLANE + y * PS;
 #pragma unroll
-      for (int x = 0; x < TW; ++x) st[x] = gated_step(s, cb + x, K_PASS | K_PX, s.xg[x + 1], use_add);
+      for (int x = 0; x < TW; ++x) st[x] = sp[x];
 #pragma unroll
-      for (int x = TW - 1; x >= 0; --x) {
-        uint64_t c = extend(lab[x + 2], st[x]);
-        if (c < lab[x + 1]) {
-          lab[x + 1] = c;
-          chg |= 1ull << x;
-        }
-      }
+      for (int x = TW - 1; x >= 0; --x) relax_to(v[x + 2], st[x], v[x + 1], chg, 1u << x);
       if (chg) {
         ch = true;
 #pragma unroll
-        for (int x = 0; x < TW; ++x)
-          if ((chg >> x) & 1) {
-            st_lab(s, lb + x + 1, lab[x + 1]);
-            s.m[cb + x] |= K_CHG;
-          }
+        for (int x = 0; x < TW; ++x) lp[x + 1] = v[x + 1];
+        s.rchg[line] |= chg;
       }
     }
-    __syncthreads();
-    // columns (lane per (layer, x))
+    quiet = phase_changed(s, ch, k++) ? 0u : quiet + 1;
+    if (k >= nphase && quiet + 1 >= nphase) return false;
+    // columns: lane per (layer, x)
+    ch = false;
     for (uint32_t line = tid; line < L * TW; line += nt) {
-      uint32_t l = line / TW, x = line - l * TW;
-      uint32_t lb = l * rh * rs + x + 1, cb = l * plane_p + x;
-      uint64_t lab[TH + 2];
+      const uint32_t l = line / TW, x = line - l * TW;
+      uint64_t* lp = s.lab + l * LAB_PLANE + x + 1;
+      const uint32_t* sp = s.syt + l * STEP_PLANE + x;
+      uint64_t v[TH + 2];
       uint32_t st[TH];
-      uint64_t chg = 0;
+      uint32_t chg = 0;
 #pragma unroll
-      for (int i = 0; i < TH + 2; ++i) lab[i] = ld_lab(s, lb + i * rs);
+      for (int i = 0; i < TH + 2; ++i) v[i] = lp[i * RS];
 #pragma unroll
-      for (int y = 0; y < TH; ++y) st[y] = gated_step(s, cb + y * PS, K_PASS | K_NY, s.yg[y], use_add);
+      for (int y = 0; y < TH; ++y) st[y] = sp[y * PS];
 #pragma unroll
-      for (int y = 0; y < TH; ++y) {
-        uint64_t c = extend(lab[y], st[y]);
-        if (c < lab[y + 1]) {
-          lab[y + 1] = c;
-          chg |= 1ull << y;
-        }
-      }
+      for (int y = 0; y < TH; ++y) relax_to(v[y], st[y], v[y + 1], chg, 1u << y);
+      sp = s.syb + l * STEP_PLANE + x;
 #pragma unroll
-      for (int y = 0; y < TH; ++y) st[y] = gated_step(s, cb + y * PS, K_PASS | K_PY, s.yg[y + 1], use_add);
+      for (int y = 0; y < TH; ++y) st[y] = sp[y * PS];
 #pragma unroll
-      for (int y = TH - 1; y >= 0; --y) {
-        uint64_t c = extend(lab[y + 2], st[y]);
-        if (c < lab[y + 1]) {
-          lab[y + 1] = c;
-          chg |= 1ull << y;
-        }
-      }
+      for (int y = TH - 1; y >= 0; --y) relax_to(v[y + 2], st[y], v[y + 1], chg, 1u << y);
       if (chg) {
         ch = true;
 #pragma unroll
-        for (int y = 0; y < TH; ++y)
-          if ((chg >> y) & 1) {
-            st_lab(s, lb + (y + 1) * rs, lab[y + 1]);
-            s.m[cb + y * PS] |= K_CHG;
-          }
+        for (int y = 0; y < TH; ++y) lp[(y + 1) * RS] = v[y + 1];
+        s.cchg[line] |= chg;
       }
     }
-    __syncthreads();
-    // via columns (lane per (y, x)); at most LM layers
+    quiet = phase_changed(s, ch, k++) ? 0u : quiet + 1;
+    if (k >= nphase && quiet + 1 >= nphase) return false;
     if (L > 1) {
+      // via columns: lane per (y, x), at most LM layers
+      ch = false;
       for (uint32_t col = tid; col < TW * TH; col += nt) {
-        uint32_t y = col / TW, x = col - y * TW;
-        uint32_t lb = (y + 1) * rs + x + 1, cb = y * PS + x;
-        uint64_t lab[LM + 2];
+        const uint32_t y = col / TW, x = col - y * TW;
+        uint64_t* lp = s.lab + (y + 1) * RS + x + 1;
+        const uint32_t* dp = s.svd + y * PS + x;
+        const uint32_t* up = s.svu + y * PS + x;
+        uint64_t v[LM + 2];
         uint32_t sf[LM], sb[LM];
-#pragma unroll
-        for (int l = 0; l < LM + 2; ++l) lab[l] = kInf;
+        uint32_t chg = 0;
+        v[0] = kUnreached;
+        v[LM + 1] = kUnreached;
 #pragma unroll
         for (int l = 0; l < LM; ++l) {
-          sf[l] = kMax;
-          sb[l] = kMax;
-          if ((uint32_t)l < L) {
-            uint32_t c = l * plane_p + cb;
-            lab[l + 1] = ld_lab(s, l * rh * rs + lb);
-            if (l > 0) sf[l] = gated_step(s, c, K_PASS | K_DN, s.vg[l - 1], use_add);
-            if ((uint32_t)l + 1 < L) sb[l] = gated_step(s, c, K_PASS | K_UP, s.vg[l], use_add);
-          }
+          const bool in = (uint32_t)l < L;
+          v[l + 1] = in ? lp[l * LAB_PLANE] : kUnreached;
+          sf[l] = in ? dp[l * STEP_PLANE] : kMax;
+          sb[l] = in ? up[l * STEP_PLANE] : kMax;
         }
-        uint64_t chg = sweep_line<LM>(lab, sf, sb);
+#pragma unroll
+        for (int l = 0; l < LM; ++l) relax_to(v[l], sf[l], v[l + 1], chg, 1u << l);
+#pragma unroll
+        for (int l = LM - 1; l >= 0; --l) relax_to(v[l + 2], sb[l], v[l + 1], chg, 1u << l);
         if (chg) {
           ch = true;
 #pragma unroll
           for (int l = 0; l < LM; ++l)
-            if ((chg >> l) & 1) {
-              st_lab(s, l * rh * rs + lb, lab[l + 1]);
-              s.m[l * plane_p + cb] |= K_CHG;
-            }
+            if ((uint32_t)l < L) lp[l * LAB_PLANE] = v[l + 1];
+          s.vchg[col] |= chg;
         }
       }
-    }
-    if (ch) s.misc[2] = rounds + 1;
-    __syncthreads();
-    if (uniform_u32(s.misc[2]) != rounds + 1) {
-      ++rounds;
-      return false;
+      quiet = phase_changed(s, ch, k++) ? 0u : quiet + 1;
+      if (k >= nphase && quiet + 1 >= nphase) return false;
     }
   }
   return true;
@@ -477,173 +503,203 @@ __device__ bool relax_local(const Lds& s, uint32_t L, bool use_add, uint32_t max
 // Result flags of one tile visit.
 enum : uint32_t { T_LEFT = 1, T_RIGHT = 2, T_TOP = 4, T_BOTTOM = 8, T_ANY = 16, T_CAPPED = 32 };
 
+__device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ? v : kUnreached; }
+
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
 template <int TW, int TH, int LM>
 __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
-  const uint32_t f = A.item_field[item];
-  const FieldDesc F = A.fields[f];
-  const BoardView& B = A.boards[F.board];
+  const uint32_t f = uniform_u32(A.item_field[item]);
+  const FieldDesc F = uniform_copy(A.fields[f]);
+  const BoardView B = uniform_copy(A.boards[F.board]);
   const uint32_t t = item - F.item0;
   const uint32_t txi = t % F.tx, tyi = t / F.tx;
-  constexpr uint32_t tw = TW, th = TH;
+  constexpr uint32_t RS = TW + 3, RH = TH + 2, PS = TW + 1;
+  constexpr uint32_t LAB_PLANE = RH * RS, STEP_PLANE = TH * PS, PT = TW * TH;
   const uint32_t L = B.layers;
-  const uint32_t ncell = L * th * tw, plane_t = th * tw;
-  const uint32_t wx0 = txi * tw, wy0 = tyi * th;
-  const bool use_add = A.use_add != 0;  // LDS holds an additive plane in this launch
-  const bool has_add = B.add != nullptr || B.halo != nullptr;
-  Lds s = carve(smem, tw, th, L, use_add);
-  const uint32_t rs = s.rs, rh = s.rh;
-  constexpr uint32_t PS = TW + 1;  // padded interior row stride (mask / weight / additive)
-  const uint64_t blockbase = F.lab_off + (uint64_t)t * ncell;
+  const uint32_t ncell = L * PT;
+  const uint32_t wx0 = txi * TW, wy0 = tyi * TH;
+  Lds s = carve<TW, TH>(smem, L);
+  const uint64_t TB = tile_block_words(L, TW, TH);
+  const uint64_t blockbase = F.lab_off + (uint64_t)t * TB;
 
+  // state of this item and its four neighbours (0 when there is none)
+  if (tid < 5) {
+    uint32_t nb = item;
+    bool ok = true;
+    if (tid == 1) ok = txi > 0, nb = item - 1;
+    if (tid == 2) ok = txi + 1 < F.tx, nb = item + 1;
+    if (tid == 3) ok = tyi > 0, nb = item - F.tx;
+    if (tid == 4) ok = tyi + 1 < F.ty, nb = item + F.tx;
+    s.misc[tid] = ok ? __hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+  }
   if (tid == 0) {
-    s.misc[0] = __hip_atomic_load(&A.state[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s.misc[1] = 0;  // result flags
-    s.misc[2] = 0;  // round changed
+    s.misc[5] = 0;  // result flags
+    s.misc[8] = 0;  // phase change stamps
+    s.misc[9] = 0;
   }
   // tile edge prices
-  for (uint32_t i = tid; i <= tw; i += nt) {
+  for (uint32_t i = tid; i <= TW; i += nt) {
     uint32_t gx = F.x0 + wx0 + i;  // entering column gx from gx-1
-    s.xg[i] = (gx >= 1 && gx - 1 + 1 < B.w) ? B.xe[gx - 1] : 0u;
+    s.xg[i] = (gx >= 1 && gx < B.w) ? gbl(B.xe)[gx - 1] : 0u;
   }
-  for (uint32_t i = tid; i <= th; i += nt) {
+  for (uint32_t i = tid; i <= TH; i += nt) {
     uint32_t gy = F.y0 + wy0 + i;
-    s.yg[i] = (gy >= 1 && gy - 1 + 1 < B.h) ? B.ye[gy - 1] : 0u;
+    s.yg[i] = (gy >= 1 && gy < B.h) ? gbl(B.ye)[gy - 1] : 0u;
   }
-  for (uint32_t i = tid; i < 16; i += nt) s.vg[i] = (i + 1 < L) ? B.ve[i] : 0u;
+  for (uint32_t i = tid; i < 16; i += nt) s.vg[i] = (i + 1 < L) ? gbl(B.ve)[i] : 0u;
+  for (uint32_t i = tid; i < L * TH; i += nt) s.rchg[i] = 0;
+  for (uint32_t i = tid; i < L * TW; i += nt) s.cchg[i] = 0;
+  for (uint32_t i = tid; i < PT; i += nt) s.vchg[i] = 0;
   __syncthreads();
   const bool touched = (uniform_u32(s.misc[0]) & S_TOUCHED) != 0;
+  const bool n_left = (uniform_u32(s.misc[1]) & S_TOUCHED) != 0;
+  const bool n_right = (uniform_u32(s.misc[2]) & S_TOUCHED) != 0;
+  const bool n_top = (uniform_u32(s.misc[3]) & S_TOUCHED) != 0;
+  const bool n_bottom = (uniform_u32(s.misc[4]) & S_TOUCHED) != 0;
 
-  // interior cells
-  for (uint32_t c = tid; c < ncell; c += nt) {
-    uint32_t l = c / plane_t, r = c - l * plane_t, y = r / tw, x = r - y * tw;
-    uint32_t wx = wx0 + x, wy = wy0 + y;
-    uint32_t li = (l * rh + y + 1) * rs + x + 1;
-    uint8_t m = 0;
-    uint32_t wgt = 0, ap = 0;
-    uint64_t lab = kInf;
-    if (wx < F.ww && wy < F.wh) {
-      uint32_t g = (l * B.h + F.y0 + wy) * B.w + F.x0 + wx;
-      uint8_t bm = B.bmask[g];
-      m = (uint8_t)((bm >> 1) & 0x7E);
-      if (!(bm & (M_TRACE | M_OBST))) m |= K_PASS;
-      wgt = B.cost[g];
-      if (has_add) ap = add_price(B, F, g);
-      if (touched)
-        lab = __hip_atomic_load(&A.labels[blockbase + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // Interior cells, four per thread per batch.  Every load of a batch is
+  // issued before the first is used (addresses are clamped into the window,
+  // so the loads are unconditional; cells outside the window are closed
+  // afterwards).  Each cell gets its per-net view (cell_info: own pads, forced
+  // endpoints, via exemptions, additive and halo pricing) and the six step
+  // prices of entering it.
+  const uint32_t* pads = A.pool + F.pad_off;
+  for (uint32_t base = 0; base < ncell; base += 4u * nt) {
+    uint64_t lv[4];
+    uint32_t gv[4], wv[4], av[4], iv[4];
+    int32_t ov[4];
+    uint8_t bv[4];
+    bool inw[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = base + u * nt + tid;
+      const uint32_t ic = i < ncell ? i : ncell - 1;
+      const uint32_t l = ic / PT, r = ic - l * PT, y = r / TW, x = r - y * TW;
+      inw[u] = i < ncell && wx0 + x < F.ww && wy0 + y < F.wh;
+      const uint32_t xx = min(wx0 + x, F.ww - 1), yy = min(wy0 + y, F.wh - 1);
+      iv[u] = ic;
+      gv[u] = (l * B.h + F.y0 + yy) * B.w + F.x0 + xx;
+      bv[u] = gbl(B.bmask)[gv[u]];
+      wv[u] = gbl(B.cost)[gv[u]];
     }
-    uint32_t pc = (l * th + y) * PS + x;
-    s.m[pc] = m;
-    s.w[pc] = wgt;
-    if (use_add) s.a[pc] = ap;
-    st_lab(s, li, lab);
-  }
-  // halo strips (neighbour tiles of the same field)
-  for (uint32_t i = tid; i < L * th; i += nt) {
-    uint32_t l = i / th, y = i - l * th;
-    uint64_t left = kInf, right = kInf;
-    if (wy0 + y < F.wh) {
-      if (txi > 0) {
-        uint32_t nb = item - 1;
-        if (__hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & S_TOUCHED)
-          left = __hip_atomic_load(&A.labels[F.lab_off + (uint64_t)(t - 1) * ncell + (l * th + y) * tw + tw - 1],
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (txi + 1 < F.tx) {
-        uint32_t nb = item + 1;
-        if (__hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & S_TOUCHED)
-          right = __hip_atomic_load(&A.labels[F.lab_off + (uint64_t)(t + 1) * ncell + (l * th + y) * tw],
-                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+    if (B.add) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) av[u] = gbl(B.add)[gv[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) av[u] = 0;
     }
-    st_lab(s, (l * rh + y + 1) * rs + 0, left);
-    st_lab(s, (l * rh + y + 1) * rs + tw + 1, right);
-  }
-  for (uint32_t i = tid; i < L * tw; i += nt) {
-    uint32_t l = i / tw, x = i - l * tw;
-    uint64_t top = kInf, bot = kInf;
-    if (wx0 + x < F.ww) {
-      if (tyi > 0) {
-        uint32_t nb = item - F.tx;
-        if (__hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & S_TOUCHED)
-          top = __hip_atomic_load(&A.labels[F.lab_off + (uint64_t)(t - F.tx) * ncell + (l * th + th - 1) * tw + x],
-                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (tyi + 1 < F.ty) {
-        uint32_t nb = item + F.tx;
-        if (__hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & S_TOUCHED)
-          bot = __hip_atomic_load(&A.labels[F.lab_off + (uint64_t)(t + F.tx) * ncell + (l * th) * tw + x],
-                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+    if (B.halo) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) ov[u] = gbl(B.halo)[gv[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) ov[u] = -1;
     }
-    st_lab(s, (l * rh + 0) * rs + x + 1, top);
-    st_lab(s, (l * rh + th + 1) * rs + x + 1, bot);
-  }
-  __syncthreads();
-
-  // per-net overrides: own pads (weight 1, passable) and forced endpoints
-  const uint32_t plane = B.plane;
-  if (F.n_pad) {
-    const uint32_t* pads = A.pool + F.pad_off;
-    for (uint32_t seg = tid; seg < L * th; seg += nt) {
-      uint32_t l = seg / th, y = seg - l * th, wy = wy0 + y;
-      if (wy >= F.wh) continue;
-      uint32_t row0 = (l * B.h + F.y0 + wy) * B.w + F.x0 + wx0;
-      uint32_t lim = min(tw, F.ww - wx0);
-      for (uint32_t k = lower_bound(pads, F.n_pad, row0); k < F.n_pad && pads[k] < row0 + lim; ++k) {
-        uint32_t x = pads[k] - row0, c = (l * th + y) * PS + x;
-        uint8_t bm = B.bmask[pads[k]];
-        if (!(bm & M_TRACE)) s.m[c] |= K_PASS;
-        if (bm & M_OBST) s.w[c] = 1u;
+    if (touched) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lv[u] = gbl(A.labels)[blockbase + iv[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lv[u] = kUnreached;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = base + u * nt + tid;
+      if (i >= ncell) continue;
+      const uint32_t l = i / PT, r = i - l * PT, y = r / TW, x = r - y * TW;
+      const uint8_t bm = inw[u] ? bv[u] : (uint8_t)M_TRACE;  // outside the window: closed
+      const uint32_t g = gv[u];
+      const bool trace = (bm & M_TRACE) != 0, obst = (bm & M_OBST) != 0;
+      uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
+      bool pass = !trace && !obst;
+      uint32_t wgt = wv[u];
+      if (obst && !trace && F.n_pad && sorted_has(pads, F.n_pad, g)) {
+        pass = true;
+        wgt = 1u;
       }
+      if (!pass && !trace && (F.flags & F_FORCE)) {
+        for (uint32_t k = 0; k < F.n_forced; ++k)
+          if (A.pool[F.forced_off + k] == g) pass = true;
+      }
+      if (pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
+      uint64_t a64 = av[u];
+      if (ov[u] != -1 && ov[u] != F.own_id) a64 += B.halo_pen;
+      const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
+      const uint32_t pc = l * STEP_PLANE + y * PS + x;
+      s.sxl[pc] = gate(pass && (m & K_NX), s.xg[x], wgt, ap);
+      s.sxr[pc] = gate(pass && (m & K_PX), s.xg[x + 1], wgt, ap);
+      s.syt[pc] = gate(pass && (m & K_NY), s.yg[y], wgt, ap);
+      s.syb[pc] = gate(pass && (m & K_PY), s.yg[y + 1], wgt, ap);
+      if (L > 1) {
+        s.svd[pc] = gate(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0], wgt, ap);
+        s.svu[pc] = gate(pass && (m & K_UP) && l + 1 < L, s.vg[l], wgt, ap);
+      }
+      s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[u] ? lab_in(lv[u]) : kUnreached;
     }
   }
-  if (F.flags & F_FORCE) {
-    for (uint32_t k = tid; k < F.n_forced; k += nt) {
-      uint32_t g = A.pool[F.forced_off + k];
-      uint32_t l = g / plane, r = g - l * plane, gy = r / B.w, gx = r - gy * B.w;
-      if (gx < F.x0 + wx0 || gy < F.y0 + wy0) continue;
-      uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0;
-      if (x >= tw || y >= th || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
-      if (!(B.bmask[g] & M_TRACE)) s.m[(l * th + y) * PS + x] |= K_PASS;
-    }
-  }
-  __syncthreads();
-  if (F.n_vpad) {
-    const uint32_t* vp = A.pool + F.vpad_off;
-    for (uint32_t seg = tid; seg < L * th; seg += nt) {
-      uint32_t l = seg / th, y = seg - l * th, wy = wy0 + y;
-      if (wy >= F.wh) continue;
-      uint32_t row0 = (l * B.h + F.y0 + wy) * B.w + F.x0 + wx0;
-      uint32_t lim = min(tw, F.ww - wx0);
-      for (uint32_t k = lower_bound(vp, F.n_vpad, row0); k < F.n_vpad && vp[k] < row0 + lim; ++k) {
-        uint32_t x = vp[k] - row0;
-        for (int dl = -1; dl <= 1; ++dl) {
-          int ll = (int)l + dl;
-          if (ll < 0 || ll >= (int)L) continue;
-          uint32_t g = vp[k] + (uint32_t)((int)plane * dl);
-          uint32_t c = ((uint32_t)ll * th + y) * PS + x;
-          uint8_t vb = via_bits(B, F, A.pool, g);
-          s.m[c] = (uint8_t)((s.m[c] & ~(K_DN | K_UP)) | vb);
+  // Halo ring, two items per thread per batch: left / right columns come from
+  // the side neighbours' border-column copies, top / bottom rows from the
+  // neighbours' label blocks.  Missing neighbours and cells outside the window
+  // read the tile's own first label (a valid address) and are replaced by
+  // kUnreached.
+  const uint32_t nhalo = 2u * L * (TH + TW);
+  for (uint32_t base = 0; base < nhalo; base += 2u * nt) {
+    uint64_t hv[2];
+    uint32_t at[2];
+    bool ok[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const uint32_t j = min(base + u * nt + tid, nhalo - 1);
+      uint64_t src = blockbase;
+      bool valid = false;
+      if (j < 2u * L * TH) {
+        const uint32_t side = j / (L * TH), k = j - side * L * TH, l = k / TH, y = k - l * TH;
+        const bool row_in = wy0 + y < F.wh;
+        if (side == 0) {
+          valid = row_in && n_left;
+          src = valid ? blockbase - TB + ncell + L * TH + k : blockbase;  // left tile's last column
+        } else {
+          valid = row_in && n_right;
+          src = valid ? blockbase + TB + ncell + k : blockbase;  // right tile's first column
         }
+        at[u] = l * LAB_PLANE + (y + 1) * RS + (side ? TW + 1 : 0);
+      } else {
+        const uint32_t jj = j - 2u * L * TH, side = jj / (L * TW), k = jj - side * L * TW;
+        const uint32_t l = k / TW, x = k - l * TW;
+        const bool col_in = wx0 + x < F.ww;
+        if (side == 0) {
+          valid = col_in && n_top;
+          src = valid ? blockbase - (uint64_t)F.tx * TB + l * PT + (TH - 1) * TW + x : blockbase;
+        } else {
+          valid = col_in && n_bottom;
+          src = valid ? blockbase + (uint64_t)F.tx * TB + l * PT + x : blockbase;
+        }
+        at[u] = l * LAB_PLANE + (side ? (TH + 1) * RS : 0) + x + 1;
       }
+      ok[u] = valid && base + u * nt + tid < nhalo;
+      hv[u] = __hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (base + u * nt + tid < nhalo) s.lab[at[u]] = ok[u] ? lab_in(hv[u]) : kUnreached;
+    }
   }
+  __syncthreads();
   // seeds (first visit only): passable sources start at label (0, 0)
   if (!touched) {
+    const uint32_t plane = B.plane;
     for (uint32_t k = tid; k < F.n_src; k += nt) {
       uint32_t g = A.pool[F.src_off + k];
       uint32_t l = g / plane, r = g - l * plane, gy = r / B.w, gx = r - gy * B.w;
       if (gx < F.x0 + wx0 || gy < F.y0 + wy0) continue;
       uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0;
-      if (x >= tw || y >= th || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
-      uint32_t c = (l * th + y) * PS + x;
-      if (s.m[c] & K_PASS) {
-        st_lab(s, (l * rh + y + 1) * rs + x + 1, 0ull);
-        s.m[c] |= K_CHG;
+      if (x >= TW || y >= TH || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
+      if (cell_info(B, F, A.pool, g).mask & K_PASS) {
+        s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = 0ull;
+        atomicOr(&s.rchg[l * TH + y], 1u << x);
       }
     }
     __syncthreads();
@@ -651,27 +707,32 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
 
   // ---- in-LDS relaxation to a local fixpoint ----------------------------------
   uint32_t rounds = 0;
-  bool capped = relax_local<TW, TH, LM>(s, L, use_add, A.max_local, rounds);
-  // ---- write back -----------------------------------------------------------
+  const bool capped = relax_local<TW, TH, LM>(s, L, A.max_local, rounds);
+  // ---- write back: labels, border-column copies, side flags -------------------
   uint32_t flags = 0;
   for (uint32_t c = tid; c < ncell; c += nt) {
-    uint32_t l = c / plane_t, r = c - l * plane_t, y = r / tw, x = r - y * tw;
-    uint8_t m = s.m[(l * th + y) * PS + x];
-    if (!touched || (m & K_CHG)) {
-      uint64_t v = ld_lab(s, (l * rh + y + 1) * rs + x + 1);
+    const uint32_t l = c / PT, r = c - l * PT, y = r / TW, x = r - y * TW;
+    const bool chg = ((s.rchg[l * TH + y] >> x) | (s.cchg[l * TW + x] >> y) | (s.vchg[r] >> l)) & 1u;
+    if (!touched || chg) {
+      const uint64_t v = s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1];
       __hip_atomic_store(&A.labels[blockbase + c], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (x == 0)
+        __hip_atomic_store(&A.labels[blockbase + ncell + l * TH + y], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (x == TW - 1)
+        __hip_atomic_store(&A.labels[blockbase + ncell + L * TH + l * TH + y], v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (m & K_CHG) {
+    if (chg) {
       flags |= T_ANY;
       if (x == 0) flags |= T_LEFT;
-      if (x == tw - 1) flags |= T_RIGHT;
+      if (x == TW - 1) flags |= T_RIGHT;
       if (y == 0) flags |= T_TOP;
-      if (y == th - 1) flags |= T_BOTTOM;
+      if (y == TH - 1) flags |= T_BOTTOM;
     }
   }
-  if (flags) atomicOr(&s.misc[1], flags);
+  if (flags) atomicOr(&s.misc[5], flags);
   __syncthreads();  // also orders this workgroup's label stores before the flag / queue updates
-  uint32_t res = uniform_u32(s.misc[1]) | (capped ? T_CAPPED : 0u);
+  const uint32_t res = uniform_u32(s.misc[5]) | (capped ? T_CAPPED : 0u);
   if (tid == 0) {
     if (!touched) __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
@@ -874,12 +935,6 @@ __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ void fill_u8(uint8_t* p, uint8_t v, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
-}
-
 // Activate items idx[0..n) for the first round: flags and list.
 __global__ void seed_list(uint32_t* act, uint32_t* list, const uint32_t* idx, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,8 +974,7 @@ __device__ uint64_t field_label(const FieldRead& R, const BoardView& B, const Fi
   uint32_t txi = wx / R.tw, tyi = wy / R.th, t = tyi * F.tx + txi;
   if (!(R.state[F.item0 + t] & S_TOUCHED)) return kInf;
   uint32_t x = wx - txi * R.tw, y = wy - tyi * R.th;
-  uint64_t ncell = (uint64_t)B.layers * R.th * R.tw;
-  return R.labels[F.lab_off + t * ncell + (l * R.th + y) * R.tw + x];
+  return R.labels[F.lab_off + t * tile_block_words(B.layers, R.tw, R.th) + (l * R.th + y) * R.tw + x];
 }
 
 __global__ void path_lengths(FieldRead R, uint32_t n, uint32_t* plen, uint32_t* pcost) {
@@ -1101,6 +1155,8 @@ void ctx_init(Ctx& c, int device) {
   c.spin_ticks = (uint64_t)rate_khz * 1000ull * 4ull;  // 4 s
   HIP_OK(hipEventCreate(&c.ws.ev0));
   HIP_OK(hipEventCreate(&c.ws.ev1));
+  HIP_OK(hipEventCreate(&c.ws.ek0));
+  HIP_OK(hipEventCreate(&c.ws.ek1));
   HIP_OK(hipEventCreate(&c.ws.ev2));
   size_t total = prop.totalGlobalMem;
   c.label_budget = std::min<size_t>(c.label_budget, total / 2);
@@ -1125,6 +1181,8 @@ void ctx_free(Ctx& c) {
   c.graph = GraphCache();
   if (w.ev0) (void)hipEventDestroy(w.ev0);
   if (w.ev1) (void)hipEventDestroy(w.ev1);
+  if (w.ek0) (void)hipEventDestroy(w.ek0);
+  if (w.ek1) (void)hipEventDestroy(w.ek1);
   if (w.ev2) (void)hipEventDestroy(w.ev2);
   w = Workspace();
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -1212,10 +1270,10 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
 
 size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t layers) {
   size_t tx = (ww + s.tw - 1) / s.tw, ty = (wh + s.th - 1) / s.th;
-  return tx * ty * (size_t)layers * s.th * s.tw;
+  return tx * ty * tile_block_words(layers, s.tw, s.th);
 }
 
-static RelaxArgs make_args(Ctx& c, uint32_t n_items, bool use_add) {
+static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   RelaxArgs A;
   memset(&A, 0, sizeof(A));
   A.boards = c.ws.boards;
@@ -1234,7 +1292,6 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items, bool use_add) {
   A.tw = c.shape.tw;
   A.th = c.shape.th;
   A.max_local = c.shape.max_local;
-  A.use_add = use_add ? 1 : 0;
   A.qcap = n_items;
   A.spin_limit = 1u << 26;
   A.spin_ticks = c.spin_ticks;
@@ -1245,11 +1302,6 @@ static void launch_fill32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
   if (!n) return;
   uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
-}
-static void launch_fill8(hipStream_t s, uint8_t* p, uint8_t v, size_t n) {
-  if (!n) return;
-  uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(fill_u8, dim3(blocks), dim3(256), 0, s, p, v, n);
 }
 
 // Capture (or reuse) `graph_rounds` rounds of relax_list + advance_round.
@@ -1288,16 +1340,14 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   auto t0 = std::chrono::steady_clock::now();
   const TileShape& S = c.shape;
   uint32_t Lmax = 1;
-  bool use_add = false;
   c.views.clear();
   for (const DevBoard* b : boards) {
     Lmax = std::max(Lmax, b->layers);
-    use_add = use_add || b->add != nullptr || b->halo != nullptr;
     c.views.push_back(b->view());
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  size_t lds = lds_bytes(S.tw, S.th, Lmax, use_add);
+  size_t lds = lds_bytes(S.tw, S.th, Lmax);
   if (lds > 160 * 1024 - 64) throw Error(1, "tile shape exceeds 160 KiB of LDS");
 
   std::vector<FieldDesc>& layout = c.layout;
@@ -1347,7 +1397,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     st_items.erase(std::unique(st_items.begin(), st_items.end()), st_items.end());
     seeds.insert(seeds.end(), st_items.begin(), st_items.end());
     item += nt;
-    lab_off += (uint64_t)nt * L * S.th * S.tw;
+    lab_off += (uint64_t)nt * tile_block_words(L, S.tw, S.th);
     field_cells += (uint64_t)F.ww * F.wh * L;
     layout.push_back(F);
   }
@@ -1378,7 +1428,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   launch_fill32(s, W.state, 0u, item);
   launch_fill32(s, W.counters, 0u, kCtrWords);
 
-  RelaxArgs A = make_args(c, item, use_add);
+  RelaxArgs A = make_args(c, item);
   SolveStats local;
   local.field_cells = field_cells;
   bool timed = false;
@@ -1392,9 +1442,15 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     auto plain_rounds = [&](uint32_t& rounds, uint32_t n_active) {
       while (n_active) {
         launch_fill32(s, W.counters + C_NEXTN * kCtrStride, 0u, 1);
+        HIP_OK(hipEventRecord(W.ek0, s));
         HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
+        HIP_OK(hipEventRecord(W.ek1, s));
         HIP_OK(hipMemcpyAsync(pinned, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        float kms = 0;
+        HIP_OK(hipEventElapsedTime(&kms, W.ek0, W.ek1));
+        local.kernel_ms += kms;
+        ++local.launches;
         ++rounds;
         if (pinned[C_ERROR * kCtrStride]) break;
         n_active = pinned[C_NEXTN * kCtrStride];
@@ -1435,6 +1491,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       }
       rounds = pinned[C_PROUNDS * kCtrStride];
       local.replays = replays;
+      local.launches = rounds;
     } else {
       throw Error(1, "the persistent grid-barrier form is disabled on this build (it did not return on MI355X); "
                      "use the plain or graph form");
@@ -1453,8 +1510,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   local.local_rounds = cnt[C_ROUNDS * kCtrStride];
   local.cells_visited = (uint64_t)cnt[C_VISITS * kCtrStride] * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (local.launches && local.kernel_ms == 0) local.kernel_ms = ms;  // graph form: replays only
   if (st) *st = local;
-  c.last = local;
+  c.last.add(local);
 }
 
 static FieldRead make_read(Ctx& c) {
@@ -1520,7 +1578,7 @@ void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, W.ev1, W.ev2));
     if (st) st->path_ms = ms;
-    c.last.path_ms = ms;
+    c.last.path_ms += ms;
   }
 }
 

@@ -73,7 +73,8 @@ class PathsS(C.Structure):
 class StatsS(C.Structure):
     _fields_ = [("relax_ms", C.c_double), ("path_ms", C.c_double), ("total_ms", C.c_double),
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
-                ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double)]
+                ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double),
+                ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32)]
 
 
 class SynthS(C.Structure):
@@ -258,6 +259,35 @@ def _windows(wins, n):
     return (WindowS * max(n, 1))(*[WindowS(*w) for w in wins])
 
 
+class PathBatch:
+    """Paths of one call as flat arrays: path i is cells[offsets[i]:offsets[i+1]]
+    when routed[i], with search cost cost[i]."""
+
+    def __init__(self, routed, cost, offsets, cells):
+        self.routed, self.cost, self.offsets, self.cells = routed, cost, offsets, cells
+
+    def __len__(self):
+        return len(self.routed)
+
+    def path(self, i: int) -> Optional[Tuple[List[int], int]]:
+        if not self.routed[i]:
+            return None
+        return self.cells[self.offsets[i]:self.offsets[i + 1]].tolist(), int(self.cost[i])
+
+
+def _take_arrays(p: PathsS) -> PathBatch:
+    try:
+        n = p.n
+        off = np.ctypeslib.as_array(p.offsets, shape=(n + 1,)).copy()
+        total = int(off[-1])
+        cells = np.ctypeslib.as_array(p.cells, shape=(max(total, 1),))[:total].copy()
+        routed = np.ctypeslib.as_array(p.routed, shape=(max(n, 1),))[:n].copy()
+        cost = np.ctypeslib.as_array(p.search_cost, shape=(max(n, 1),))[:n].copy()
+    finally:
+        lib().mrf_free_paths(C.byref(p))
+    return PathBatch(routed, cost, off, cells)
+
+
 def _take_paths(p: PathsS) -> List[Optional[Tuple[List[int], int]]]:
     out = []
     try:
@@ -270,7 +300,7 @@ def _take_paths(p: PathsS) -> List[Optional[Tuple[List[int], int]]]:
             cost = np.ctypeslib.as_array(p.search_cost, shape=(n,)).copy()
             for i in range(n):
                 if routed[i]:
-                    out.append(([int(c) for c in cells[off[i]:off[i + 1]]], int(cost[i])))
+                    out.append((cells[off[i]:off[i + 1]].tolist(), int(cost[i])))
                 else:
                     out.append(None)
     finally:
@@ -392,13 +422,20 @@ class Board:
     def bytes(self) -> int:
         return int(lib().mrf_board_bytes(self._h))
 
-    def solve(self, nets, windows=None, retry_full=True):
+    def _solve(self, nets, windows, retry_full):
         arr, keep = _nets(nets)
         out = PathsS()
         err = ErrorS()
         _check(lib().mrf_solve(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)),
                                1 if retry_full else 0, C.byref(out), C.byref(err)), err)
-        return _take_paths(out)
+        return out
+
+    def solve(self, nets, windows=None, retry_full=True):
+        return _take_paths(self._solve(nets, windows, retry_full))
+
+    def solve_arrays(self, nets, windows=None, retry_full=True) -> PathBatch:
+        """Like solve(), with the paths left as flat numpy arrays."""
+        return _take_arrays(self._solve(nets, windows, retry_full))
 
     def solve_fields(self, sources, dsts=None, net_ids=None, windows=None):
         n = len(sources)
