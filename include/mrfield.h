@@ -56,7 +56,8 @@ enum {
     MRF_OK = 0,
     MRF_BACKEND_UNAVAILABLE = 1,
     MRF_INVALID_ENDPOINT = 2,
-    MRF_MALFORMED_GRID = 3
+    MRF_MALFORMED_GRID = 3,
+    MRF_NOT_READY = 4 /* serving loop only: the request has not finished yet */
 };
 
 /* mr-core Grid::BOARD_* bits (crates/mr-core/src/lib.rs:411-416) */
@@ -231,6 +232,67 @@ int mrf_solve_fields(mrf_ctx *ctx, mrf_board *board, const uint32_t *const *sour
                      size_t n_fields, const mrf_window *windows, mrf_error *err);
 int mrf_read_field(mrf_ctx *ctx, mrf_board *board, size_t field, uint32_t *dist_out,
                    uint32_t *hops_out, mrf_error *err);
+
+/* ---- serving loop ------------------------------------------------------------
+ * A server keeps board planes resident, queues solve requests from any thread
+ * and runs one relaxation on a worker thread.  Requests are admitted in
+ * arrival order at round boundaries while the label budget (HBM) and the
+ * request slots allow; the fields of every admitted request, on any of the
+ * resident boards, relax together, and a request leaves as soon as its own
+ * fields have converged (continuous batching).  A request's paths do not
+ * depend on what it was batched with: each field is the unique fixpoint of its
+ * own relaxation.  The reference's /solve path runs one request per call
+ * (crates/mr-server/src/lib.rs:362-453); this is the resident, batched form of
+ * that loop, with the provider's semantics per request (window pass, then
+ * full-board retry of unreachable nets when asked). */
+typedef struct mrf_server mrf_server;
+
+typedef struct {
+    size_t label_bytes;     /* HBM for labels of admitted requests (the admission budget);
+                               0 = a third of device memory, at most 96 GiB */
+    uint32_t max_boards;    /* resident board slots (0 = 64) */
+    uint32_t max_layers;    /* largest layer count of any board (0 = 8) */
+    uint32_t max_inflight;  /* requests on the device at once (0 = 256) */
+    uint32_t tile_w, tile_h, threads; /* tile shape (0 = 16 x 16, 256 threads) */
+} mrf_server_config;
+
+typedef struct {
+    double queue_ms;    /* submit -> admitted */
+    double solve_ms;    /* admitted -> paths ready */
+    double total_ms;    /* submit -> paths ready */
+    uint32_t rounds;    /* relaxation rounds the request took part in */
+    uint32_t peers_max; /* most other requests relaxing in the same round */
+} mrf_request_stats;
+
+typedef struct {
+    uint64_t submitted, completed, failed;
+    uint64_t rounds, tile_visits;
+    double kernel_ms;         /* relaxation kernels, HIP events */
+    size_t label_bytes;       /* admission budget */
+    size_t label_bytes_peak;  /* most label bytes admitted at once */
+    uint32_t inflight_peak;   /* most requests admitted at once */
+} mrf_server_stats;
+
+int mrf_server_create(int device, const mrf_server_config *cfg, mrf_server **out, mrf_error *err);
+/* Stops the worker and releases the device; unfinished requests are dropped.
+ * Call it only after every mrf_server_wait has returned. */
+void mrf_server_destroy(mrf_server *srv);
+int mrf_server_add_board(mrf_server *srv, mrf_dims dims, const mrf_grid *grid, const mrf_edges *edges,
+                         const mrf_pricing *pricing, uint32_t *board_id, mrf_error *err);
+/* MRF_BACKEND_UNAVAILABLE while a queued or running request uses the board. */
+int mrf_server_remove_board(mrf_server *srv, uint32_t board_id, mrf_error *err);
+/* Queue nets on a resident board: one pass inside `windows` (NULL = full
+ * board), then, when retry_full != 0 and windows were given, the unreachable
+ * nets on the full board.  net_ids (or NULL) are the halo owner ids. */
+int mrf_server_submit(mrf_server *srv, uint32_t board_id, const mrf_net *nets, size_t n_nets,
+                      const mrf_window *windows, int retry_full, const int32_t *net_ids,
+                      uint64_t *request_id, mrf_error *err);
+/* Wait up to timeout_ms (< 0: no limit) for a request.  MRF_NOT_READY when
+ * it has not finished (the request stays queued); otherwise the request is
+ * retired and its paths or error returned. */
+int mrf_server_wait(mrf_server *srv, uint64_t request_id, int64_t timeout_ms, mrf_paths *out,
+                    mrf_request_stats *stats, mrf_error *err);
+int mrf_server_get_stats(mrf_server *srv, mrf_server_stats *out);
 
 /* ---- synthetic workloads --------------------------------------------------- */
 /* Seeded generators for the benchmark configurations (documented in

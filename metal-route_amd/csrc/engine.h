@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -243,5 +244,87 @@ void ctx_init(Ctx& c, int device);
 void ctx_free(Ctx& c);
 size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t layers);
 bool tile_shape_supported(uint32_t tw, uint32_t th);
+
+// ---- round driver: fields join and leave a running relaxation -------------
+//
+// The serving loop keeps one relaxation running.  A group (the fields of one
+// request) is admitted at a round boundary: it gets contiguous ranges of
+// field descriptors, work items, label words and pool words, and its seed
+// tiles join the next round's active list.  After every round the driver
+// reports the groups with no active item left; their labels are final (the
+// fixpoint of a field does not depend on the other fields), so their paths
+// are reconstructed and their ranges freed while the other groups keep
+// relaxing.
+
+// First-fit allocator of ranges in [0, cap).
+class RangeAlloc {
+ public:
+  void reset(uint64_t cap);
+  bool alloc(uint64_t n, uint64_t& off);
+  void release(uint64_t off, uint64_t n);
+  uint64_t cap() const { return cap_; }
+  uint64_t used() const { return used_; }
+
+ private:
+  uint64_t cap_ = 0, used_ = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> length
+};
+
+struct Group {
+  bool live = false;
+  uint64_t field0 = 0, nfields = 0;
+  uint64_t item0 = 0, nitems = 0;
+  uint64_t lab0 = 0, labwords = 0;
+  uint64_t pool0 = 0, npool = 0;
+  bool active = false;  // has an item on the current active list
+};
+
+struct RoundDriver {
+  Ctx* c = nullptr;
+  uint32_t max_layers = 0, max_boards = 0, max_groups = 0;
+  // device arrays
+  uint64_t* labels = nullptr;
+  uint32_t* state = nullptr;
+  uint32_t* act = nullptr;   // 2 x max_items
+  uint32_t* list = nullptr;  // 2 x max_items
+  uint32_t* item_field = nullptr;
+  uint32_t* item_group = nullptr;
+  uint32_t* seeds = nullptr;  // staging for seed items of newly admitted groups
+  FieldDesc* fields = nullptr;
+  uint32_t* pool = nullptr;
+  BoardView* boards = nullptr;
+  uint32_t* group_active = nullptr;
+  uint32_t* pinned = nullptr;  // counters + group_active readback
+  RangeAlloc fa, ia, la, pa;   // fields, items, label words, pool words
+  std::vector<BoardView> views;
+  std::vector<uint8_t> board_used;
+  std::vector<Group> groups;
+  std::vector<uint32_t> pending_seeds;
+  std::vector<std::vector<uint8_t>> staging;  // host buffers of uploads in flight
+  uint32_t n_cur = 0;
+  uint32_t cur = 0;  // which half of act/list is the current round's
+  uint64_t rounds = 0, tile_visits = 0, local_rounds = 0;
+  double kernel_ms = 0;
+};
+
+// label_bytes: HBM for labels of admitted groups (the admission budget).
+void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards, uint32_t max_layers,
+                 uint32_t max_groups);
+void driver_free(RoundDriver& d);
+size_t driver_device_bytes(const RoundDriver& d);
+int driver_add_board(RoundDriver& d, const DevBoard& b);  // slot, or -1 when the table is full
+void driver_remove_board(RoundDriver& d, int slot);
+// Label words a group of fields needs (FieldReq::board is a driver slot).
+uint64_t driver_group_words(const RoundDriver& d, const std::vector<FieldReq>& reqs);
+// Admit a group at the next round boundary; false when it does not fit now.
+bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& group);
+// One relaxation round over every active item (new groups' seeds included).
+// Returns the number of items active for the next round.
+uint32_t driver_step(RoundDriver& d);
+// Live groups with no item on the active list: their fields are complete.
+void driver_finished(RoundDriver& d, std::vector<int>& out);
+void driver_paths(RoundDriver& d, int group, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
+                  std::vector<uint64_t>& off, std::vector<uint32_t>& cells);
+void driver_release(RoundDriver& d, int group);
 
 }  // namespace mrf

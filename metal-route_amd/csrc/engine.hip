@@ -1526,9 +1526,11 @@ static FieldRead make_read(Ctx& c) {
   return R;
 }
 
-void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
-                       std::vector<uint32_t>& cost, std::vector<uint64_t>& off, std::vector<uint32_t>& cells,
-                       SolveStats* st) {
+// Canonical paths of fields R.fields[0..n) (pool: the cell pool their
+// FieldDescs index); path buffers come from the context's workspace.
+static void reconstruct_with(Ctx& c, const FieldRead& R, const uint32_t* pool, size_t n,
+                             std::vector<uint8_t>& routed, std::vector<uint32_t>& cost, std::vector<uint64_t>& off,
+                             std::vector<uint32_t>& cells, SolveStats* st) {
   HIP_OK(hipSetDevice(c.device));
   Workspace& W = c.ws;
   hipStream_t s = c.stream;
@@ -1540,7 +1542,6 @@ void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
   grow(W.plen, W.plen_cap, n);
   grow(W.pcost, W.pcost_cap, n);
   grow(W.poff, W.poff_cap, n + 1);
-  FieldRead R = make_read(c);
   HIP_OK(hipEventRecord(W.ev1, s));
   hipLaunchKernelGGL(path_lengths, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, R, (uint32_t)n, W.plen,
                      W.pcost);
@@ -1560,7 +1561,7 @@ void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
     WalkArgs WA;
     memset(&WA, 0, sizeof(WA));
     WA.R = R;
-    WA.pool = W.pool;
+    WA.pool = pool;
     WA.poff = W.poff;
     WA.plen = W.plen;
     WA.cells = W.pcells;
@@ -1580,6 +1581,11 @@ void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed,
     if (st) st->path_ms = ms;
     c.last.path_ms += ms;
   }
+}
+
+void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
+                       std::vector<uint64_t>& off, std::vector<uint32_t>& cells, SolveStats* st) {
+  reconstruct_with(c, make_read(c), c.ws.pool, n, routed, cost, off, cells, st);
 }
 
 void read_field(Ctx& c, size_t f, uint32_t* dist, uint32_t* hops) {
@@ -1607,6 +1613,360 @@ void read_field(Ctx& c, size_t f, uint32_t* dist, uint32_t* hops) {
       memcpy(dist + dst, tmp.data() + src, F.ww * 4);
       if (hops) memcpy(hops + dst, tmp.data() + n + src, F.ww * 4);
     }
+}
+
+// ---------------------------------------------------------------------------
+// round driver (serving loop)
+// ---------------------------------------------------------------------------
+
+void RangeAlloc::reset(uint64_t cap) {
+  cap_ = cap;
+  used_ = 0;
+  free_.clear();
+  if (cap) free_[0] = cap;
+}
+
+bool RangeAlloc::alloc(uint64_t n, uint64_t& off) {
+  if (n == 0) {
+    off = 0;
+    return true;
+  }
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < n) continue;
+    off = it->first;
+    const uint64_t rest = it->second - n;
+    free_.erase(it);
+    if (rest) free_[off + n] = rest;
+    used_ += n;
+    return true;
+  }
+  return false;
+}
+
+void RangeAlloc::release(uint64_t off, uint64_t n) {
+  if (n == 0) return;
+  used_ -= n;
+  auto it = free_.emplace(off, n).first;
+  auto nx = std::next(it);
+  if (nx != free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    free_.erase(nx);
+  }
+  if (it != free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+// Flag the groups that own an item of the next round's list.
+__global__ void mark_groups(const uint32_t* list, const uint32_t* counters, const uint32_t* item_group,
+                            uint32_t* group_active) {
+  const uint32_t n = counters[C_NEXTN * kCtrStride];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    group_active[item_group[list[i]]] = 1u;
+}
+
+namespace {
+uint64_t driver_items(const RoundDriver& d) { return d.ia.cap(); }
+
+template <typename T>
+void dmalloc(T*& p, uint64_t n) {
+  HIP_OK(hipMalloc((void**)&p, std::max<uint64_t>(n, 1) * sizeof(T)));
+}
+
+template <typename T>
+void stage_copy(RoundDriver& d, T* dst, const std::vector<T>& src) {
+  if (src.empty()) return;
+  d.staging.emplace_back((const uint8_t*)src.data(), (const uint8_t*)src.data() + src.size() * sizeof(T));
+  HIP_OK(hipMemcpyAsync(dst, d.staging.back().data(), src.size() * sizeof(T), hipMemcpyHostToDevice, d.c->stream));
+}
+}  // namespace
+
+void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards, uint32_t max_layers,
+                 uint32_t max_groups) {
+  HIP_OK(hipSetDevice(c.device));
+  const TileShape& S = c.shape;
+  if (!tile_shape_supported(S.tw, S.th)) throw Error(1, "unsupported tile shape");
+  if (max_layers < 1 || max_layers > 16) throw Error(1, "max_layers must be in 1..16");
+  if (lds_bytes(S.tw, S.th, max_layers) > 160 * 1024 - 64) throw Error(1, "tile shape too large for max_layers");
+  if (max_boards < 1 || max_groups < 1) throw Error(1, "server needs at least one board and one group slot");
+  d.c = &c;
+  d.max_layers = max_layers;
+  d.max_boards = max_boards;
+  d.max_groups = max_groups;
+  const uint64_t words = std::max<uint64_t>(label_bytes / 8, 1);
+  const uint64_t items = std::min<uint64_t>(words / tile_block_words(1, S.tw, S.th) + 1, 1ull << 26);
+  const uint64_t nfields = std::min<uint64_t>(items, 1ull << 22);
+  const uint64_t pool_words = 1ull << 24;
+  dmalloc(d.labels, words);
+  dmalloc(d.state, items);
+  dmalloc(d.act, 2 * items);
+  dmalloc(d.list, 2 * items);
+  dmalloc(d.item_field, items);
+  dmalloc(d.item_group, items);
+  dmalloc(d.seeds, items);
+  dmalloc(d.fields, nfields);
+  dmalloc(d.pool, pool_words);
+  dmalloc(d.boards, max_boards);
+  dmalloc(d.group_active, max_groups);
+  HIP_OK(hipHostMalloc((void**)&d.pinned, (kCtrWords + max_groups) * sizeof(uint32_t), hipHostMallocDefault));
+  launch_fill32(c.stream, d.act, 0u, 2 * items);
+  launch_fill32(c.stream, d.group_active, 0u, max_groups);
+  HIP_OK(hipStreamSynchronize(c.stream));
+  d.fa.reset(nfields);
+  d.ia.reset(items);
+  d.la.reset(words);
+  d.pa.reset(pool_words);
+  d.views.assign(max_boards, BoardView{});
+  d.board_used.assign(max_boards, 0);
+  d.groups.assign(max_groups, Group{});
+  d.pending_seeds.clear();
+  d.n_cur = 0;
+  d.cur = 0;
+}
+
+void driver_free(RoundDriver& d) {
+  if (!d.c) return;
+  (void)hipSetDevice(d.c->device);
+  void* ptrs[] = {d.labels, d.state,  d.act,   d.list,   d.item_field,  d.item_group,
+                  d.seeds,  d.fields, d.pool,  d.boards, d.group_active};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (d.pinned) (void)hipHostFree(d.pinned);
+  d = RoundDriver();
+}
+
+size_t driver_device_bytes(const RoundDriver& d) {
+  const uint64_t items = d.ia.cap();
+  return d.la.cap() * 8 + items * 4 * 8 + d.fa.cap() * sizeof(FieldDesc) + d.pa.cap() * 4 +
+         (size_t)d.max_boards * sizeof(BoardView) + (size_t)d.max_groups * 4;
+}
+
+int driver_add_board(RoundDriver& d, const DevBoard& b) {
+  if (b.layers > d.max_layers) throw Error(1, "board has more layers than the server was configured for");
+  for (uint32_t k = 0; k < d.max_boards; ++k) {
+    if (d.board_used[k]) continue;
+    d.views[k] = b.view();
+    d.board_used[k] = 1;
+    HIP_OK(hipMemcpyAsync(d.boards + k, &d.views[k], sizeof(BoardView), hipMemcpyHostToDevice, d.c->stream));
+    HIP_OK(hipStreamSynchronize(d.c->stream));
+    return (int)k;
+  }
+  return -1;
+}
+
+void driver_remove_board(RoundDriver& d, int slot) {
+  if (slot >= 0 && (uint32_t)slot < d.max_boards) d.board_used[slot] = 0;
+}
+
+uint64_t driver_group_words(const RoundDriver& d, const std::vector<FieldReq>& reqs) {
+  uint64_t w = 0;
+  for (const FieldReq& q : reqs) {
+    if (q.board >= d.max_boards || !d.board_used[q.board]) throw Error(1, "field names an unknown board");
+    w += field_label_words(d.c->shape, q.x1 - q.x0 + 1, q.y1 - q.y0 + 1, d.views[q.board].layers);
+  }
+  return w;
+}
+
+bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
+  const TileShape& S = d.c->shape;
+  int g = -1;
+  for (uint32_t k = 0; k < d.max_groups; ++k)
+    if (!d.groups[k].live) {
+      g = (int)k;
+      break;
+    }
+  if (g < 0) return false;
+  uint64_t nitems = 0, words = 0, npool = 0;
+  for (const FieldReq& q : reqs) {
+    if (q.board >= d.max_boards || !d.board_used[q.board]) throw Error(1, "field names an unknown board");
+    const uint32_t ww = q.x1 - q.x0 + 1, wh = q.y1 - q.y0 + 1;
+    nitems += (uint64_t)((ww + S.tw - 1) / S.tw) * ((wh + S.th - 1) / S.th);
+    words += field_label_words(S, ww, wh, d.views[q.board].layers);
+    npool += q.sources.size() + q.forced.size() + q.pads.size() + q.via_pads.size();
+  }
+  Group G;
+  if (!d.fa.alloc(reqs.size(), G.field0)) return false;
+  if (!d.ia.alloc(nitems, G.item0)) {
+    d.fa.release(G.field0, reqs.size());
+    return false;
+  }
+  if (!d.la.alloc(words, G.lab0)) {
+    d.fa.release(G.field0, reqs.size());
+    d.ia.release(G.item0, nitems);
+    return false;
+  }
+  if (!d.pa.alloc(npool, G.pool0)) {
+    d.fa.release(G.field0, reqs.size());
+    d.ia.release(G.item0, nitems);
+    d.la.release(G.lab0, words);
+    return false;
+  }
+  G.nfields = reqs.size();
+  G.nitems = nitems;
+  G.labwords = words;
+  G.npool = npool;
+  std::vector<FieldDesc> descs(reqs.size());
+  std::vector<uint32_t> pool, item_field;
+  pool.reserve(npool);
+  item_field.reserve(nitems);
+  uint64_t item = G.item0, lab = G.lab0;
+  std::vector<uint32_t> seeds;
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    const FieldReq& q = reqs[i];
+    const BoardView& b = d.views[q.board];
+    FieldDesc& F = descs[i];
+    memset(&F, 0, sizeof(F));
+    F.x0 = q.x0;
+    F.y0 = q.y0;
+    F.ww = q.x1 - q.x0 + 1;
+    F.wh = q.y1 - q.y0 + 1;
+    F.tx = (F.ww + S.tw - 1) / S.tw;
+    F.ty = (F.wh + S.th - 1) / S.th;
+    F.item0 = (uint32_t)item;
+    F.lab_off = lab;
+    auto put = [&](const std::vector<uint32_t>& v, uint32_t& off, uint32_t& n) {
+      off = (uint32_t)(G.pool0 + pool.size());
+      n = (uint32_t)v.size();
+      pool.insert(pool.end(), v.begin(), v.end());
+    };
+    put(q.sources, F.src_off, F.n_src);
+    put(q.forced, F.forced_off, F.n_forced);
+    put(q.pads, F.pad_off, F.n_pad);
+    put(q.via_pads, F.vpad_off, F.n_vpad);
+    F.dst = q.dst;
+    F.own_id = q.own_id;
+    F.flags = q.flags;
+    F.board = q.board;
+    const uint32_t nt = F.tx * F.ty;
+    for (uint32_t k = 0; k < nt; ++k) item_field.push_back((uint32_t)(G.field0 + i));
+    std::vector<uint32_t> st;
+    for (uint32_t src : q.sources) {
+      const uint32_t r = src % b.plane, gy = r / b.w, gx = r % b.w;
+      if ((size_t)src >= (size_t)b.plane * b.layers || gx < F.x0 || gy < F.y0 || gx > q.x1 || gy > q.y1) continue;
+      st.push_back((uint32_t)item + ((gy - F.y0) / S.th) * F.tx + (gx - F.x0) / S.tw);
+    }
+    std::sort(st.begin(), st.end());
+    st.erase(std::unique(st.begin(), st.end()), st.end());
+    seeds.insert(seeds.end(), st.begin(), st.end());
+    item += nt;
+    lab += (uint64_t)nt * tile_block_words(b.layers, S.tw, S.th);
+  }
+  hipStream_t s = d.c->stream;
+  stage_copy(d, d.fields + G.field0, descs);
+  stage_copy(d, d.pool + G.pool0, pool);
+  stage_copy(d, d.item_field + G.item0, item_field);
+  launch_fill32(s, d.item_group + G.item0, (uint32_t)g, nitems);
+  launch_fill32(s, d.state + G.item0, 0u, nitems);
+  G.live = true;
+  G.active = !seeds.empty();
+  d.pending_seeds.insert(d.pending_seeds.end(), seeds.begin(), seeds.end());
+  d.groups[g] = G;
+  gid = g;
+  return true;
+}
+
+uint32_t driver_step(RoundDriver& d) {
+  Ctx& c = *d.c;
+  HIP_OK(hipSetDevice(c.device));
+  hipStream_t s = c.stream;
+  const TileShape& S = c.shape;
+  const uint64_t items = driver_items(d);
+  uint32_t* act_cur = d.act + (uint64_t)d.cur * items;
+  uint32_t* act_next = d.act + (uint64_t)(1 - d.cur) * items;
+  uint32_t* list_cur = d.list + (uint64_t)d.cur * items;
+  uint32_t* list_next = d.list + (uint64_t)(1 - d.cur) * items;
+  if (!d.pending_seeds.empty()) {
+    const uint32_t k = (uint32_t)d.pending_seeds.size();
+    stage_copy(d, d.seeds, d.pending_seeds);
+    hipLaunchKernelGGL(seed_list, dim3((k + 255) / 256), dim3(256), 0, s, act_cur, list_cur + d.n_cur, d.seeds, k);
+    d.n_cur += k;
+    d.pending_seeds.clear();
+  }
+  if (d.n_cur == 0) {
+    HIP_OK(hipStreamSynchronize(s));
+    d.staging.clear();
+    return 0;
+  }
+  launch_fill32(s, c.ws.counters, 0u, kCtrWords);
+  launch_fill32(s, d.group_active, 0u, d.max_groups);
+  RelaxArgs A;
+  memset(&A, 0, sizeof(A));
+  A.boards = d.boards;
+  A.fields = d.fields;
+  A.item_field = d.item_field;
+  A.pool = d.pool;
+  A.labels = d.labels;
+  A.state = d.state;
+  A.act_cur = act_cur;
+  A.act_next = act_next;
+  A.list_cur = list_cur;
+  A.list_next = list_next;
+  A.counters = c.ws.counters;
+  A.n_items = (uint32_t)items;
+  A.tw = S.tw;
+  A.th = S.th;
+  A.max_local = S.max_local;
+  A.spin_ticks = c.spin_ticks;
+  const KernelSet& K = pick_kernels(S.tw, S.th, d.max_layers);
+  const size_t lds = lds_bytes(S.tw, S.th, d.max_layers);
+  void* args[] = {&A};
+  HIP_OK(hipEventRecord(c.ws.ek0, s));
+  HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
+  HIP_OK(hipEventRecord(c.ws.ek1, s));
+  hipLaunchKernelGGL(mark_groups, dim3(1024), dim3(256), 0, s, list_next, c.ws.counters, d.item_group,
+                     d.group_active);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(d.pinned, c.ws.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(d.pinned + kCtrWords, d.group_active, d.max_groups * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  d.staging.clear();
+  if (d.pinned[C_ERROR * kCtrStride])
+    throw Error(1, "device relaxation reported error flags " + std::to_string(d.pinned[C_ERROR * kCtrStride]));
+  float kms = 0;
+  HIP_OK(hipEventElapsedTime(&kms, c.ws.ek0, c.ws.ek1));
+  d.kernel_ms += kms;
+  d.rounds += 1;
+  d.tile_visits += d.pinned[C_VISITS * kCtrStride];
+  d.local_rounds += d.pinned[C_ROUNDS * kCtrStride];
+  d.n_cur = d.pinned[C_NEXTN * kCtrStride];
+  d.cur = 1 - d.cur;
+  for (uint32_t g = 0; g < d.max_groups; ++g)
+    if (d.groups[g].live) d.groups[g].active = d.pinned[kCtrWords + g] != 0;
+  return d.n_cur;
+}
+
+void driver_finished(RoundDriver& d, std::vector<int>& out) {
+  out.clear();
+  for (uint32_t g = 0; g < d.max_groups; ++g)
+    if (d.groups[g].live && !d.groups[g].active) out.push_back((int)g);
+}
+
+void driver_paths(RoundDriver& d, int g, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
+                  std::vector<uint64_t>& off, std::vector<uint32_t>& cells) {
+  const Group& G = d.groups.at(g);
+  FieldRead R;
+  R.boards = d.boards;
+  R.fields = d.fields + G.field0;
+  R.labels = d.labels;
+  R.state = d.state;
+  R.tw = d.c->shape.tw;
+  R.th = d.c->shape.th;
+  reconstruct_with(*d.c, R, d.pool, G.nfields, routed, cost, off, cells, nullptr);
+}
+
+void driver_release(RoundDriver& d, int g) {
+  Group& G = d.groups.at(g);
+  if (!G.live) return;
+  d.fa.release(G.field0, G.nfields);
+  d.ia.release(G.item0, G.nitems);
+  d.la.release(G.lab0, G.labwords);
+  d.pa.release(G.pool0, G.npool);
+  G = Group{};
 }
 
 }  // namespace mrf

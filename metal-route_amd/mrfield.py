@@ -77,6 +77,23 @@ class StatsS(C.Structure):
                 ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32)]
 
 
+class ServerConfigS(C.Structure):
+    _fields_ = [("label_bytes", C.c_size_t), ("max_boards", C.c_uint32), ("max_layers", C.c_uint32),
+                ("max_inflight", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
+                ("threads", C.c_uint32)]
+
+
+class RequestStatsS(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("solve_ms", C.c_double), ("total_ms", C.c_double),
+                ("rounds", C.c_uint32), ("peers_max", C.c_uint32)]
+
+
+class ServerStatsS(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("rounds", C.c_uint64), ("tile_visits", C.c_uint64), ("kernel_ms", C.c_double),
+                ("label_bytes", C.c_size_t), ("label_bytes_peak", C.c_size_t), ("inflight_peak", C.c_uint32)]
+
+
 class SynthS(C.Structure):
     _fields_ = [("dims", Dims), ("cost", U32P), ("board_constraint", U8P), ("additive", U32P),
                 ("halo_owner", I32P), ("halo_penalty", C.c_uint32), ("x_edges", U32P), ("y_edges", U32P),
@@ -134,6 +151,16 @@ def lib():
         "mrf_solve_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, I32P,
                                        C.c_size_t, C.POINTER(WindowS), E]),
         "mrf_read_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, U32P, U32P, E]),
+        "mrf_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfigS), C.POINTER(C.c_void_p), E]),
+        "mrf_server_destroy": (None, [C.c_void_p]),
+        "mrf_server_add_board": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(EdgesS),
+                                           C.POINTER(PricingS), C.POINTER(C.c_uint32), E]),
+        "mrf_server_remove_board": (C.c_int, [C.c_void_p, C.c_uint32, E]),
+        "mrf_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(NetS), C.c_size_t, C.POINTER(WindowS),
+                                        C.c_int, I32P, C.POINTER(C.c_uint64), E]),
+        "mrf_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.POINTER(PathsS),
+                                      C.POINTER(RequestStatsS), E]),
+        "mrf_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsS)]),
         "mrf_synth_config": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(SynthS), E]),
         "mrf_synth_board": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(SynthS)]),
         "mrf_free_synth": (None, [C.POINTER(SynthS)]),
@@ -455,6 +482,74 @@ class Board:
         err = ErrorS()
         _check(lib().mrf_read_field(self.eng._h, self._h, i, _p32(dist), _p32(hops), C.byref(err)), err)
         return dist, hops
+
+
+NOT_READY = 4
+
+
+class Server:
+    """Serving loop: resident boards, HBM-budget admission, continuous batching."""
+
+    def __init__(self, device: int = 0, label_bytes: int = 0, max_boards: int = 0, max_layers: int = 0,
+                 max_inflight: int = 0, tile_w: int = 0, tile_h: int = 0, threads: int = 0):
+        self._h = C.c_void_p()
+        cfg = ServerConfigS(label_bytes, max_boards, max_layers, max_inflight, tile_w, tile_h, threads)
+        err = ErrorS()
+        _check(lib().mrf_server_create(device, C.byref(cfg), C.byref(self._h), C.byref(err)), err)
+        self._boards = {}
+
+    def close(self):
+        if self._h:
+            lib().mrf_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_board(self, grid: Grid, edges: Edges, pricing: Optional[Pricing] = None) -> int:
+        g = grid.cstruct()
+        e = edges.cstruct()
+        pr = pricing.cstruct() if pricing is not None else None
+        bid = C.c_uint32()
+        err = ErrorS()
+        _check(lib().mrf_server_add_board(self._h, grid.dims(), C.byref(g), C.byref(e),
+                                          C.byref(pr) if pr else None, C.byref(bid), C.byref(err)), err)
+        return int(bid.value)
+
+    def remove_board(self, board_id: int) -> None:
+        err = ErrorS()
+        _check(lib().mrf_server_remove_board(self._h, board_id, C.byref(err)), err)
+
+    def submit(self, board_id: int, nets, windows=None, retry_full=True, net_ids=None) -> int:
+        arr, keep = _nets(nets)
+        ids = np.ascontiguousarray(np.asarray(list(net_ids), dtype=np.int32)) if net_ids is not None else None
+        rid = C.c_uint64()
+        err = ErrorS()
+        _check(lib().mrf_server_submit(self._h, board_id, arr, len(nets), _windows(windows, len(nets)),
+                                       1 if retry_full else 0, ids.ctypes.data_as(I32P) if ids is not None else None,
+                                       C.byref(rid), C.byref(err)), err)
+        return int(rid.value)
+
+    def wait(self, request_id: int, timeout_ms: int = -1, arrays: bool = False):
+        """(paths, stats); paths as solve() returns them (or a PathBatch when
+        arrays is true).  None when the request has not finished in time."""
+        out = PathsS()
+        st = RequestStatsS()
+        err = ErrorS()
+        rc = lib().mrf_server_wait(self._h, request_id, timeout_ms, C.byref(out), C.byref(st), C.byref(err))
+        if rc == NOT_READY:
+            return None
+        _check(rc, err)
+        paths = _take_arrays(out) if arrays else _take_paths(out)
+        return paths, {k: getattr(st, k) for k, _ in RequestStatsS._fields_}
+
+    def stats(self) -> dict:
+        s = ServerStatsS()
+        lib().mrf_server_get_stats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStatsS._fields_}
 
 
 @dataclass

@@ -37,6 +37,16 @@ def test_open_without_device_fails_closed():
     assert e.value.code == mrfield.BACKEND_UNAVAILABLE
 
 
+def test_server_without_device_fails_closed():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    with pytest.raises(mrfield.MrfError) as e:
+        mrfield.Server(0)
+    assert e.value.code == mrfield.BACKEND_UNAVAILABLE
+
+
 def _blocked_fraction(s):
     return float((s.grid.cost == mrfield.MAX).mean())
 

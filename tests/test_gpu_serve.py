@@ -1,0 +1,165 @@
+"""GPU: the serving loop (resident boards, HBM-budget admission, continuous
+batching) through the C ABI.
+
+Every request's paths must equal the CPU oracle's alone search for the same
+nets (window pass, then full-board retry), whatever else was on the device at
+the same time: each field is the unique fixpoint of its own relaxation, so
+solving a request alone and in a mix of other requests on other boards gives
+the same bytes.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mrcpu
+import mrfield
+
+pytestmark = pytest.mark.gpu
+
+# (side, layers, nets, seed): boards of the config-2 generator at small sizes
+BOARDS = [(64, 2, 24, 11), (96, 4, 16, 12), (128, 3, 24, 13)]
+PER_REQ = 8
+
+
+def oracle_spec(s):
+    g = s.grid
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost, board=g.board)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    return mrcpu.Spec(og, oe, additive=s.additive)
+
+
+def oracle_paths(s, nets, windows=None, retry=True):
+    g = s.grid
+    wins = windows if windows is not None else [(0, 0, g.w - 1, g.h - 1)] * len(nets)
+    on = [mrcpu.Net(n.src, n.dst) for n in nets]
+    paths, costs = mrcpu.alone_batch(oracle_spec(s), on, wins, retry_full=retry, threads=8)
+    return [None if p is None else (p, c) for p, c in zip(paths, costs)]
+
+
+@pytest.fixture(scope="module")
+def boards():
+    return [mrfield.synth_board(side, L, n, seed) for side, L, n, seed in BOARDS]
+
+
+@pytest.fixture(scope="module")
+def requests(boards):
+    """(board index, nets) in a fixed interleaved order across boards."""
+    per_board = []
+    for bi, s in enumerate(boards):
+        nets = s.nets()
+        per_board.append([(bi, nets[k:k + PER_REQ]) for k in range(0, len(nets), PER_REQ)])
+    out = []
+    for k in range(max(len(p) for p in per_board)):
+        for p in per_board:
+            if k < len(p):
+                out.append(p[k])
+    return out
+
+
+@pytest.fixture(scope="module")
+def expected(boards, requests):
+    return [oracle_paths(boards[bi], nets) for bi, nets in requests]
+
+
+def make_server(boards, **kw):
+    srv = mrfield.Server(0, **kw)
+    ids = [srv.add_board(s.grid, s.edges, s.pricing()) for s in boards]
+    return srv, ids
+
+
+def submit_all(srv, ids, requests, **kw):
+    rids = [srv.submit(ids[bi], nets, **kw) for bi, nets in requests]
+    return [srv.wait(r) for r in rids]
+
+
+def test_requests_match_oracle(boards, requests, expected):
+    srv, ids = make_server(boards, label_bytes=1 << 30)
+    try:
+        got = submit_all(srv, ids, requests)
+        for k, ((paths, st), exp) in enumerate(zip(got, expected)):
+            assert paths == exp, k
+            assert st["total_ms"] >= st["solve_ms"] >= 0.0
+        s = srv.stats()
+        assert s["completed"] == len(requests) and s["failed"] == 0
+        assert s["inflight_peak"] >= 2  # requests from several boards relaxed together
+    finally:
+        srv.close()
+
+
+def test_alone_equals_mixed(boards, requests, expected):
+    srv, ids = make_server(boards, label_bytes=1 << 30)
+    try:
+        alone = []
+        for bi, nets in requests:
+            alone.append(srv.wait(srv.submit(ids[bi], nets))[0])
+        mixed = [p for p, _ in submit_all(srv, ids, requests)]
+        assert alone == mixed
+        assert alone == expected
+    finally:
+        srv.close()
+
+
+def test_admission_by_label_budget(boards, requests, expected):
+    # room for about one request's labels at a time
+    budget = 4 << 20
+    srv, ids = make_server(boards, label_bytes=budget)
+    try:
+        got = submit_all(srv, ids, requests)
+        assert [p for p, _ in got] == expected
+        s = srv.stats()
+        assert 0 < s["label_bytes_peak"] <= budget
+        assert s["label_bytes"] == budget
+    finally:
+        srv.close()
+
+
+def test_windows_then_full_retry(boards):
+    s = boards[2]
+    nets = s.nets()[:16]
+    g = s.grid
+    # even nets: the endpoints' bounding box; odd nets: a window holding the
+    # source column only, so their target is reachable on the full board only
+    wins = []
+    for k, n in enumerate(nets):
+        ax, ay = n.src % g.w, (n.src % (g.w * g.h)) // g.w
+        bx, by = n.dst % g.w, (n.dst % (g.w * g.h)) // g.w
+        if k % 2 == 0:
+            wins.append((min(ax, bx), min(ay, by), max(ax, bx), max(ay, by)))
+        else:
+            wins.append((ax, ay, ax, ay))
+    srv, ids = make_server([s], label_bytes=1 << 28)
+    try:
+        with_retry = srv.wait(srv.submit(ids[0], nets, windows=wins, retry_full=True))[0]
+        one_pass = srv.wait(srv.submit(ids[0], nets, windows=wins, retry_full=False))[0]
+    finally:
+        srv.close()
+    assert with_retry == oracle_paths(s, nets, wins, retry=True)
+    assert one_pass == oracle_paths(s, nets, wins, retry=False)
+    assert sum(p is None for p in one_pass) > sum(p is None for p in with_retry)
+
+
+def test_errors_fail_closed(boards):
+    s = boards[0]
+    # room for one 64x64x2 field (74 KB of labels) but not for eight
+    srv, ids = make_server([s], label_bytes=256 << 10)
+    try:
+        with pytest.raises(mrfield.MrfError):
+            srv.submit(ids[0] + 100, s.nets()[:1])
+        bad = mrfield.Net(int(np.flatnonzero(s.grid.cost == mrfield.MAX)[0]), s.nets()[0].dst)
+        with pytest.raises(mrfield.MrfError) as e:
+            srv.submit(ids[0], [bad])
+        assert e.value.code == mrfield.INVALID_ENDPOINT
+        # a request whose labels exceed the whole budget fails on its own ...
+        big = srv.submit(ids[0], s.nets()[:8])
+        with pytest.raises(mrfield.MrfError) as e:
+            srv.wait(big)
+        assert e.value.code == mrfield.BACKEND_UNAVAILABLE
+        # ... and the server keeps serving requests that fit
+        small = srv.wait(srv.submit(ids[0], s.nets()[:1]))[0]
+        assert small == oracle_paths(s, s.nets()[:1])
+        srv.remove_board(ids[0])
+        with pytest.raises(mrfield.MrfError):
+            srv.submit(ids[0], s.nets()[:1])
+    finally:
+        srv.close()
