@@ -39,7 +39,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["fields", "serve"], default="fields",
+                    help="fields: one batch per step of --config (default, the headline); "
+                         "serve: Poisson request mix through the serving loop vs one request at a time")
     ap.add_argument("--config", type=int, default=2, help="synthetic config (2 = 512x512x4, 256 nets)")
+    ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
     ap.add_argument("--tiles", default="16,16,256,32", help="tw,th,threads,max_local")
@@ -105,6 +109,38 @@ def cpu_baseline(s, sample, got_paths):
     }
 
 
+def serve_main(a, tiles):
+    import serve_bench
+
+    t0 = time.perf_counter()
+    r = serve_bench.run(seed=a.seed, n_req=a.serve_requests, tiles=tiles)
+    top = r["rows"][-1]
+    out = {
+        "metric": "serving nets/sec, Poisson request mix on resident boards (sides 128-1024, 2-8 layers, "
+                  "16-512 nets); p50/p99 latency",
+        "value": top["serve"]["nets_s"],
+        "unit": "nets/s",
+        "n_gpus": 1,
+        "steps": r["requests"],
+        "warmup": 1,
+        "ms_per_step": round(r["plain_mean_service_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{r['requests']} requests over {len(r['boards'])} resident boards, "
+                               f"{r['nets']} nets, Poisson arrivals at 0.5x-4x the one-request-at-a-time "
+                               f"capacity (seed {a.seed})", "boards": r["boards"], "tiles": a.tiles},
+        "rows": r["rows"],
+        "speedup_vs_plain_at_top_load": round(top["serve"]["nets_s"] / top["plain"]["nets_s"], 3),
+        "paths_equal_plain": r["paths_equal_plain"],
+        "server": r["server"],
+        "wall_s": round(time.perf_counter() - t0, 1),
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -119,6 +155,9 @@ def main():
     import mrfield
 
     tw, th, threads, ml = (int(v) for v in a.tiles.split(","))
+    if a.workload == "serve":
+        serve_main(a, (tw, th, threads, ml))
+        return
     eng = mrfield.Engine(0)
     eng.set_tiles(tw, th, threads, ml)
     eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)

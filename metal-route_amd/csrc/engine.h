@@ -276,6 +276,7 @@ struct Group {
   uint64_t item0 = 0, nitems = 0;
   uint64_t lab0 = 0, labwords = 0;
   uint64_t pool0 = 0, npool = 0;
+  uint32_t layers = 0;  // most layers of any of its boards
   bool active = false;  // has an item on the current active list
 };
 

@@ -1807,6 +1807,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   }
   G.nfields = reqs.size();
   G.nitems = nitems;
+  for (const FieldReq& q : reqs) G.layers = std::max(G.layers, d.views[q.board].layers);
   G.labwords = words;
   G.npool = npool;
   std::vector<FieldDesc> descs(reqs.size());
@@ -1911,8 +1912,13 @@ uint32_t driver_step(RoundDriver& d) {
   A.th = S.th;
   A.max_local = S.max_local;
   A.spin_ticks = c.spin_ticks;
-  const KernelSet& K = pick_kernels(S.tw, S.th, d.max_layers);
-  const size_t lds = lds_bytes(S.tw, S.th, d.max_layers);
+  // The tile kernel and its LDS are sized for the most layers of any live
+  // group's board, not for the server's maximum.
+  uint32_t Lmax = 1;
+  for (uint32_t g = 0; g < d.max_groups; ++g)
+    if (d.groups[g].live) Lmax = std::max(Lmax, d.groups[g].layers);
+  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax);
+  const size_t lds = lds_bytes(S.tw, S.th, Lmax);
   void* args[] = {&A};
   HIP_OK(hipEventRecord(c.ws.ek0, s));
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
