@@ -1,22 +1,26 @@
 """Benchmark: nets/s (complete fields + canonical paths, bit-exact) on the
 512x512x4 board with 256 two-pin nets, plus p50/p99 per-batch solve latency.
 
-One step = one batch of 256 nets solved on a board resident in HBM: complete
-u64 (dist, hops) label fields over the full board for every net, canonical
-path reconstruction, and the host re-validation of every path.  Inputs (board
-planes) are uploaded once before the timed region.
+One step = one batch of nets solved on a board resident in HBM: complete
+u64 (dist, hops) label fields for every net (full board, or the net's window
+for --config 3), canonical path reconstruction, and the host re-validation of
+every path.  Inputs (board planes) are uploaded once before the timed region.
+--config 4 solves multi-source fields (one per multi-pin net, no paths).
 
 Output: one JSON line (see the keys below).  ``roofline`` is per launch of
 the relaxation kernel (relax_round in the plain form): algorithmic bytes =
-19 B per cell visit (8 B label read + 8 B label write + 2 B history + 1 B
-mask) times the cell visits of one launch (device counter of tile loads x
-tile cells, averaged over the step's launches), divided by the kernel's
-average launch duration from HIP events recorded around every launch on the
-engine's stream.  ``traffic`` is the measured HBM bytes per launch from a
-committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated
+B per cell visit (8 B label read + 8 B label write + the per-cell planes of
+the workload: 1 B mask, + 2 B u16 history for configs 2 and 3, + 4 B u32
+history for config 4) times the cell visits of one launch (device counter of
+tile loads x tile cells, averaged over the step's launches), divided by the
+kernel's average launch duration from HIP events recorded around every launch
+on the engine's stream.  ``traffic`` is the measured HBM bytes per launch
+from a committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated
 against a copy of known size).  ``cpu_baseline`` times the CPU oracle (a C
 restatement of mr-cpu, not the Rust reference) on a bounded sample of the
-same nets.
+same nets on the host cores.
+
+--workload serve runs the serving comparison instead (serve_bench.py).
 """
 from __future__ import annotations
 
@@ -31,7 +35,8 @@ sys.path.insert(0, os.path.join(ROOT, "metal-route_amd"))
 
 METRIC = "nets/sec (fields+paths, bit-exact) at 512²×4L; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_VISIT = 19
+BYTES_PER_CELL_VISIT = {0: 17, 1: 17, 2: 19, 3: 19, 4: 21}
+CPU_SECONDS = 10.0
 
 
 def parse():
@@ -42,12 +47,15 @@ def parse():
     ap.add_argument("--workload", choices=["fields", "serve"], default="fields",
                     help="fields: one batch per step of --config (default, the headline); "
                          "serve: Poisson request mix through the serving loop vs one request at a time")
-    ap.add_argument("--config", type=int, default=2, help="synthetic config (2 = 512x512x4, 256 nets)")
-    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--config", type=int, default=2,
+                    help="synthetic config: 1 = 128x128x2/64 nets, 2 = 512x512x4/256 nets (headline), "
+                         "3 = 2048x2048x2/1024 windowed nets, 4 = 1024x1024x6/512 multi-pin fields")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
     ap.add_argument("--tiles", default="16,16,256,32", help="tw,th,threads,max_local")
-    ap.add_argument("--cpu-sample", type=int, default=16, help="nets in the CPU baseline sample")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick tiles / round cap / form with the library's autotuner (always on for config 3)")
+    ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (rocprof counter calibration)")
@@ -79,33 +87,88 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(s, sample, got_paths):
-    """Time the CPU oracle on `sample` nets of the same workload (fields + paths)."""
+def oracle_spec(s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import mrcpu
 
     g = s.grid
     og = mrcpu.Grid(g.w, g.h, g.layers, g.cost, board=g.board)
     oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
-    spec = mrcpu.Spec(og, oe, additive=s.additive)
-    nets = [mrcpu.Net(int(a), int(b)) for a, b in zip(s.src[:sample], s.dst[:sample])]
-    threads = min(os.cpu_count() or 1, 16, sample)
+    return mrcpu, mrcpu.Spec(og, oe, additive=s.additive, halo_owner=s.halo_owner, halo_penalty=s.halo_penalty)
+
+
+def cpu_threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def cpu_baseline_paths(s, got):
+    """The oracle's alone search (window, then full-board retry) over the
+    step's nets in chunks on the host cores until about CPU_SECONDS; the first
+    chunk's paths are compared with the GPU's."""
+    mrcpu, spec = oracle_spec(s)
+    g = s.grid
+    nets = s.nets()
+    wins = s.windows or [(0, 0, g.w - 1, g.h - 1)] * len(nets)
+    threads = cpu_threads()
+    chunk = max(threads, 16)
+    done, exact, k = 0, True, 0
     t0 = time.perf_counter()
-    dist, hops = mrcpu.fields_batch(spec, nets, [[n.src] for n in nets], threads=threads)
-    paths = [mrcpu.reconstruct(spec, dist[i], hops[i], n.dst) for i, n in enumerate(nets)]
+    while True:
+        idx = [(k + j) % len(nets) for j in range(chunk)]
+        on = [mrcpu.Net(nets[i].src, nets[i].dst) for i in idx]
+        paths, _ = mrcpu.alone_batch(spec, on, [wins[i] for i in idx], retry_full=True, threads=threads)
+        if k == 0:
+            for i, p in zip(idx, paths):
+                gp = got.path(i)
+                exact = exact and ((p is None and gp is None) or (gp is not None and p == gp[0]))
+        done += len(idx)
+        k += chunk
+        if time.perf_counter() - t0 >= CPU_SECONDS:
+            break
     dt = time.perf_counter() - t0
-    exact = True
-    for i, p in enumerate(paths):
-        gp = got_paths.path(i)
-        exact = exact and ((p is None and gp is None) or (gp is not None and p == gp[0]))
     return {
-        "value": round(sample / dt, 3),
+        "value": round(done / dt, 3),
         "unit": "nets/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample} nets of the same board: complete lexicographic (dist,hops) Dijkstra fields "
-                  f"+ canonical reconstruction, {threads} threads, {dt:.2f} s",
+        "sample": f"{done} nets of the same workload in chunks of {chunk}: alone search (complete "
+                  f"lexicographic labels, window then full-board retry) + canonical path, {threads} threads, "
+                  f"{dt:.1f} s",
         "paths_bit_exact_vs_gpu": bool(exact),
+    }
+
+
+def cpu_baseline_fields(s, board):
+    """The oracle's complete multi-source fields for nets in chunks until about
+    CPU_SECONDS; the first chunk's dist and hops planes are compared with the
+    GPU's fields."""
+    import numpy as np
+
+    mrcpu, spec = oracle_spec(s)
+    threads = cpu_threads()
+    pins = s.pins
+    done, exact, k = 0, True, 0
+    t0 = time.perf_counter()
+    while True:
+        idx = [(k + j) % len(pins) for j in range(threads)]
+        nets = [mrcpu.Net(0, 0xFFFFFFFF, own_id=i) for i in idx]  # no target: sources only
+        dist, hops = mrcpu.fields_batch(spec, nets, [pins[i] for i in idx], threads=threads)
+        if k == 0:
+            for j, i in enumerate(idx[:2]):
+                gd, gh = board.read_field(i)
+                exact = exact and np.array_equal(gd, dist[j]) and np.array_equal(gh, hops[j])
+        done += len(idx)
+        k += len(idx)
+        if time.perf_counter() - t0 >= CPU_SECONDS:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(done / dt, 3),
+        "unit": "fields/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} multi-source fields of the same board, {threads} threads, {dt:.1f} s",
+        "fields_bit_exact_vs_gpu": bool(exact),
     }
 
 
@@ -163,9 +226,23 @@ def main():
     eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)
     s = mrfield.synth_config(a.config, seed=a.seed)
     g = s.grid
-    board = mrfield.Board(eng, g, s.edges, s.pricing())
+    multi = s.pins is not None
+    board = mrfield.Board(eng, g, s.edges, s.pricing(list(range(len(s.pins))) if multi else None))
     nets = s.nets()
     wins = s.windows
+    tuning = None
+    form, tiles = a.mode, a.tiles
+    if (a.autotune or a.config == 3) and not multi:
+        tuning = board.autotune(nets, windows=wins, sample=64)
+        form = "graph" if tuning["mode"] == mrfield.MODE_GRAPH else "plain"
+        tw, th = tuning["tile_w"], tuning["tile_h"]
+        tiles = f"{tw},{th},{tuning['threads']},{tuning['max_local']}"
+
+    def step():
+        if multi:
+            board.solve_fields(s.pins, net_ids=list(range(len(s.pins))))
+            return None
+        return board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
     kern_ms, launches = [], []
@@ -174,7 +251,7 @@ def main():
         if i == a.warmup:
             t_start = time.perf_counter()
         t0 = time.perf_counter()
-        got = board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
+        got = step()
         t1 = time.perf_counter()
         if i >= a.warmup:
             st = eng.stats()
@@ -188,19 +265,30 @@ def main():
             kern_ms.append(st["relax_kernel_ms"])
             launches.append(st["relax_launches"])
     wall = time.perf_counter() - t_start
-    n_nets = len(nets)
+    n_nets = len(s.pins) if multi else len(nets)
     value = n_nets * a.steps / wall
-    routed = int(np.count_nonzero(got.routed))
+    routed = int(np.count_nonzero(got.routed)) if got is not None else None
+    bpc = BYTES_PER_CELL_VISIT[a.config]
     # per launch of the relaxation kernel: algorithmic bytes / average duration
     n_launch = float(np.mean(launches))
     cells_per_launch = float(np.mean(cells_visited)) / max(n_launch, 1.0)
     avg_launch_ms = float(np.mean(kern_ms)) / max(n_launch, 1.0)
-    achieved = cells_per_launch * BYTES_PER_CELL_VISIT / (avg_launch_ms * 1e-3) / 1e9
+    achieved = cells_per_launch * bpc / (avg_launch_ms * 1e-3) / 1e9
     workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
+    kernel = "relax_list" if form == "graph" else "relax_round"
+    if a.config == 2:
+        metric, desc = METRIC, "two-pin nets, complete fields + canonical paths per step"
+    elif multi:
+        metric = f"fields/sec (multi-source, complete, bit-exact) at {g.w}x{g.h}x{g.layers}"
+        desc = "multi-pin nets, one complete multi-source field per net per step (no paths)"
+    else:
+        metric = f"nets/sec (fields+paths, bit-exact) at {g.w}x{g.h}x{g.layers}; p50/p99 solve latency"
+        desc = ("two-pin nets in their windows (full-board retry), fields + canonical paths per step"
+                if wins is not None else "two-pin nets, complete fields + canonical paths per step")
     out = {
-        "metric": METRIC,
+        "metric": metric,
         "value": round(value, 3),
-        "unit": "nets/s",
+        "unit": "fields/s" if multi else "nets/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
@@ -211,10 +299,10 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"{g.w}x{g.h}x{g.layers} board, {n_nets} two-pin nets, complete fields + canonical "
-                        f"paths per step (synthetic config {a.config}, seed {a.seed})",
-            "form": a.mode,
-            "tiles": a.tiles,
+            "workload": f"{g.w}x{g.h}x{g.layers} board, {n_nets} {desc} (synthetic config {a.config}, "
+                        f"seed {a.seed})",
+            "form": form,
+            "tiles": tiles,
         },
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -234,15 +322,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic("relax_list" if a.mode == "graph" else "relax_round", workload),
-            "kernel": "relax_list" if a.mode == "graph" else "relax_round",
-            "bytes_per_cell_visit": BYTES_PER_CELL_VISIT,
+            "traffic": load_traffic(kernel, workload),
+            "kernel": kernel,
+            "bytes_per_cell_visit": bpc,
             "cell_visits_per_launch": int(cells_per_launch),
         },
         "cpu_baseline": None,
     }
+    if tuning is not None:
+        out["autotune"] = tuning
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, min(a.cpu_sample, n_nets), got)
+        out["cpu_baseline"] = cpu_baseline_fields(s, board) if multi else cpu_baseline_paths(s, got)
     board.close()
     eng.close()
     print(json.dumps(out))

@@ -233,6 +233,28 @@ int mrf_solve_fields(mrf_ctx *ctx, mrf_board *board, const uint32_t *const *sour
 int mrf_read_field(mrf_ctx *ctx, mrf_board *board, size_t field, uint32_t *dist_out,
                    uint32_t *hops_out, mrf_error *err);
 
+/* ---- autotuner -----------------------------------------------------------------
+ * Picks the tile shape, workgroup size, in-LDS round cap (sweeps per tile
+ * visit) and solve form (plain rounds, or hipGraph replays of `graph_rounds`
+ * rounds: the work-list granularity) for a workload by measuring candidates
+ * on a sample of its nets on a resident board.  No reference counterpart: the
+ * Metal engine uses fixed threadgroup sizes (crates/mr-metal/src/gpu.rs). */
+typedef struct {
+    uint32_t tile_w, tile_h, threads, max_local;
+    int mode;               /* MRF_MODE_PLAIN or MRF_MODE_GRAPH */
+    uint32_t graph_rounds;  /* rounds per hipGraph replay (graph form) */
+    double sample_ms;       /* wall time of the chosen configuration on the sample */
+    double default_ms;      /* wall time of the context's configuration before tuning */
+    uint32_t tried;         /* configurations measured */
+} mrf_tuning;
+
+/* Measures candidates on every k-th net (at most `sample` nets, windows as in
+ * mrf_solve with the full-board retry), keeps the fastest on the context and
+ * reports it.  The context's results are the same for every configuration;
+ * only the time differs. */
+int mrf_autotune(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets, const mrf_window *windows,
+                 size_t sample, mrf_tuning *out, mrf_error *err);
+
 /* ---- serving loop ------------------------------------------------------------
  * A server keeps board planes resident, queues solve requests from any thread
  * and runs one relaxation on a worker thread.  Requests are admitted in

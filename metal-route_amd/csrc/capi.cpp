@@ -2,6 +2,7 @@
 // adapter's window -> full-board retry, chunking by the label budget, and the
 // fail-closed host re-validation of every returned path.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -579,7 +580,6 @@ int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, c
     if (windows) wins.assign(windows, windows + n_nets);
     Batch all;
     solve_nets(ctx->c, bd->b, g, nets, idx, wins, false, nullptr, all);
-    mrf::SolveStats first = ctx->c.last;
     if (retry_full && windows) {
       std::vector<size_t> retry_idx;
       for (size_t i = 0; i < n_nets; ++i)
@@ -594,15 +594,6 @@ int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, c
           all.cost[i] = second.cost[k];
           all.paths[i] = std::move(second.paths[k]);
         }
-        mrf::SolveStats& s = ctx->c.last;
-        s.relax_ms += first.relax_ms;
-        s.path_ms += first.path_ms;
-        s.tile_visits += first.tile_visits;
-        s.local_rounds += first.local_rounds;
-        s.cells_visited += first.cells_visited;
-        s.field_cells += first.field_cells;
-        s.rounds += first.rounds;
-        s.persistent_ms += first.persistent_ms;
       }
     }
     export_paths(all, out);
@@ -613,6 +604,102 @@ int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, c
     return fail(err, e);
   } catch (const std::exception& e) {
     mrf_free_paths(out);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
+                 size_t sample, mrf_tuning* out, mrf_error* err) {
+  if (!ctx || !bd || !out || bd->ctx != ctx) return MRF_BACKEND_UNAVAILABLE;
+  memset(out, 0, sizeof(*out));
+  try {
+    Lane lane(ctx->c);
+    HostGrid g = board_host(bd);
+    const mrf_dims dims = bd->dims;
+    if (n_nets == 0 || !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "autotune needs nets");
+    for (size_t i = 0; windows && i < n_nets; ++i) check_window(dims, windows[i], i);
+    for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
+    // every k-th net
+    const size_t want = std::max<size_t>(1, std::min(sample ? sample : 64, n_nets));
+    const size_t step = n_nets / want;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < n_nets && idx.size() < want; i += std::max<size_t>(step, 1)) idx.push_back(i);
+    const mrf_window full{0, 0, dims.w - 1, dims.h - 1};
+    std::vector<mrf_window> wins;
+    for (size_t i : idx) wins.push_back(windows ? windows[i] : full);
+    mrf::Ctx& c = ctx->c;
+    auto run = [&]() {
+      auto t0 = std::chrono::steady_clock::now();
+      Batch b;
+      solve_nets(c, bd->b, g, nets, idx, wins, false, nullptr, b);
+      if (windows) {
+        std::vector<size_t> retry;
+        for (size_t k = 0; k < idx.size(); ++k)
+          if (!b.routed[k]) retry.push_back(idx[k]);
+        if (!retry.empty()) {
+          Batch b2;
+          solve_nets(c, bd->b, g, nets, retry, std::vector<mrf_window>(retry.size(), full), false, nullptr, b2);
+        }
+      }
+      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    struct Cand {
+      uint32_t tw, th, threads, max_local;
+      int mode;
+      uint32_t graph_rounds;
+    };
+    const mrf::TileShape keep_shape = c.shape;
+    const int keep_mode = c.mode == MRF_MODE_PERSISTENT ? MRF_MODE_PLAIN : c.mode;
+    const uint32_t keep_rounds = c.graph_rounds;
+    auto apply = [&](const Cand& k) {
+      c.shape.tw = k.tw;
+      c.shape.th = k.th;
+      c.shape.threads = k.threads;
+      c.shape.max_local = k.max_local;
+      c.mode = k.mode;
+      c.graph_rounds = k.graph_rounds;
+    };
+    auto timed = [&](const Cand& k) {
+      apply(k);
+      run();  // warm-up: workspace growth, graph capture
+      return std::min(run(), run());
+    };
+    Cand base{keep_shape.tw, keep_shape.th, keep_shape.threads, keep_shape.max_local, keep_mode, keep_rounds};
+    double best_ms = timed(base);
+    out->default_ms = best_ms;
+    Cand best = base;
+    uint32_t tried = 1;
+    const uint32_t shapes[][2] = {{32, 16}, {16, 16}, {16, 8}, {8, 4}};
+    for (auto& sh : shapes) {
+      if (mrf::lds_bytes_for(sh[0], sh[1], dims.layers) > 160 * 1024 - 64) continue;
+      for (uint32_t threads : {128u, 256u})
+        for (uint32_t ml : {4u, 32u})
+          for (int mode : {(int)MRF_MODE_PLAIN, (int)MRF_MODE_GRAPH})
+            for (uint32_t gr : mode == MRF_MODE_GRAPH ? std::vector<uint32_t>{4, 16} : std::vector<uint32_t>{8}) {
+              Cand k{sh[0], sh[1], threads, ml, mode, gr};
+              double ms = timed(k);
+              ++tried;
+              if (ms < best_ms) {
+                best_ms = ms;
+                best = k;
+              }
+            }
+    }
+    apply(best);
+    out->tile_w = best.tw;
+    out->tile_h = best.th;
+    out->threads = best.threads;
+    out->max_local = best.max_local;
+    out->mode = best.mode;
+    out->graph_rounds = best.graph_rounds;
+    out->sample_ms = best_ms;
+    out->tried = tried;
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  } catch (const std::exception& e) {
     set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
     return MRF_BACKEND_UNAVAILABLE;
   }

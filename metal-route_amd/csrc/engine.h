@@ -244,6 +244,7 @@ void ctx_init(Ctx& c, int device);
 void ctx_free(Ctx& c);
 size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t layers);
 bool tile_shape_supported(uint32_t tw, uint32_t th);
+size_t lds_bytes_for(uint32_t tw, uint32_t th, uint32_t layers);  // LDS of one tile visit
 
 // ---- round driver: fields join and leave a running relaxation -------------
 //

@@ -1126,6 +1126,8 @@ static const std::vector<KernelSet>& kernel_sets() {
   return v;
 }
 
+size_t lds_bytes_for(uint32_t tw, uint32_t th, uint32_t layers) { return lds_bytes(tw, th, layers); }
+
 bool tile_shape_supported(uint32_t tw, uint32_t th) {
   for (const KernelSet& k : kernel_sets())
     if (k.tw == tw && k.th == th) return true;

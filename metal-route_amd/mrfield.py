@@ -77,6 +77,12 @@ class StatsS(C.Structure):
                 ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32)]
 
 
+class TuningS(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("threads", C.c_uint32), ("max_local", C.c_uint32),
+                ("mode", C.c_int), ("graph_rounds", C.c_uint32), ("sample_ms", C.c_double),
+                ("default_ms", C.c_double), ("tried", C.c_uint32)]
+
+
 class ServerConfigS(C.Structure):
     _fields_ = [("label_bytes", C.c_size_t), ("max_boards", C.c_uint32), ("max_layers", C.c_uint32),
                 ("max_inflight", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
@@ -151,6 +157,8 @@ def lib():
         "mrf_solve_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, I32P,
                                        C.c_size_t, C.POINTER(WindowS), E]),
         "mrf_read_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, U32P, U32P, E]),
+        "mrf_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t, C.POINTER(WindowS),
+                                   C.c_size_t, C.POINTER(TuningS), E]),
         "mrf_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfigS), C.POINTER(C.c_void_p), E]),
         "mrf_server_destroy": (None, [C.c_void_p]),
         "mrf_server_add_board": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(EdgesS),
@@ -459,6 +467,16 @@ class Board:
 
     def solve(self, nets, windows=None, retry_full=True):
         return _take_paths(self._solve(nets, windows, retry_full))
+
+    def autotune(self, nets, windows=None, sample=64) -> dict:
+        """Measure tile shapes / workgroup sizes / round caps / forms on a
+        sample of `nets`; the fastest stays set on the engine."""
+        arr, keep = _nets(nets)
+        t = TuningS()
+        err = ErrorS()
+        _check(lib().mrf_autotune(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)), sample,
+                                  C.byref(t), C.byref(err)), err)
+        return {k: getattr(t, k) for k, _ in TuningS._fields_}
 
     def solve_arrays(self, nets, windows=None, retry_full=True) -> PathBatch:
         """Like solve(), with the paths left as flat numpy arrays."""
