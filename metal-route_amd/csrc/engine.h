@@ -83,6 +83,8 @@ struct DevBoard {
   uint32_t* ye = nullptr;
   uint32_t* ve = nullptr;
   std::vector<uint8_t> via_allowed;  // host copy
+  std::vector<uint8_t> obst;         // host copy: cost == kMax per cell
+  uint64_t max_step = 0;  // bound on any step price (obstacles priced at weight 1, as own pads)
   size_t bytes = 0;
   BoardView view() const;
 };
@@ -273,6 +275,7 @@ class RangeAlloc {
 
 struct Group {
   bool live = false;
+  bool narrow = false;  // every step price fits the narrow step planes
   uint64_t field0 = 0, nfields = 0;
   uint64_t item0 = 0, nitems = 0;
   uint64_t lab0 = 0, labwords = 0;
@@ -299,6 +302,7 @@ struct RoundDriver {
   uint32_t* pinned = nullptr;  // counters + group_active readback
   RangeAlloc fa, ia, la, pa;   // fields, items, label words, pool words
   std::vector<BoardView> views;
+  std::vector<const DevBoard*> devs;  // caller-owned boards, by slot
   std::vector<uint8_t> board_used;
   std::vector<Group> groups;
   std::vector<uint32_t> pending_seeds;
@@ -314,6 +318,7 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
                  uint32_t max_groups);
 void driver_free(RoundDriver& d);
 size_t driver_device_bytes(const RoundDriver& d);
+// The board must outlive its slot (driver_remove_board).
 int driver_add_board(RoundDriver& d, const DevBoard& b);  // slot, or -1 when the table is full
 void driver_remove_board(RoundDriver& d, int slot);
 // Label words a group of fields needs (FieldReq::board is a driver slot).

@@ -32,8 +32,6 @@
 //   relax_list        the same visits as a grid-stride loop over a device
 //                     work list; rounds captured in a hipGraph with
 //                     advance_round between them (graph form).
-//   relax_persistent  resident grid with a device barrier between rounds;
-//                     disabled on the host side.
 //   path_lengths / walk_paths
 //                     canonical reconstruction: from dst, the lowest-CellIdx
 //                     predecessor p with label(p) + (step(p, c), 1) == label(c)
@@ -47,6 +45,7 @@
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
+#include <type_traits>
 
 #include "engine.h"
 
@@ -287,21 +286,26 @@ __host__ __device__ inline uint64_t tile_block_words(uint32_t L, uint32_t tw, ui
 //   lab   u64 labels [L][TH+2][RS] with a one-cell halo ring; RS = TW+3 is odd
 //         so lanes that each own a row hit distinct banks.
 //   sxl / sxr / syt / syb / svd / svu
-//         u32 step prices [L][TH][PS] for entering each cell from x-1, x+1,
-//         y-1, y+1, layer-1, layer+1 (kMax = closed).  Computed once per
-//         visit while the tile loads; the relaxation rounds only read them.
-//         PS = TW+1 is odd.
+//         step prices [L][TH][PS] for entering each cell from x-1, x+1, y-1,
+//         y+1, layer-1, layer+1.  Computed once per visit while the tile
+//         loads; the relaxation rounds only read them.  PS = TW+1 is odd.
+//         Wide form: u32 with kMax = closed.  Narrow form (boards whose every
+//         step price is below 0x7FFF): i16 with -1 = closed, so a
+//         sign-extending load yields the same u32 value (kMax when closed)
+//         from half the LDS.
 //   rchg / cchg / vchg
 //         per-line change masks (row lines, column lines, via columns): bit i
 //         set when cell i of the line improved during this visit.
+template <bool NARROW>
 struct Lds {
+  using Step = typename std::conditional<NARROW, int16_t, uint32_t>::type;
   uint64_t* lab;
-  uint32_t* sxl;
-  uint32_t* sxr;
-  uint32_t* syt;
-  uint32_t* syb;
-  uint32_t* svd;
-  uint32_t* svu;
+  Step* sxl;
+  Step* sxr;
+  Step* syt;
+  Step* syb;
+  Step* svd;
+  Step* svu;
   uint32_t* rchg;
   uint32_t* cchg;
   uint32_t* vchg;
@@ -311,25 +315,36 @@ struct Lds {
   uint32_t* misc;
 };
 
+// Largest step price the narrow form holds (0x7FFF is reserved).
+constexpr uint32_t kNarrowMaxStep = 0x7FFE;
+
+__device__ __forceinline__ uint32_t step_of(uint32_t v) { return v; }
+__device__ __forceinline__ uint32_t step_of(int16_t v) { return (uint32_t)(int32_t)v; }
+template <typename ST>
+__device__ __forceinline__ ST to_step(uint32_t v) {
+  return (ST)v;  // narrow: kMax -> -1, prices <= kNarrowMaxStep unchanged
+}
+
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L) {
+__host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L, bool narrow) {
   const size_t rs = tw + 3, rh = th + 2, ps = tw + 1;
   const size_t ncell = (size_t)L * th * ps, nlab = (size_t)L * rh * rs;
   const size_t nstep = L > 1 ? 6 : 4;
   size_t s = 0;
   s += align16(nlab * 8);
-  s += nstep * align16(ncell * 4);
+  s += nstep * align16(ncell * (narrow ? 2 : 4));
   s += align16((size_t)L * th * 4) + align16((size_t)L * tw * 4) + align16((size_t)th * tw * 4);
   s += align16((tw + 1) * 4) + align16((th + 1) * 4) + align16(16 * 4) + 64;
   return s;
 }
 
-template <int TW, int TH>
-__device__ inline Lds carve(char* base, uint32_t L) {
+template <int TW, int TH, bool NARROW>
+__device__ inline Lds<NARROW> carve(char* base, uint32_t L) {
+  using ST = typename Lds<NARROW>::Step;
   constexpr size_t RS = TW + 3, RH = TH + 2, PS = TW + 1;
   const size_t ncell = (size_t)L * TH * PS, nlab = (size_t)L * RH * RS;
-  Lds s;
+  Lds<NARROW> s;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     char* p = base + o;
@@ -337,13 +352,13 @@ __device__ inline Lds carve(char* base, uint32_t L) {
     return p;
   };
   s.lab = (uint64_t*)take(nlab * 8);
-  s.sxl = (uint32_t*)take(ncell * 4);
-  s.sxr = (uint32_t*)take(ncell * 4);
-  s.syt = (uint32_t*)take(ncell * 4);
-  s.syb = (uint32_t*)take(ncell * 4);
+  s.sxl = (ST*)take(ncell * sizeof(ST));
+  s.sxr = (ST*)take(ncell * sizeof(ST));
+  s.syt = (ST*)take(ncell * sizeof(ST));
+  s.syb = (ST*)take(ncell * sizeof(ST));
   if (L > 1) {
-    s.svd = (uint32_t*)take(ncell * 4);
-    s.svu = (uint32_t*)take(ncell * 4);
+    s.svd = (ST*)take(ncell * sizeof(ST));
+    s.svu = (ST*)take(ncell * sizeof(ST));
   } else {
     s.svd = s.svu = nullptr;
   }
@@ -384,7 +399,8 @@ __device__ __forceinline__ uint32_t gate(bool open, uint32_t e, uint32_t w, uint
 // Per-phase change flags.  Phase k writes slot k & 1 with stamp k + 1; the
 // slot is next written in phase k + 2, after the barrier that ends phase
 // k + 1, so every thread reads phase k's flag before anyone overwrites it.
-__device__ __forceinline__ bool phase_changed(const Lds& s, bool ch, uint32_t k) {
+template <bool NARROW>
+__device__ __forceinline__ bool phase_changed(const Lds<NARROW>& s, bool ch, uint32_t k) {
   if (ch) s.misc[8 + (k & 1)] = k + 1;
   __syncthreads();
   return uniform_u32(s.misc[8 + (k & 1)]) == k + 1;
@@ -395,8 +411,8 @@ __device__ __forceinline__ bool phase_changed(const Lds& s, bool ch, uint32_t k)
 // the whole line in registers; a forward + backward sweep leaves a line at its
 // own fixpoint, so the tile is at a fixpoint once every other phase after a
 // sweep has changed nothing.  Returns true when the round cap was hit.
-template <int TW, int TH, int LM>
-__device__ bool relax_local(const Lds& s, uint32_t L, uint32_t max_local, uint32_t& rounds) {
+template <int TW, int TH, int LM, bool NARROW>
+__device__ bool relax_local(const Lds<NARROW>& s, uint32_t L, uint32_t max_local, uint32_t& rounds) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   constexpr uint32_t RS = TW + 3, RH = TH + 2, PS = TW + 1;
   constexpr uint32_t LAB_PLANE = RH * RS, STEP_PLANE = TH * PS;
@@ -408,19 +424,19 @@ __device__ bool relax_local(const Lds& s, uint32_t L, uint32_t max_local, uint32
     for (uint32_t line = tid; line < L * TH; line += nt) {
       const uint32_t l = line / TH, y = line - l * TH;
       uint64_t* lp = s.lab + l * LAB_PLANE + (y + 1) * RS;
-      const uint32_t* sp = s.sxl + l * STEP_PLANE + y * PS;
+      const auto* sp = s.sxl + l * STEP_PLANE + y * PS;
       uint64_t v[TW + 2];
       uint32_t st[TW];
       uint32_t chg = 0;
 #pragma unroll
       for (int i = 0; i < TW + 2; ++i) v[i] = lp[i];
 #pragma unroll
-      for (int x = 0; x < TW; ++x) st[x] = sp[x];
+      for (int x = 0; x < TW; ++x) st[x] = step_of(sp[x]);
 #pragma unroll
       for (int x = 0; x < TW; ++x) relax_to(v[x], st[x], v[x + 1], chg, 1u << x);
       sp = s.sxr + l * STEP_PLANE + y * PS;
 #pragma unroll
-      for (int x = 0; x < TW; ++x) st[x] = sp[x];
+      for (int x = 0; x < TW; ++x) st[x] = step_of(sp[x]);
 #pragma unroll
       for (int x = TW - 1; x >= 0; --x) relax_to(v[x + 2], st[x], v[x + 1], chg, 1u << x);
       if (chg) {
@@ -437,19 +453,19 @@ __device__ bool relax_local(const Lds& s, uint32_t L, uint32_t max_local, uint32
     for (uint32_t line = tid; line < L * TW; line += nt) {
       const uint32_t l = line / TW, x = line - l * TW;
       uint64_t* lp = s.lab + l * LAB_PLANE + x + 1;
-      const uint32_t* sp = s.syt + l * STEP_PLANE + x;
+      const auto* sp = s.syt + l * STEP_PLANE + x;
       uint64_t v[TH + 2];
       uint32_t st[TH];
       uint32_t chg = 0;
 #pragma unroll
       for (int i = 0; i < TH + 2; ++i) v[i] = lp[i * RS];
 #pragma unroll
-      for (int y = 0; y < TH; ++y) st[y] = sp[y * PS];
+      for (int y = 0; y < TH; ++y) st[y] = step_of(sp[y * PS]);
 #pragma unroll
       for (int y = 0; y < TH; ++y) relax_to(v[y], st[y], v[y + 1], chg, 1u << y);
       sp = s.syb + l * STEP_PLANE + x;
 #pragma unroll
-      for (int y = 0; y < TH; ++y) st[y] = sp[y * PS];
+      for (int y = 0; y < TH; ++y) st[y] = step_of(sp[y * PS]);
 #pragma unroll
       for (int y = TH - 1; y >= 0; --y) relax_to(v[y + 2], st[y], v[y + 1], chg, 1u << y);
       if (chg) {
@@ -467,8 +483,8 @@ __device__ bool relax_local(const Lds& s, uint32_t L, uint32_t max_local, uint32
       for (uint32_t col = tid; col < TW * TH; col += nt) {
         const uint32_t y = col / TW, x = col - y * TW;
         uint64_t* lp = s.lab + (y + 1) * RS + x + 1;
-        const uint32_t* dp = s.svd + y * PS + x;
-        const uint32_t* up = s.svu + y * PS + x;
+        const auto* dp = s.svd + y * PS + x;
+        const auto* up = s.svu + y * PS + x;
         uint64_t v[LM + 2];
         uint32_t sf[LM], sb[LM];
         uint32_t chg = 0;
@@ -478,8 +494,8 @@ __device__ bool relax_local(const Lds& s, uint32_t L, uint32_t max_local, uint32
         for (int l = 0; l < LM; ++l) {
           const bool in = (uint32_t)l < L;
           v[l + 1] = in ? lp[l * LAB_PLANE] : kUnreached;
-          sf[l] = in ? dp[l * STEP_PLANE] : kMax;
-          sb[l] = in ? up[l * STEP_PLANE] : kMax;
+          sf[l] = in ? step_of(dp[l * STEP_PLANE]) : kMax;
+          sb[l] = in ? step_of(up[l * STEP_PLANE]) : kMax;
         }
 #pragma unroll
         for (int l = 0; l < LM; ++l) relax_to(v[l], sf[l], v[l + 1], chg, 1u << l);
@@ -507,7 +523,7 @@ __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ?
 
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
-template <int TW, int TH, int LM>
+template <int TW, int TH, int LM, bool NARROW>
 __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = uniform_u32(A.item_field[item]);
@@ -520,7 +536,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
   const uint32_t L = B.layers;
   const uint32_t ncell = L * PT;
   const uint32_t wx0 = txi * TW, wy0 = tyi * TH;
-  Lds s = carve<TW, TH>(smem, L);
+  Lds<NARROW> s = carve<TW, TH, NARROW>(smem, L);
   const uint64_t TB = tile_block_words(L, TW, TH);
   const uint64_t blockbase = F.lab_off + (uint64_t)t * TB;
 
@@ -629,13 +645,14 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       if (ov[u] != -1 && ov[u] != F.own_id) a64 += B.halo_pen;
       const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
       const uint32_t pc = l * STEP_PLANE + y * PS + x;
-      s.sxl[pc] = gate(pass && (m & K_NX), s.xg[x], wgt, ap);
-      s.sxr[pc] = gate(pass && (m & K_PX), s.xg[x + 1], wgt, ap);
-      s.syt[pc] = gate(pass && (m & K_NY), s.yg[y], wgt, ap);
-      s.syb[pc] = gate(pass && (m & K_PY), s.yg[y + 1], wgt, ap);
+      using ST = typename Lds<NARROW>::Step;
+      s.sxl[pc] = to_step<ST>(gate(pass && (m & K_NX), s.xg[x], wgt, ap));
+      s.sxr[pc] = to_step<ST>(gate(pass && (m & K_PX), s.xg[x + 1], wgt, ap));
+      s.syt[pc] = to_step<ST>(gate(pass && (m & K_NY), s.yg[y], wgt, ap));
+      s.syb[pc] = to_step<ST>(gate(pass && (m & K_PY), s.yg[y + 1], wgt, ap));
       if (L > 1) {
-        s.svd[pc] = gate(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0], wgt, ap);
-        s.svu[pc] = gate(pass && (m & K_UP) && l + 1 < L, s.vg[l], wgt, ap);
+        s.svd[pc] = to_step<ST>(gate(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0], wgt, ap));
+        s.svu[pc] = to_step<ST>(gate(pass && (m & K_UP) && l + 1 < L, s.vg[l], wgt, ap));
       }
       s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[u] ? lab_in(lv[u]) : kUnreached;
     }
@@ -707,7 +724,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
 
   // ---- in-LDS relaxation to a local fixpoint ----------------------------------
   uint32_t rounds = 0;
-  const bool capped = relax_local<TW, TH, LM>(s, L, A.max_local, rounds);
+  const bool capped = relax_local<TW, TH, LM, NARROW>(s, L, A.max_local, rounds);
   // ---- write back: labels, border-column copies, side flags -------------------
   uint32_t flags = 0;
   for (uint32_t c = tid; c < ncell; c += nt) {
@@ -755,11 +772,11 @@ __device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb) {
   if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[(C_NEXTN) * kCtrStride], 1u)] = nb;
 }
 
-template <int TW, int TH, int LM>
+template <int TW, int TH, int LM, bool NARROW>
 __global__ void __launch_bounds__(256, 2) relax_round(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   uint32_t item = A.list_cur[blockIdx.x];
-  uint32_t res = process_tile<TW, TH, LM>(A, item, smem);
+  uint32_t res = process_tile<TW, TH, LM, NARROW>(A, item, smem);
   if (threadIdx.x == 0) {
     A.act_cur[item] = 0;
     const FieldDesc F = A.fields[A.item_field[item]];
@@ -774,109 +791,6 @@ __global__ void __launch_bounds__(256, 2) relax_round(RelaxArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// persistent form: device-side rounds over a work list
-// ---------------------------------------------------------------------------
-//
-// All workgroups stay resident.  A round pops items from the current active
-// list with one atomicAdd per item (a device work queue), processes them and
-// appends activated neighbours to the next list (deduplicated by flags).  A
-// grid barrier with agent-scope fences separates rounds, so every round reads
-// labels written by earlier rounds exactly as the host-driven form does, and
-// the loop ends when a round activates nothing.  Per-round counters are
-// triple-buffered so a slow workgroup can still read round r's counters while
-// round r+1 runs.  Every wait is bounded by spin_limit (error flag + exit).
-
-__device__ bool grid_barrier(const RelaxArgs& A, uint32_t nblocks, uint32_t& my_gen) {
-  __syncthreads();
-  bool ok = true;
-  if (threadIdx.x == 0) {
-    __threadfence();  // release this workgroup's label stores
-    uint32_t g = my_gen;
-    if (atomicAdd(&A.counters[(C_BAR_ARRIVE) * kCtrStride], 1u) == nblocks - 1) {
-      atomicExch(&A.counters[(C_BAR_ARRIVE) * kCtrStride], 0u);
-      __hip_atomic_fetch_add(&A.counters[(C_BAR_GEN) * kCtrStride], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      // bounded by the real-time clock, with exponential backoff
-      uint64_t t0 = wall_clock64();
-      uint32_t naps = 1;
-      while (__hip_atomic_load(&A.counters[(C_BAR_GEN) * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        if (wall_clock64() - t0 > A.spin_ticks) {
-          atomicOr(&A.counters[(C_ERROR) * kCtrStride], 8u);
-          ok = false;
-          break;
-        }
-        for (uint32_t i = 0; i < naps; ++i) __builtin_amdgcn_s_sleep(8);
-        naps = naps < 16 ? naps * 2 : 16;
-      }
-    }
-    __threadfence();  // acquire labels written before the barrier
-    my_gen = g + 1;
-  }
-  __shared__ uint32_t s_ok;
-  if (threadIdx.x == 0) s_ok = ok ? 1u : 0u;
-  __syncthreads();
-  return uniform_u32(s_ok) != 0;
-}
-
-template <int TW, int TH, int LM>
-__global__ void __launch_bounds__(256, 2) relax_persistent(RelaxArgs A, uint32_t n_first) {
-  extern __shared__ __align__(16) char smem[];
-  __shared__ uint32_t s_k, s_n;
-  RelaxArgs R = A;
-  uint32_t my_gen = 0;
-  uint32_t n_cur = n_first;
-  for (uint32_t round = 0;; ++round) {
-    uint32_t* taken = &A.counters[(C_TAKEN0 + round % 3) * kCtrStride];
-    uint32_t* nextn = &A.counters[(C_NEXT0 + round % 3) * kCtrStride];
-    R.counters = A.counters;
-    // pop / process this round's list
-    for (;;) {
-      if (threadIdx.x == 0) s_k = atomicAdd(taken, 1u);
-      __syncthreads();
-      uint32_t k = uniform_u32(s_k);
-      __syncthreads();
-      if (k >= n_cur) break;
-      uint32_t item = uniform_u32(R.list_cur[k]);
-      uint32_t res = process_tile<TW, TH, LM>(R, item, smem);
-      if (threadIdx.x == 0) {
-        R.act_cur[item] = 0;
-        const FieldDesc F = R.fields[R.item_field[item]];
-        uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-        auto act = [&](uint32_t nb) {
-          if (atomicExch(&R.act_next[nb], 1u) == 0u) R.list_next[atomicAdd(nextn, 1u)] = nb;
-        };
-        if ((res & T_LEFT) && txi > 0) act(item - 1);
-        if ((res & T_RIGHT) && txi + 1 < F.tx) act(item + 1);
-        if ((res & T_TOP) && tyi > 0) act(item - F.tx);
-        if ((res & T_BOTTOM) && tyi + 1 < F.ty) act(item + F.tx);
-        if (res & T_CAPPED) act(item);
-      }
-    }
-    // the counters of round + 1 were last used in round - 2: clear them now
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      atomicExch(&A.counters[(C_TAKEN0 + (round + 1) % 3) * kCtrStride], 0u);
-      atomicExch(&A.counters[(C_NEXT0 + (round + 1) % 3) * kCtrStride], 0u);
-    }
-    if (!grid_barrier(A, gridDim.x, my_gen)) return;
-    if (threadIdx.x == 0)
-      s_n = __hip_atomic_load(nextn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    n_cur = uniform_u32(s_n);
-    __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.counters[(C_PROUNDS) * kCtrStride], 1u);
-    if (n_cur == 0 ||
-        uniform_u32(__hip_atomic_load(&A.counters[(C_ERROR) * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-      return;
-    uint32_t* ta = R.act_cur;
-    R.act_cur = R.act_next;
-    R.act_next = ta;
-    const uint32_t* tl = R.list_cur;
-    R.list_cur = R.list_next;
-    R.list_next = const_cast<uint32_t*>(tl);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // graph form: one grid-stride launch per round, rounds replayed from a hipGraph
 // ---------------------------------------------------------------------------
 //
@@ -887,7 +801,7 @@ __global__ void __launch_bounds__(256, 2) relax_persistent(RelaxArgs A, uint32_t
 // coherent labels; no workgroup ever waits on another.  A launch whose list is
 // empty returns at once, so a captured sequence can overshoot the fixpoint.
 
-template <int TW, int TH, int LM>
+template <int TW, int TH, int LM, bool NARROW>
 __global__ void __launch_bounds__(256, 2) relax_list(RelaxArgs E, RelaxArgs O) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t r =
@@ -903,7 +817,7 @@ __global__ void __launch_bounds__(256, 2) relax_list(RelaxArgs E, RelaxArgs O) {
   // the compiler so lanes other than 0 re-run the body without a new pop.
   for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
     uint32_t item = uniform_u32(R.list_cur[k]);
-    uint32_t res = process_tile<TW, TH, LM>(R, item, smem);
+    uint32_t res = process_tile<TW, TH, LM, NARROW>(R, item, smem);
     if (threadIdx.x == 0) {
       R.act_cur[item] = 0;
       const FieldDesc F = R.fields[R.item_field[item]];
@@ -1090,43 +1004,49 @@ static T* dev_copy(const T* h, size_t n, hipStream_t s) {
   return d;
 }
 
-// Instantiated tile shapes (width x height) and layer bounds.
+// Instantiated tile shapes (width x height), layer bounds and step widths.
 struct KernelSet {
   uint32_t tw, th, lm;
+  bool narrow;
   const void* round;
-  const void* persistent;
   const void* list;
 };
 
-template <int TW, int TH, int LM>
+template <int TW, int TH, int LM, bool NARROW>
 static KernelSet make_set() {
-  return KernelSet{TW, TH, LM, (const void*)relax_round<TW, TH, LM>, (const void*)relax_persistent<TW, TH, LM>,
-                   (const void*)relax_list<TW, TH, LM>};
+  return KernelSet{TW, TH, LM, NARROW, (const void*)relax_round<TW, TH, LM, NARROW>,
+                   (const void*)relax_list<TW, TH, LM, NARROW>};
 }
 
-template <int TW, int TH>
+// Wide step planes for every layer bound; narrow ones up to 8 layers.
+template <int TW, int TH, bool WITH_NARROW>
 static void add_shape(std::vector<KernelSet>& v) {
-  v.push_back(make_set<TW, TH, 2>());
-  v.push_back(make_set<TW, TH, 4>());
-  v.push_back(make_set<TW, TH, 8>());
-  v.push_back(make_set<TW, TH, 16>());
+  v.push_back(make_set<TW, TH, 2, false>());
+  v.push_back(make_set<TW, TH, 4, false>());
+  v.push_back(make_set<TW, TH, 8, false>());
+  v.push_back(make_set<TW, TH, 16, false>());
+  if (WITH_NARROW) {
+    v.push_back(make_set<TW, TH, 2, WITH_NARROW>());
+    v.push_back(make_set<TW, TH, 4, WITH_NARROW>());
+    v.push_back(make_set<TW, TH, 8, WITH_NARROW>());
+  }
 }
 
 static const std::vector<KernelSet>& kernel_sets() {
   static const std::vector<KernelSet> v = [] {
     std::vector<KernelSet> k;
-    add_shape<32, 32>(k);
-    add_shape<32, 16>(k);
-    add_shape<16, 16>(k);
-    add_shape<16, 8>(k);
-    add_shape<8, 4>(k);
-    add_shape<4, 1>(k);
+    add_shape<32, 32, true>(k);
+    add_shape<32, 16, true>(k);
+    add_shape<16, 16, true>(k);
+    add_shape<16, 8, true>(k);
+    add_shape<8, 4, true>(k);
+    add_shape<4, 1, false>(k);
     return k;
   }();
   return v;
 }
 
-size_t lds_bytes_for(uint32_t tw, uint32_t th, uint32_t layers) { return lds_bytes(tw, th, layers); }
+size_t lds_bytes_for(uint32_t tw, uint32_t th, uint32_t layers) { return lds_bytes(tw, th, layers, false); }
 
 bool tile_shape_supported(uint32_t tw, uint32_t th) {
   for (const KernelSet& k : kernel_sets())
@@ -1134,10 +1054,16 @@ bool tile_shape_supported(uint32_t tw, uint32_t th) {
   return false;
 }
 
-static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers) {
-  uint32_t lm = layers <= 2 ? 2 : layers <= 4 ? 4 : layers <= 8 ? 8 : 16;
-  for (const KernelSet& k : kernel_sets())
-    if (k.tw == tw && k.th == th && k.lm == lm) return k;
+// The narrow variant when asked for and instantiated, else the wide one.
+static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers, bool narrow) {
+  const uint32_t lm = layers <= 2 ? 2 : layers <= 4 ? 4 : layers <= 8 ? 8 : 16;
+  const KernelSet* wide = nullptr;
+  for (const KernelSet& k : kernel_sets()) {
+    if (k.tw != tw || k.th != th || k.lm != lm) continue;
+    if (k.narrow == narrow) return k;
+    if (!k.narrow) wide = &k;
+  }
+  if (wide) return *wide;
   throw Error(1, "tile shape " + std::to_string(tw) + "x" + std::to_string(th) + " is not instantiated");
 }
 
@@ -1166,7 +1092,6 @@ void ctx_init(Ctx& c, int device) {
   HIP_OK(hipHostMalloc((void**)&c.ws.pinned, kCtrWords * sizeof(uint32_t), hipHostMallocDefault));
   for (const KernelSet& k : kernel_sets()) {
     HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
-    HIP_OK(hipFuncSetAttribute(k.persistent, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
   }
 }
@@ -1239,7 +1164,26 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
       if (l > 0 && via_allowed[l - 1] && ok(i) && ok(i - plane)) bm[i] |= M_DN;
       if (l + 1 < layers && via_allowed[l] && ok(i) && ok(i + plane)) bm[i] |= M_UP;
     }
+  // Bound on every step price min(E*w + a, MAX-1): the largest edge price
+  // times the largest enter weight (obstacles count as weight 1, the price of
+  // an own pad) plus the largest additive price.  Boards under the narrow
+  // limit relax with 16-bit step planes in LDS.
+  uint64_t max_e = 0, max_w = 1, max_a = 0;
+  for (uint32_t k = 0; k + 1 < w; ++k) max_e = std::max<uint64_t>(max_e, xe[k]);
+  for (uint32_t k = 0; k + 1 < h; ++k) max_e = std::max<uint64_t>(max_e, ye[k]);
+  for (uint32_t k = 0; k + 1 < layers; ++k)
+    if (via_allowed[k] && ve) max_e = std::max<uint64_t>(max_e, ve[k]);
+  std::vector<uint8_t> obst(n);
+  for (size_t i = 0; i < n; ++i) {
+    obst[i] = cost[i] == kMax;
+    if (!obst[i]) max_w = std::max<uint64_t>(max_w, cost[i]);
+    uint64_t a = add ? add[i] : 0;
+    if (halo) a += halo_pen;
+    max_a = std::max(max_a, a);
+  }
   board_free(b);
+  b.obst = std::move(obst);
+  b.max_step = std::min<uint64_t>(max_e * max_w + max_a, kMax - 1);
   b.w = w;
   b.h = h;
   b.layers = layers;
@@ -1336,6 +1280,16 @@ static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_
   return g.exec;
 }
 
+// Whether a field's step prices all fit the narrow step planes: its board's
+// bound, and no forced endpoint on an obstacle (priced at the obstacle's
+// weight unless it is an own pad).
+static bool field_narrow(const DevBoard& b, const FieldReq& q) {
+  if (b.max_step > kNarrowMaxStep) return false;
+  for (uint32_t g : q.forced)
+    if (g < b.obst.size() && b.obst[g] && !std::binary_search(q.pads.begin(), q.pads.end(), g)) return false;
+  return true;
+}
+
 void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std::vector<FieldReq>& reqs,
                   SolveStats* st) {
   HIP_OK(hipSetDevice(c.device));
@@ -1349,7 +1303,13 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  size_t lds = lds_bytes(S.tw, S.th, Lmax);
+  bool narrow = true;
+  for (const FieldReq& q : reqs) {
+    if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
+    narrow = narrow && field_narrow(*boards[q.board], q);
+  }
+  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow);
+  size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   if (lds > 160 * 1024 - 64) throw Error(1, "tile shape exceeds 160 KiB of LDS");
 
   std::vector<FieldDesc>& layout = c.layout;
@@ -1435,7 +1395,6 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   local.field_cells = field_cells;
   bool timed = false;
   if (item && !seeds.empty()) {
-    const KernelSet& K = pick_kernels(S.tw, S.th, Lmax);
     uint32_t* pinned = W.pinned;
     void* args[] = {&A};
     // Rounds of relax_round over the active-item list until no item is
@@ -1693,7 +1652,7 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   const TileShape& S = c.shape;
   if (!tile_shape_supported(S.tw, S.th)) throw Error(1, "unsupported tile shape");
   if (max_layers < 1 || max_layers > 16) throw Error(1, "max_layers must be in 1..16");
-  if (lds_bytes(S.tw, S.th, max_layers) > 160 * 1024 - 64) throw Error(1, "tile shape too large for max_layers");
+  if (lds_bytes(S.tw, S.th, max_layers, false) > 160 * 1024 - 64) throw Error(1, "tile shape too large for max_layers");
   if (max_boards < 1 || max_groups < 1) throw Error(1, "server needs at least one board and one group slot");
   d.c = &c;
   d.max_layers = max_layers;
@@ -1723,6 +1682,7 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   d.la.reset(words);
   d.pa.reset(pool_words);
   d.views.assign(max_boards, BoardView{});
+  d.devs.assign(max_boards, nullptr);
   d.board_used.assign(max_boards, 0);
   d.groups.assign(max_groups, Group{});
   d.pending_seeds.clear();
@@ -1752,6 +1712,7 @@ int driver_add_board(RoundDriver& d, const DevBoard& b) {
   for (uint32_t k = 0; k < d.max_boards; ++k) {
     if (d.board_used[k]) continue;
     d.views[k] = b.view();
+    d.devs[k] = &b;
     d.board_used[k] = 1;
     HIP_OK(hipMemcpyAsync(d.boards + k, &d.views[k], sizeof(BoardView), hipMemcpyHostToDevice, d.c->stream));
     HIP_OK(hipStreamSynchronize(d.c->stream));
@@ -1761,7 +1722,10 @@ int driver_add_board(RoundDriver& d, const DevBoard& b) {
 }
 
 void driver_remove_board(RoundDriver& d, int slot) {
-  if (slot >= 0 && (uint32_t)slot < d.max_boards) d.board_used[slot] = 0;
+  if (slot >= 0 && (uint32_t)slot < d.max_boards) {
+    d.board_used[slot] = 0;
+    d.devs[slot] = nullptr;
+  }
 }
 
 uint64_t driver_group_words(const RoundDriver& d, const std::vector<FieldReq>& reqs) {
@@ -1809,7 +1773,11 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   }
   G.nfields = reqs.size();
   G.nitems = nitems;
-  for (const FieldReq& q : reqs) G.layers = std::max(G.layers, d.views[q.board].layers);
+  G.narrow = true;
+  for (const FieldReq& q : reqs) {
+    G.layers = std::max(G.layers, d.views[q.board].layers);
+    G.narrow = G.narrow && field_narrow(*d.devs[q.board], q);
+  }
   G.labwords = words;
   G.npool = npool;
   std::vector<FieldDesc> descs(reqs.size());
@@ -1917,10 +1885,14 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
+  bool narrow = true;
   for (uint32_t g = 0; g < d.max_groups; ++g)
-    if (d.groups[g].live) Lmax = std::max(Lmax, d.groups[g].layers);
-  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax);
-  const size_t lds = lds_bytes(S.tw, S.th, Lmax);
+    if (d.groups[g].live) {
+      Lmax = std::max(Lmax, d.groups[g].layers);
+      narrow = narrow && d.groups[g].narrow;
+    }
+  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow);
+  const size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   void* args[] = {&A};
   HIP_OK(hipEventRecord(c.ws.ek0, s));
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
