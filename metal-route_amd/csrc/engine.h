@@ -181,6 +181,8 @@ struct Workspace {
   uint32_t* fout = nullptr;      // extracted field output staging
   size_t fout_cap = 0;
   uint32_t* pinned = nullptr;    // host-pinned counter readback
+  uint32_t* hostctr = nullptr;   // host-mapped words the plain form's kernel publishes to
+  uint32_t* hostctr_dev = nullptr;
   BoardView* boards = nullptr;   // device board table of the current batch
   size_t boards_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ek0 = nullptr, ek1 = nullptr;

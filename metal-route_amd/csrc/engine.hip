@@ -92,7 +92,8 @@ enum : int {
   C_MISC = 17,
   C_GROUND = 18,  // graph form: current round index
   C_CURN = 19,    // graph form: length of the current round's list
-  C_NCOUNTERS = 20
+  C_DONE = 20,    // plain form: workgroups of the launch that finished
+  C_NCOUNTERS = 21
 };
 // Each device counter sits on its own 128-byte line: polled words and the
 // work-list atomics must not share a line, or every pop and push queues
@@ -272,6 +273,12 @@ struct RelaxArgs {
   uint32_t qcap;
   uint32_t spin_limit;
   uint64_t spin_ticks;  // real-time bound of any spin wait (wall_clock64 ticks)
+  // Plain form: the round's activation count goes to counter next_ctr;
+  // counter clear_ctr (the previous round's, already read) is zeroed for the
+  // next round; the last workgroup to finish publishes the counts to the
+  // host-mapped words host_out (next count, error flags, visits, rounds).
+  uint32_t next_ctr, clear_ctr;
+  uint32_t* host_out;
 };
 
 // Words of one tile block in the label buffer: the L x TH x TW labels
@@ -769,12 +776,24 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 
 // Activate `nb` for the next round once (flag + append to the next list).
 __device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb) {
-  if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[(C_NEXTN) * kCtrStride], 1u)] = nb;
+  if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[A.next_ctr * kCtrStride], 1u)] = nb;
 }
 
+// Waves per SIMD the kernels are compiled for, the most that fit without
+// register spills: tiles up to 16 wide with up to 8 layers fit six (at most
+// 80 VGPRs); 16-layer via columns five; 32-wide line sweeps three.  The
+// grid-stride form keeps a few more values live.
+template <int TW, int LM>
+struct MinWaves {
+  static constexpr int round = TW >= 32 ? 3 : (LM >= 16 ? 5 : 6);
+  static constexpr int list = TW >= 32 ? 2 : 4;
+};
+
 template <int TW, int TH, int LM, bool NARROW>
-__global__ void __launch_bounds__(256, 2) relax_round(RelaxArgs A) {
+__global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && A.clear_ctr != A.next_ctr)
+    __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t item = A.list_cur[blockIdx.x];
   uint32_t res = process_tile<TW, TH, LM, NARROW>(A, item, smem);
   if (threadIdx.x == 0) {
@@ -787,6 +806,27 @@ __global__ void __launch_bounds__(256, 2) relax_round(RelaxArgs A) {
     if ((res & T_BOTTOM) && tyi + 1 < F.ty) activate(A, item + F.tx);
     if (res & T_CAPPED) activate(A, item);
     if (res & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
+    if (A.host_out) {
+      // The last workgroup of the launch publishes the counts.  Every
+      // workgroup's activation atomics returned (their results index the
+      // list) before its done increment was issued, and atomics are performed
+      // at the device's coherence point, so the last one reads final counts;
+      // a relaxed increment keeps the caches untouched (an acquire-release
+      // one would write back and invalidate L2 in every workgroup).
+      const uint32_t prev = __hip_atomic_fetch_add(&A.counters[C_DONE * kCtrStride], 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == gridDim.x - 1) {
+        __hip_atomic_store(&A.counters[C_DONE * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t v[4] = {
+            __hip_atomic_load(&A.counters[A.next_ctr * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+            __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+            __hip_atomic_load(&A.counters[C_VISITS * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+            __hip_atomic_load(&A.counters[C_ROUNDS * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+        for (int k = 0; k < 4; ++k)
+          __hip_atomic_store(&A.host_out[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+      }
+    }
   }
 }
 
@@ -802,7 +842,7 @@ __global__ void __launch_bounds__(256, 2) relax_round(RelaxArgs A) {
 // empty returns at once, so a captured sequence can overshoot the fixpoint.
 
 template <int TW, int TH, int LM, bool NARROW>
-__global__ void __launch_bounds__(256, 2) relax_list(RelaxArgs E, RelaxArgs O) {
+__global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(RelaxArgs E, RelaxArgs O) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t r =
       uniform_u32(__hip_atomic_load(&E.counters[C_GROUND * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -1090,6 +1130,9 @@ void ctx_init(Ctx& c, int device) {
   c.label_budget = std::min<size_t>(c.label_budget, total / 2);
   HIP_OK(hipMalloc((void**)&c.ws.counters, kCtrWords * sizeof(uint32_t)));
   HIP_OK(hipHostMalloc((void**)&c.ws.pinned, kCtrWords * sizeof(uint32_t), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&c.ws.hostctr, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  memset(c.ws.hostctr, 0, 64);
+  HIP_OK(hipHostGetDevicePointer((void**)&c.ws.hostctr_dev, c.ws.hostctr, 0));
   for (const KernelSet& k : kernel_sets()) {
     HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
@@ -1104,6 +1147,7 @@ void ctx_free(Ctx& c) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (w.pinned) (void)hipHostFree(w.pinned);
+  if (w.hostctr) (void)hipHostFree(w.hostctr);
   if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
   c.graph = GraphCache();
   if (w.ev0) (void)hipEventDestroy(w.ev0);
@@ -1241,6 +1285,9 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   A.qcap = n_items;
   A.spin_limit = 1u << 26;
   A.spin_ticks = c.spin_ticks;
+  A.next_ctr = C_NEXTN;
+  A.clear_ctr = C_NEXTN;
+  A.host_out = nullptr;
   return A;
 }
 
@@ -1400,21 +1447,27 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     // Rounds of relax_round over the active-item list until no item is
     // activated.  Every round starts at a kernel boundary, so it reads
     // coherent labels; an empty next list certifies the fixpoint.
+    // Per round: one launch and one stream synchronisation.  The activation
+    // count alternates between two counters (the kernel clears the one the
+    // host has read), and the launch's last workgroup publishes the counts to
+    // host-mapped memory, so no fill or copy is queued between rounds.
     auto plain_rounds = [&](uint32_t& rounds, uint32_t n_active) {
+      A.host_out = W.hostctr_dev;
       while (n_active) {
-        launch_fill32(s, W.counters + C_NEXTN * kCtrStride, 0u, 1);
+        A.next_ctr = C_NEXT0 + (rounds & 1);
+        A.clear_ctr = C_NEXT0 + ((rounds + 1) & 1);
         HIP_OK(hipEventRecord(W.ek0, s));
         HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
         HIP_OK(hipEventRecord(W.ek1, s));
-        HIP_OK(hipMemcpyAsync(pinned, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         float kms = 0;
         HIP_OK(hipEventElapsedTime(&kms, W.ek0, W.ek1));
         local.kernel_ms += kms;
         ++local.launches;
         ++rounds;
-        if (pinned[C_ERROR * kCtrStride]) break;
-        n_active = pinned[C_NEXTN * kCtrStride];
+        const volatile uint32_t* hc = W.hostctr;
+        if (hc[1]) break;
+        n_active = hc[0];
         std::swap(A.act_cur, A.act_next);
         uint32_t* lc = const_cast<uint32_t*>(A.list_cur);
         A.list_cur = A.list_next;
@@ -1882,6 +1935,9 @@ uint32_t driver_step(RoundDriver& d) {
   A.th = S.th;
   A.max_local = S.max_local;
   A.spin_ticks = c.spin_ticks;
+  A.next_ctr = C_NEXTN;
+  A.clear_ctr = C_NEXTN;
+  A.host_out = nullptr;
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
