@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
     ap.add_argument("--tiles", default="16,16,256,32", help="tw,th,threads,max_local")
+    ap.add_argument("--delta", type=int, default=0, help="delta-stepping window of the plain form (0 = off)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick tiles / round cap / form with the library's autotuner (always on for config 3)")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -224,6 +225,8 @@ def main():
     eng = mrfield.Engine(0)
     eng.set_tiles(tw, th, threads, ml)
     eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)
+    if a.delta:
+        eng.set_delta(a.delta)
     s = mrfield.synth_config(a.config, seed=a.seed)
     g = s.grid
     multi = s.pins is not None
@@ -245,7 +248,7 @@ def main():
         return board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
-    kern_ms, launches = [], []
+    kern_ms, launches, changed = [], [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
@@ -263,6 +266,7 @@ def main():
             rounds_l.append(st["local_rounds"])
             host_rounds.append(st["host_rounds"])
             kern_ms.append(st["relax_kernel_ms"])
+            changed.append(st["changed_visits"])
             launches.append(st["relax_launches"])
     wall = time.perf_counter() - t_start
     n_nets = len(s.pins) if multi else len(nets)
@@ -303,6 +307,7 @@ def main():
                         f"seed {a.seed})",
             "form": form,
             "tiles": tiles,
+            "delta": (tuning["delta"] if tuning is not None else (a.delta or None)),
         },
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -314,6 +319,7 @@ def main():
         "relax_avg_launch_ms": round(avg_launch_ms, 5),
         "path_ms": round(float(np.mean(path_ms)), 4),
         "tile_visits": int(np.mean(visits)),
+        "changed_visits": int(np.mean(changed)),
         "local_rounds_per_visit": round(float(np.mean(rounds_l)) / max(float(np.mean(visits)), 1), 3),
         "host_rounds": int(np.mean(host_rounds)),
         "roofline": {

@@ -156,6 +156,7 @@ typedef struct {
   double relax_kernel_ms;  /* relaxation kernels alone: sum of per-launch HIP event times (plain
                               form; the graph form reports its replay time) */
   uint32_t relax_launches; /* relaxation kernel launches */
+  uint64_t changed_visits; /* tile visits that improved at least one label */
 } mrf_stats;
 
 /* Solve form of a context.
@@ -177,6 +178,12 @@ void mrf_close(mrf_ctx *ctx);
 int mrf_set_tiles(mrf_ctx *ctx, uint32_t tw, uint32_t th, uint32_t threads, uint32_t max_local,
                   mrf_error *err);
 int mrf_set_mode(mrf_ctx *ctx, int mode);
+/* Plain form: process, each round, only the tiles whose activating border
+ * cost is within `delta` of the smallest pending one; the others wait for a
+ * later round (delta stepping over tiles: fewer revisits, more rounds).
+ * UINT32_MAX (the default) processes every active tile each round.  Results
+ * do not depend on it. */
+int mrf_set_delta(mrf_ctx *ctx, uint32_t delta);
 int mrf_last_stats(mrf_ctx *ctx, mrf_stats *out);
 
 /* IsolatedRouteProvider::route_isolated_batch + MetalIsolatedProvider adapter
@@ -243,6 +250,7 @@ typedef struct {
     uint32_t tile_w, tile_h, threads, max_local;
     int mode;               /* MRF_MODE_PLAIN or MRF_MODE_GRAPH */
     uint32_t graph_rounds;  /* rounds per hipGraph replay (graph form) */
+    uint32_t delta;         /* delta-stepping window (plain form; UINT32_MAX = off) */
     double sample_ms;       /* wall time of the chosen configuration on the sample */
     double default_ms;      /* wall time of the context's configuration before tuning */
     uint32_t tried;         /* configurations measured */

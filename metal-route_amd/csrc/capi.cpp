@@ -312,6 +312,13 @@ int mrf_set_mode(mrf_ctx* ctx, int mode) {
   return MRF_OK;
 }
 
+int mrf_set_delta(mrf_ctx* ctx, uint32_t delta) {
+  if (!ctx) return MRF_BACKEND_UNAVAILABLE;
+  std::lock_guard<std::mutex> g(ctx->c.lane);
+  ctx->c.delta = delta == 0 ? 1u : delta;
+  return MRF_OK;
+}
+
 int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   if (!ctx || !out) return MRF_BACKEND_UNAVAILABLE;
   const mrf::SolveStats& s = ctx->c.last;
@@ -326,6 +333,7 @@ int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   out->persistent_ms = s.persistent_ms;
   out->relax_kernel_ms = s.kernel_ms;
   out->relax_launches = s.launches;
+  out->changed_visits = s.changed_visits;
   return MRF_OK;
 }
 
@@ -648,6 +656,7 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
       uint32_t tw, th, threads, max_local;
       int mode;
       uint32_t graph_rounds;
+      uint32_t delta;
     };
     const mrf::TileShape keep_shape = c.shape;
     const int keep_mode = c.mode == MRF_MODE_PERSISTENT ? MRF_MODE_PLAIN : c.mode;
@@ -659,13 +668,29 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
       c.shape.max_local = k.max_local;
       c.mode = k.mode;
       c.graph_rounds = k.graph_rounds;
+      c.delta = k.delta;
     };
+    // Typical step price of the board (mean planar edge price x mean enter
+    // weight + mean additive price) sets the scale of the delta candidates:
+    // a few tile crossings.
+    double e_mean = 0, w_mean = 0, a_mean = 0;
+    size_t ne = 0, nw = 0;
+    for (uint32_t v : bd->xe) e_mean += v, ++ne;
+    for (uint32_t v : bd->ye) e_mean += v, ++ne;
+    e_mean = ne ? e_mean / ne : 1.0;
+    for (uint32_t v : bd->cost)
+      if (v != kMax) w_mean += v, ++nw;
+    w_mean = nw ? w_mean / nw : 1.0;
+    for (uint32_t v : bd->add) a_mean += v;
+    a_mean = bd->add.empty() ? 0.0 : a_mean / bd->add.size();
+    const double typical = std::max(1.0, e_mean * w_mean + a_mean);
     auto timed = [&](const Cand& k) {
       apply(k);
       run();  // warm-up: workspace growth, graph capture
       return std::min(run(), run());
     };
-    Cand base{keep_shape.tw, keep_shape.th, keep_shape.threads, keep_shape.max_local, keep_mode, keep_rounds};
+    Cand base{keep_shape.tw, keep_shape.th, keep_shape.threads, keep_shape.max_local, keep_mode, keep_rounds,
+              c.delta};
     double best_ms = timed(base);
     out->default_ms = best_ms;
     Cand best = base;
@@ -675,9 +700,17 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
       if (mrf::lds_bytes_for(sh[0], sh[1], dims.layers) > 160 * 1024 - 64) continue;
       for (uint32_t threads : {128u, 256u})
         for (uint32_t ml : {4u, 32u})
-          for (int mode : {(int)MRF_MODE_PLAIN, (int)MRF_MODE_GRAPH})
-            for (uint32_t gr : mode == MRF_MODE_GRAPH ? std::vector<uint32_t>{4, 16} : std::vector<uint32_t>{8}) {
-              Cand k{sh[0], sh[1], threads, ml, mode, gr};
+          for (int mode : {(int)MRF_MODE_PLAIN, (int)MRF_MODE_GRAPH}) {
+            std::vector<std::pair<uint32_t, uint32_t>> forms;  // (graph rounds, delta)
+            if (mode == MRF_MODE_GRAPH) {
+              forms = {{4, kMax}, {16, kMax}};
+            } else {
+              forms = {{8, kMax}};
+              for (double crossings : {2.0, 8.0})
+                forms.push_back({8, (uint32_t)std::min(4e9, crossings * sh[0] * typical)});
+            }
+            for (auto& fm : forms) {
+              Cand k{sh[0], sh[1], threads, ml, mode, fm.first, fm.second};
               double ms = timed(k);
               ++tried;
               if (ms < best_ms) {
@@ -685,6 +718,7 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
                 best = k;
               }
             }
+          }
     }
     apply(best);
     out->tile_w = best.tw;
@@ -693,6 +727,7 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
     out->max_local = best.max_local;
     out->mode = best.mode;
     out->graph_rounds = best.graph_rounds;
+    out->delta = best.delta;
     out->sample_ms = best_ms;
     out->tried = tried;
     if (err) set_err(err, MRF_OK, "ok");

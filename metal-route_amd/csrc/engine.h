@@ -105,6 +105,7 @@ struct FieldReq {
 struct SolveStats {
   double relax_ms = 0, path_ms = 0, total_ms = 0;
   uint64_t tile_visits = 0;    // device counter of tile loads (HBM sweep visits)
+  uint64_t changed_visits = 0; // visits that improved at least one label
   uint64_t local_rounds = 0;   // in-LDS relaxation rounds
   uint64_t cells_visited = 0;  // tile_visits x tile cells
   uint32_t rounds = 0;         // host rounds (plain form) or 1 (persistent)
@@ -118,6 +119,7 @@ struct SolveStats {
     path_ms += o.path_ms;
     total_ms += o.total_ms;
     tile_visits += o.tile_visits;
+    changed_visits += o.changed_visits;
     local_rounds += o.local_rounds;
     cells_visited += o.cells_visited;
     rounds += o.rounds;
@@ -159,6 +161,8 @@ struct Workspace {
   size_t state_cap = 0;
   uint32_t* act = nullptr;    // 2 x items activation flags
   size_t act_cap = 0;
+  uint32_t* key = nullptr;    // 2 x items activation keys (delta stepping)
+  size_t key_cap = 0;
   uint32_t* list = nullptr;   // 2 x items active-item lists (round form)
   size_t list_cap = 0;
   uint32_t* queue = nullptr;  // persistent work queue
@@ -202,6 +206,7 @@ struct Ctx {
   TileShape shape;
   int mode = MODE_GRAPH;
   uint32_t graph_rounds = 8;  // rounds captured per hipGraph replay
+  uint32_t delta = kMax;      // plain form: delta-stepping window in search cost (kMax = off)
   size_t label_budget = size_t(48) << 30;  // bytes of labels per chunk
   int cus = 256;
   uint64_t spin_ticks = 400000000ull;  // bound of device spin waits (wall-clock ticks)

@@ -93,7 +93,9 @@ enum : int {
   C_GROUND = 18,  // graph form: current round index
   C_CURN = 19,    // graph form: length of the current round's list
   C_DONE = 20,    // plain form: workgroups of the launch that finished
-  C_NCOUNTERS = 21
+  C_KEY0 = 21,    // 21..22: smallest activation key of the next list (delta stepping)
+  C_THRESH = 23,  // delta stepping: the round processes items whose key is at most this
+  C_NCOUNTERS = 24
 };
 // Each device counter sits on its own 128-byte line: polled words and the
 // work-list atomics must not share a line, or every pop and push queues
@@ -279,6 +281,15 @@ struct RelaxArgs {
   // host-mapped words host_out (next count, error flags, visits, rounds).
   uint32_t next_ctr, clear_ctr;
   uint32_t* host_out;
+  // Delta stepping (key_cur non-null): every activation carries a key, the
+  // smallest search cost among the border labels that changed; a round
+  // processes the items whose key is at most counters[C_THRESH] and carries
+  // the others over with their keys.  The launch's last workgroup sets the
+  // next threshold to the next list's smallest key plus delta.
+  uint32_t* key_cur;
+  uint32_t* key_next;
+  uint32_t key_ctr, key_clear;
+  uint32_t delta;
 };
 
 // Words of one tile block in the label buffer: the L x TH x TW labels
@@ -526,12 +537,20 @@ __device__ bool relax_local(const Lds<NARROW>& s, uint32_t L, uint32_t max_local
 // Result flags of one tile visit.
 enum : uint32_t { T_LEFT = 1, T_RIGHT = 2, T_TOP = 4, T_BOTTOM = 8, T_ANY = 16, T_CAPPED = 32 };
 
+// Result of one tile visit (uniform across the workgroup): flags and, per
+// side (left, right, top, bottom, any), the smallest search cost among the
+// cells that improved there (kMax when none did).
+struct TileResult {
+  uint32_t flags;
+  uint32_t key[5];
+};
+
 __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ? v : kUnreached; }
 
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
 template <int TW, int TH, int LM, bool NARROW>
-__device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
+__device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = uniform_u32(A.item_field[item]);
   const FieldDesc F = uniform_copy(A.fields[f]);
@@ -561,6 +580,7 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
     s.misc[5] = 0;  // result flags
     s.misc[8] = 0;  // phase change stamps
     s.misc[9] = 0;
+    for (int k = 10; k < 15; ++k) s.misc[k] = kMax;  // side keys
   }
   // tile edge prices
   for (uint32_t i = tid; i <= TW; i += nt) {
@@ -653,13 +673,19 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
       const uint32_t pc = l * STEP_PLANE + y * PS + x;
       using ST = typename Lds<NARROW>::Step;
-      s.sxl[pc] = to_step<ST>(gate(pass && (m & K_NX), s.xg[x], wgt, ap));
-      s.sxr[pc] = to_step<ST>(gate(pass && (m & K_PX), s.xg[x + 1], wgt, ap));
-      s.syt[pc] = to_step<ST>(gate(pass && (m & K_NY), s.yg[y], wgt, ap));
-      s.syb[pc] = to_step<ST>(gate(pass && (m & K_PY), s.yg[y + 1], wgt, ap));
+      // Narrow boards bound every open step price below 0x7FFF, so it is
+      // exact in 32-bit arithmetic without the saturation of step_price.
+      auto g6 = [&](bool open, uint32_t e) -> ST {
+        if (NARROW) return to_step<ST>(open ? e * wgt + ap : kMax);
+        return to_step<ST>(gate(open, e, wgt, ap));
+      };
+      s.sxl[pc] = g6(pass && (m & K_NX), s.xg[x]);
+      s.sxr[pc] = g6(pass && (m & K_PX), s.xg[x + 1]);
+      s.syt[pc] = g6(pass && (m & K_NY), s.yg[y]);
+      s.syb[pc] = g6(pass && (m & K_PY), s.yg[y + 1]);
       if (L > 1) {
-        s.svd[pc] = to_step<ST>(gate(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0], wgt, ap));
-        s.svu[pc] = to_step<ST>(gate(pass && (m & K_UP) && l + 1 < L, s.vg[l], wgt, ap));
+        s.svd[pc] = g6(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0]);
+        s.svu[pc] = g6(pass && (m & K_UP) && l + 1 < L, s.vg[l]);
       }
       s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[u] ? lab_in(lv[u]) : kUnreached;
     }
@@ -732,8 +758,9 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
   // ---- in-LDS relaxation to a local fixpoint ----------------------------------
   uint32_t rounds = 0;
   const bool capped = relax_local<TW, TH, LM, NARROW>(s, L, A.max_local, rounds);
-  // ---- write back: labels, border-column copies, side flags -------------------
+  // ---- write back: labels, border-column copies, side flags and keys -----------
   uint32_t flags = 0;
+  uint32_t kl = kMax, kr = kMax, kt = kMax, kb = kMax, ka = kMax;
   for (uint32_t c = tid; c < ncell; c += nt) {
     const uint32_t l = c / PT, r = c - l * PT, y = r / TW, x = r - y * TW;
     const bool chg = ((s.rchg[l * TH + y] >> x) | (s.cchg[l * TW + x] >> y) | (s.vchg[r] >> l)) & 1u;
@@ -745,18 +772,37 @@ __device__ uint32_t process_tile(const RelaxArgs& A, uint32_t item, char* smem) 
       if (x == TW - 1)
         __hip_atomic_store(&A.labels[blockbase + ncell + L * TH + l * TH + y], v, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (chg) {
-      flags |= T_ANY;
-      if (x == 0) flags |= T_LEFT;
-      if (x == TW - 1) flags |= T_RIGHT;
-      if (y == 0) flags |= T_TOP;
-      if (y == TH - 1) flags |= T_BOTTOM;
+      if (chg) {
+        // A changed border cell wakes the neighbour across that side only if
+        // it can improve the neighbour's adjacent cell: every candidate it
+        // offers is at least (dist, hops + 1) = v + 1, so a halo label (the
+        // neighbour's label at load time, which only decreases) at or below
+        // v + 1 cannot improve, and no path into the neighbour tile through
+        // this cell can either.
+        const uint32_t d = (uint32_t)(v >> 32);
+        const uint64_t lb = v + 1;
+        const uint64_t* row = s.lab + l * LAB_PLANE + (y + 1) * RS;
+        flags |= T_ANY;
+        ka = min(ka, d);
+        if (x == 0 && lb < row[0]) flags |= T_LEFT, kl = min(kl, d);
+        if (x == TW - 1 && lb < row[TW + 1]) flags |= T_RIGHT, kr = min(kr, d);
+        if (y == 0 && lb < s.lab[l * LAB_PLANE + x + 1]) flags |= T_TOP, kt = min(kt, d);
+        if (y == TH - 1 && lb < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1]) flags |= T_BOTTOM, kb = min(kb, d);
+      }
     }
   }
-  if (flags) atomicOr(&s.misc[5], flags);
+  if (flags) {
+    atomicOr(&s.misc[5], flags);
+    atomicMin(&s.misc[14], ka);
+    if (kl != kMax) atomicMin(&s.misc[10], kl);
+    if (kr != kMax) atomicMin(&s.misc[11], kr);
+    if (kt != kMax) atomicMin(&s.misc[12], kt);
+    if (kb != kMax) atomicMin(&s.misc[13], kb);
+  }
   __syncthreads();  // also orders this workgroup's label stores before the flag / queue updates
-  const uint32_t res = uniform_u32(s.misc[5]) | (capped ? T_CAPPED : 0u);
+  TileResult res;
+  res.flags = uniform_u32(s.misc[5]) | (capped ? T_CAPPED : 0u);
+  for (int k = 0; k < 5; ++k) res.key[k] = uniform_u32(s.misc[10 + k]);
   if (tid == 0) {
     if (!touched) __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
@@ -775,7 +821,8 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 }
 
 // Activate `nb` for the next round once (flag + append to the next list).
-__device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb) {
+__device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb, uint32_t key) {
+  if (A.key_next) atomicMin(&A.key_next[nb], key);
   if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[A.next_ctr * kCtrStride], 1u)] = nb;
 }
 
@@ -792,20 +839,45 @@ struct MinWaves {
 template <int TW, int TH, int LM, bool NARROW>
 __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
-  if (blockIdx.x == 0 && threadIdx.x == 0 && A.clear_ctr != A.next_ctr)
-    __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint32_t item = A.list_cur[blockIdx.x];
-  uint32_t res = process_tile<TW, TH, LM, NARROW>(A, item, smem);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (A.clear_ctr != A.next_ctr)
+      __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (A.key_cur && A.key_clear != A.key_ctr)
+      __hip_atomic_store(&A.counters[A.key_clear * kCtrStride], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const uint32_t item = uniform_u32(A.list_cur[blockIdx.x]);
+  uint32_t key = 0;
+  bool defer = false;
+  if (A.key_cur) {
+    key = uniform_u32(A.key_cur[item]);
+    defer = key > uniform_u32(__hip_atomic_load(&A.counters[C_THRESH * kCtrStride], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT));
+  }
+  TileResult res{};
+  if (!defer) res = process_tile<TW, TH, LM, NARROW>(A, item, smem);
   if (threadIdx.x == 0) {
     A.act_cur[item] = 0;
-    const FieldDesc F = A.fields[A.item_field[item]];
-    uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-    if ((res & T_LEFT) && txi > 0) activate(A, item - 1);
-    if ((res & T_RIGHT) && txi + 1 < F.tx) activate(A, item + 1);
-    if ((res & T_TOP) && tyi > 0) activate(A, item - F.tx);
-    if ((res & T_BOTTOM) && tyi + 1 < F.ty) activate(A, item + F.tx);
-    if (res & T_CAPPED) activate(A, item);
-    if (res & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
+    uint32_t kmin = kMax;
+    if (A.key_cur) A.key_cur[item] = kMax;
+    if (defer) {
+      activate(A, item, key);  // carried over to the next round with its key
+      kmin = key;
+    } else {
+      const FieldDesc F = A.fields[A.item_field[item]];
+      const uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
+      const uint32_t f = res.flags;
+      auto act = [&](uint32_t nb, uint32_t k) {
+        activate(A, nb, k);
+        kmin = min(kmin, k);
+      };
+      if ((f & T_LEFT) && txi > 0) act(item - 1, res.key[0]);
+      if ((f & T_RIGHT) && txi + 1 < F.tx) act(item + 1, res.key[1]);
+      if ((f & T_TOP) && tyi > 0) act(item - F.tx, res.key[2]);
+      if ((f & T_BOTTOM) && tyi + 1 < F.ty) act(item + F.tx, res.key[3]);
+      if (f & T_CAPPED) act(item, res.key[4] == kMax ? key : res.key[4]);
+      if (f & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
+    }
+    if (A.key_cur && kmin != kMax) atomicMin(&A.counters[A.key_ctr * kCtrStride], kmin);
     if (A.host_out) {
       // The last workgroup of the launch publishes the counts.  Every
       // workgroup's activation atomics returned (their results index the
@@ -817,6 +889,13 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
                                                    __HIP_MEMORY_SCOPE_AGENT);
       if (prev == gridDim.x - 1) {
         __hip_atomic_store(&A.counters[C_DONE * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (A.key_cur) {
+          const uint64_t kn = __hip_atomic_load(&A.counters[A.key_ctr * kCtrStride], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t th = kn + A.delta;
+          __hip_atomic_store(&A.counters[C_THRESH * kCtrStride], th < kMax ? (uint32_t)th : kMax, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        }
         const uint32_t v[4] = {
             __hip_atomic_load(&A.counters[A.next_ctr * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
             __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
@@ -857,16 +936,17 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
   // the compiler so lanes other than 0 re-run the body without a new pop.
   for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
     uint32_t item = uniform_u32(R.list_cur[k]);
-    uint32_t res = process_tile<TW, TH, LM, NARROW>(R, item, smem);
+    const TileResult res = process_tile<TW, TH, LM, NARROW>(R, item, smem);
     if (threadIdx.x == 0) {
       R.act_cur[item] = 0;
       const FieldDesc F = R.fields[R.item_field[item]];
       uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-      if ((res & T_LEFT) && txi > 0) activate(R, item - 1);
-      if ((res & T_RIGHT) && txi + 1 < F.tx) activate(R, item + 1);
-      if ((res & T_TOP) && tyi > 0) activate(R, item - F.tx);
-      if ((res & T_BOTTOM) && tyi + 1 < F.ty) activate(R, item + F.tx);
-      if (res & T_CAPPED) activate(R, item);
+      const uint32_t f = res.flags;
+      if ((f & T_LEFT) && txi > 0) activate(R, item - 1, res.key[0]);
+      if ((f & T_RIGHT) && txi + 1 < F.tx) activate(R, item + 1, res.key[1]);
+      if ((f & T_TOP) && tyi > 0) activate(R, item - F.tx, res.key[2]);
+      if ((f & T_BOTTOM) && tyi + 1 < F.ty) activate(R, item + F.tx, res.key[3]);
+      if (f & T_CAPPED) activate(R, item, res.key[4]);
     }
     __syncthreads();
   }
@@ -890,11 +970,12 @@ __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
 }
 
 // Activate items idx[0..n) for the first round: flags and list.
-__global__ void seed_list(uint32_t* act, uint32_t* list, const uint32_t* idx, uint32_t n) {
+__global__ void seed_list(uint32_t* act, uint32_t* list, const uint32_t* idx, uint32_t n, uint32_t* key) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     act[idx[i]] = 1u;
     list[i] = idx[i];
+    if (key) key[idx[i]] = 0u;  // sources start at search cost 0
   }
 }
 
@@ -1141,7 +1222,7 @@ void ctx_init(Ctx& c, int device) {
 
 void ctx_free(Ctx& c) {
   Workspace& w = c.ws;
-  void* ptrs[] = {w.labels, w.state, w.act, w.list, w.queue, w.fields, w.item_field, w.pool, w.counters,
+  void* ptrs[] = {w.labels, w.state, w.act, w.key, w.list, w.queue, w.fields, w.item_field, w.pool, w.counters,
                   w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards};
   (void)hipSetDevice(c.device);
   for (void* p : ptrs)
@@ -1288,6 +1369,9 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
   A.host_out = nullptr;
+  A.key_cur = A.key_next = nullptr;
+  A.key_ctr = A.key_clear = C_KEY0;
+  A.delta = kMax;
   return A;
 }
 
@@ -1417,6 +1501,8 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   grow(W.state, W.state_cap, std::max<size_t>(item, 1));
   grow(W.queue, W.queue_cap, std::max<size_t>(item, 1));
   grow(W.act, W.act_cap, std::max<size_t>((size_t)item * 2, 2));
+  const bool keyed = c.mode == MODE_PLAIN && c.delta != kMax;
+  if (keyed) grow(W.key, W.key_cap, std::max<size_t>((size_t)item * 2, 2));
   grow(W.list, W.list_cap, std::max<size_t>((size_t)item * 2, 2));
   grow(W.fields, W.fields_cap, std::max<size_t>(reqs.size(), 1));
   grow(W.item_field, W.item_field_cap, std::max<size_t>(item, 1));
@@ -1456,6 +1542,8 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       while (n_active) {
         A.next_ctr = C_NEXT0 + (rounds & 1);
         A.clear_ctr = C_NEXT0 + ((rounds + 1) & 1);
+        A.key_ctr = C_KEY0 + (rounds & 1);
+        A.key_clear = C_KEY0 + ((rounds + 1) & 1);
         HIP_OK(hipEventRecord(W.ek0, s));
         HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
         HIP_OK(hipEventRecord(W.ek1, s));
@@ -1469,6 +1557,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         if (hc[1]) break;
         n_active = hc[0];
         std::swap(A.act_cur, A.act_next);
+        std::swap(A.key_cur, A.key_next);
         uint32_t* lc = const_cast<uint32_t*>(A.list_cur);
         A.list_cur = A.list_next;
         A.list_next = lc;
@@ -1478,14 +1567,24 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     uint32_t rounds = 0;
     launch_fill32(s, W.act, 0u, (size_t)item * 2);
     if (c.mode == MODE_PLAIN) {
+      if (keyed) {
+        // every item starts unkeyed; seeds at key 0, so the first threshold is delta
+        launch_fill32(s, W.key, kMax, (size_t)item * 2);
+        launch_fill32(s, W.counters + C_KEY0 * kCtrStride, kMax, 1);
+        launch_fill32(s, W.counters + (C_KEY0 + 1) * kCtrStride, kMax, 1);
+        launch_fill32(s, W.counters + C_THRESH * kCtrStride, c.delta, 1);
+        A.key_cur = W.key;
+        A.key_next = W.key + item;
+        A.delta = c.delta;
+      }
       hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
-                         d_seeds, (uint32_t)seeds.size());
+                         d_seeds, (uint32_t)seeds.size(), keyed ? W.key : (uint32_t*)nullptr);
       HIP_OK(hipEventRecord(W.ev0, s));
       timed = true;
       plain_rounds(rounds, (uint32_t)seeds.size());
     } else if (c.mode == MODE_GRAPH) {
       hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
-                         d_seeds, (uint32_t)seeds.size());
+                         d_seeds, (uint32_t)seeds.size(), (uint32_t*)nullptr);
       launch_fill32(s, W.counters + C_CURN * kCtrStride, (uint32_t)seeds.size(), 1);
       int occ = 0;
       HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.list, S.threads, lds));
@@ -1521,6 +1620,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   if (timed) HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   local.relax_ms = ms;
   local.tile_visits = cnt[C_VISITS * kCtrStride];
+  local.changed_visits = cnt[C_CHANGED * kCtrStride];
   local.local_rounds = cnt[C_ROUNDS * kCtrStride];
   local.cells_visited = (uint64_t)cnt[C_VISITS * kCtrStride] * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1906,7 +2006,8 @@ uint32_t driver_step(RoundDriver& d) {
   if (!d.pending_seeds.empty()) {
     const uint32_t k = (uint32_t)d.pending_seeds.size();
     stage_copy(d, d.seeds, d.pending_seeds);
-    hipLaunchKernelGGL(seed_list, dim3((k + 255) / 256), dim3(256), 0, s, act_cur, list_cur + d.n_cur, d.seeds, k);
+    hipLaunchKernelGGL(seed_list, dim3((k + 255) / 256), dim3(256), 0, s, act_cur, list_cur + d.n_cur, d.seeds, k,
+                       (uint32_t*)nullptr);
     d.n_cur += k;
     d.pending_seeds.clear();
   }
@@ -1938,6 +2039,9 @@ uint32_t driver_step(RoundDriver& d) {
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
   A.host_out = nullptr;
+  A.key_cur = A.key_next = nullptr;
+  A.key_ctr = A.key_clear = C_KEY0;
+  A.delta = kMax;
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;

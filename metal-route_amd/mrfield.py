@@ -74,12 +74,12 @@ class StatsS(C.Structure):
     _fields_ = [("relax_ms", C.c_double), ("path_ms", C.c_double), ("total_ms", C.c_double),
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
                 ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double),
-                ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32)]
+                ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32), ("changed_visits", C.c_uint64)]
 
 
 class TuningS(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("threads", C.c_uint32), ("max_local", C.c_uint32),
-                ("mode", C.c_int), ("graph_rounds", C.c_uint32), ("sample_ms", C.c_double),
+                ("mode", C.c_int), ("graph_rounds", C.c_uint32), ("delta", C.c_uint32), ("sample_ms", C.c_double),
                 ("default_ms", C.c_double), ("tried", C.c_uint32)]
 
 
@@ -136,6 +136,7 @@ def lib():
         "mrf_close": (None, [C.c_void_p]),
         "mrf_set_tiles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, E]),
         "mrf_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
+        "mrf_set_delta": (C.c_int, [C.c_void_p, C.c_uint32]),
         "mrf_last_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsS)]),
         "mrf_route_isolated_batch": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(NetS), C.c_size_t,
                                                C.POINTER(WindowS), C.POINTER(EdgesS), C.POINTER(PathsS), E]),
@@ -369,6 +370,11 @@ class Engine:
     def set_mode(self, mode: int):
         if lib().mrf_set_mode(self._h, mode) != OK:
             raise MrfError(BACKEND_UNAVAILABLE, "bad mode")
+
+    def set_delta(self, delta: int):
+        """Delta-stepping window of the plain form (MAX = off)."""
+        if lib().mrf_set_delta(self._h, delta) != OK:
+            raise MrfError(BACKEND_UNAVAILABLE, "bad delta")
 
     def stats(self) -> dict:
         s = StatsS()
