@@ -347,3 +347,28 @@ def test_fail_closed_contracts(eng):
         eng.route_isolated_batch(g, [mrfield.Net(0, 1)], [(0, 0, 2, 1)], mrfield.Edges([16], [16], []))
     assert x.value.code == mrfield.BACKEND_UNAVAILABLE
     assert eng.route_isolated_batch(g, [], [], e) == []
+
+
+def test_autotune_keeps_results(eng):
+    """The autotuner only changes time: paths after tuning equal the oracle's."""
+    s = mrfield.synth_config(3, seed=2, scale_nets=48)
+    g = s.grid
+    board = mrfield.Board(eng, g, s.edges, s.pricing())
+    try:
+        t = board.autotune(s.nets(), windows=s.windows, sample=16)
+        assert t["tried"] > 10 and t["sample_ms"] > 0
+        assert (t["tile_w"], t["tile_h"]) in [(32, 16), (16, 16), (16, 8), (8, 4)]
+        got = board.solve(s.nets(), windows=s.windows, retry_full=True)
+    finally:
+        board.close()
+        configure(eng, (16, 16, 256, 32), mrfield.MODE_PLAIN)
+        eng.set_delta(MAX)
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    spec = mrcpu.Spec(og, oe, additive=s.additive)
+    nets = [mrcpu.Net(int(a), int(b)) for a, b in zip(s.src, s.dst)]
+    ref_p, ref_c = mrcpu.alone_batch(spec, nets, s.windows, retry_full=True, threads=8)
+    for i, r in enumerate(got):
+        assert (r is None) == (ref_p[i] is None), i
+        if r is not None:
+            assert r[0] == ref_p[i] and r[1] == ref_c[i], i
