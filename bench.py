@@ -248,7 +248,7 @@ def main():
         return board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
-    kern_ms, launches, changed = [], [], []
+    kern_ms, launches, changed, valid_ms, call_ms = [], [], [], [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
@@ -267,6 +267,8 @@ def main():
             host_rounds.append(st["host_rounds"])
             kern_ms.append(st["relax_kernel_ms"])
             changed.append(st["changed_visits"])
+            valid_ms.append(st["validate_ms"])
+            call_ms.append(st["total_ms"])
             launches.append(st["relax_launches"])
     wall = time.perf_counter() - t_start
     n_nets = len(s.pins) if multi else len(nets)
@@ -318,6 +320,8 @@ def main():
         "relax_launches": int(n_launch),
         "relax_avg_launch_ms": round(avg_launch_ms, 5),
         "path_ms": round(float(np.mean(path_ms)), 4),
+        "validate_ms": round(float(np.mean(valid_ms)), 4),
+        "solve_call_ms": round(float(np.mean(call_ms)), 4),
         "tile_visits": int(np.mean(visits)),
         "changed_visits": int(np.mean(changed)),
         "local_rounds_per_visit": round(float(np.mean(rounds_l)) / max(float(np.mean(visits)), 1), 3),

@@ -157,6 +157,7 @@ typedef struct {
                               form; the graph form reports its replay time) */
   uint32_t relax_launches; /* relaxation kernel launches */
   uint64_t changed_visits; /* tile visits that improved at least one label */
+  double validate_ms;      /* host re-validation of the device paths */
 } mrf_stats;
 
 /* Solve form of a context.

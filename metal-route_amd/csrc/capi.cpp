@@ -103,6 +103,7 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
     if (c.inject == 1) throw Error(MRF_BACKEND_UNAVAILABLE, "injected backend failure");
     if (c.inject == 2 && !cells.empty()) cells[cells.size() / 2] ^= 1u;
     // Re-validate on up to 8 host threads; any failure fails the whole call.
+    const auto tv0 = std::chrono::steady_clock::now();
     std::vector<uint8_t> bad(k1 - k0, 0);
     auto check = [&](size_t a, size_t b) {
       for (size_t k = a; k < b; ++k) {
@@ -132,6 +133,8 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
       }
       for (auto& th : pool) th.join();
     }
+    c.last.validate_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tv0).count();
     for (uint8_t v : bad)
       if (v) throw Error(MRF_BACKEND_UNAVAILABLE, "device path failed host re-validation");
     k0 = k1;
@@ -334,6 +337,7 @@ int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   out->relax_kernel_ms = s.kernel_ms;
   out->relax_launches = s.launches;
   out->changed_visits = s.changed_visits;
+  out->validate_ms = s.validate_ms;
   return MRF_OK;
 }
 

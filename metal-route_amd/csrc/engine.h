@@ -113,6 +113,7 @@ struct SolveStats {
   double persistent_ms = 0;    // persistent kernel alone (persistent form)
   uint32_t replays = 0;        // hipGraph replays (graph form)
   double kernel_ms = 0;        // relaxation kernels alone: sum of per-launch event times (plain form)
+  double validate_ms = 0;      // host re-validation of device paths
   uint32_t launches = 0;       // relaxation kernel launches
   void add(const SolveStats& o) {
     relax_ms += o.relax_ms;
@@ -127,6 +128,7 @@ struct SolveStats {
     persistent_ms += o.persistent_ms;
     replays += o.replays;
     kernel_ms += o.kernel_ms;
+    validate_ms += o.validate_ms;
     launches += o.launches;
   }
 };

@@ -278,7 +278,7 @@ struct RelaxArgs {
   // Plain form: the round's activation count goes to counter next_ctr;
   // counter clear_ctr (the previous round's, already read) is zeroed for the
   // next round; the last workgroup to finish publishes the counts to the
-  // host-mapped words host_out (next count, error flags, visits, rounds).
+  // host-mapped words host_out (next count, error flags).
   uint32_t next_ctr, clear_ctr;
   uint32_t* host_out;
   // Delta stepping (key_cur non-null): every activation carries a key, the
@@ -538,11 +538,11 @@ __device__ bool relax_local(const Lds<NARROW>& s, uint32_t L, uint32_t max_local
 enum : uint32_t { T_LEFT = 1, T_RIGHT = 2, T_TOP = 4, T_BOTTOM = 8, T_ANY = 16, T_CAPPED = 32 };
 
 // Result of one tile visit (uniform across the workgroup): flags and, per
-// side (left, right, top, bottom, any), the smallest search cost among the
-// cells that improved there (kMax when none did).
+// side (left, right, top, bottom) and over the tile (ka), the smallest search
+// cost among the cells that improved there (kMax when none did).
 struct TileResult {
   uint32_t flags;
-  uint32_t key[5];
+  uint32_t kl, kr, kt, kb, ka;
 };
 
 __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ? v : kUnreached; }
@@ -566,7 +566,13 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   const uint64_t TB = tile_block_words(L, TW, TH);
   const uint64_t blockbase = F.lab_off + (uint64_t)t * TB;
 
-  // state of this item and its four neighbours (0 when there is none)
+  // ---- loads that do not depend on the tile states go first -----------------
+  // The states of this item and its neighbours decide whether labels exist;
+  // the board planes of the first batch of cells and the neighbours' border
+  // labels (speculatively: the address is valid whenever the neighbour
+  // exists, the value is kept only if it was visited) are issued before the
+  // barrier that publishes the states, so one memory round trip covers them.
+  uint32_t stv = 0;
   if (tid < 5) {
     uint32_t nb = item;
     bool ok = true;
@@ -574,47 +580,60 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     if (tid == 2) ok = txi + 1 < F.tx, nb = item + 1;
     if (tid == 3) ok = tyi > 0, nb = item - F.tx;
     if (tid == 4) ok = tyi + 1 < F.ty, nb = item + F.tx;
-    s.misc[tid] = ok ? __hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    stv = ok ? __hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
   }
-  if (tid == 0) {
-    s.misc[5] = 0;  // result flags
-    s.misc[8] = 0;  // phase change stamps
-    s.misc[9] = 0;
-    for (int k = 10; k < 15; ++k) s.misc[k] = kMax;  // side keys
+  // halo ring, two entries per thread: sides 0..3 = left, right, top, bottom
+  const uint32_t nhalo = 2u * L * (TH + TW);
+  auto halo_entry = [&](uint32_t j, uint64_t& src, uint32_t& at, uint32_t& side, bool& exists) {
+    if (j < 2u * L * TH) {
+      side = j / (L * TH);
+      const uint32_t k = j - side * L * TH, l = k / TH, y = k - l * TH;
+      const bool row_in = wy0 + y < F.wh;
+      if (side == 0) {
+        exists = row_in && txi > 0;
+        src = blockbase - TB + ncell + L * TH + k;  // left tile's last column
+      } else {
+        exists = row_in && txi + 1 < F.tx;
+        src = blockbase + TB + ncell + k;  // right tile's first column
+      }
+      at = l * LAB_PLANE + (y + 1) * RS + (side ? TW + 1 : 0);
+    } else {
+      const uint32_t jj = j - 2u * L * TH, sd = jj / (L * TW), k = jj - sd * L * TW;
+      const uint32_t l = k / TW, x = k - l * TW;
+      const bool col_in = wx0 + x < F.ww;
+      side = 2 + sd;
+      if (sd == 0) {
+        exists = col_in && tyi > 0;
+        src = blockbase - (uint64_t)F.tx * TB + l * PT + (TH - 1) * TW + x;
+      } else {
+        exists = col_in && tyi + 1 < F.ty;
+        src = blockbase + (uint64_t)F.tx * TB + l * PT + x;
+      }
+      at = l * LAB_PLANE + (sd ? (TH + 1) * RS : 0) + x + 1;
+    }
+    if (!exists) src = blockbase;
+  };
+  uint64_t hv[2];
+  uint32_t hat[2], hside[2];
+  bool hok[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const uint32_t j = u * nt + tid;
+    uint64_t src = blockbase;
+    hat[u] = 0;
+    hside[u] = 0;
+    hok[u] = false;
+    if (j < nhalo) halo_entry(j, src, hat[u], hside[u], hok[u]);
+    hv[u] = __hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  // tile edge prices
-  for (uint32_t i = tid; i <= TW; i += nt) {
-    uint32_t gx = F.x0 + wx0 + i;  // entering column gx from gx-1
-    s.xg[i] = (gx >= 1 && gx < B.w) ? gbl(B.xe)[gx - 1] : 0u;
-  }
-  for (uint32_t i = tid; i <= TH; i += nt) {
-    uint32_t gy = F.y0 + wy0 + i;
-    s.yg[i] = (gy >= 1 && gy < B.h) ? gbl(B.ye)[gy - 1] : 0u;
-  }
-  for (uint32_t i = tid; i < 16; i += nt) s.vg[i] = (i + 1 < L) ? gbl(B.ve)[i] : 0u;
-  for (uint32_t i = tid; i < L * TH; i += nt) s.rchg[i] = 0;
-  for (uint32_t i = tid; i < L * TW; i += nt) s.cchg[i] = 0;
-  for (uint32_t i = tid; i < PT; i += nt) s.vchg[i] = 0;
-  __syncthreads();
-  const bool touched = (uniform_u32(s.misc[0]) & S_TOUCHED) != 0;
-  const bool n_left = (uniform_u32(s.misc[1]) & S_TOUCHED) != 0;
-  const bool n_right = (uniform_u32(s.misc[2]) & S_TOUCHED) != 0;
-  const bool n_top = (uniform_u32(s.misc[3]) & S_TOUCHED) != 0;
-  const bool n_bottom = (uniform_u32(s.misc[4]) & S_TOUCHED) != 0;
-
-  // Interior cells, four per thread per batch.  Every load of a batch is
-  // issued before the first is used (addresses are clamped into the window,
-  // so the loads are unconditional; cells outside the window are closed
-  // afterwards).  Each cell gets its per-net view (cell_info: own pads, forced
-  // endpoints, via exemptions, additive and halo pricing) and the six step
-  // prices of entering it.
-  const uint32_t* pads = A.pool + F.pad_off;
-  for (uint32_t base = 0; base < ncell; base += 4u * nt) {
-    uint64_t lv[4];
-    uint32_t gv[4], wv[4], av[4], iv[4];
-    int32_t ov[4];
-    uint8_t bv[4];
-    bool inw[4];
+  // interior planes, four cells per thread per batch (addresses clamped into
+  // the window, so the loads are unconditional; cells outside the window are
+  // closed afterwards)
+  uint32_t gv[4], wv[4], av[4], iv[4];
+  int32_t ov[4];
+  uint8_t bv[4];
+  bool inw[4];
+  auto load_planes = [&](uint32_t base) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t i = base + u * nt + tid;
@@ -641,6 +660,53 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
 #pragma unroll
       for (int u = 0; u < 4; ++u) ov[u] = -1;
     }
+  };
+  load_planes(0);
+  if (tid < 5) s.misc[tid] = stv;
+  if (tid == 0) {
+    s.misc[5] = 0;  // result flags
+    s.misc[8] = 0;  // phase change stamps
+    s.misc[9] = 0;
+    for (int k = 10; k < 15; ++k) s.misc[k] = kMax;  // side keys
+  }
+  // tile edge prices
+  for (uint32_t i = tid; i <= TW; i += nt) {
+    uint32_t gx = F.x0 + wx0 + i;  // entering column gx from gx-1
+    s.xg[i] = (gx >= 1 && gx < B.w) ? gbl(B.xe)[gx - 1] : 0u;
+  }
+  for (uint32_t i = tid; i <= TH; i += nt) {
+    uint32_t gy = F.y0 + wy0 + i;
+    s.yg[i] = (gy >= 1 && gy < B.h) ? gbl(B.ye)[gy - 1] : 0u;
+  }
+  for (uint32_t i = tid; i < 16; i += nt) s.vg[i] = (i + 1 < L) ? gbl(B.ve)[i] : 0u;
+  for (uint32_t i = tid; i < L * TH; i += nt) s.rchg[i] = 0;
+  for (uint32_t i = tid; i < L * TW; i += nt) s.cchg[i] = 0;
+  for (uint32_t i = tid; i < PT; i += nt) s.vchg[i] = 0;
+  __syncthreads();
+  const bool touched = (uniform_u32(s.misc[0]) & S_TOUCHED) != 0;
+  const uint32_t nbv = ((uniform_u32(s.misc[1]) & S_TOUCHED) ? 1u : 0u) | ((uniform_u32(s.misc[2]) & S_TOUCHED) ? 2u : 0u) |
+                       ((uniform_u32(s.misc[3]) & S_TOUCHED) ? 4u : 0u) | ((uniform_u32(s.misc[4]) & S_TOUCHED) ? 8u : 0u);
+  // halo labels into the ring (a neighbour never visited has none)
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    if (u * nt + tid < nhalo) s.lab[hat[u]] = (hok[u] && ((nbv >> hside[u]) & 1u)) ? lab_in(hv[u]) : kUnreached;
+  for (uint32_t j = 2 * nt + tid; j < nhalo; j += nt) {
+    uint64_t src;
+    uint32_t at, side;
+    bool ex;
+    halo_entry(j, src, at, side, ex);
+    s.lab[at] = (ex && ((nbv >> side) & 1u))
+                    ? lab_in(__hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    : kUnreached;
+  }
+
+  // Interior cells: labels (when the tile was visited before), the per-net
+  // view of each cell (cell_info: own pads, forced endpoints, via exemptions,
+  // additive and halo pricing) and the six step prices of entering it.
+  const uint32_t* pads = A.pool + F.pad_off;
+  for (uint32_t base = 0; base < ncell; base += 4u * nt) {
+    if (base) load_planes(base);
+    uint64_t lv[4];
     if (touched) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) lv[u] = gbl(A.labels)[blockbase + iv[u]];
@@ -688,53 +754,6 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         s.svu[pc] = g6(pass && (m & K_UP) && l + 1 < L, s.vg[l]);
       }
       s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[u] ? lab_in(lv[u]) : kUnreached;
-    }
-  }
-  // Halo ring, two items per thread per batch: left / right columns come from
-  // the side neighbours' border-column copies, top / bottom rows from the
-  // neighbours' label blocks.  Missing neighbours and cells outside the window
-  // read the tile's own first label (a valid address) and are replaced by
-  // kUnreached.
-  const uint32_t nhalo = 2u * L * (TH + TW);
-  for (uint32_t base = 0; base < nhalo; base += 2u * nt) {
-    uint64_t hv[2];
-    uint32_t at[2];
-    bool ok[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const uint32_t j = min(base + u * nt + tid, nhalo - 1);
-      uint64_t src = blockbase;
-      bool valid = false;
-      if (j < 2u * L * TH) {
-        const uint32_t side = j / (L * TH), k = j - side * L * TH, l = k / TH, y = k - l * TH;
-        const bool row_in = wy0 + y < F.wh;
-        if (side == 0) {
-          valid = row_in && n_left;
-          src = valid ? blockbase - TB + ncell + L * TH + k : blockbase;  // left tile's last column
-        } else {
-          valid = row_in && n_right;
-          src = valid ? blockbase + TB + ncell + k : blockbase;  // right tile's first column
-        }
-        at[u] = l * LAB_PLANE + (y + 1) * RS + (side ? TW + 1 : 0);
-      } else {
-        const uint32_t jj = j - 2u * L * TH, side = jj / (L * TW), k = jj - side * L * TW;
-        const uint32_t l = k / TW, x = k - l * TW;
-        const bool col_in = wx0 + x < F.ww;
-        if (side == 0) {
-          valid = col_in && n_top;
-          src = valid ? blockbase - (uint64_t)F.tx * TB + l * PT + (TH - 1) * TW + x : blockbase;
-        } else {
-          valid = col_in && n_bottom;
-          src = valid ? blockbase + (uint64_t)F.tx * TB + l * PT + x : blockbase;
-        }
-        at[u] = l * LAB_PLANE + (side ? (TH + 1) * RS : 0) + x + 1;
-      }
-      ok[u] = valid && base + u * nt + tid < nhalo;
-      hv[u] = __hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (base + u * nt + tid < nhalo) s.lab[at[u]] = ok[u] ? lab_in(hv[u]) : kUnreached;
     }
   }
   __syncthreads();
@@ -802,7 +821,11 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   __syncthreads();  // also orders this workgroup's label stores before the flag / queue updates
   TileResult res;
   res.flags = uniform_u32(s.misc[5]) | (capped ? T_CAPPED : 0u);
-  for (int k = 0; k < 5; ++k) res.key[k] = uniform_u32(s.misc[10 + k]);
+  res.kl = uniform_u32(s.misc[10]);
+  res.kr = uniform_u32(s.misc[11]);
+  res.kt = uniform_u32(s.misc[12]);
+  res.kb = uniform_u32(s.misc[13]);
+  res.ka = uniform_u32(s.misc[14]);
   if (tid == 0) {
     if (!touched) __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
@@ -870,11 +893,11 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
         activate(A, nb, k);
         kmin = min(kmin, k);
       };
-      if ((f & T_LEFT) && txi > 0) act(item - 1, res.key[0]);
-      if ((f & T_RIGHT) && txi + 1 < F.tx) act(item + 1, res.key[1]);
-      if ((f & T_TOP) && tyi > 0) act(item - F.tx, res.key[2]);
-      if ((f & T_BOTTOM) && tyi + 1 < F.ty) act(item + F.tx, res.key[3]);
-      if (f & T_CAPPED) act(item, res.key[4] == kMax ? key : res.key[4]);
+      if ((f & T_LEFT) && txi > 0) act(item - 1, res.kl);
+      if ((f & T_RIGHT) && txi + 1 < F.tx) act(item + 1, res.kr);
+      if ((f & T_TOP) && tyi > 0) act(item - F.tx, res.kt);
+      if ((f & T_BOTTOM) && tyi + 1 < F.ty) act(item + F.tx, res.kb);
+      if (f & T_CAPPED) act(item, res.ka == kMax ? key : res.ka);
       if (f & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
     }
     if (A.key_cur && kmin != kMax) atomicMin(&A.counters[A.key_ctr * kCtrStride], kmin);
@@ -896,13 +919,11 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
           __hip_atomic_store(&A.counters[C_THRESH * kCtrStride], th < kMax ? (uint32_t)th : kMax, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
         }
-        const uint32_t v[4] = {
-            __hip_atomic_load(&A.counters[A.next_ctr * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-            __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-            __hip_atomic_load(&A.counters[C_VISITS * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-            __hip_atomic_load(&A.counters[C_ROUNDS * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
-        for (int k = 0; k < 4; ++k)
-          __hip_atomic_store(&A.host_out[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint32_t nn =
+            __hip_atomic_load(&A.counters[A.next_ctr * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t er = __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&A.host_out[0], nn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&A.host_out[1], er, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __threadfence_system();
       }
     }
@@ -942,11 +963,11 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
       const FieldDesc F = R.fields[R.item_field[item]];
       uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
       const uint32_t f = res.flags;
-      if ((f & T_LEFT) && txi > 0) activate(R, item - 1, res.key[0]);
-      if ((f & T_RIGHT) && txi + 1 < F.tx) activate(R, item + 1, res.key[1]);
-      if ((f & T_TOP) && tyi > 0) activate(R, item - F.tx, res.key[2]);
-      if ((f & T_BOTTOM) && tyi + 1 < F.ty) activate(R, item + F.tx, res.key[3]);
-      if (f & T_CAPPED) activate(R, item, res.key[4]);
+      if ((f & T_LEFT) && txi > 0) activate(R, item - 1, res.kl);
+      if ((f & T_RIGHT) && txi + 1 < F.tx) activate(R, item + 1, res.kr);
+      if ((f & T_TOP) && tyi > 0) activate(R, item - F.tx, res.kt);
+      if ((f & T_BOTTOM) && tyi + 1 < F.ty) activate(R, item + F.tx, res.kb);
+      if (f & T_CAPPED) activate(R, item, res.ka);
     }
     __syncthreads();
   }
@@ -1007,9 +1028,12 @@ __device__ uint64_t field_label(const FieldRead& R, const BoardView& B, const Fi
   uint32_t wx = gx - F.x0, wy = gy - F.y0;
   if (wx >= F.ww || wy >= F.wh) return kInf;
   uint32_t txi = wx / R.tw, tyi = wy / R.th, t = tyi * F.tx + txi;
-  if (!(R.state[F.item0 + t] & S_TOUCHED)) return kInf;
   uint32_t x = wx - txi * R.tw, y = wy - tyi * R.th;
-  return R.labels[F.lab_off + t * tile_block_words(B.layers, R.tw, R.th) + (l * R.th + y) * R.tw + x];
+  // state and label are loaded together (the label slot exists for every
+  // tile of the field); the label counts only if the tile was visited
+  const uint32_t st = R.state[F.item0 + t];
+  const uint64_t lab = R.labels[F.lab_off + t * tile_block_words(B.layers, R.tw, R.th) + (l * R.th + y) * R.tw + x];
+  return (st & S_TOUCHED) ? lab : kInf;
 }
 
 __global__ void path_lengths(FieldRead R, uint32_t n, uint32_t* plen, uint32_t* pcost) {

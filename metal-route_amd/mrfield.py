@@ -74,7 +74,8 @@ class StatsS(C.Structure):
     _fields_ = [("relax_ms", C.c_double), ("path_ms", C.c_double), ("total_ms", C.c_double),
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
                 ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double),
-                ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32), ("changed_visits", C.c_uint64)]
+                ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32), ("changed_visits", C.c_uint64),
+                ("validate_ms", C.c_double)]
 
 
 class TuningS(C.Structure):
@@ -278,14 +279,41 @@ class Pricing:
                         self._i.ctypes.data_as(I32P) if self._i is not None else None)
 
 
+_NO_PADS = np.zeros(1, dtype=np.uint32)
+
+
+_NET_DTYPE = np.dtype([("src", np.uint32), ("dst", np.uint32), ("pads", np.uintp), ("n_pads", np.uintp),
+                       ("via_pads", np.uintp), ("n_via_pads", np.uintp)], align=True)
+
+
 def _nets(nets: Sequence[Net]):
-    keep = []
+    if all(not n.pads and not n.via_pads for n in nets):
+        # vectorised: one numpy record per net, laid out as mrf_net
+        assert _NET_DTYPE.itemsize == C.sizeof(NetS)
+        rec = np.zeros(max(len(nets), 1), dtype=_NET_DTYPE)
+        if nets:
+            rec["src"] = [n.src for n in nets]
+            rec["dst"] = [n.dst for n in nets]
+        rec["pads"] = _NO_PADS.ctypes.data
+        rec["via_pads"] = _NO_PADS.ctypes.data
+        return C.cast(rec.ctypes.data, C.POINTER(NetS)), [rec, _NO_PADS]
+    keep = [_NO_PADS]
     arr = (NetS * max(len(nets), 1))()
+    empty = _p32(_NO_PADS)
     for i, n in enumerate(nets):
-        p = _u32(list(n.pads) + [0])
-        v = _u32(list(n.via_pads) + [0])
-        keep += [p, v]
-        arr[i] = NetS(n.src, n.dst, _p32(p), len(n.pads), _p32(v), len(n.via_pads))
+        if n.pads:
+            p = _u32(list(n.pads) + [0])
+            keep.append(p)
+            pp = _p32(p)
+        else:
+            pp = empty
+        if n.via_pads:
+            v = _u32(list(n.via_pads) + [0])
+            keep.append(v)
+            vp = _p32(v)
+        else:
+            vp = empty
+        arr[i] = NetS(n.src, n.dst, pp, len(n.pads), vp, len(n.via_pads))
     return arr, keep
 
 
