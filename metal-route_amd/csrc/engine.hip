@@ -778,12 +778,23 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   uint32_t rounds = 0;
   const bool capped = relax_local<TW, TH, LM, NARROW>(s, L, A.max_local, rounds);
   // ---- write back: labels, border-column copies, side flags and keys -----------
+  // Four cells per thread per batch: the change masks of a batch are read
+  // together before any of them is used.
   uint32_t flags = 0;
   uint32_t kl = kMax, kr = kMax, kt = kMax, kb = kMax, ka = kMax;
-  for (uint32_t c = tid; c < ncell; c += nt) {
-    const uint32_t l = c / PT, r = c - l * PT, y = r / TW, x = r - y * TW;
-    const bool chg = ((s.rchg[l * TH + y] >> x) | (s.cchg[l * TW + x] >> y) | (s.vchg[r] >> l)) & 1u;
-    if (!touched || chg) {
+  for (uint32_t base = 0; base < ncell; base += 4u * nt) {
+    bool chg[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t c = min(base + u * nt + tid, ncell - 1);
+      const uint32_t l = c / PT, r = c - l * PT, y = r / TW, x = r - y * TW;
+      chg[u] = ((s.rchg[l * TH + y] >> x) | (s.cchg[l * TW + x] >> y) | (s.vchg[r] >> l)) & 1u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t c = base + u * nt + tid;
+      if (c >= ncell || !(chg[u] || !touched)) continue;
+      const uint32_t l = c / PT, r = c - l * PT, y = r / TW, x = r - y * TW;
       const uint64_t v = s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1];
       __hip_atomic_store(&A.labels[blockbase + c], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (x == 0)
@@ -791,7 +802,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
       if (x == TW - 1)
         __hip_atomic_store(&A.labels[blockbase + ncell + L * TH + l * TH + y], v, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-      if (chg) {
+      if (chg[u]) {
         // A changed border cell wakes the neighbour across that side only if
         // it can improve the neighbour's adjacent cell: every candidate it
         // offers is at least (dist, hops + 1) = v + 1, so a halo label (the
