@@ -177,6 +177,17 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gbl(const 
   return (const __attribute__((address_space(1))) T*)p;
 }
 
+// Four consecutive plane elements in one load (element alignment only).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef u32x4 u32x4_a4 __attribute__((aligned(4)));
+typedef uint8_t u8x4 __attribute__((ext_vector_type(4)));
+typedef u8x4 u8x4_a1 __attribute__((aligned(1)));
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef u64x2 u64x2_a8 __attribute__((aligned(8)));
+
+__device__ __forceinline__ u32x4 load4(const uint32_t* p) { return *(const __attribute__((address_space(1))) u32x4_a4*)p; }
+__device__ __forceinline__ u8x4 load4(const uint8_t* p) { return *(const __attribute__((address_space(1))) u8x4_a1*)p; }
+
 __device__ __forceinline__ bool sorted_has(const uint32_t* a, uint32_t n, uint32_t v) {
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
@@ -626,42 +637,56 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     if (j < nhalo) halo_entry(j, src, hat[u], hside[u], hok[u]);
     hv[u] = __hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  // interior planes, four cells per thread per batch (addresses clamped into
-  // the window, so the loads are unconditional; cells outside the window are
-  // closed afterwards)
-  uint32_t gv[4], wv[4], av[4], iv[4];
+  // Interior planes by quads: four consecutive cells of one row per thread
+  // per batch, loaded with one vector load per plane when the quad lies
+  // inside the window (clamped scalar loads at the window's edge; cells
+  // outside the window are closed afterwards).
+  constexpr uint32_t QPR = TW / 4;  // quads per tile row
+  const uint32_t nquad = L * TH * QPR;
+  uint32_t qrowg = 0, qwxb = 0, qc0 = 0, ql = 0, qy = 0, qx0 = 0;
+  bool qvalid = false;
+  uint32_t wv[4], av[4];
   int32_t ov[4];
   uint8_t bv[4];
   bool inw[4];
-  auto load_planes = [&](uint32_t base) {
+  auto load_planes = [&](uint32_t q) {
+    qvalid = q < nquad;
+    const uint32_t qq = qvalid ? q : 0;
+    ql = qq / (TH * QPR);
+    const uint32_t rq = qq - ql * TH * QPR;
+    qy = rq / QPR;
+    qx0 = (rq - qy * QPR) * 4;
+    qc0 = ql * PT + qy * TW + qx0;
+    const uint32_t wy = wy0 + qy;
+    qwxb = wx0 + qx0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t i = base + u * nt + tid;
-      const uint32_t ic = i < ncell ? i : ncell - 1;
-      const uint32_t l = ic / PT, r = ic - l * PT, y = r / TW, x = r - y * TW;
-      inw[u] = i < ncell && wx0 + x < F.ww && wy0 + y < F.wh;
-      const uint32_t xx = min(wx0 + x, F.ww - 1), yy = min(wy0 + y, F.wh - 1);
-      iv[u] = ic;
-      gv[u] = (l * B.h + F.y0 + yy) * B.w + F.x0 + xx;
-      bv[u] = gbl(B.bmask)[gv[u]];
-      wv[u] = gbl(B.cost)[gv[u]];
-    }
-    if (B.add) {
+    for (int k = 0; k < 4; ++k) inw[k] = qvalid && wy < F.wh && qwxb + k < F.ww;
+    qrowg = (ql * B.h + F.y0 + min(wy, F.wh - 1)) * B.w + F.x0;
+    if (inw[3]) {
+      const uint32_t g0 = qrowg + qwxb;
+      const u8x4 b4 = load4(B.bmask + g0);
+      const u32x4 c4 = load4(B.cost + g0);
+      const u32x4 a4 = B.add ? load4(B.add + g0) : u32x4{0, 0, 0, 0};
+      const u32x4 o4 = B.halo ? load4((const uint32_t*)B.halo + g0) : u32x4{kMax, kMax, kMax, kMax};
 #pragma unroll
-      for (int u = 0; u < 4; ++u) av[u] = gbl(B.add)[gv[u]];
+      for (int k = 0; k < 4; ++k) {
+        bv[k] = b4[k];
+        wv[k] = c4[k];
+        av[k] = a4[k];
+        ov[k] = (int32_t)o4[k];
+      }
     } else {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) av[u] = 0;
-    }
-    if (B.halo) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) ov[u] = gbl(B.halo)[gv[u]];
-    } else {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) ov[u] = -1;
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t g = qrowg + min(qwxb + k, F.ww - 1);
+        bv[k] = gbl(B.bmask)[g];
+        wv[k] = gbl(B.cost)[g];
+        av[k] = B.add ? gbl(B.add)[g] : 0u;
+        ov[k] = B.halo ? gbl(B.halo)[g] : -1;
+      }
     }
   };
-  load_planes(0);
+  load_planes(tid);
   if (tid < 5) s.misc[tid] = stv;
   if (tid == 0) {
     s.misc[5] = 0;  // result flags
@@ -704,38 +729,40 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   // view of each cell (cell_info: own pads, forced endpoints, via exemptions,
   // additive and halo pricing) and the six step prices of entering it.
   const uint32_t* pads = A.pool + F.pad_off;
-  for (uint32_t base = 0; base < ncell; base += 4u * nt) {
-    if (base) load_planes(base);
-    uint64_t lv[4];
+  for (uint32_t qbase = 0; qbase < nquad; qbase += nt) {
+    if (qbase) load_planes(qbase + tid);
+    if (!qvalid) continue;
+    uint64_t lv[4] = {kUnreached, kUnreached, kUnreached, kUnreached};
     if (touched) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) lv[u] = gbl(A.labels)[blockbase + iv[u]];
-    } else {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) lv[u] = kUnreached;
+      // a quad is contiguous in the tile block whatever the window
+      const auto* lp = (const __attribute__((address_space(1))) u64x2_a8*)(A.labels + blockbase + qc0);
+      const u64x2 l01 = lp[0], l23 = lp[1];
+      lv[0] = l01[0];
+      lv[1] = l01[1];
+      lv[2] = l23[0];
+      lv[3] = l23[1];
     }
+    const uint32_t l = ql, y = qy;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t i = base + u * nt + tid;
-      if (i >= ncell) continue;
-      const uint32_t l = i / PT, r = i - l * PT, y = r / TW, x = r - y * TW;
-      const uint8_t bm = inw[u] ? bv[u] : (uint8_t)M_TRACE;  // outside the window: closed
-      const uint32_t g = gv[u];
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t x = qx0 + k;
+      const uint8_t bm = inw[k] ? bv[k] : (uint8_t)M_TRACE;  // outside the window: closed
+      const uint32_t g = qrowg + min(qwxb + k, F.ww - 1);
       const bool trace = (bm & M_TRACE) != 0, obst = (bm & M_OBST) != 0;
       uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
       bool pass = !trace && !obst;
-      uint32_t wgt = wv[u];
+      uint32_t wgt = wv[k];
       if (obst && !trace && F.n_pad && sorted_has(pads, F.n_pad, g)) {
         pass = true;
         wgt = 1u;
       }
       if (!pass && !trace && (F.flags & F_FORCE)) {
-        for (uint32_t k = 0; k < F.n_forced; ++k)
-          if (A.pool[F.forced_off + k] == g) pass = true;
+        for (uint32_t j = 0; j < F.n_forced; ++j)
+          if (A.pool[F.forced_off + j] == g) pass = true;
       }
       if (pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
-      uint64_t a64 = av[u];
-      if (ov[u] != -1 && ov[u] != F.own_id) a64 += B.halo_pen;
+      uint64_t a64 = av[k];
+      if (ov[k] != -1 && ov[k] != F.own_id) a64 += B.halo_pen;
       const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
       const uint32_t pc = l * STEP_PLANE + y * PS + x;
       using ST = typename Lds<NARROW>::Step;
@@ -753,7 +780,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         s.svd[pc] = g6(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0]);
         s.svu[pc] = g6(pass && (m & K_UP) && l + 1 < L, s.vg[l]);
       }
-      s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[u] ? lab_in(lv[u]) : kUnreached;
+      s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[k] ? lab_in(lv[k]) : kUnreached;
     }
   }
   __syncthreads();
@@ -809,15 +836,19 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         // neighbour's label at load time, which only decreases) at or below
         // v + 1 cannot improve, and no path into the neighbour tile through
         // this cell can either.
+        // The planar edges are symmetric: when entering this (passable) cell
+        // from a side is closed, leaving it towards that side is too.
         const uint32_t d = (uint32_t)(v >> 32);
         const uint64_t lb = v + 1;
         const uint64_t* row = s.lab + l * LAB_PLANE + (y + 1) * RS;
+        const uint32_t pc = l * STEP_PLANE + y * PS + x;
         flags |= T_ANY;
         ka = min(ka, d);
-        if (x == 0 && lb < row[0]) flags |= T_LEFT, kl = min(kl, d);
-        if (x == TW - 1 && lb < row[TW + 1]) flags |= T_RIGHT, kr = min(kr, d);
-        if (y == 0 && lb < s.lab[l * LAB_PLANE + x + 1]) flags |= T_TOP, kt = min(kt, d);
-        if (y == TH - 1 && lb < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1]) flags |= T_BOTTOM, kb = min(kb, d);
+        if (x == 0 && lb < row[0] && step_of(s.sxl[pc]) != kMax) flags |= T_LEFT, kl = min(kl, d);
+        if (x == TW - 1 && lb < row[TW + 1] && step_of(s.sxr[pc]) != kMax) flags |= T_RIGHT, kr = min(kr, d);
+        if (y == 0 && lb < s.lab[l * LAB_PLANE + x + 1] && step_of(s.syt[pc]) != kMax) flags |= T_TOP, kt = min(kt, d);
+        if (y == TH - 1 && lb < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1] && step_of(s.syb[pc]) != kMax)
+          flags |= T_BOTTOM, kb = min(kb, d);
       }
     }
   }
