@@ -552,8 +552,9 @@ enum : uint32_t { T_LEFT = 1, T_RIGHT = 2, T_TOP = 4, T_BOTTOM = 8, T_ANY = 16, 
 // side (left, right, top, bottom) and over the tile (ka), the smallest search
 // cost among the cells that improved there (kMax when none did).
 struct TileResult {
-  uint32_t flags;
+  uint32_t flags;  // side flags only where that neighbour exists
   uint32_t kl, kr, kt, kb, ka;
+  uint32_t tx;  // tiles per row of the field (item distance of top / bottom)
 };
 
 __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ? v : kUnreached; }
@@ -564,6 +565,10 @@ template <int TW, int TH, int LM, bool NARROW>
 __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = uniform_u32(A.item_field[item]);
+  // the item's own state travels with the descriptor loads: whether the tile
+  // holds labels is known before any plane load is issued
+  const bool touched =
+      (uniform_u32(__hip_atomic_load(&A.state[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & S_TOUCHED) != 0;
   const FieldDesc F = uniform_copy(A.fields[f]);
   const BoardView B = uniform_copy(A.boards[F.board]);
   const uint32_t t = item - F.item0;
@@ -578,13 +583,14 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   const uint64_t blockbase = F.lab_off + (uint64_t)t * TB;
 
   // ---- loads that do not depend on the tile states go first -----------------
-  // The states of this item and its neighbours decide whether labels exist;
-  // the board planes of the first batch of cells and the neighbours' border
-  // labels (speculatively: the address is valid whenever the neighbour
-  // exists, the value is kept only if it was visited) are issued before the
-  // barrier that publishes the states, so one memory round trip covers them.
+  // The states of the neighbours decide whether their border labels exist;
+  // the board planes and labels of the first batch of cells and the
+  // neighbours' border labels (speculatively: the address is valid whenever
+  // the neighbour exists, the value is kept only if it was visited) are
+  // issued before the barrier that publishes the states, so one memory round
+  // trip covers them.
   uint32_t stv = 0;
-  if (tid < 5) {
+  if (tid >= 1 && tid < 5) {
     uint32_t nb = item;
     bool ok = true;
     if (tid == 1) ok = txi > 0, nb = item - 1;
@@ -649,6 +655,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   int32_t ov[4];
   uint8_t bv[4];
   bool inw[4];
+  uint64_t lv[4];
   auto load_planes = [&](uint32_t q) {
     qvalid = q < nquad;
     const uint32_t qq = qvalid ? q : 0;
@@ -685,9 +692,21 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         ov[k] = B.halo ? gbl(B.halo)[g] : -1;
       }
     }
+    if (touched) {
+      // a quad is contiguous in the tile block whatever the window
+      const auto* lp = (const __attribute__((address_space(1))) u64x2_a8*)(A.labels + blockbase + qc0);
+      const u64x2 l01 = lp[0], l23 = lp[1];
+      lv[0] = l01[0];
+      lv[1] = l01[1];
+      lv[2] = l23[0];
+      lv[3] = l23[1];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) lv[k] = kUnreached;
+    }
   };
   load_planes(tid);
-  if (tid < 5) s.misc[tid] = stv;
+  if (tid >= 1 && tid < 5) s.misc[tid] = stv;
   if (tid == 0) {
     s.misc[5] = 0;  // result flags
     s.misc[8] = 0;  // phase change stamps
@@ -708,7 +727,6 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   for (uint32_t i = tid; i < L * TW; i += nt) s.cchg[i] = 0;
   for (uint32_t i = tid; i < PT; i += nt) s.vchg[i] = 0;
   __syncthreads();
-  const bool touched = (uniform_u32(s.misc[0]) & S_TOUCHED) != 0;
   const uint32_t nbv = ((uniform_u32(s.misc[1]) & S_TOUCHED) ? 1u : 0u) | ((uniform_u32(s.misc[2]) & S_TOUCHED) ? 2u : 0u) |
                        ((uniform_u32(s.misc[3]) & S_TOUCHED) ? 4u : 0u) | ((uniform_u32(s.misc[4]) & S_TOUCHED) ? 8u : 0u);
   // halo labels into the ring (a neighbour never visited has none)
@@ -732,16 +750,6 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   for (uint32_t qbase = 0; qbase < nquad; qbase += nt) {
     if (qbase) load_planes(qbase + tid);
     if (!qvalid) continue;
-    uint64_t lv[4] = {kUnreached, kUnreached, kUnreached, kUnreached};
-    if (touched) {
-      // a quad is contiguous in the tile block whatever the window
-      const auto* lp = (const __attribute__((address_space(1))) u64x2_a8*)(A.labels + blockbase + qc0);
-      const u64x2 l01 = lp[0], l23 = lp[1];
-      lv[0] = l01[0];
-      lv[1] = l01[1];
-      lv[2] = l23[0];
-      lv[3] = l23[1];
-    }
     const uint32_t l = ql, y = qy;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -862,7 +870,10 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   }
   __syncthreads();  // also orders this workgroup's label stores before the flag / queue updates
   TileResult res;
-  res.flags = uniform_u32(s.misc[5]) | (capped ? T_CAPPED : 0u);
+  const uint32_t sides = (txi > 0 ? T_LEFT : 0u) | (txi + 1 < F.tx ? T_RIGHT : 0u) | (tyi > 0 ? T_TOP : 0u) |
+                         (tyi + 1 < F.ty ? T_BOTTOM : 0u) | T_ANY;
+  res.flags = (uniform_u32(s.misc[5]) & sides) | (capped ? T_CAPPED : 0u);
+  res.tx = F.tx;
   res.kl = uniform_u32(s.misc[10]);
   res.kr = uniform_u32(s.misc[11]);
   res.kt = uniform_u32(s.misc[12]);
@@ -889,6 +900,42 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 __device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb, uint32_t key) {
   if (A.key_next) atomicMin(&A.key_next[nb], key);
   if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[A.next_ctr * kCtrStride], 1u)] = nb;
+}
+
+// Activations after a visit, run by every lane of the first wave: lanes 0..3
+// take the left, right, top and bottom neighbour, lane 4 the tile itself when
+// its visit hit the round cap (key `self_key` when no cell improved).  The
+// flag exchanges run side by side and one atomicAdd reserves the list slots of
+// all new entries, so the visit ends after two memory round trips instead of
+// two per neighbour.  Returns the smallest key activated (kMax when none);
+// every atomic whose result the list depends on has returned on exit.
+__device__ __forceinline__ uint32_t activate_sides(const RelaxArgs& A, uint32_t item, const TileResult& res,
+                                                   uint32_t self_key) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t f = res.flags;
+  bool want = false;
+  uint32_t nb = item, k = kMax;
+  if (lane == 0) want = (f & T_LEFT) != 0, nb = item - 1, k = res.kl;
+  if (lane == 1) want = (f & T_RIGHT) != 0, nb = item + 1, k = res.kr;
+  if (lane == 2) want = (f & T_TOP) != 0, nb = item - res.tx, k = res.kt;
+  if (lane == 3) want = (f & T_BOTTOM) != 0, nb = item + res.tx, k = res.kb;
+  if (lane == 4) want = (f & T_CAPPED) != 0, k = res.ka == kMax ? self_key : res.ka;
+  bool fresh = false;
+  if (want) {
+    if (A.key_next) atomicMin(&A.key_next[nb], k);
+    fresh = atomicExch(&A.act_next[nb], 1u) == 0u;
+  }
+  const uint64_t m = __ballot(fresh);
+  if (m) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&A.counters[A.next_ctr * kCtrStride], (uint32_t)__popcll(m));
+    base = uniform_u32(base);
+    if (fresh) A.list_next[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = nb;
+  }
+  uint32_t kmin = kMax;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane(want ? k : kMax, j));
+  return kmin;
 }
 
 // Waves per SIMD the kernels are compiled for, the most that fit without
@@ -920,28 +967,18 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
   }
   TileResult res{};
   if (!defer) res = process_tile<TW, TH, LM, NARROW>(A, item, smem);
+  if (threadIdx.x >= 64) return;
+  uint32_t kmin = kMax;
+  if (defer) {
+    if (threadIdx.x == 0) activate(A, item, key);  // carried over to the next round with its key
+    kmin = key;
+  } else {
+    kmin = activate_sides(A, item, res, key);
+  }
   if (threadIdx.x == 0) {
     A.act_cur[item] = 0;
-    uint32_t kmin = kMax;
     if (A.key_cur) A.key_cur[item] = kMax;
-    if (defer) {
-      activate(A, item, key);  // carried over to the next round with its key
-      kmin = key;
-    } else {
-      const FieldDesc F = A.fields[A.item_field[item]];
-      const uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-      const uint32_t f = res.flags;
-      auto act = [&](uint32_t nb, uint32_t k) {
-        activate(A, nb, k);
-        kmin = min(kmin, k);
-      };
-      if ((f & T_LEFT) && txi > 0) act(item - 1, res.kl);
-      if ((f & T_RIGHT) && txi + 1 < F.tx) act(item + 1, res.kr);
-      if ((f & T_TOP) && tyi > 0) act(item - F.tx, res.kt);
-      if ((f & T_BOTTOM) && tyi + 1 < F.ty) act(item + F.tx, res.kb);
-      if (f & T_CAPPED) act(item, res.ka == kMax ? key : res.ka);
-      if (f & (T_ANY | T_CAPPED)) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
-    }
+    if (!defer && (res.flags & (T_ANY | T_CAPPED))) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
     if (A.key_cur && kmin != kMax) atomicMin(&A.counters[A.key_ctr * kCtrStride], kmin);
     if (A.host_out) {
       // The last workgroup of the launch publishes the counts.  Every
@@ -1000,16 +1037,9 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
   for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
     uint32_t item = uniform_u32(R.list_cur[k]);
     const TileResult res = process_tile<TW, TH, LM, NARROW>(R, item, smem);
-    if (threadIdx.x == 0) {
-      R.act_cur[item] = 0;
-      const FieldDesc F = R.fields[R.item_field[item]];
-      uint32_t t = item - F.item0, txi = t % F.tx, tyi = t / F.tx;
-      const uint32_t f = res.flags;
-      if ((f & T_LEFT) && txi > 0) activate(R, item - 1, res.kl);
-      if ((f & T_RIGHT) && txi + 1 < F.tx) activate(R, item + 1, res.kr);
-      if ((f & T_TOP) && tyi > 0) activate(R, item - F.tx, res.kt);
-      if ((f & T_BOTTOM) && tyi + 1 < F.ty) activate(R, item + F.tx, res.kb);
-      if (f & T_CAPPED) activate(R, item, res.ka);
+    if (threadIdx.x < 64) {
+      if (threadIdx.x == 0) R.act_cur[item] = 0;
+      activate_sides(R, item, res, res.ka);
     }
     __syncthreads();
   }
