@@ -241,6 +241,52 @@ int mrf_solve_fields(mrf_ctx *ctx, mrf_board *board, const uint32_t *const *sour
 int mrf_read_field(mrf_ctx *ctx, mrf_board *board, size_t field, uint32_t *dist_out,
                    uint32_t *hops_out, mrf_error *err);
 
+/* Replace the resident board's additive price plane (n == w*h*layers).
+ * The negotiation loop reprices the board once per iteration
+ * (negotiated.rs:1610-1667); mrf_route_negotiated_batch calls this itself. */
+int mrf_board_set_additive(mrf_board *board, const uint32_t *additive, size_t n, mrf_error *err);
+
+/* ---- negotiated-congestion batches --------------------------------------------
+ * One negotiation iteration's searches in one batch: the rayon map over the
+ * dirty nets of the Jacobi loop (negotiated.rs:1679-1778), each net routed by
+ * route_negotiated (negotiated.rs:2746-2856) in its window with a full-board
+ * retry.  Entering cell v over base step price E*w costs
+ *   passable_search_cost(E*w + history[v] + foreign(v))
+ * with pf = pfac * 16 (SCALE == CLEARANCE_NEG_WEIGHT, negotiated.rs:71, :99),
+ * self_present(v) = [v on the net's own path] (own_path_off != NULL) and
+ * self_halo(v) = multiplicity of v in the net's own halo list:
+ *   fused (shared_congestion, negotiated.rs:1654-1667):
+ *     foreign = (pf*present[v] + pf*present_halo[v]) - pf*self_present - pf*self_halo
+ *   otherwise:
+ *     foreign = pf*(present[v] - self_present) + pf*(present_halo[v] - self_halo)
+ * every sum and difference saturating (negotiated.rs:2781-2795).  The library
+ * writes history + the shared term into the board's additive plane and prices
+ * each net's own footprint apart on the device; the board must have no halo-
+ * owner plane.  Paths and costs as mrf_solve. */
+typedef struct {
+    const uint32_t *history;      /* n_cells */
+    const uint32_t *present;      /* n_cells */
+    const uint32_t *present_halo; /* n_cells, or NULL for all zero (no clearance) */
+    size_t n_cells;
+    uint32_t pfac;                /* 1 + iteration */
+    int fused;                    /* shared congestion snapshot (Some) or separate terms */
+    /* Own copper of each net: own_path[own_path_off[i] .. own_path_off[i+1]),
+     * or own_path_off == NULL when the batch passes no own path (None). */
+    const uint64_t *own_path_off;
+    const uint32_t *own_path;
+    /* Own halo of each net (CountedCellSet): entries
+     * [own_halo_off[i] .. own_halo_off[i+1]) of own_halo, each adding
+     * own_halo_count[k] (1 when own_halo_count == NULL) to its cell's
+     * multiplicity; own_halo_off == NULL for none. */
+    const uint64_t *own_halo_off;
+    const uint32_t *own_halo;
+    const uint32_t *own_halo_count;
+} mrf_negotiation;
+
+int mrf_route_negotiated_batch(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets,
+                               const mrf_window *windows, int retry_full, const mrf_negotiation *neg,
+                               mrf_paths *out, mrf_error *err);
+
 /* ---- autotuner -----------------------------------------------------------------
  * Picks the tile shape, workgroup size, in-LDS round cap (sweeps per tile
  * visit) and solve form (plain rounds, or hipGraph replays of `graph_rounds`

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <new>
 #include <thread>
 #include <vector>
@@ -52,9 +53,10 @@ int fail(mrf_error* err, const Error& e) {
 
 // Solve one pass of nets (each inside its window) on a resident board, chunked
 // by the label budget, re-validating every path on the host.
+// `prices` (optional, indexed like nets) carries per-net additive overrides.
 void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mrf_net* nets,
                 const std::vector<size_t>& idx, const std::vector<mrf_window>& wins, bool lee,
-                const int32_t* net_ids, Batch& out) {
+                const int32_t* net_ids, Batch& out, const std::vector<NetPlan>* prices = nullptr) {
   size_t n = idx.size();
   out.routed.assign(n, 0);
   out.cost.assign(n, kMax);
@@ -64,6 +66,10 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
     const mrf_net& nt = nets[idx[k]];
     plans[k].pads = sorted_unique(nt.pads, nt.n_pads);
     plans[k].vpads = sorted_unique(nt.via_pads, nt.n_via_pads);
+    if (prices) {
+      plans[k].ovr_cells = (*prices)[idx[k]].ovr_cells;
+      plans[k].ovr_add = (*prices)[idx[k]].ovr_add;
+    }
   }
   size_t budget_words = std::max<size_t>(c.label_budget / 8, 1);
   size_t k0 = 0;
@@ -89,6 +95,8 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
       q.pads = plans[k].pads;
       q.via_pads = plans[k].vpads;
       q.own_id = net_ids ? net_ids[idx[k]] : -1;
+      q.ovr_cells = plans[k].ovr_cells;
+      q.ovr_add = plans[k].ovr_add;
       if (!lee) {
         q.forced = {nt.src, nt.dst};
         q.flags = mrf::F_FORCE;
@@ -574,41 +582,168 @@ static HostGrid board_host(const mrf_board* bd) {
   return g;
 }
 
+// Nets on a resident board: windows (full board when NULL), then the
+// unreachable nets on the full board when retry_full.  Caller holds the lane.
+static void solve_board(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
+                        int retry_full, const std::vector<NetPlan>* prices, mrf_paths* out) {
+  HostGrid g = board_host(bd);
+  mrf_dims dims = bd->dims;
+  if (n_nets && !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "null nets");
+  for (size_t i = 0; windows && i < n_nets; ++i) check_window(dims, windows[i], i);
+  for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
+  mrf_window full{0, 0, dims.w - 1, dims.h - 1};
+  std::vector<size_t> idx(n_nets);
+  for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
+  std::vector<mrf_window> wins(n_nets, full);
+  if (windows) wins.assign(windows, windows + n_nets);
+  Batch all;
+  solve_nets(ctx->c, bd->b, g, nets, idx, wins, false, nullptr, all, prices);
+  if (retry_full && windows) {
+    std::vector<size_t> retry_idx;
+    for (size_t i = 0; i < n_nets; ++i)
+      if (!all.routed[i]) retry_idx.push_back(i);
+    if (!retry_idx.empty()) {
+      std::vector<mrf_window> fw(retry_idx.size(), full);
+      Batch second;
+      solve_nets(ctx->c, bd->b, g, nets, retry_idx, fw, false, nullptr, second, prices);
+      for (size_t k = 0; k < retry_idx.size(); ++k) {
+        size_t i = retry_idx[k];
+        all.routed[i] = second.routed[k];
+        all.cost[i] = second.cost[k];
+        all.paths[i] = std::move(second.paths[k]);
+      }
+    }
+  }
+  export_paths(all, out);
+}
+
 int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
               int retry_full, mrf_paths* out, mrf_error* err) {
   if (out) memset(out, 0, sizeof(*out));
   if (!ctx || !bd || !out || bd->ctx != ctx) return MRF_BACKEND_UNAVAILABLE;
   try {
     Lane lane(ctx->c);
-    HostGrid g = board_host(bd);
-    mrf_dims dims = bd->dims;
-    if (n_nets && !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "null nets");
-    for (size_t i = 0; windows && i < n_nets; ++i) check_window(dims, windows[i], i);
-    for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
-    mrf_window full{0, 0, dims.w - 1, dims.h - 1};
-    std::vector<size_t> idx(n_nets);
-    for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
-    std::vector<mrf_window> wins(n_nets, full);
-    if (windows) wins.assign(windows, windows + n_nets);
-    Batch all;
-    solve_nets(ctx->c, bd->b, g, nets, idx, wins, false, nullptr, all);
-    if (retry_full && windows) {
-      std::vector<size_t> retry_idx;
-      for (size_t i = 0; i < n_nets; ++i)
-        if (!all.routed[i]) retry_idx.push_back(i);
-      if (!retry_idx.empty()) {
-        std::vector<mrf_window> fw(retry_idx.size(), full);
-        Batch second;
-        solve_nets(ctx->c, bd->b, g, nets, retry_idx, fw, false, nullptr, second);
-        for (size_t k = 0; k < retry_idx.size(); ++k) {
-          size_t i = retry_idx[k];
-          all.routed[i] = second.routed[k];
-          all.cost[i] = second.cost[k];
-          all.paths[i] = std::move(second.paths[k]);
+    solve_board(ctx, bd, nets, n_nets, windows, retry_full, nullptr, out);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf_free_paths(out);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf_free_paths(out);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+int mrf_board_set_additive(mrf_board* bd, const uint32_t* additive, size_t n, mrf_error* err) {
+  if (!bd || !bd->ctx) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(bd->ctx->c, false);
+    if (n != dims_len(bd->dims) || !additive) throw Error(MRF_MALFORMED_GRID, "additive plane length does not match dims");
+    mrf::board_set_additive(bd->ctx->c, bd->b, additive);
+    bd->add.assign(additive, additive + n);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+namespace {
+
+inline uint64_t sat_add64(uint64_t a, uint64_t b) { return a + b < a ? UINT64_MAX : a + b; }
+inline uint64_t sat_mul64(uint64_t a, uint64_t b) { return (b && a > UINT64_MAX / b) ? UINT64_MAX : a * b; }
+inline uint64_t sat_sub64(uint64_t a, uint64_t b) { return a > b ? a - b : 0; }
+inline uint32_t clamp32(uint64_t v) { return v < 0xFFFFFFFFull ? (uint32_t)v : 0xFFFFFFFFu; }
+constexpr uint64_t kNegScale = 16;  // SCALE == CLEARANCE_NEG_WEIGHT (negotiated.rs:71, :99)
+
+// Own cells of net i from a CSR list, sorted and unique; `counts` gets the
+// summed multiplicities (entry weights `weight`, 1 each when null).
+void own_cells(const uint64_t* off, const uint32_t* cells, const uint32_t* weight, size_t i, size_t n_cells,
+               std::vector<uint32_t>& out, std::vector<uint64_t>* counts) {
+  out.clear();
+  if (counts) counts->clear();
+  if (!off) return;
+  if (off[i + 1] < off[i]) throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint offsets are not ascending");
+  std::vector<std::pair<uint32_t, uint64_t>> v;
+  v.reserve(off[i + 1] - off[i]);
+  for (uint64_t k = off[i]; k < off[i + 1]; ++k) {
+    if (cells[k] >= n_cells) throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint cell outside the board");
+    v.emplace_back(cells[k], weight ? weight[k] : 1u);
+  }
+  std::sort(v.begin(), v.end());
+  for (size_t k = 0; k < v.size();) {
+    uint64_t sum = 0;
+    size_t e = k;
+    for (; e < v.size() && v[e].first == v[k].first; ++e) sum += v[e].second;
+    out.push_back(v[k].first);
+    if (counts) counts->push_back(sum);
+    k = e;
+  }
+}
+
+}  // namespace
+
+int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets,
+                               const mrf_window* windows, int retry_full, const mrf_negotiation* neg, mrf_paths* out,
+                               mrf_error* err) {
+  if (out) memset(out, 0, sizeof(*out));
+  if (!ctx || !bd || !out || !neg || bd->ctx != ctx) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(ctx->c);
+    const size_t n = dims_len(bd->dims);
+    if (!bd->halo.empty()) throw Error(MRF_BACKEND_UNAVAILABLE, "negotiated batches take boards without a halo-owner plane");
+    if (neg->n_cells != n || !neg->history || !neg->present)
+      throw Error(MRF_MALFORMED_GRID, "negotiation planes do not match dims");
+    if ((neg->own_path_off && !neg->own_path) || (neg->own_halo_off && !neg->own_halo))
+      throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint offsets without cells");
+    const uint64_t pf = (uint64_t)neg->pfac * kNegScale;
+    auto halo_at = [&](size_t c) -> uint64_t { return neg->present_halo ? neg->present_halo[c] : 0u; };
+    // Shared congestion snapshot (negotiated.rs:1654-1667), folded with
+    // history into the board's additive plane: what every net pays on cells
+    // outside its own footprint.
+    std::vector<uint32_t> plane(n);
+    for (size_t c = 0; c < n; ++c) {
+      const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
+      plane[c] = clamp32(sat_add64(neg->history[c], shared));
+    }
+    // Own footprint of each net: priced_with_base's self-subtraction
+    // (negotiated.rs:2777-2797) as additive overrides.
+    std::vector<NetPlan> prices(n_nets);
+    std::vector<uint32_t> path, halo;
+    std::vector<uint64_t> hcount;
+    for (size_t i = 0; i < n_nets; ++i) {
+      own_cells(neg->own_path_off, neg->own_path, nullptr, i, n, path, nullptr);
+      own_cells(neg->own_halo_off, neg->own_halo, neg->own_halo_count, i, n, halo, &hcount);
+      std::vector<uint32_t> u;
+      std::set_union(path.begin(), path.end(), halo.begin(), halo.end(), std::back_inserter(u));
+      NetPlan& p = prices[i];
+      for (uint32_t c : u) {
+        const uint64_t self_present = std::binary_search(path.begin(), path.end(), c) ? 1u : 0u;
+        const auto h = std::lower_bound(halo.begin(), halo.end(), c);
+        const uint64_t self_halo = (h != halo.end() && *h == c) ? hcount[h - halo.begin()] : 0u;
+        uint64_t foreign;
+        if (neg->fused) {
+          const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
+          foreign = sat_sub64(sat_sub64(shared, sat_mul64(pf, self_present)), sat_mul64(pf, self_halo));
+        } else {
+          foreign = sat_add64(sat_mul64(pf, sat_sub64(neg->present[c], self_present)),
+                              sat_mul64(pf, sat_sub64(halo_at(c), self_halo)));
+        }
+        const uint32_t a = clamp32(sat_add64(neg->history[c], foreign));
+        if (a != plane[c]) {
+          p.ovr_cells.push_back(c);
+          p.ovr_add.push_back(a);
         }
       }
     }
-    export_paths(all, out);
+    mrf::board_set_additive(ctx->c, bd->b, plane.data());
+    bd->add = std::move(plane);
+    solve_board(ctx, bd, nets, n_nets, windows, retry_full, &prices, out);
     if (err) set_err(err, MRF_OK, "ok");
     return MRF_OK;
   } catch (const Error& e) {

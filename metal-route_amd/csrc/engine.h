@@ -63,6 +63,11 @@ struct FieldDesc {
   int32_t own_id;      // halo owner id of this net
   uint32_t flags;
   uint32_t board;      // index into the launch's board table
+  // Additive overrides of the net (negotiated batches), n_ovr entries in the
+  // pool at ovr_off: cells ascending [n], their additive prices [n], per-tile
+  // start indices [tx*ty+1], then the same entries ordered by tile: cells [n],
+  // prices [n].
+  uint32_t ovr_off, n_ovr;
 };
 
 struct TileShape {
@@ -85,6 +90,7 @@ struct DevBoard {
   std::vector<uint8_t> via_allowed;  // host copy
   std::vector<uint8_t> obst;         // host copy: cost == kMax per cell
   uint64_t max_step = 0;  // bound on any step price (obstacles priced at weight 1, as own pads)
+  uint64_t max_ew = 0;    // largest edge price x largest enter weight (max_step without the additive part)
   size_t bytes = 0;
   BoardView view() const;
 };
@@ -100,6 +106,9 @@ struct FieldReq {
   int32_t own_id = -1;
   uint32_t flags = 0;
   uint32_t board = 0;  // index into the boards passed to solve_fields
+  // Per-net additive prices replacing the board plane's on these cells
+  // (ascending, unique); cells outside the window are ignored.
+  std::vector<uint32_t> ovr_cells, ovr_add;
 };
 
 struct SolveStats {
@@ -226,6 +235,8 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
                   const uint8_t* via_allowed, const uint32_t* add, const int32_t* halo,
                   uint32_t halo_pen);
 void board_free(DevBoard& b);
+// Replace (or add) the board's additive price plane; host array of w*h*layers.
+void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add);
 
 // Solve complete fields for `reqs` (FieldReq::board indexes `boards`) in one
 // launch.  Labels stay on the device in the workspace until the next solve.

@@ -212,8 +212,31 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
   return lo;
 }
 
-__device__ __forceinline__ uint32_t add_price(const BoardView& b, const FieldDesc& F, uint32_t g) {
+// Index of g in the field's additive overrides (cells ascending), or kMax.
+__device__ __forceinline__ uint32_t override_index(const FieldDesc& F, const uint32_t* pool, uint32_t g) {
+  const uint32_t* c = pool + F.ovr_off;
+  uint32_t lo = 0, hi = F.n_ovr;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (c[mid] < g)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < F.n_ovr && c[lo] == g ? lo : kMax;
+}
+
+// Additive price of entering g for field F: the net's own override when it
+// has one (negotiated batches price the net's own footprint apart,
+// negotiated.rs:2781-2795), else the board plane; plus the foreign-halo
+// penalty.
+__device__ __forceinline__ uint32_t add_price(const BoardView& b, const FieldDesc& F, const uint32_t* pool,
+                                              uint32_t g) {
   uint64_t a = b.add ? b.add[g] : 0u;
+  if (F.n_ovr) {
+    const uint32_t k = override_index(F, pool, g);
+    if (k != kMax) a = pool[F.ovr_off + F.n_ovr + k];
+  }
   if (b.halo) {
     int32_t o = b.halo[g];
     if (o != -1 && o != F.own_id) a += b.halo_pen;
@@ -264,7 +287,7 @@ __device__ CellInfo cell_info(const BoardView& b, const FieldDesc& F, const uint
   if (F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(b, F, pool, g));
   ci.mask = m;
   ci.w = (obst && pad) ? 1u : b.cost[g];
-  ci.a = (b.add || b.halo) ? add_price(b, F, g) : 0u;
+  ci.a = (b.add || b.halo || F.n_ovr) ? add_price(b, F, pool, g) : 0u;
   return ci;
 }
 
@@ -747,6 +770,43 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   // view of each cell (cell_info: own pads, forced endpoints, via exemptions,
   // additive and halo pricing) and the six step prices of entering it.
   const uint32_t* pads = A.pool + F.pad_off;
+  // The six step prices of entering cell (l, y, x) = global g with board mask
+  // bm, cost weight w0, additive price a0 and halo owner o.
+  auto cell_steps = [&](uint32_t l, uint32_t y, uint32_t x, uint32_t g, uint8_t bm, uint32_t w0, uint32_t a0,
+                        int32_t o) {
+    const bool trace = (bm & M_TRACE) != 0, obst = (bm & M_OBST) != 0;
+    uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
+    bool pass = !trace && !obst;
+    uint32_t wgt = w0;
+    if (obst && !trace && F.n_pad && sorted_has(pads, F.n_pad, g)) {
+      pass = true;
+      wgt = 1u;
+    }
+    if (!pass && !trace && (F.flags & F_FORCE)) {
+      for (uint32_t j = 0; j < F.n_forced; ++j)
+        if (A.pool[F.forced_off + j] == g) pass = true;
+    }
+    if (pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
+    uint64_t a64 = a0;
+    if (o != -1 && o != F.own_id) a64 += B.halo_pen;
+    const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
+    const uint32_t pc = l * STEP_PLANE + y * PS + x;
+    using ST = typename Lds<NARROW>::Step;
+    // Narrow boards bound every open step price below 0x7FFF, so it is
+    // exact in 32-bit arithmetic without the saturation of step_price.
+    auto g6 = [&](bool open, uint32_t e) -> ST {
+      if (NARROW) return to_step<ST>(open ? e * wgt + ap : kMax);
+      return to_step<ST>(gate(open, e, wgt, ap));
+    };
+    s.sxl[pc] = g6(pass && (m & K_NX), s.xg[x]);
+    s.sxr[pc] = g6(pass && (m & K_PX), s.xg[x + 1]);
+    s.syt[pc] = g6(pass && (m & K_NY), s.yg[y]);
+    s.syb[pc] = g6(pass && (m & K_PY), s.yg[y + 1]);
+    if (L > 1) {
+      s.svd[pc] = g6(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0]);
+      s.svu[pc] = g6(pass && (m & K_UP) && l + 1 < L, s.vg[l]);
+    }
+  };
   for (uint32_t qbase = 0; qbase < nquad; qbase += nt) {
     if (qbase) load_planes(qbase + tid);
     if (!qvalid) continue;
@@ -755,43 +815,28 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     for (int k = 0; k < 4; ++k) {
       const uint32_t x = qx0 + k;
       const uint8_t bm = inw[k] ? bv[k] : (uint8_t)M_TRACE;  // outside the window: closed
-      const uint32_t g = qrowg + min(qwxb + k, F.ww - 1);
-      const bool trace = (bm & M_TRACE) != 0, obst = (bm & M_OBST) != 0;
-      uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
-      bool pass = !trace && !obst;
-      uint32_t wgt = wv[k];
-      if (obst && !trace && F.n_pad && sorted_has(pads, F.n_pad, g)) {
-        pass = true;
-        wgt = 1u;
-      }
-      if (!pass && !trace && (F.flags & F_FORCE)) {
-        for (uint32_t j = 0; j < F.n_forced; ++j)
-          if (A.pool[F.forced_off + j] == g) pass = true;
-      }
-      if (pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
-      uint64_t a64 = av[k];
-      if (ov[k] != -1 && ov[k] != F.own_id) a64 += B.halo_pen;
-      const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
-      const uint32_t pc = l * STEP_PLANE + y * PS + x;
-      using ST = typename Lds<NARROW>::Step;
-      // Narrow boards bound every open step price below 0x7FFF, so it is
-      // exact in 32-bit arithmetic without the saturation of step_price.
-      auto g6 = [&](bool open, uint32_t e) -> ST {
-        if (NARROW) return to_step<ST>(open ? e * wgt + ap : kMax);
-        return to_step<ST>(gate(open, e, wgt, ap));
-      };
-      s.sxl[pc] = g6(pass && (m & K_NX), s.xg[x]);
-      s.sxr[pc] = g6(pass && (m & K_PX), s.xg[x + 1]);
-      s.syt[pc] = g6(pass && (m & K_NY), s.yg[y]);
-      s.syb[pc] = g6(pass && (m & K_PY), s.yg[y + 1]);
-      if (L > 1) {
-        s.svd[pc] = g6(pass && (m & K_DN) && l > 0, s.vg[l > 0 ? l - 1 : 0]);
-        s.svu[pc] = g6(pass && (m & K_UP) && l + 1 < L, s.vg[l]);
-      }
+      cell_steps(l, y, x, qrowg + min(qwxb + k, F.ww - 1), bm, wv[k], av[k], ov[k]);
       s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[k] ? lab_in(lv[k]) : kUnreached;
     }
   }
   __syncthreads();
+  // The net's own additive overrides inside this tile (negotiated batches):
+  // their cells are priced again over the board-plane prices just written.
+  if (F.n_ovr) {
+    const uint32_t* tab = A.pool + F.ovr_off + 2 * F.n_ovr;
+    const uint32_t o0 = uniform_u32(tab[t]), o1 = uniform_u32(tab[t + 1]);
+    if (o1 > o0) {
+      const uint32_t* tc = tab + F.tx * F.ty + 1;
+      const uint32_t plane = B.plane;
+      for (uint32_t j = o0 + tid; j < o1; j += nt) {
+        const uint32_t g = tc[j], a = tc[F.n_ovr + j];
+        const uint32_t l = g / plane, r = g - l * plane, gy = r / B.w, gx = r - gy * B.w;
+        cell_steps(l, gy - F.y0 - wy0, gx - F.x0 - wx0, g, gbl(B.bmask)[g], gbl(B.cost)[g], a,
+                   B.halo ? gbl(B.halo)[g] : -1);
+      }
+      __syncthreads();
+    }
+  }
   // seeds (first visit only): passable sources start at label (0, 0)
   if (!touched) {
     const uint32_t plane = B.plane;
@@ -1404,7 +1449,8 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   }
   board_free(b);
   b.obst = std::move(obst);
-  b.max_step = std::min<uint64_t>(max_e * max_w + max_a, kMax - 1);
+  b.max_ew = max_e * max_w;
+  b.max_step = std::min<uint64_t>(b.max_ew + max_a, kMax - 1);
   b.w = w;
   b.h = h;
   b.layers = layers;
@@ -1432,6 +1478,22 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
     b.bytes += n * 4;
   }
   b.halo_pen = halo_pen;
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add) {
+  HIP_OK(hipSetDevice(c.device));
+  const size_t n = (size_t)b.w * b.h * b.layers;
+  uint64_t max_a = 0;
+  for (size_t i = 0; i < n; ++i) max_a = std::max<uint64_t>(max_a, add[i]);
+  if (b.halo) max_a += b.halo_pen;
+  if (!b.add) {
+    b.add = dev_copy(add, n, c.stream);
+    b.bytes += n * 4;
+  } else {
+    HIP_OK(hipMemcpyAsync(b.add, add, n * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
+  }
+  b.max_step = std::min<uint64_t>(b.max_ew + max_a, kMax - 1);
   HIP_OK(hipStreamSynchronize(c.stream));
 }
 
@@ -1507,11 +1569,49 @@ static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_
   return g.exec;
 }
 
+// Lay out a field's additive overrides inside its window in the pool (see
+// FieldDesc::ovr_off): by cell for point lookups, and by tile for the loader.
+static void put_overrides(const FieldReq& q, const DevBoard& b, const TileShape& S, FieldDesc& F,
+                          std::vector<uint32_t>& pool) {
+  F.ovr_off = (uint32_t)pool.size();
+  F.n_ovr = 0;
+  if (q.ovr_cells.empty()) return;
+  if (q.ovr_add.size() != q.ovr_cells.size()) throw Error(1, "override cells and prices differ in length");
+  const uint32_t plane = b.w * b.h, nt = F.tx * F.ty;
+  std::vector<uint32_t> cells, adds, tile;
+  for (size_t k = 0; k < q.ovr_cells.size(); ++k) {
+    const uint32_t g = q.ovr_cells[k];
+    if (k && g <= q.ovr_cells[k - 1]) throw Error(1, "override cells must be strictly ascending");
+    if ((size_t)g >= (size_t)plane * b.layers) throw Error(1, "override cell outside the board");
+    const uint32_t r = g % plane, gy = r / b.w, gx = r % b.w;
+    if (gx < q.x0 || gx > q.x1 || gy < q.y0 || gy > q.y1) continue;
+    cells.push_back(g);
+    adds.push_back(q.ovr_add[k]);
+    tile.push_back(((gy - q.y0) / S.th) * F.tx + (gx - q.x0) / S.tw);
+  }
+  const uint32_t n = (uint32_t)cells.size();
+  F.n_ovr = n;
+  if (!n) return;
+  std::vector<uint32_t> order(n);
+  for (uint32_t k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return tile[a] < tile[c]; });
+  pool.insert(pool.end(), cells.begin(), cells.end());
+  pool.insert(pool.end(), adds.begin(), adds.end());
+  std::vector<uint32_t> tab(nt + 1, 0);
+  for (uint32_t k = 0; k < n; ++k) tab[tile[k] + 1]++;
+  for (uint32_t t = 0; t < nt; ++t) tab[t + 1] += tab[t];
+  pool.insert(pool.end(), tab.begin(), tab.end());
+  for (uint32_t k : order) pool.push_back(cells[k]);
+  for (uint32_t k : order) pool.push_back(adds[k]);
+}
+
 // Whether a field's step prices all fit the narrow step planes: its board's
 // bound, and no forced endpoint on an obstacle (priced at the obstacle's
 // weight unless it is an own pad).
 static bool field_narrow(const DevBoard& b, const FieldReq& q) {
   if (b.max_step > kNarrowMaxStep) return false;
+  for (uint32_t a : q.ovr_add)
+    if (b.max_ew + a + (b.halo ? b.halo_pen : 0u) > kNarrowMaxStep) return false;
   for (uint32_t g : q.forced)
     if (g < b.obst.size() && b.obst[g] && !std::binary_search(q.pads.begin(), q.pads.end(), g)) return false;
   return true;
@@ -1573,6 +1673,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     F.flags = q.flags;
     F.board = q.board;
     uint32_t nt = F.tx * F.ty;
+    put_overrides(q, b, S, F, pool);
     for (uint32_t k = 0; k < nt; ++k) item_field.push_back((uint32_t)i);
     // seed tiles: tiles holding a source inside the window
     std::vector<uint32_t> st_items;
@@ -1998,6 +2099,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   uint64_t nitems = 0, words = 0, npool = 0;
   for (const FieldReq& q : reqs) {
     if (q.board >= d.max_boards || !d.board_used[q.board]) throw Error(1, "field names an unknown board");
+    if (!q.ovr_cells.empty()) throw Error(1, "the round driver takes no additive overrides");
     const uint32_t ww = q.x1 - q.x0 + 1, wh = q.y1 - q.y0 + 1;
     nitems += (uint64_t)((ww + S.tw - 1) / S.tw) * ((wh + S.th - 1) / S.th);
     words += field_label_words(S, ww, wh, d.views[q.board].layers);

@@ -107,6 +107,8 @@ inline void check_net(const HostGrid& g, const mrf_net& n, size_t i) {
 
 struct NetPlan {
   std::vector<uint32_t> pads, vpads;
+  // Per-net additive prices replacing the board plane's (cells ascending).
+  std::vector<uint32_t> ovr_cells, ovr_add;
 };
 
 inline uint32_t step_price(uint32_t e, uint32_t w, uint64_t a) {
@@ -149,7 +151,10 @@ inline bool path_ok(const HostGrid& g, const mrf_net& n, const NetPlan& p, const
     }
     uint32_t wgt = (g.obst(c) && sorted_has(p.pads, c)) ? 1u : g.cost[c];
     uint64_t add = 0;
-    if (g.add) add += g.add[c];
+    if (!p.ovr_cells.empty() && std::binary_search(p.ovr_cells.begin(), p.ovr_cells.end(), c))
+      add += p.ovr_add[std::lower_bound(p.ovr_cells.begin(), p.ovr_cells.end(), c) - p.ovr_cells.begin()];
+    else if (g.add)
+      add += g.add[c];
     if (g.halo && g.halo[c] != -1 && g.halo[c] != own_id) add += g.halo_pen;
     cost += step_price(e, wgt, add);
     if (cost >= kMax) return false;

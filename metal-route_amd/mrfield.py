@@ -108,6 +108,15 @@ class SynthS(C.Structure):
                 ("pin_off", U64P), ("pins", U32P), ("windows", C.POINTER(WindowS))]
 
 
+class NegotiationS(C.Structure):
+    _fields_ = [
+        ("history", U32P), ("present", U32P), ("present_halo", U32P), ("n_cells", C.c_size_t),
+        ("pfac", C.c_uint32), ("fused", C.c_int),
+        ("own_path_off", C.POINTER(C.c_uint64)), ("own_path", U32P),
+        ("own_halo_off", C.POINTER(C.c_uint64)), ("own_halo", U32P), ("own_halo_count", U32P),
+    ]
+
+
 class MrfError(RuntimeError):
     def __init__(self, code: int, message: str, net: int = 0):
         super().__init__(f"[{code}] {message}")
@@ -159,6 +168,10 @@ def lib():
         "mrf_solve_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, I32P,
                                        C.c_size_t, C.POINTER(WindowS), E]),
         "mrf_read_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, U32P, U32P, E]),
+        "mrf_board_set_additive": (C.c_int, [C.c_void_p, U32P, C.c_size_t, E]),
+        "mrf_route_negotiated_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t,
+                                                 C.POINTER(WindowS), C.c_int, C.POINTER(NegotiationS),
+                                                 C.POINTER(PathsS), E]),
         "mrf_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t, C.POINTER(WindowS),
                                    C.c_size_t, C.POINTER(TuningS), E]),
         "mrf_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfigS), C.POINTER(C.c_void_p), E]),
@@ -515,6 +528,61 @@ class Board:
     def solve_arrays(self, nets, windows=None, retry_full=True) -> PathBatch:
         """Like solve(), with the paths left as flat numpy arrays."""
         return _take_arrays(self._solve(nets, windows, retry_full))
+
+    def set_additive(self, additive) -> None:
+        """Replace the board's additive price plane (w*h*layers u32)."""
+        a = _u32(additive)
+        err = ErrorS()
+        _check(lib().mrf_board_set_additive(self._h, _p32(a), a.size, C.byref(err)), err)
+
+    def route_negotiated(self, nets, history, present, pfac, *, present_halo=None, fused=False, own_paths=None,
+                         own_halos=None, windows=None, retry_full=True, arrays=False):
+        """One negotiation iteration's searches (route_negotiated per net,
+        negotiated.rs:2746-2856) in one device batch.
+
+        own_paths: per net its own copper (sequence of cells), or None for the
+        batch passing no own path.  own_halos: per net its own halo, a
+        {cell: count} dict or cells repeated by multiplicity, or None.
+        """
+        arr, keep = _nets(nets)
+        hist, pres = _u32(history), _u32(present)
+        halo = _u32(present_halo) if present_halo is not None else None
+        if pres.size != hist.size or (halo is not None and halo.size != hist.size):
+            raise MrfError(MALFORMED_GRID, "negotiation planes differ in length")
+        ng = NegotiationS(_p32(hist), _p32(pres), _p32(halo), hist.size, int(pfac), 1 if fused else 0)
+        if own_paths is not None:
+            lens = [len(p) for p in own_paths]
+            poff = np.zeros(len(nets) + 1, dtype=np.uint64)
+            poff[1:] = np.cumsum(lens)
+            pc = _u32(np.concatenate([np.asarray(p, dtype=np.uint32) for p in own_paths]) if sum(lens) else [0])
+            keep += [poff, pc]
+            ng.own_path_off = poff.ctypes.data_as(C.POINTER(C.c_uint64))
+            ng.own_path = _p32(pc)
+        if own_halos is not None:
+            cells, counts, lens = [], [], []
+            for hset in own_halos:
+                if isinstance(hset, dict):
+                    ks = [int(c) for c in hset]
+                    cells += ks
+                    counts += [int(hset[c]) for c in hset]
+                    lens.append(len(ks))
+                else:
+                    hs = [int(c) for c in hset]
+                    cells += hs
+                    counts += [1] * len(hs)
+                    lens.append(len(hs))
+            hoff = np.zeros(len(nets) + 1, dtype=np.uint64)
+            hoff[1:] = np.cumsum(lens)
+            hc, hn = _u32(cells + [0]), _u32(counts + [0])
+            keep += [hoff, hc, hn]
+            ng.own_halo_off = hoff.ctypes.data_as(C.POINTER(C.c_uint64))
+            ng.own_halo = _p32(hc)
+            ng.own_halo_count = _p32(hn)
+        out = PathsS()
+        err = ErrorS()
+        _check(lib().mrf_route_negotiated_batch(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)),
+                                                1 if retry_full else 0, C.byref(ng), C.byref(out), C.byref(err)), err)
+        return _take_arrays(out) if arrays else _take_paths(out)
 
     def solve_fields(self, sources, dsts=None, net_ids=None, windows=None):
         n = len(sources)

@@ -16,7 +16,8 @@
  *       axis_gap_cost/axis_edge_costs :1197-1201/:1268-1272,
  *       manhattan_scaled/axis_cost_prefix/axis_leg_cost :1245-1282,
  *       provider_path_is_valid :1284-1331, grid_path_cost :1106-1115,
- *       CPU alone batch :2115-2184, route_negotiated :2746-2856,
+ *       CPU alone batch :2115-2184, route_negotiated :2746-2856
+ *       (priced_with_base with self-subtraction :2777-2797),
  *       route_legal_once :2885-3052.
  *
  * Parity pins (what checks each function; tests/test_oracle*.py):
@@ -40,6 +41,13 @@
  *                                single-source fields.
  *   mrc_provider_path_is_valid   accepts every oracle path, rejects hand-built
  *                                illegal paths (negotiated.rs:4486-4517 cases).
+ *   priced_negotiated            the two properties negotiated.rs:6560-6660
+ *                                asserts: a net whose own copper and halo are
+ *                                the whole snapshot routes as on an empty
+ *                                snapshot, and fused == term-by-term pricing
+ *                                (randomized like :6665-6800); plus brute-force
+ *                                simple-path enumeration under the restated
+ *                                price on tiny grids.
  */
 #include "mrcpu.h"
 
@@ -489,7 +497,44 @@ static int blocked(const mrc_spec *s, uint32_t c, const uint32_t *forced, size_t
 
 /* priced_with_base (negotiated.rs:2777-2797) generalised to an additive plane
  * plus a foreign-halo penalty; without them it is passable_search_cost (:2929). */
+static inline uint64_t sat_sub64(uint64_t a, uint64_t b) { return a > b ? a - b : 0; }
+
+/* CountedCellSet::count of the own halo (0 when absent). */
+static uint32_t own_halo_count(const mrc_spec *s, uint32_t c) {
+    size_t lo = 0, hi = s->n_own_halo;
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (s->own_halo[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (lo < s->n_own_halo && s->own_halo[lo] == c) ? s->own_halo_count[lo] : 0;
+}
+
+/* priced_with_base of route_negotiated (negotiated.rs:2777-2797): base plus
+ * history plus the foreign congestion -- the snapshot less the net's own
+ * copper (self_present) and own halo multiplicity (self_halo), either from
+ * the fused shared term or term by term -- then passable_search_cost. */
+static uint32_t priced_negotiated(const mrc_spec *s, uint32_t c, uint64_t base) {
+    const uint64_t scale = 16; /* SCALE == CLEARANCE_NEG_WEIGHT (negotiated.rs:71, :99) */
+    uint64_t self_present = (s->has_own_path && sorted_contains(s->own_path, s->n_own_path, c)) ? 1 : 0;
+    uint64_t self_halo = own_halo_count(s, c);
+    uint64_t pf = (uint64_t)s->neg_pfac * scale;
+    uint64_t priced;
+    if (s->neg_shared) {
+        uint64_t foreign = sat_sub64(sat_sub64(s->neg_shared[c], pf * self_present), pf * self_halo);
+        priced = sat_add64(sat_add64(base, s->neg_history[c]), foreign);
+    } else {
+        uint64_t fp = sat_sub64(s->neg_present[c], self_present);
+        uint64_t fh = sat_sub64(s->neg_halo ? s->neg_halo[c] : 0, self_halo);
+        priced = sat_add64(sat_add64(sat_add64(base, s->neg_history[c]), pf * fp), pf * fh);
+    }
+    return mrc_passable_search_cost(priced);
+}
+
 static inline uint32_t priced(const mrc_spec *s, uint32_t c, uint64_t base) {
+    if (s->neg_history) return priced_negotiated(s, c, base);
     uint64_t p = base;
     if (s->additive) p = sat_add64(p, s->additive[c]);
     if (s->halo_owner) {
@@ -901,6 +946,12 @@ static void spec_for_net(const batch_job *j, size_t i, mrc_spec *s) {
     s->via_pads = j->nets[i].via_pads;
     s->n_via_pads = j->nets[i].n_via_pads;
     s->own_id = j->nets[i].own_id;
+    s->has_own_path = j->nets[i].has_own_path;
+    s->own_path = j->nets[i].own_path;
+    s->n_own_path = j->nets[i].n_own_path;
+    s->own_halo = j->nets[i].own_halo;
+    s->own_halo_count = j->nets[i].own_halo_count;
+    s->n_own_halo = j->nets[i].n_own_halo;
     s->window = j->windows ? j->windows[i] : mrc_window_full(j->tmpl->grid->dims);
 }
 

@@ -89,6 +89,21 @@ typedef struct {
     /* astar heuristic: 0 = none (Dijkstra), 1 = manhattan_scaled. */
     int use_heuristic;
     uint32_t min_via_cost;
+    /* route_negotiated snapshot pricing (negotiated.rs:2777-2797), active when
+     * neg_history != NULL: entering v costs passable_search_cost(base +
+     * history[v] + foreign(v)) with the net's own copper and counted halo
+     * subtracted from the snapshot (fused when neg_shared != NULL). */
+    const uint32_t *neg_history;
+    const uint32_t *neg_present;
+    const uint32_t *neg_halo;    /* NULL = all zero */
+    const uint64_t *neg_shared;  /* fused shared congestion, or NULL */
+    uint32_t neg_pfac;
+    int has_own_path;            /* own_path Some / None */
+    const uint32_t *own_path;    /* sorted unique */
+    size_t n_own_path;
+    const uint32_t *own_halo;    /* sorted unique cells ... */
+    const uint32_t *own_halo_count; /* ... and their multiplicities */
+    size_t n_own_halo;
 } mrc_spec;
 
 /* ---- scalar helpers ---------------------------------------------------- */
@@ -158,6 +173,13 @@ typedef struct {
     const uint32_t *via_pads;
     size_t n_via_pads;
     int32_t own_id;
+    /* negotiated searches: the net's own copper and counted halo */
+    int has_own_path;
+    const uint32_t *own_path;
+    size_t n_own_path;
+    const uint32_t *own_halo;
+    const uint32_t *own_halo_count;
+    size_t n_own_halo;
 } mrc_net;
 
 /* For each net: search its window, and on failure the full board.  Paths are

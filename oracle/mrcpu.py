@@ -68,6 +68,17 @@ class SpecS(C.Structure):
         ("mode", C.c_int),
         ("use_heuristic", C.c_int),
         ("min_via_cost", C.c_uint32),
+        ("neg_history", U32P),
+        ("neg_present", U32P),
+        ("neg_halo", U32P),
+        ("neg_shared", C.POINTER(C.c_uint64)),
+        ("neg_pfac", C.c_uint32),
+        ("has_own_path", C.c_int),
+        ("own_path", U32P),
+        ("n_own_path", C.c_size_t),
+        ("own_halo", U32P),
+        ("own_halo_count", U32P),
+        ("n_own_halo", C.c_size_t),
     ]
 
 
@@ -80,6 +91,12 @@ class NetS(C.Structure):
         ("via_pads", U32P),
         ("n_via_pads", C.c_size_t),
         ("own_id", C.c_int32),
+        ("has_own_path", C.c_int),
+        ("own_path", U32P),
+        ("n_own_path", C.c_size_t),
+        ("own_halo", U32P),
+        ("own_halo_count", U32P),
+        ("n_own_halo", C.c_size_t),
     ]
 
 
@@ -137,6 +154,15 @@ def lib():
 
 def _u32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _counted(own_halo):
+    """CountedCellSet from a {cell: count} dict or a list of cells repeated by multiplicity."""
+    if isinstance(own_halo, dict):
+        keys = sorted(int(c) for c, k in own_halo.items() if k)
+        return keys, [int(own_halo[c]) for c in keys]
+    hc, hn = np.unique(np.asarray(list(own_halo), dtype=np.int64), return_counts=True)
+    return [int(c) for c in hc], [int(k) for k in hn]
 
 
 def _p32(a: Optional[np.ndarray]):
@@ -206,6 +232,30 @@ class Net:
     pads: List[int] = field(default_factory=list)
     via_pads: List[int] = field(default_factory=list)
     own_id: int = -1
+    # negotiated searches: own copper (None = not passed) and own halo
+    # ({cell: count}, or cells repeated by multiplicity)
+    own_path: Optional[List[int]] = None
+    own_halo: object = field(default_factory=list)
+
+
+@dataclass
+class Negotiation:
+    """Snapshot of route_negotiated's pricing inputs (negotiated.rs:2746-2797)."""
+    history: np.ndarray
+    present: np.ndarray
+    present_halo: Optional[np.ndarray]
+    pfac: int
+    fused: bool
+
+    def shared(self) -> np.ndarray:
+        """shared_congestion of the Jacobi batch (negotiated.rs:1654-1667)."""
+        pf = np.uint64(self.pfac * 16)
+        p = np.asarray(self.present, dtype=np.uint64) * pf
+        h = (np.asarray(self.present_halo, dtype=np.uint64) * pf if self.present_halo is not None
+             else np.zeros_like(p))
+        s = p + h
+        s[s < p] = np.uint64(0xFFFFFFFFFFFFFFFF)
+        return s
 
 
 class Spec:
@@ -213,7 +263,8 @@ class Spec:
 
     def __init__(self, grid: Grid, edges: Optional[Edges] = None, *, mode=MODE_ALONE,
                  pads=(), via_pads=(), window=None, additive=None, halo_owner=None,
-                 halo_penalty=0, own_id=-1, use_heuristic=False, min_via_cost=0):
+                 halo_penalty=0, own_id=-1, use_heuristic=False, min_via_cost=0,
+                 negotiation: Optional[Negotiation] = None, own_path=None, own_halo=()):
         self.grid = grid
         self.edges = edges if edges is not None else Edges.uniform(grid.w, grid.h, grid.layers, 1, 0)
         self.pads = _u32(sorted(set(int(p) for p in pads)) + [0])
@@ -230,6 +281,28 @@ class Spec:
             self.halo_owner.ctypes.data_as(I32P) if self.halo_owner is not None else None,
             halo_penalty, own_id, _p32(self.pads), self.n_pads, _p32(self.via_pads), self.n_via_pads,
             WindowS(*window), mode, 1 if use_heuristic else 0, min_via_cost)
+        self.negotiation = negotiation
+        if negotiation is not None:
+            self._neg = [_u32(negotiation.history), _u32(negotiation.present)]
+            self._neg_halo = _u32(negotiation.present_halo) if negotiation.present_halo is not None else None
+            self._neg_shared = (np.ascontiguousarray(negotiation.shared(), dtype=np.uint64)
+                                if negotiation.fused else None)
+            self.s.neg_history = _p32(self._neg[0])
+            self.s.neg_present = _p32(self._neg[1])
+            self.s.neg_halo = _p32(self._neg_halo)
+            self.s.neg_shared = (self._neg_shared.ctypes.data_as(C.POINTER(C.c_uint64))
+                                 if self._neg_shared is not None else None)
+            self.s.neg_pfac = int(negotiation.pfac)
+        if own_path is not None:
+            self._own = _u32(sorted(set(int(c) for c in own_path)) + [0])
+            self.s.has_own_path = 1
+            self.s.own_path = _p32(self._own)
+            self.s.n_own_path = len(self._own) - 1
+        hc, hn = _counted(own_halo)
+        self._own_halo = (_u32(list(hc) + [0]), _u32(list(hn) + [0]))
+        self.s.own_halo = _p32(self._own_halo[0])
+        self.s.own_halo_count = _p32(self._own_halo[1])
+        self.s.n_own_halo = len(hc)
 
 
 # ---- thin wrappers ---------------------------------------------------------
@@ -339,8 +412,14 @@ def _nets_array(nets: Sequence[Net]):
     for i, n in enumerate(nets):
         p = _u32(sorted(set(n.pads)) + [0])
         vp = _u32(sorted(set(n.via_pads)) + [0])
-        keep += [p, vp]
-        arr[i] = NetS(n.src, n.dst, _p32(p), len(set(n.pads)), _p32(vp), len(set(n.via_pads)), n.own_id)
+        own = sorted(set(n.own_path)) if n.own_path is not None else []
+        op = _u32(own + [0])
+        hc, hn = _counted(n.own_halo)
+        hcell = _u32(list(hc) + [0])
+        hcnt = _u32(list(hn) + [0])
+        keep += [p, vp, op, hcell, hcnt]
+        arr[i] = NetS(n.src, n.dst, _p32(p), len(set(n.pads)), _p32(vp), len(set(n.via_pads)), n.own_id,
+                      1 if n.own_path is not None else 0, _p32(op), len(own), _p32(hcell), _p32(hcnt), len(hc))
     return arr, keep
 
 
