@@ -21,6 +21,9 @@ restatement of mr-cpu, not the Rust reference) on a bounded sample of the
 same nets on the host cores.
 
 --workload serve runs the serving comparison instead (serve_bench.py).
+--workload negotiate runs the negotiated-congestion loop (negotiate.py) on
+the same board and nets: one step = one negotiation iteration, its dirty nets
+routed in one device batch with per-net self-subtracted prices.
 """
 from __future__ import annotations
 
@@ -44,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["fields", "serve"], default="fields",
+    ap.add_argument("--workload", choices=["fields", "serve", "negotiate"], default="fields",
                     help="fields: one batch per step of --config (default, the headline); "
                          "serve: Poisson request mix through the serving loop vs one request at a time")
     ap.add_argument("--config", type=int, default=2,
@@ -205,6 +208,92 @@ def serve_main(a, tiles):
     print(json.dumps(out))
 
 
+def negotiate_main(a, tiles):
+    """Negotiation iterations on the config's board: searches/s over the
+    timed iterations, per-iteration latency, and the oracle's route_negotiated
+    on the first iteration's snapshot as the CPU baseline."""
+    import numpy as np
+
+    import mrfield
+    import negotiate as ng
+
+    eng = mrfield.Engine(0)
+    eng.set_tiles(*tiles)
+    eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)
+    s = mrfield.synth_config(a.config, seed=a.seed)
+    g = s.grid
+    board = mrfield.Board(eng, g, s.edges, None)
+    nets = s.nets()
+    hist0 = s.additive  # the configuration's congestion-history plane
+    if a.warmup:
+        ng.negotiate_board(board, nets, max_iters=a.warmup, history0=hist0)
+    times, dirty = [], []
+    last = [time.perf_counter()]
+
+    def on_iter(it, n):
+        t = time.perf_counter()
+        times.append(t - last[0])
+        dirty.append(n)
+        last[0] = t
+
+    t0 = time.perf_counter()
+    last[0] = t0
+    res = ng.negotiate_board(board, nets, max_iters=a.steps, history0=hist0, on_iter=on_iter)
+    wall = time.perf_counter() - t0
+    searches = int(sum(dirty))
+    out = {
+        "metric": "negotiation searches/sec (route_negotiated batches, bit-exact) at "
+                  f"{g.w}x{g.h}x{g.layers}; per-iteration latency",
+        "value": round(searches / wall, 3),
+        "unit": "searches/s",
+        "n_gpus": 1,
+        "steps": res.iterations,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / max(res.iterations, 1), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{g.w}x{g.h}x{g.layers} board, {len(nets)} two-pin nets, negotiation "
+                               f"iterations (synthetic config {a.config}, seed {a.seed}; history seeded from "
+                               "the configuration's history plane)", "tiles": a.tiles},
+        "dirty_per_iteration": dirty,
+        "ms_per_iteration": [round(t * 1e3, 2) for t in times],
+        "converged": res.converged,
+        "unrouted": int(sum(1 for p in res.paths if not p)),
+    }
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import mrcpu
+
+        og = mrcpu.Grid(g.w, g.h, g.layers, g.cost, board=g.board)
+        oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+        n = g.n
+        tmpl = mrcpu.Spec(og, oe, negotiation=mrcpu.Negotiation(
+            history=np.asarray(hist0, dtype=np.uint32), present=np.zeros(n, dtype=np.uint32), present_halo=None,
+            pfac=1, fused=True))
+        wins = [ng.net_window(g.w, g.h, x.src, x.dst) for x in nets]
+        threads = cpu_threads()
+        chunk = max(threads, 16)
+        done, k = 0, 0
+        c0 = time.perf_counter()
+        while time.perf_counter() - c0 < CPU_SECONDS:
+            idx = [(k + j) % len(nets) for j in range(chunk)]
+            on = [mrcpu.Net(nets[i].src, nets[i].dst, own_halo=[]) for i in idx]
+            mrcpu.alone_batch(tmpl, on, [wins[i] for i in idx], retry_full=True, threads=threads)
+            done += len(idx)
+            k += chunk
+        dt = time.perf_counter() - c0
+        out["cpu_baseline"] = {
+            "value": round(done / dt, 3), "unit": "searches/s", "cores": threads, "kind": "port",
+            "sample": f"{done} route_negotiated searches of the first iteration (window then full board), "
+                      f"{threads} threads, {dt:.1f} s"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -221,6 +310,9 @@ def main():
     tw, th, threads, ml = (int(v) for v in a.tiles.split(","))
     if a.workload == "serve":
         serve_main(a, (tw, th, threads, ml))
+        return
+    if a.workload == "negotiate":
+        negotiate_main(a, (tw, th, threads, ml))
         return
     eng = mrfield.Engine(0)
     eng.set_tiles(tw, th, threads, ml)

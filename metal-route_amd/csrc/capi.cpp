@@ -8,6 +8,7 @@
 #include <cstring>
 #include <iterator>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -265,6 +266,58 @@ int isolated(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* n
     set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
     return MRF_BACKEND_UNAVAILABLE;
   }
+}
+
+}  // namespace
+
+namespace {
+
+inline uint64_t sat_add64(uint64_t a, uint64_t b) { return a + b < a ? UINT64_MAX : a + b; }
+inline uint64_t sat_mul64(uint64_t a, uint64_t b) { return (b && a > UINT64_MAX / b) ? UINT64_MAX : a * b; }
+inline uint64_t sat_sub64(uint64_t a, uint64_t b) { return a > b ? a - b : 0; }
+inline uint32_t clamp32(uint64_t v) { return v < 0xFFFFFFFFull ? (uint32_t)v : 0xFFFFFFFFu; }
+constexpr uint64_t kNegScale = 16;  // SCALE == CLEARANCE_NEG_WEIGHT (negotiated.rs:71, :99)
+
+// Own cells of net i from a CSR list, sorted and unique; `counts` gets the
+// summed multiplicities (entry weights `weight`, 1 each when null).
+void own_cells(const uint64_t* off, const uint32_t* cells, const uint32_t* weight, size_t i, size_t n_cells,
+               std::vector<uint32_t>& out, std::vector<uint64_t>* counts) {
+  out.clear();
+  if (counts) counts->clear();
+  if (!off) return;
+  if (off[i + 1] < off[i]) throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint offsets are not ascending");
+  std::vector<std::pair<uint32_t, uint64_t>> v;
+  v.reserve(off[i + 1] - off[i]);
+  for (uint64_t k = off[i]; k < off[i + 1]; ++k) {
+    if (cells[k] >= n_cells) throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint cell outside the board");
+    v.emplace_back(cells[k], weight ? weight[k] : 1u);
+  }
+  std::sort(v.begin(), v.end());
+  for (size_t k = 0; k < v.size();) {
+    uint64_t sum = 0;
+    size_t e = k;
+    for (; e < v.size() && v[e].first == v[k].first; ++e) sum += v[e].second;
+    out.push_back(v[k].first);
+    if (counts) counts->push_back(sum);
+    k = e;
+  }
+}
+
+// body(begin, end) over [0, n) split across up to 8 host threads.
+template <typename F>
+void parallel_for(size_t n, size_t min_chunk, F&& body) {
+  const size_t nt = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), (n + min_chunk - 1) / min_chunk});
+  if (nt <= 1) {
+    body(size_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const size_t per = (n + nt - 1) / nt;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t a = t * per, b = std::min(n, a + per);
+    if (a < b) pool.emplace_back([&body, a, b] { body(a, b); });
+  }
+  for (auto& th : pool) th.join();
 }
 
 }  // namespace
@@ -653,41 +706,6 @@ int mrf_board_set_additive(mrf_board* bd, const uint32_t* additive, size_t n, mr
   }
 }
 
-namespace {
-
-inline uint64_t sat_add64(uint64_t a, uint64_t b) { return a + b < a ? UINT64_MAX : a + b; }
-inline uint64_t sat_mul64(uint64_t a, uint64_t b) { return (b && a > UINT64_MAX / b) ? UINT64_MAX : a * b; }
-inline uint64_t sat_sub64(uint64_t a, uint64_t b) { return a > b ? a - b : 0; }
-inline uint32_t clamp32(uint64_t v) { return v < 0xFFFFFFFFull ? (uint32_t)v : 0xFFFFFFFFu; }
-constexpr uint64_t kNegScale = 16;  // SCALE == CLEARANCE_NEG_WEIGHT (negotiated.rs:71, :99)
-
-// Own cells of net i from a CSR list, sorted and unique; `counts` gets the
-// summed multiplicities (entry weights `weight`, 1 each when null).
-void own_cells(const uint64_t* off, const uint32_t* cells, const uint32_t* weight, size_t i, size_t n_cells,
-               std::vector<uint32_t>& out, std::vector<uint64_t>* counts) {
-  out.clear();
-  if (counts) counts->clear();
-  if (!off) return;
-  if (off[i + 1] < off[i]) throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint offsets are not ascending");
-  std::vector<std::pair<uint32_t, uint64_t>> v;
-  v.reserve(off[i + 1] - off[i]);
-  for (uint64_t k = off[i]; k < off[i + 1]; ++k) {
-    if (cells[k] >= n_cells) throw Error(MRF_BACKEND_UNAVAILABLE, "own-footprint cell outside the board");
-    v.emplace_back(cells[k], weight ? weight[k] : 1u);
-  }
-  std::sort(v.begin(), v.end());
-  for (size_t k = 0; k < v.size();) {
-    uint64_t sum = 0;
-    size_t e = k;
-    for (; e < v.size() && v[e].first == v[k].first; ++e) sum += v[e].second;
-    out.push_back(v[k].first);
-    if (counts) counts->push_back(sum);
-    k = e;
-  }
-}
-
-}  // namespace
-
 int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets,
                                const mrf_window* windows, int retry_full, const mrf_negotiation* neg, mrf_paths* out,
                                mrf_error* err) {
@@ -707,40 +725,52 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
     // history into the board's additive plane: what every net pays on cells
     // outside its own footprint.
     std::vector<uint32_t> plane(n);
-    for (size_t c = 0; c < n; ++c) {
-      const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
-      plane[c] = clamp32(sat_add64(neg->history[c], shared));
-    }
+    parallel_for(n, 1 << 16, [&](size_t a, size_t b) {
+      for (size_t c = a; c < b; ++c) {
+        const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
+        plane[c] = clamp32(sat_add64(neg->history[c], shared));
+      }
+    });
     // Own footprint of each net: priced_with_base's self-subtraction
     // (negotiated.rs:2777-2797) as additive overrides.
     std::vector<NetPlan> prices(n_nets);
-    std::vector<uint32_t> path, halo;
-    std::vector<uint64_t> hcount;
-    for (size_t i = 0; i < n_nets; ++i) {
-      own_cells(neg->own_path_off, neg->own_path, nullptr, i, n, path, nullptr);
-      own_cells(neg->own_halo_off, neg->own_halo, neg->own_halo_count, i, n, halo, &hcount);
-      std::vector<uint32_t> u;
-      std::set_union(path.begin(), path.end(), halo.begin(), halo.end(), std::back_inserter(u));
-      NetPlan& p = prices[i];
-      for (uint32_t c : u) {
-        const uint64_t self_present = std::binary_search(path.begin(), path.end(), c) ? 1u : 0u;
-        const auto h = std::lower_bound(halo.begin(), halo.end(), c);
-        const uint64_t self_halo = (h != halo.end() && *h == c) ? hcount[h - halo.begin()] : 0u;
-        uint64_t foreign;
-        if (neg->fused) {
-          const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
-          foreign = sat_sub64(sat_sub64(shared, sat_mul64(pf, self_present)), sat_mul64(pf, self_halo));
-        } else {
-          foreign = sat_add64(sat_mul64(pf, sat_sub64(neg->present[c], self_present)),
-                              sat_mul64(pf, sat_sub64(halo_at(c), self_halo)));
+    std::vector<std::string> bad(n_nets);
+    parallel_for(n_nets, 16, [&](size_t a, size_t b) {
+      std::vector<uint32_t> path, halo;
+      std::vector<uint64_t> hcount;
+      for (size_t i = a; i < b; ++i) {
+        try {
+          own_cells(neg->own_path_off, neg->own_path, nullptr, i, n, path, nullptr);
+          own_cells(neg->own_halo_off, neg->own_halo, neg->own_halo_count, i, n, halo, &hcount);
+        } catch (const Error& e) {
+          bad[i] = e.what();
+          continue;
         }
-        const uint32_t a = clamp32(sat_add64(neg->history[c], foreign));
-        if (a != plane[c]) {
-          p.ovr_cells.push_back(c);
-          p.ovr_add.push_back(a);
+        std::vector<uint32_t> u;
+        std::set_union(path.begin(), path.end(), halo.begin(), halo.end(), std::back_inserter(u));
+        NetPlan& p = prices[i];
+        for (uint32_t c : u) {
+          const uint64_t self_present = std::binary_search(path.begin(), path.end(), c) ? 1u : 0u;
+          const auto h = std::lower_bound(halo.begin(), halo.end(), c);
+          const uint64_t self_halo = (h != halo.end() && *h == c) ? hcount[h - halo.begin()] : 0u;
+          uint64_t foreign;
+          if (neg->fused) {
+            const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
+            foreign = sat_sub64(sat_sub64(shared, sat_mul64(pf, self_present)), sat_mul64(pf, self_halo));
+          } else {
+            foreign = sat_add64(sat_mul64(pf, sat_sub64(neg->present[c], self_present)),
+                                sat_mul64(pf, sat_sub64(halo_at(c), self_halo)));
+          }
+          const uint32_t price = clamp32(sat_add64(neg->history[c], foreign));
+          if (price != plane[c]) {
+            p.ovr_cells.push_back(c);
+            p.ovr_add.push_back(price);
+          }
         }
       }
-    }
+    });
+    for (const std::string& m : bad)
+      if (!m.empty()) throw Error(MRF_BACKEND_UNAVAILABLE, m.c_str());
     mrf::board_set_additive(ctx->c, bd->b, plane.data());
     bd->add = std::move(plane);
     solve_board(ctx, bd, nets, n_nets, windows, retry_full, &prices, out);

@@ -1484,8 +1484,7 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
 void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add) {
   HIP_OK(hipSetDevice(c.device));
   const size_t n = (size_t)b.w * b.h * b.layers;
-  uint64_t max_a = 0;
-  for (size_t i = 0; i < n; ++i) max_a = std::max<uint64_t>(max_a, add[i]);
+  uint64_t max_a = *std::max_element(add, add + n);
   if (b.halo) max_a += b.halo_pen;
   if (!b.add) {
     b.add = dev_copy(add, n, c.stream);

@@ -551,33 +551,35 @@ class Board:
             raise MrfError(MALFORMED_GRID, "negotiation planes differ in length")
         ng = NegotiationS(_p32(hist), _p32(pres), _p32(halo), hist.size, int(pfac), 1 if fused else 0)
         if own_paths is not None:
-            lens = [len(p) for p in own_paths]
+            arrs = [np.asarray(p, dtype=np.uint32).ravel() for p in own_paths]
             poff = np.zeros(len(nets) + 1, dtype=np.uint64)
-            poff[1:] = np.cumsum(lens)
-            pc = _u32(np.concatenate([np.asarray(p, dtype=np.uint32) for p in own_paths]) if sum(lens) else [0])
+            np.cumsum([a.size for a in arrs], out=poff[1:])
+            pc = np.concatenate(arrs) if poff[-1] else np.zeros(1, dtype=np.uint32)
             keep += [poff, pc]
             ng.own_path_off = poff.ctypes.data_as(C.POINTER(C.c_uint64))
             ng.own_path = _p32(pc)
         if own_halos is not None:
-            cells, counts, lens = [], [], []
+            cells, counts = [], []
+            weighted = any(isinstance(h, dict) for h in own_halos)
             for hset in own_halos:
                 if isinstance(hset, dict):
-                    ks = [int(c) for c in hset]
-                    cells += ks
-                    counts += [int(hset[c]) for c in hset]
-                    lens.append(len(ks))
+                    cells.append(np.fromiter(hset.keys(), dtype=np.uint32, count=len(hset)))
+                    counts.append(np.fromiter(hset.values(), dtype=np.uint32, count=len(hset)))
                 else:
-                    hs = [int(c) for c in hset]
-                    cells += hs
-                    counts += [1] * len(hs)
-                    lens.append(len(hs))
+                    a = np.asarray(hset, dtype=np.uint32).ravel()
+                    cells.append(a)
+                    if weighted:
+                        counts.append(np.ones(a.size, dtype=np.uint32))
             hoff = np.zeros(len(nets) + 1, dtype=np.uint64)
-            hoff[1:] = np.cumsum(lens)
-            hc, hn = _u32(cells + [0]), _u32(counts + [0])
-            keep += [hoff, hc, hn]
+            np.cumsum([a.size for a in cells], out=hoff[1:])
+            hc = np.concatenate(cells) if hoff[-1] else np.zeros(1, dtype=np.uint32)
+            keep += [hoff, hc]
             ng.own_halo_off = hoff.ctypes.data_as(C.POINTER(C.c_uint64))
             ng.own_halo = _p32(hc)
-            ng.own_halo_count = _p32(hn)
+            if weighted:
+                hn = np.concatenate(counts) if hoff[-1] else np.zeros(1, dtype=np.uint32)
+                keep.append(hn)
+                ng.own_halo_count = _p32(hn)
         out = PathsS()
         err = ErrorS()
         _check(lib().mrf_route_negotiated_batch(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)),

@@ -124,3 +124,31 @@ def isolated_stress_case(case: int):
     else:
         via = [None, 0]     # with_allowed_steps(3, 0, [(1,2)])
     return mrcpu.Grid(w, h, L, cost), nets, mrcpu.Edges(ex, ey, via)
+
+
+def oracle_parts(s):
+    """Oracle grid and edge prices of a synthetic board (mrfield.Synth)."""
+    g = s.grid
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost, board=g.board)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    return og, oe
+
+
+def oracle_negotiation_router(s, nets, windows, threads=8):
+    """The negotiation loop's search (RouteBatch of metal-route_amd/negotiate.py)
+    answered by the oracle's route_negotiated restatement."""
+    og, oe = oracle_parts(s)
+
+    def route(idx, history, present, pfac, fused, own_paths, own_halos):
+        ng = mrcpu.Negotiation(history=np.array(history, dtype=np.uint32), present=np.array(present, dtype=np.uint32),
+                               present_halo=None, pfac=pfac, fused=fused)
+        tmpl = mrcpu.Spec(og, oe, negotiation=ng)
+        onets = []
+        for k, i in enumerate(idx):
+            onets.append(mrcpu.Net(nets[i].src, nets[i].dst, pads=list(nets[i].pads), via_pads=list(nets[i].via_pads),
+                                   own_path=own_paths[k] if own_paths is not None else None,
+                                   own_halo=own_halos[k] if own_halos is not None else []))
+        paths, _ = mrcpu.alone_batch(tmpl, onets, [windows[i] for i in idx], retry_full=True, threads=threads)
+        return paths
+
+    return route

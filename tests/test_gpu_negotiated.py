@@ -197,3 +197,33 @@ def test_negotiated_batch_rejects_bad_planes():
             board.close()
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("side,layers,n,seed,iters", [(64, 4, 24, 11, 60), (40, 1, 40, 3, 6), (24, 2, 12, 5, 10)])
+def test_negotiation_loop_matches_oracle_loop(side, layers, n, seed, iters):
+    """The whole loop (negotiate.py) with device batches equals the loop with
+    the oracle's searches, iteration by iteration: Jacobi with term-by-term
+    snapshots (converging), Jacobi with fused snapshots, and Gauss-Seidel."""
+    import negotiate as ng
+    from cases import oracle_negotiation_router
+
+    s = mrfield.synth_board(side, layers, n, seed)
+    nets = s.nets()
+    g = s.grid
+    wins = [ng.net_window(g.w, g.h, x.src, x.dst, x.pads) for x in nets]
+    eng = mrfield.Engine(0)
+    try:
+        eng.set_tiles(16, 16, 256, 32)
+        board = mrfield.Board(eng, g, s.edges, None)
+        try:
+            gpu = ng.negotiate(ng.board_router(board, nets, wins), g.n, len(nets), max_iters=iters, keep_trace=True)
+        finally:
+            board.close()
+    finally:
+        eng.close()
+    cpu = ng.negotiate(oracle_negotiation_router(s, nets, wins), g.n, len(nets), max_iters=iters, keep_trace=True)
+    assert (gpu.iterations, gpu.converged) == (cpu.iterations, cpu.converged)
+    for a, b in zip(gpu.trace, cpu.trace):
+        assert a == b, a.iter
+    assert gpu.paths == cpu.paths
+    assert np.array_equal(gpu.history, cpu.history)
