@@ -361,6 +361,7 @@ struct Lds {
   uint32_t* rchg;
   uint32_t* cchg;
   uint32_t* vchg;
+  uint32_t* hstep;  // [2*L*(TH+TW)] lower bound of the price of entering each halo cell from the tile
   uint32_t* xg;  // [TW+1] x gap prices entering column i from the left
   uint32_t* yg;  // [TH+1]
   uint32_t* vg;  // [16]
@@ -387,6 +388,7 @@ __host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L
   s += align16(nlab * 8);
   s += nstep * align16(ncell * (narrow ? 2 : 4));
   s += align16((size_t)L * th * 4) + align16((size_t)L * tw * 4) + align16((size_t)th * tw * 4);
+  s += align16((size_t)2 * L * (th + tw) * 4);
   s += align16((tw + 1) * 4) + align16((th + 1) * 4) + align16(16 * 4) + 64;
   return s;
 }
@@ -417,6 +419,7 @@ __device__ inline Lds<NARROW> carve(char* base, uint32_t L) {
   s.rchg = (uint32_t*)take((size_t)L * TH * 4);
   s.cchg = (uint32_t*)take((size_t)L * TW * 4);
   s.vchg = (uint32_t*)take((size_t)TH * TW * 4);
+  s.hstep = (uint32_t*)take((size_t)2 * L * (TH + TW) * 4);
   s.xg = (uint32_t*)take((TW + 1) * 4);
   s.yg = (uint32_t*)take((TH + 1) * 4);
   s.vg = (uint32_t*)take(16 * 4);
@@ -624,7 +627,9 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   }
   // halo ring, two entries per thread: sides 0..3 = left, right, top, bottom
   const uint32_t nhalo = 2u * L * (TH + TW);
-  auto halo_entry = [&](uint32_t j, uint64_t& src, uint32_t& at, uint32_t& side, bool& exists) {
+  // Entry j: the neighbour's label block word `src`, its ring slot `at`, the
+  // side, whether the neighbour exists, and the board cell `gc` it holds.
+  auto halo_entry = [&](uint32_t j, uint64_t& src, uint32_t& at, uint32_t& side, bool& exists, uint32_t& gc) {
     if (j < 2u * L * TH) {
       side = j / (L * TH);
       const uint32_t k = j - side * L * TH, l = k / TH, y = k - l * TH;
@@ -637,6 +642,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         src = blockbase + TB + ncell + k;  // right tile's first column
       }
       at = l * LAB_PLANE + (y + 1) * RS + (side ? TW + 1 : 0);
+      gc = (l * B.h + F.y0 + wy0 + y) * B.w + F.x0 + wx0 + (side ? TW : 0u) - (side ? 0u : 1u);
     } else {
       const uint32_t jj = j - 2u * L * TH, sd = jj / (L * TW), k = jj - sd * L * TW;
       const uint32_t l = k / TW, x = k - l * TW;
@@ -650,11 +656,13 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         src = blockbase + (uint64_t)F.tx * TB + l * PT + x;
       }
       at = l * LAB_PLANE + (sd ? (TH + 1) * RS : 0) + x + 1;
+      gc = (l * B.h + F.y0 + wy0 + (sd ? TH : 0u) - (sd ? 0u : 1u)) * B.w + F.x0 + wx0 + x;
     }
-    if (!exists) src = blockbase;
+    if (!exists) src = blockbase, gc = 0;
   };
   uint64_t hv[2];
-  uint32_t hat[2], hside[2];
+  uint32_t hat[2], hside[2], hgc[2], hw[2], ha[2];
+  uint8_t hb[2];
   bool hok[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
@@ -662,9 +670,13 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     uint64_t src = blockbase;
     hat[u] = 0;
     hside[u] = 0;
+    hgc[u] = 0;
     hok[u] = false;
-    if (j < nhalo) halo_entry(j, src, hat[u], hside[u], hok[u]);
+    if (j < nhalo) halo_entry(j, src, hat[u], hside[u], hok[u], hgc[u]);
     hv[u] = __hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    hb[u] = gbl(B.bmask)[hgc[u]];
+    hw[u] = gbl(B.cost)[hgc[u]];
+    ha[u] = B.add ? gbl(B.add)[hgc[u]] : 0u;
   }
   // Interior planes by quads: four consecutive cells of one row per thread
   // per batch, loaded with one vector load per plane when the quad lies
@@ -752,18 +764,45 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   __syncthreads();
   const uint32_t nbv = ((uniform_u32(s.misc[1]) & S_TOUCHED) ? 1u : 0u) | ((uniform_u32(s.misc[2]) & S_TOUCHED) ? 2u : 0u) |
                        ((uniform_u32(s.misc[3]) & S_TOUCHED) ? 4u : 0u) | ((uniform_u32(s.misc[4]) & S_TOUCHED) ? 8u : 0u);
-  // halo labels into the ring (a neighbour never visited has none)
+  // halo labels into the ring (a neighbour never visited has none), and a
+  // lower bound of the price of stepping from the tile into each halo cell:
+  // the neighbour's gap price times the cell's weight (1 for an obstacle
+  // that is an own pad) plus its additive price (0 when the net carries
+  // overrides, which only lower it); closed when the cell is a trace node,
+  // a foreign obstacle or its edge towards the tile is closed.
+  const uint32_t* hpads = A.pool + F.pad_off;
+  auto entry_price = [&](uint32_t side, uint32_t gc, uint8_t bm, uint32_t w, uint32_t a) -> uint32_t {
+    constexpr uint8_t kEdge[4] = {K_PX, K_NX, K_PY, K_NY};  // the cell's edge facing the tile
+    const uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
+    if ((bm & M_TRACE) || !(m & kEdge[side])) return kMax;
+    uint32_t wgt = w;
+    if (bm & M_OBST) {
+      bool forced = false;
+      if (F.flags & F_FORCE)
+        for (uint32_t k = 0; k < F.n_forced; ++k) forced = forced || A.pool[F.forced_off + k] == gc;
+      if (F.n_pad && sorted_has(hpads, F.n_pad, gc))
+        wgt = 1u;
+      else if (!forced)
+        return kMax;
+    }
+    const uint32_t e = side == 0 ? s.xg[0] : side == 1 ? s.xg[TW] : side == 2 ? s.yg[0] : s.yg[TH];
+    return step_price(e, wgt, F.n_ovr ? 0u : a);
+  };
 #pragma unroll
   for (int u = 0; u < 2; ++u)
-    if (u * nt + tid < nhalo) s.lab[hat[u]] = (hok[u] && ((nbv >> hside[u]) & 1u)) ? lab_in(hv[u]) : kUnreached;
+    if (u * nt + tid < nhalo) {
+      s.lab[hat[u]] = (hok[u] && ((nbv >> hside[u]) & 1u)) ? lab_in(hv[u]) : kUnreached;
+      s.hstep[u * nt + tid] = hok[u] ? entry_price(hside[u], hgc[u], hb[u], hw[u], ha[u]) : kMax;
+    }
   for (uint32_t j = 2 * nt + tid; j < nhalo; j += nt) {
     uint64_t src;
-    uint32_t at, side;
+    uint32_t at, side, gc;
     bool ex;
-    halo_entry(j, src, at, side, ex);
+    halo_entry(j, src, at, side, ex, gc);
     s.lab[at] = (ex && ((nbv >> side) & 1u))
                     ? lab_in(__hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     : kUnreached;
+    s.hstep[j] = ex ? entry_price(side, gc, gbl(B.bmask)[gc], gbl(B.cost)[gc], B.add ? gbl(B.add)[gc] : 0u) : kMax;
   }
 
   // Interior cells: labels (when the tile was visited before), the per-net
@@ -884,23 +923,29 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
                            __HIP_MEMORY_SCOPE_AGENT);
       if (chg[u]) {
         // A changed border cell wakes the neighbour across that side only if
-        // it can improve the neighbour's adjacent cell: every candidate it
-        // offers is at least (dist, hops + 1) = v + 1, so a halo label (the
-        // neighbour's label at load time, which only decreases) at or below
-        // v + 1 cannot improve, and no path into the neighbour tile through
-        // this cell can either.
+        // it can improve the neighbour's adjacent cell: the candidate it
+        // offers is at least (dist + p, hops + 1) for p the lower bound of the
+        // entry price (hstep), so a halo label (the neighbour's label at load
+        // time, which only decreases) at or below that cannot improve, and
+        // no path into the neighbour tile through this cell can either.
         // The planar edges are symmetric: when entering this (passable) cell
         // from a side is closed, leaving it towards that side is too.
         const uint32_t d = (uint32_t)(v >> 32);
-        const uint64_t lb = v + 1;
+        auto offers = [&](uint32_t p) {
+          return ((uint64_t)__builtin_elementwise_add_sat(d, p) << 32) | (uint32_t)((uint32_t)v + 1u);
+        };
         const uint64_t* row = s.lab + l * LAB_PLANE + (y + 1) * RS;
         const uint32_t pc = l * STEP_PLANE + y * PS + x;
         flags |= T_ANY;
         ka = min(ka, d);
-        if (x == 0 && lb < row[0] && step_of(s.sxl[pc]) != kMax) flags |= T_LEFT, kl = min(kl, d);
-        if (x == TW - 1 && lb < row[TW + 1] && step_of(s.sxr[pc]) != kMax) flags |= T_RIGHT, kr = min(kr, d);
-        if (y == 0 && lb < s.lab[l * LAB_PLANE + x + 1] && step_of(s.syt[pc]) != kMax) flags |= T_TOP, kt = min(kt, d);
-        if (y == TH - 1 && lb < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1] && step_of(s.syb[pc]) != kMax)
+        if (x == 0 && offers(s.hstep[l * TH + y]) < row[0] && step_of(s.sxl[pc]) != kMax) flags |= T_LEFT, kl = min(kl, d);
+        if (x == TW - 1 && offers(s.hstep[L * TH + l * TH + y]) < row[TW + 1] && step_of(s.sxr[pc]) != kMax)
+          flags |= T_RIGHT, kr = min(kr, d);
+        if (y == 0 && offers(s.hstep[2 * L * TH + l * TW + x]) < s.lab[l * LAB_PLANE + x + 1] &&
+            step_of(s.syt[pc]) != kMax)
+          flags |= T_TOP, kt = min(kt, d);
+        if (y == TH - 1 && offers(s.hstep[2 * L * TH + L * TW + l * TW + x]) < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1] &&
+            step_of(s.syb[pc]) != kMax)
           flags |= T_BOTTOM, kb = min(kb, d);
       }
     }
