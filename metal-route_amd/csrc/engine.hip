@@ -170,6 +170,23 @@ __device__ __forceinline__ T uniform_copy(const T& v) {
   return out;
 }
 
+// A workgroup-uniform load through the scalar data cache (constant address
+// space): for data no workgroup of the running kernel writes -- descriptors
+// the host uploaded, work lists and item states written by earlier launches
+// (each launch starts with invalidated caches).  The value lands in scalar
+// registers directly.
+template <typename T>
+__device__ __forceinline__ T const_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor size must be a whole number of dwords");
+  const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t w[sizeof(T) / 4];
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 // The same pointer in the global address space: loads through it are
 // global_load (no flat address check, no LDS counter in the wait).
 template <typename T>
@@ -361,7 +378,7 @@ struct Lds {
   uint32_t* rchg;
   uint32_t* cchg;
   uint32_t* vchg;
-  uint32_t* hstep;  // [2*L*(TH+TW)] lower bound of the price of entering each halo cell from the tile
+  Step* hstep;  // [2*L*(TH+TW)] lower bound of the price of entering each halo cell from the tile
   uint32_t* xg;  // [TW+1] x gap prices entering column i from the left
   uint32_t* yg;  // [TH+1]
   uint32_t* vg;  // [16]
@@ -388,7 +405,7 @@ __host__ __device__ inline size_t lds_bytes(uint32_t tw, uint32_t th, uint32_t L
   s += align16(nlab * 8);
   s += nstep * align16(ncell * (narrow ? 2 : 4));
   s += align16((size_t)L * th * 4) + align16((size_t)L * tw * 4) + align16((size_t)th * tw * 4);
-  s += align16((size_t)2 * L * (th + tw) * 4);
+  s += align16((size_t)2 * L * (th + tw) * (narrow ? 2 : 4));
   s += align16((tw + 1) * 4) + align16((th + 1) * 4) + align16(16 * 4) + 64;
   return s;
 }
@@ -419,7 +436,7 @@ __device__ inline Lds<NARROW> carve(char* base, uint32_t L) {
   s.rchg = (uint32_t*)take((size_t)L * TH * 4);
   s.cchg = (uint32_t*)take((size_t)L * TW * 4);
   s.vchg = (uint32_t*)take((size_t)TH * TW * 4);
-  s.hstep = (uint32_t*)take((size_t)2 * L * (TH + TW) * 4);
+  s.hstep = (ST*)take((size_t)2 * L * (TH + TW) * sizeof(ST));
   s.xg = (uint32_t*)take((TW + 1) * 4);
   s.yg = (uint32_t*)take((TH + 1) * 4);
   s.vg = (uint32_t*)take(16 * 4);
@@ -590,13 +607,14 @@ __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ?
 template <int TW, int TH, int LM, bool NARROW>
 __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
-  const uint32_t f = uniform_u32(A.item_field[item]);
+  const uint32_t f = const_load(A.item_field + item);
   // the item's own state travels with the descriptor loads: whether the tile
   // holds labels is known before any plane load is issued
-  const bool touched =
-      (uniform_u32(__hip_atomic_load(&A.state[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & S_TOUCHED) != 0;
-  const FieldDesc F = uniform_copy(A.fields[f]);
-  const BoardView B = uniform_copy(A.boards[F.board]);
+  // (an item is visited at most once per launch, so no workgroup of this
+  // launch has written its state yet)
+  const bool touched = (const_load(A.state + item) & S_TOUCHED) != 0;
+  const FieldDesc F = const_load(A.fields + f);
+  const BoardView B = const_load(A.boards + F.board);
   const uint32_t t = item - F.item0;
   const uint32_t txi = t % F.tx, tyi = t / F.tx;
   constexpr uint32_t RS = TW + 3, RH = TH + 2, PS = TW + 1;
@@ -605,6 +623,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   const uint32_t ncell = L * PT;
   const uint32_t wx0 = txi * TW, wy0 = tyi * TH;
   Lds<NARROW> s = carve<TW, TH, NARROW>(smem, L);
+  using ST = typename Lds<NARROW>::Step;
   const uint64_t TB = tile_block_words(L, TW, TH);
   const uint64_t blockbase = F.lab_off + (uint64_t)t * TB;
 
@@ -786,13 +805,14 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         return kMax;
     }
     const uint32_t e = side == 0 ? s.xg[0] : side == 1 ? s.xg[TW] : side == 2 ? s.yg[0] : s.yg[TH];
-    return step_price(e, wgt, F.n_ovr ? 0u : a);
+    const uint32_t p = step_price(e, wgt, F.n_ovr ? 0u : a);
+    return NARROW ? min(p, kNarrowMaxStep) : p;  // a smaller bound stays a bound
   };
 #pragma unroll
   for (int u = 0; u < 2; ++u)
     if (u * nt + tid < nhalo) {
       s.lab[hat[u]] = (hok[u] && ((nbv >> hside[u]) & 1u)) ? lab_in(hv[u]) : kUnreached;
-      s.hstep[u * nt + tid] = hok[u] ? entry_price(hside[u], hgc[u], hb[u], hw[u], ha[u]) : kMax;
+      s.hstep[u * nt + tid] = to_step<ST>(hok[u] ? entry_price(hside[u], hgc[u], hb[u], hw[u], ha[u]) : kMax);
     }
   for (uint32_t j = 2 * nt + tid; j < nhalo; j += nt) {
     uint64_t src;
@@ -802,7 +822,8 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     s.lab[at] = (ex && ((nbv >> side) & 1u))
                     ? lab_in(__hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     : kUnreached;
-    s.hstep[j] = ex ? entry_price(side, gc, gbl(B.bmask)[gc], gbl(B.cost)[gc], B.add ? gbl(B.add)[gc] : 0u) : kMax;
+    s.hstep[j] =
+        to_step<ST>(ex ? entry_price(side, gc, gbl(B.bmask)[gc], gbl(B.cost)[gc], B.add ? gbl(B.add)[gc] : 0u) : kMax);
   }
 
   // Interior cells: labels (when the tile was visited before), the per-net
@@ -830,7 +851,6 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     if (o != -1 && o != F.own_id) a64 += B.halo_pen;
     const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
     const uint32_t pc = l * STEP_PLANE + y * PS + x;
-    using ST = typename Lds<NARROW>::Step;
     // Narrow boards bound every open step price below 0x7FFF, so it is
     // exact in 32-bit arithmetic without the saturation of step_price.
     auto g6 = [&](bool open, uint32_t e) -> ST {
@@ -938,13 +958,13 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         const uint32_t pc = l * STEP_PLANE + y * PS + x;
         flags |= T_ANY;
         ka = min(ka, d);
-        if (x == 0 && offers(s.hstep[l * TH + y]) < row[0] && step_of(s.sxl[pc]) != kMax) flags |= T_LEFT, kl = min(kl, d);
-        if (x == TW - 1 && offers(s.hstep[L * TH + l * TH + y]) < row[TW + 1] && step_of(s.sxr[pc]) != kMax)
+        if (x == 0 && offers(step_of(s.hstep[l * TH + y])) < row[0] && step_of(s.sxl[pc]) != kMax) flags |= T_LEFT, kl = min(kl, d);
+        if (x == TW - 1 && offers(step_of(s.hstep[L * TH + l * TH + y])) < row[TW + 1] && step_of(s.sxr[pc]) != kMax)
           flags |= T_RIGHT, kr = min(kr, d);
-        if (y == 0 && offers(s.hstep[2 * L * TH + l * TW + x]) < s.lab[l * LAB_PLANE + x + 1] &&
+        if (y == 0 && offers(step_of(s.hstep[2 * L * TH + l * TW + x])) < s.lab[l * LAB_PLANE + x + 1] &&
             step_of(s.syt[pc]) != kMax)
           flags |= T_TOP, kt = min(kt, d);
-        if (y == TH - 1 && offers(s.hstep[2 * L * TH + L * TW + l * TW + x]) < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1] &&
+        if (y == TH - 1 && offers(step_of(s.hstep[2 * L * TH + L * TW + l * TW + x])) < s.lab[l * LAB_PLANE + (TH + 1) * RS + x + 1] &&
             step_of(s.syb[pc]) != kMax)
           flags |= T_BOTTOM, kb = min(kb, d);
       }
@@ -1047,7 +1067,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
     if (A.key_cur && A.key_clear != A.key_ctr)
       __hip_atomic_store(&A.counters[A.key_clear * kCtrStride], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  const uint32_t item = uniform_u32(A.list_cur[blockIdx.x]);
+  const uint32_t item = const_load(A.list_cur + blockIdx.x);
   uint32_t key = 0;
   bool defer = false;
   if (A.key_cur) {
@@ -1125,7 +1145,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
   // `if (tid == 0) k = atomicAdd(..); __syncthreads(); ..` is restructured by
   // the compiler so lanes other than 0 re-run the body without a new pop.
   for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
-    uint32_t item = uniform_u32(R.list_cur[k]);
+    uint32_t item = const_load(R.list_cur + k);
     const TileResult res = process_tile<TW, TH, LM, NARROW>(R, item, smem);
     if (threadIdx.x < 64) {
       if (threadIdx.x == 0) R.act_cur[item] = 0;
