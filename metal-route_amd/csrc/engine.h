@@ -176,8 +176,6 @@ struct Workspace {
   size_t key_cap = 0;
   uint32_t* list = nullptr;   // 2 x items active-item lists (round form)
   size_t list_cap = 0;
-  uint32_t* queue = nullptr;  // persistent work queue
-  size_t queue_cap = 0;
   FieldDesc* fields = nullptr;
   size_t fields_cap = 0;
   uint32_t* item_field = nullptr;
@@ -220,7 +218,6 @@ struct Ctx {
   uint32_t delta = kMax;      // plain form: delta-stepping window in search cost (kMax = off)
   size_t label_budget = size_t(48) << 30;  // bytes of labels per chunk
   int cus = 256;
-  uint64_t spin_ticks = 400000000ull;  // bound of device spin waits (wall-clock ticks)
   SolveStats last;
   std::vector<FieldDesc> layout;  // fields of the last solve (for paths / reads)
   std::vector<BoardView> views;   // boards of the last solve

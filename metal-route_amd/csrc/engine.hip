@@ -73,29 +73,22 @@ enum : uint8_t {
   K_CHG = 0x80,
 };
 // per-item state bits
-enum : uint32_t { S_TOUCHED = 1u, S_PEND = 2u, S_ACTIVE = 4u };
+enum : uint32_t { S_TOUCHED = 1u };
 // device counters
 enum : int {
   C_CHANGED = 0,
   C_VISITS = 1,
   C_ROUNDS = 2,
   C_ERROR = 3,
-  C_QHEAD = 4,
-  C_QTAIL = 5,
-  C_PENDING = 6,
-  C_NEXTN = 7,
-  C_BAR_ARRIVE = 8,
-  C_BAR_GEN = 9,
-  C_TAKEN0 = 10,  // 10..12
-  C_NEXT0 = 13,   // 13..15
-  C_PROUNDS = 16,
-  C_MISC = 17,
-  C_GROUND = 18,  // graph form: current round index
-  C_CURN = 19,    // graph form: length of the current round's list
-  C_DONE = 20,    // plain form: workgroups of the launch that finished
-  C_KEY0 = 21,    // 21..22: smallest activation key of the next list (delta stepping)
-  C_THRESH = 23,  // delta stepping: the round processes items whose key is at most this
-  C_NCOUNTERS = 24
+  C_NEXTN = 4,    // graph form and round driver: length of the next round's list
+  C_NEXT0 = 5,    // 5..6: plain form, next list length, alternating per round
+  C_PROUNDS = 7,  // graph form: rounds that had work
+  C_GROUND = 8,   // graph form: current round index
+  C_CURN = 9,     // graph form: length of the current round's list
+  C_DONE = 10,    // plain form: workgroups of the launch that finished
+  C_KEY0 = 11,    // 11..12: smallest activation key of the next list (delta stepping)
+  C_THRESH = 13,  // delta stepping: the round processes items whose key is at most this
+  C_NCOUNTERS = 14
 };
 // Each device counter sits on its own 128-byte line: polled words and the
 // work-list atomics must not share a line, or every pop and push queues
@@ -320,12 +313,7 @@ struct RelaxArgs {
   const uint32_t* list_cur;
   uint32_t* list_next;
   uint32_t* counters;
-  uint32_t* queue;
-  uint32_t n_items;
-  uint32_t tw, th, max_local;
-  uint32_t qcap;
-  uint32_t spin_limit;
-  uint64_t spin_ticks;  // real-time bound of any spin wait (wall_clock64 ticks)
+  uint32_t max_local;
   // Plain form: the round's activation count goes to counter next_ctr;
   // counter clear_ctr (the previous round's, already read) is zeroed for the
   // next round; the last workgroup to finish publishes the counts to the
@@ -1162,7 +1150,6 @@ __global__ void advance_round(uint32_t* ctr) {
   if (ctr[C_CURN * kCtrStride]) ctr[C_PROUNDS * kCtrStride] += 1;
   ctr[C_CURN * kCtrStride] = n;
   ctr[C_NEXTN * kCtrStride] = 0;
-  ctr[C_TAKEN0 * kCtrStride] = 0;
   ctr[C_GROUND * kCtrStride] += 1;
 }
 
@@ -1404,10 +1391,6 @@ void ctx_init(Ctx& c, int device) {
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, device));
   c.cus = prop.multiProcessorCount;
-  int rate_khz = 0;
-  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
-    rate_khz = 100000;
-  c.spin_ticks = (uint64_t)rate_khz * 1000ull * 4ull;  // 4 s
   HIP_OK(hipEventCreate(&c.ws.ev0));
   HIP_OK(hipEventCreate(&c.ws.ev1));
   HIP_OK(hipEventCreate(&c.ws.ek0));
@@ -1428,7 +1411,7 @@ void ctx_init(Ctx& c, int device) {
 
 void ctx_free(Ctx& c) {
   Workspace& w = c.ws;
-  void* ptrs[] = {w.labels, w.state, w.act, w.key, w.list, w.queue, w.fields, w.item_field, w.pool, w.counters,
+  void* ptrs[] = {w.labels, w.state, w.act, w.key, w.list, w.fields, w.item_field, w.pool, w.counters,
                   w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards};
   (void)hipSetDevice(c.device);
   for (void* p : ptrs)
@@ -1580,14 +1563,7 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   A.list_cur = c.ws.list;
   A.list_next = c.ws.list + n_items;
   A.counters = c.ws.counters;
-  A.queue = c.ws.queue;
-  A.n_items = n_items;
-  A.tw = c.shape.tw;
-  A.th = c.shape.th;
   A.max_local = c.shape.max_local;
-  A.qcap = n_items;
-  A.spin_limit = 1u << 26;
-  A.spin_ticks = c.spin_ticks;
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
   A.host_out = nullptr;
@@ -1760,7 +1736,6 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   Workspace& W = c.ws;
   grow(W.labels, W.labels_cap, std::max<size_t>(lab_off, 1));
   grow(W.state, W.state_cap, std::max<size_t>(item, 1));
-  grow(W.queue, W.queue_cap, std::max<size_t>(item, 1));
   grow(W.act, W.act_cap, std::max<size_t>((size_t)item * 2, 2));
   const bool keyed = c.mode == MODE_PLAIN && c.delta != kMax;
   if (keyed) grow(W.key, W.key_cap, std::max<size_t>((size_t)item * 2, 2));
@@ -2293,11 +2268,7 @@ uint32_t driver_step(RoundDriver& d) {
   A.list_cur = list_cur;
   A.list_next = list_next;
   A.counters = c.ws.counters;
-  A.n_items = (uint32_t)items;
-  A.tw = S.tw;
-  A.th = S.th;
   A.max_local = S.max_local;
-  A.spin_ticks = c.spin_ticks;
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
   A.host_out = nullptr;
