@@ -241,9 +241,10 @@ int mrf_solve_fields(mrf_ctx *ctx, mrf_board *board, const uint32_t *const *sour
 int mrf_read_field(mrf_ctx *ctx, mrf_board *board, size_t field, uint32_t *dist_out,
                    uint32_t *hops_out, mrf_error *err);
 
-/* Replace the resident board's additive price plane (n == w*h*layers).
- * The negotiation loop reprices the board once per iteration
- * (negotiated.rs:1610-1667); mrf_route_negotiated_batch calls this itself. */
+/* Replace the resident board's additive price plane (n == w*h*layers), or
+ * drop it (additive == NULL, n == 0).  The negotiation loop reprices the
+ * board once per iteration (negotiated.rs:1610-1667);
+ * mrf_route_negotiated_batch calls this itself. */
 int mrf_board_set_additive(mrf_board *board, const uint32_t *additive, size_t n, mrf_error *err);
 
 /* ---- negotiated-congestion batches --------------------------------------------
@@ -286,6 +287,23 @@ typedef struct {
 int mrf_route_negotiated_batch(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets,
                                const mrf_window *windows, int retry_full, const mrf_negotiation *neg,
                                mrf_paths *out, mrf_error *err);
+
+/* ---- legalization searches ------------------------------------------------------
+ * route_legal_once (negotiated.rs:2885-3052) for a batch of nets against one
+ * committed snapshot, as the staged legalization's parallel phase runs them
+ * (negotiated.rs:3640-3700): each net searches its window (full-board retry
+ * when retry_full) with copper owned by another group blocked even at its
+ * endpoints, and a foreign or mixed (-2) clearance halo blocked except at its
+ * endpoints and on its group's own copper (negotiated.rs:2937-2970); steps
+ * cost E*w as in the alone search.  owner / halo: w*h*layers group ids (-1
+ * free), either may be NULL; values must fit 32 bits.  The via guards are not
+ * modelled: this is route_legal_once with an unoccupied ViaGuard, clearance 0
+ * and no per-net via-forbidden XYs.  The board must have no additive or halo-
+ * owner pricing plane (mrf_board_set_additive(board, NULL, 0) drops one). */
+int mrf_route_legal_batch(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets,
+                          const mrf_window *windows, int retry_full, const int64_t *owner,
+                          const int64_t *halo, size_t n_cells, const int64_t *own_groups,
+                          mrf_paths *out, mrf_error *err);
 
 /* ---- autotuner -----------------------------------------------------------------
  * Picks the tile shape, workgroup size, in-LDS round cap (sweeps per tile

@@ -2,6 +2,7 @@
 // adapter's window -> full-board retry, chunking by the label budget, and the
 // fail-closed host re-validation of every returned path.
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,9 @@ struct mrf_board {
   std::vector<uint8_t> va;
   std::vector<int32_t> halo;
   uint32_t halo_pen = 0;
+  // Legal-search planes of the running mrf_route_legal_batch call (empty
+  // otherwise).
+  std::vector<int32_t> owner, lhalo;
 };
 
 namespace {
@@ -70,6 +74,7 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
     if (prices) {
       plans[k].ovr_cells = (*prices)[idx[k]].ovr_cells;
       plans[k].ovr_add = (*prices)[idx[k]].ovr_add;
+      plans[k].own_group = (*prices)[idx[k]].own_group;
     }
   }
   size_t budget_words = std::max<size_t>(c.label_budget / 8, 1);
@@ -98,6 +103,7 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
       q.own_id = net_ids ? net_ids[idx[k]] : -1;
       q.ovr_cells = plans[k].ovr_cells;
       q.ovr_add = plans[k].ovr_add;
+      q.own_group = plans[k].own_group;
       if (!lee) {
         q.forced = {nt.src, nt.dst};
         q.flags = mrf::F_FORCE;
@@ -632,6 +638,8 @@ static HostGrid board_host(const mrf_board* bd) {
   g.add = bd->add.empty() ? nullptr : bd->add.data();
   g.halo = bd->halo.empty() ? nullptr : bd->halo.data();
   g.halo_pen = bd->halo_pen;
+  g.owner = bd->owner.empty() ? nullptr : bd->owner.data();
+  g.lhalo = bd->lhalo.empty() ? nullptr : bd->lhalo.data();
   return g;
 }
 
@@ -693,9 +701,15 @@ int mrf_board_set_additive(mrf_board* bd, const uint32_t* additive, size_t n, mr
   if (!bd || !bd->ctx) return MRF_BACKEND_UNAVAILABLE;
   try {
     Lane lane(bd->ctx->c, false);
-    if (n != dims_len(bd->dims) || !additive) throw Error(MRF_MALFORMED_GRID, "additive plane length does not match dims");
-    mrf::board_set_additive(bd->ctx->c, bd->b, additive);
-    bd->add.assign(additive, additive + n);
+    if (!additive && n == 0) {
+      mrf::board_clear_additive(bd->b);
+      bd->add.clear();
+    } else {
+      if (n != dims_len(bd->dims) || !additive)
+        throw Error(MRF_MALFORMED_GRID, "additive plane length does not match dims");
+      mrf::board_set_additive(bd->ctx->c, bd->b, additive);
+      bd->add.assign(additive, additive + n);
+    }
     if (err) set_err(err, MRF_OK, "ok");
     return MRF_OK;
   } catch (const Error& e) {
@@ -774,6 +788,59 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
     mrf::board_set_additive(ctx->c, bd->b, plane.data());
     bd->add = std::move(plane);
     solve_board(ctx, bd, nets, n_nets, windows, retry_full, &prices, out);
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    mrf_free_paths(out);
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    mrf_free_paths(out);
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+int mrf_route_legal_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets,
+                          const mrf_window* windows, int retry_full, const int64_t* owner, const int64_t* halo,
+                          size_t n_cells, const int64_t* own_groups, mrf_paths* out, mrf_error* err) {
+  if (out) memset(out, 0, sizeof(*out));
+  if (!ctx || !bd || !out || bd->ctx != ctx) return MRF_BACKEND_UNAVAILABLE;
+  try {
+    Lane lane(ctx->c);
+    const size_t n = dims_len(bd->dims);
+    if (!bd->add.empty() || !bd->halo.empty())
+      throw Error(MRF_BACKEND_UNAVAILABLE, "legal batches take boards without additive or halo-owner pricing planes");
+    if ((owner || halo) && n_cells != n) throw Error(MRF_MALFORMED_GRID, "owner / halo plane length does not match dims");
+    if (n_nets && !own_groups) throw Error(MRF_BACKEND_UNAVAILABLE, "null own groups");
+    auto narrow32 = [](int64_t v) {
+      if (v < INT32_MIN || v > INT32_MAX) throw Error(MRF_BACKEND_UNAVAILABLE, "group id outside the 32-bit range");
+      return (int32_t)v;
+    };
+    std::vector<int32_t> o32, h32;
+    if (owner) {
+      o32.resize(n);
+      for (size_t c = 0; c < n; ++c) o32[c] = narrow32(owner[c]);
+    }
+    if (halo) {
+      h32.resize(n);
+      for (size_t c = 0; c < n; ++c) h32[c] = narrow32(halo[c]);
+    }
+    std::vector<NetPlan> plans(n_nets);
+    for (size_t i = 0; i < n_nets; ++i) plans[i].own_group = narrow32(own_groups[i]);
+    mrf::board_set_legal(ctx->c, bd->b, owner ? o32.data() : nullptr, halo ? h32.data() : nullptr);
+    bd->owner = std::move(o32);
+    bd->lhalo = std::move(h32);
+    try {
+      solve_board(ctx, bd, nets, n_nets, windows, retry_full, &plans, out);
+    } catch (...) {
+      mrf::board_clear_legal(bd->b);
+      bd->owner.clear();
+      bd->lhalo.clear();
+      throw;
+    }
+    mrf::board_clear_legal(bd->b);
+    bd->owner.clear();
+    bd->lhalo.clear();
     if (err) set_err(err, MRF_OK, "ok");
     return MRF_OK;
   } catch (const Error& e) {

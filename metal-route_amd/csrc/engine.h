@@ -46,6 +46,10 @@ struct BoardView {
   const uint32_t* ve;       // via prices [layers-1] (+1 pad)
   uint32_t va;              // bit k: vias allowed between layers k and k+1
   uint32_t pad_;
+  // Legal searches: committed copper owner and clearance-halo planes (-1
+  // free, halo -2 mixed, else a group), or nullptr.
+  const int32_t* owner;
+  const int32_t* lhalo;
 };
 
 // One complete field: a window (all layers), its sources, overrides.
@@ -68,6 +72,8 @@ struct FieldDesc {
   // start indices [tx*ty+1], then the same entries ordered by tile: cells [n],
   // prices [n].
   uint32_t ovr_off, n_ovr;
+  int32_t own_group;  // legal searches: the net's group
+  uint32_t pad_;
 };
 
 struct TileShape {
@@ -91,6 +97,10 @@ struct DevBoard {
   std::vector<uint8_t> obst;         // host copy: cost == kMax per cell
   uint64_t max_step = 0;  // bound on any step price (obstacles priced at weight 1, as own pads)
   uint64_t max_ew = 0;    // largest edge price x largest enter weight (max_step without the additive part)
+  int32_t* owner = nullptr;  // legal-search planes (allocated on first use)
+  int32_t* lhalo = nullptr;
+  bool legal = false;        // the owner / halo planes apply to the current call
+  bool legal_owner = false, legal_halo = false;  // which of them the call passed
   size_t bytes = 0;
   BoardView view() const;
 };
@@ -109,6 +119,7 @@ struct FieldReq {
   // Per-net additive prices replacing the board plane's on these cells
   // (ascending, unique); cells outside the window are ignored.
   std::vector<uint32_t> ovr_cells, ovr_add;
+  int32_t own_group = -1;  // legal searches: the net's group
 };
 
 struct SolveStats {
@@ -234,6 +245,12 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
 void board_free(DevBoard& b);
 // Replace (or add) the board's additive price plane; host array of w*h*layers.
 void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add);
+// Upload the legal-search owner / halo planes (either may be null: none) and
+// make them apply to the board's solves until board_clear_legal.
+void board_set_legal(Ctx& c, DevBoard& b, const int32_t* owner, const int32_t* halo);
+void board_clear_legal(DevBoard& b);
+// Drop the board's additive price plane.
+void board_clear_additive(DevBoard& b);
 
 // Solve complete fields for `reqs` (FieldReq::board indexes `boards`) in one
 // launch.  Labels stay on the device in the workspace until the next solve.

@@ -115,6 +115,8 @@ BoardView DevBoard::view() const {
   v.ve = ve;
   v.va = 0;
   v.pad_ = 0;
+  v.owner = legal && legal_owner ? owner : nullptr;
+  v.lhalo = legal && legal_halo ? lhalo : nullptr;
   for (int i = 0; i < 16; ++i)
     if (via_allowed.size() > (size_t)i && via_allowed[i]) v.va |= 1u << i;
   return v;
@@ -276,6 +278,16 @@ __device__ uint8_t via_bits(const BoardView& b, const FieldDesc& F, const uint32
   return bits;
 }
 
+// Legal-search blocks of a cell with copper owner o and halo h for group grp
+// (route_legal_once, negotiated.rs:2946-2966): bit 0 foreign copper (blocks
+// even the endpoints), bit 1 a foreign or mixed halo on a cell that is not
+// the group's own copper (blocks all but the endpoints).
+__device__ __forceinline__ uint32_t legal_code(int32_t o, int32_t h, int32_t grp) {
+  const bool copper = o >= 0 && o != grp;
+  const bool halo = (h == -2 || (h >= 0 && h != grp)) && o != grp;
+  return (copper ? 1u : 0u) | (halo ? 2u : 0u);
+}
+
 // Complete per-field view of one cell (must match the tile loader exactly).
 struct CellInfo {
   uint8_t mask;
@@ -289,10 +301,14 @@ __device__ CellInfo cell_info(const BoardView& b, const FieldDesc& F, const uint
   bool trace = bm & M_TRACE, obst = bm & M_OBST;
   bool pad = F.n_pad && obst && sorted_has(pool + F.pad_off, F.n_pad, g);
   bool pass = !trace && (!obst || pad);
+  const uint32_t legal =
+      (b.owner || b.lhalo) ? legal_code(b.owner ? b.owner[g] : -1, b.lhalo ? b.lhalo[g] : -1, F.own_group) : 0u;
+  if (legal & 2u) pass = false;
   if (!pass && !trace && (F.flags & F_FORCE)) {
     for (uint32_t k = 0; k < F.n_forced; ++k)
       if (pool[F.forced_off + k] == g) pass = true;
   }
+  if (legal & 1u) pass = false;
   if (pass) m |= K_PASS;
   if (F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(b, F, pool, g));
   ci.mask = m;
@@ -696,8 +712,10 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   uint32_t wv[4], av[4];
   int32_t ov[4];
   uint8_t bv[4];
+  uint8_t lg[4];  // legal-search block codes
   bool inw[4];
   uint64_t lv[4];
+  const bool legal_planes = B.owner || B.lhalo;
   auto load_planes = [&](uint32_t q) {
     qvalid = q < nquad;
     const uint32_t qq = qvalid ? q : 0;
@@ -733,6 +751,17 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         av[k] = B.add ? gbl(B.add)[g] : 0u;
         ov[k] = B.halo ? gbl(B.halo)[g] : -1;
       }
+    }
+    if (legal_planes) {
+      const uint32_t g0 = qrowg + (inw[3] ? qwxb : 0u);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t g = inw[3] ? g0 + k : qrowg + min(qwxb + k, F.ww - 1);
+        lg[k] = (uint8_t)legal_code(B.owner ? gbl(B.owner)[g] : -1, B.lhalo ? gbl(B.lhalo)[g] : -1, F.own_group);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) lg[k] = 0;
     }
     if (touched) {
       // a quad is contiguous in the tile block whatever the window
@@ -821,7 +850,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   // The six step prices of entering cell (l, y, x) = global g with board mask
   // bm, cost weight w0, additive price a0 and halo owner o.
   auto cell_steps = [&](uint32_t l, uint32_t y, uint32_t x, uint32_t g, uint8_t bm, uint32_t w0, uint32_t a0,
-                        int32_t o) {
+                        int32_t o, uint32_t legal) {
     const bool trace = (bm & M_TRACE) != 0, obst = (bm & M_OBST) != 0;
     uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
     bool pass = !trace && !obst;
@@ -830,10 +859,12 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
       pass = true;
       wgt = 1u;
     }
+    if (legal & 2u) pass = false;  // foreign halo: only the endpoints stay open
     if (!pass && !trace && (F.flags & F_FORCE)) {
       for (uint32_t j = 0; j < F.n_forced; ++j)
         if (A.pool[F.forced_off + j] == g) pass = true;
     }
+    if (legal & 1u) pass = false;  // foreign copper: closed even at the endpoints
     if (pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
     uint64_t a64 = a0;
     if (o != -1 && o != F.own_id) a64 += B.halo_pen;
@@ -862,7 +893,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     for (int k = 0; k < 4; ++k) {
       const uint32_t x = qx0 + k;
       const uint8_t bm = inw[k] ? bv[k] : (uint8_t)M_TRACE;  // outside the window: closed
-      cell_steps(l, y, x, qrowg + min(qwxb + k, F.ww - 1), bm, wv[k], av[k], ov[k]);
+      cell_steps(l, y, x, qrowg + min(qwxb + k, F.ww - 1), bm, wv[k], av[k], ov[k], lg[k]);
       s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1] = inw[k] ? lab_in(lv[k]) : kUnreached;
     }
   }
@@ -879,7 +910,10 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         const uint32_t g = tc[j], a = tc[F.n_ovr + j];
         const uint32_t l = g / plane, r = g - l * plane, gy = r / B.w, gx = r - gy * B.w;
         cell_steps(l, gy - F.y0 - wy0, gx - F.x0 - wx0, g, gbl(B.bmask)[g], gbl(B.cost)[g], a,
-                   B.halo ? gbl(B.halo)[g] : -1);
+                   B.halo ? gbl(B.halo)[g] : -1,
+                   legal_planes ? legal_code(B.owner ? gbl(B.owner)[g] : -1, B.lhalo ? gbl(B.lhalo)[g] : -1,
+                                             F.own_group)
+                                : 0u);
       }
       __syncthreads();
     }
@@ -1431,7 +1465,7 @@ void ctx_free(Ctx& c) {
 }
 
 void board_free(DevBoard& b) {
-  void* ptrs[] = {b.cost, b.add, b.halo, b.bmask, b.raw, b.xe, b.ye, b.ve};
+  void* ptrs[] = {b.cost, b.add, b.halo, b.bmask, b.raw, b.xe, b.ye, b.ve, b.owner, b.lhalo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b = DevBoard();
@@ -1527,6 +1561,36 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   }
   b.halo_pen = halo_pen;
   HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+void board_set_legal(Ctx& c, DevBoard& b, const int32_t* owner, const int32_t* halo) {
+  HIP_OK(hipSetDevice(c.device));
+  const size_t n = (size_t)b.w * b.h * b.layers;
+  auto put = [&](int32_t*& dst, const int32_t* src) {
+    if (!src) return;
+    if (!dst) {
+      HIP_OK(hipMalloc((void**)&dst, n * sizeof(int32_t)));
+      b.bytes += n * 4;
+    }
+    HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  };
+  put(b.owner, owner);
+  put(b.lhalo, halo);
+  HIP_OK(hipStreamSynchronize(c.stream));
+  b.legal_owner = owner != nullptr;
+  b.legal_halo = halo != nullptr;
+  b.legal = true;
+}
+
+void board_clear_legal(DevBoard& b) { b.legal = false; }
+
+void board_clear_additive(DevBoard& b) {
+  if (b.add) {
+    (void)hipFree(b.add);
+    b.add = nullptr;
+    b.bytes -= (size_t)b.w * b.h * b.layers * 4;
+  }
+  b.max_step = std::min<uint64_t>(b.max_ew + (b.halo ? b.halo_pen : 0u), kMax - 1);
 }
 
 void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add) {
@@ -1710,6 +1774,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     put(q.via_pads, F.vpad_off, F.n_vpad);
     F.dst = q.dst;
     F.own_id = q.own_id;
+    F.own_group = q.own_group;
     F.flags = q.flags;
     F.board = q.board;
     uint32_t nt = F.tx * F.ty;
@@ -2200,6 +2265,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
     put(q.via_pads, F.vpad_off, F.n_vpad);
     F.dst = q.dst;
     F.own_id = q.own_id;
+    F.own_group = q.own_group;
     F.flags = q.flags;
     F.board = q.board;
     const uint32_t nt = F.tx * F.ty;

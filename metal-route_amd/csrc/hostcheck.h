@@ -31,9 +31,14 @@ struct HostGrid {
   const uint32_t* add = nullptr;
   const int32_t* halo = nullptr;
   uint32_t halo_pen = 0;
+  // legal searches: committed copper owners and clearance halo (or null)
+  const int32_t* owner = nullptr;
+  const int32_t* lhalo = nullptr;
   bool obst(uint32_t c) const { return cost[c] == kMax; }
   bool trace(uint32_t c) const { return board && (board[c] & MRF_BOARD_TRACE_NODE); }
   bool vianode(uint32_t c) const { return board && (board[c] & MRF_BOARD_VIA_NODE); }
+  // whether c is not the group's own copper (a foreign halo blocks it)
+  bool owner_free_of(uint32_t c, int32_t grp) const { return !(owner && owner[c] == grp); }
   bool viaforb(uint32_t c) const { return vf && vf[c]; }
   // mr-core is_board_planar_step_forbidden (lib.rs:466-496) for neighbours.
   bool planar_forbidden(uint32_t u, uint32_t v) const {
@@ -109,6 +114,7 @@ struct NetPlan {
   std::vector<uint32_t> pads, vpads;
   // Per-net additive prices replacing the board plane's (cells ascending).
   std::vector<uint32_t> ovr_cells, ovr_add;
+  int32_t own_group = -1;  // legal searches: the net's group
 };
 
 inline uint32_t step_price(uint32_t e, uint32_t w, uint64_t a) {
@@ -132,7 +138,13 @@ inline bool path_ok(const HostGrid& g, const mrf_net& n, const NetPlan& p, const
   uint64_t cost = 0;
   for (size_t i = 0; i < len; ++i) {
     uint32_t c = path[i];
-    if (c >= total || g.trace(c) || (g.obst(c) && !sorted_has(p.pads, c))) return false;
+    if (c >= total || g.trace(c)) return false;
+    const bool endpoint = c == n.src || c == n.dst;
+    if (g.owner && g.owner[c] >= 0 && g.owner[c] != p.own_group) return false;  // foreign copper
+    if (!endpoint && g.lhalo && g.owner_free_of(c, p.own_group) &&
+        (g.lhalo[c] == -2 || (g.lhalo[c] >= 0 && g.lhalo[c] != p.own_group)))
+      return false;  // foreign halo
+    if (g.obst(c) && !sorted_has(p.pads, c)) return false;
     uint32_t r = c % plane, x = r % g.d.w, y = r / g.d.w;
     if (x < w.x0 || x > w.x1 || y < w.y0 || y > w.y1) return false;
     if (i == 0) continue;

@@ -169,6 +169,9 @@ def lib():
                                        C.c_size_t, C.POINTER(WindowS), E]),
         "mrf_read_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, U32P, U32P, E]),
         "mrf_board_set_additive": (C.c_int, [C.c_void_p, U32P, C.c_size_t, E]),
+        "mrf_route_legal_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t, C.POINTER(WindowS),
+                                            C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_size_t,
+                                            C.POINTER(C.c_int64), C.POINTER(PathsS), E]),
         "mrf_route_negotiated_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t,
                                                  C.POINTER(WindowS), C.c_int, C.POINTER(NegotiationS),
                                                  C.POINTER(PathsS), E]),
@@ -530,10 +533,36 @@ class Board:
         return _take_arrays(self._solve(nets, windows, retry_full))
 
     def set_additive(self, additive) -> None:
-        """Replace the board's additive price plane (w*h*layers u32)."""
-        a = _u32(additive)
+        """Replace the board's additive price plane (w*h*layers u32); None drops it."""
         err = ErrorS()
+        if additive is None:
+            _check(lib().mrf_board_set_additive(self._h, None, 0, C.byref(err)), err)
+            return
+        a = _u32(additive)
         _check(lib().mrf_board_set_additive(self._h, _p32(a), a.size, C.byref(err)), err)
+
+    def route_legal(self, nets, owner, halo, own_groups, windows=None, retry_full=True, arrays=False):
+        """Legalization searches (route_legal_once, negotiated.rs:2885-3052)
+        against committed copper owners and clearance halo (w*h*layers group
+        ids, -1 free, halo -2 mixed; either may be None), one group per net."""
+        arr, keep = _nets(nets)
+        P64 = C.POINTER(C.c_int64)
+        o = np.ascontiguousarray(np.asarray(owner, dtype=np.int64)) if owner is not None else None
+        h = np.ascontiguousarray(np.asarray(halo, dtype=np.int64)) if halo is not None else None
+        n = o.size if o is not None else (h.size if h is not None else 0)
+        if o is not None and h is not None and o.size != h.size:
+            raise MrfError(MALFORMED_GRID, "owner and halo planes differ in length")
+        gid = np.ascontiguousarray(np.asarray(list(own_groups), dtype=np.int64))
+        if gid.size != len(nets):
+            raise MrfError(BACKEND_UNAVAILABLE, "one group per net")
+        out = PathsS()
+        err = ErrorS()
+        _check(lib().mrf_route_legal_batch(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)),
+                                           1 if retry_full else 0, o.ctypes.data_as(P64) if o is not None else None,
+                                           h.ctypes.data_as(P64) if h is not None else None, n,
+                                           gid.ctypes.data_as(P64) if gid.size else None, C.byref(out),
+                                           C.byref(err)), err)
+        return _take_arrays(out) if arrays else _take_paths(out)
 
     def route_negotiated(self, nets, history, present, pfac, *, present_halo=None, fused=False, own_paths=None,
                          own_halos=None, windows=None, retry_full=True, arrays=False):

@@ -41,6 +41,11 @@
  *                                single-source fields.
  *   mrc_provider_path_is_valid   accepts every oracle path, rejects hand-built
  *                                illegal paths (negotiated.rs:4486-4517 cases).
+ *   blocked (legal owner / halo) equal to the alone search on the board with
+ *                                every blocked cell turned into a board
+ *                                trace node (endpoint exemptions applied), on
+ *                                random tiny boards; hand cases for the
+ *                                endpoint / own-copper / mixed-halo ordering.
  *   priced_negotiated            the two properties negotiated.rs:6560-6660
  *                                asserts: a net whose own copper and halo are
  *                                the whole snapshot routes as on an empty
@@ -484,14 +489,24 @@ static inline uint32_t enter_weight(const mrc_spec *s, uint32_t c) {
 
 /* blocked_fn.  `forced` cells (src/dst of the alone search) stay passable
  * unless outside the window or on a board trace-node (negotiated.rs:2942-2955). */
+/* halo_is_foreign (negotiated.rs:190-192): HALO_MIXED (-2) or another group. */
+static int halo_is_foreign(int64_t h, int64_t own) { return h == -2 || (h >= 0 && h != own); }
+
 static int blocked(const mrc_spec *s, uint32_t c, const uint32_t *forced, size_t n_forced) {
     const mrc_grid *g = s->grid;
     if (s->mode == MRC_MODE_LEE) /* lee.rs:61 */
         return mrc_is_board_forbidden(g, c) || (mrc_is_obstacle(g, c) && !is_pad(s, c));
     if (!in_window(s, c)) return 1;
     if (mrc_is_board_forbidden(g, c)) return 1;
+    /* route_legal_once: foreign copper blocks even the endpoints (:2946-2953) */
+    if (s->legal_owner && s->legal_owner[c] >= 0 && s->legal_owner[c] != s->own_group) return 1;
     for (size_t i = 0; i < n_forced; ++i)
         if (forced[i] == c) return 0;
+    /* ... the endpoints stay reachable, then a foreign halo blocks unless the
+     * cell is this group's own copper (:2954-2966) */
+    if (s->legal_halo && halo_is_foreign(s->legal_halo[c], s->own_group) &&
+        !(s->legal_owner && s->legal_owner[c] == s->own_group))
+        return 1;
     return mrc_is_obstacle(g, c) && !is_pad(s, c);
 }
 
@@ -952,6 +967,7 @@ static void spec_for_net(const batch_job *j, size_t i, mrc_spec *s) {
     s->own_halo = j->nets[i].own_halo;
     s->own_halo_count = j->nets[i].own_halo_count;
     s->n_own_halo = j->nets[i].n_own_halo;
+    if (j->tmpl->legal_owner || j->tmpl->legal_halo) s->own_group = j->nets[i].own_group;
     s->window = j->windows ? j->windows[i] : mrc_window_full(j->tmpl->grid->dims);
 }
 

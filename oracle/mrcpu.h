@@ -104,6 +104,12 @@ typedef struct {
     const uint32_t *own_halo;    /* sorted unique cells ... */
     const uint32_t *own_halo_count; /* ... and their multiplicities */
     size_t n_own_halo;
+    /* route_legal_once hard blocks (negotiated.rs:2937-2970): committed copper
+     * owners and clearance halo (-1 free, -2 mixed, else a group), either may
+     * be NULL; the searching net's group is own_group. */
+    const int64_t *legal_owner;
+    const int64_t *legal_halo;
+    int64_t own_group;
 } mrc_spec;
 
 /* ---- scalar helpers ---------------------------------------------------- */
@@ -180,6 +186,7 @@ typedef struct {
     const uint32_t *own_halo;
     const uint32_t *own_halo_count;
     size_t n_own_halo;
+    int64_t own_group; /* legal searches: the net's group */
 } mrc_net;
 
 /* For each net: search its window, and on failure the full board.  Paths are

@@ -79,6 +79,9 @@ class SpecS(C.Structure):
         ("own_halo", U32P),
         ("own_halo_count", U32P),
         ("n_own_halo", C.c_size_t),
+        ("legal_owner", C.POINTER(C.c_int64)),
+        ("legal_halo", C.POINTER(C.c_int64)),
+        ("own_group", C.c_int64),
     ]
 
 
@@ -97,6 +100,7 @@ class NetS(C.Structure):
         ("own_halo", U32P),
         ("own_halo_count", U32P),
         ("n_own_halo", C.c_size_t),
+        ("own_group", C.c_int64),
     ]
 
 
@@ -236,6 +240,7 @@ class Net:
     # ({cell: count}, or cells repeated by multiplicity)
     own_path: Optional[List[int]] = None
     own_halo: object = field(default_factory=list)
+    own_group: int = -1      # legal searches: the net's group
 
 
 @dataclass
@@ -264,7 +269,8 @@ class Spec:
     def __init__(self, grid: Grid, edges: Optional[Edges] = None, *, mode=MODE_ALONE,
                  pads=(), via_pads=(), window=None, additive=None, halo_owner=None,
                  halo_penalty=0, own_id=-1, use_heuristic=False, min_via_cost=0,
-                 negotiation: Optional[Negotiation] = None, own_path=None, own_halo=()):
+                 negotiation: Optional[Negotiation] = None, own_path=None, own_halo=(),
+                 legal_owner=None, legal_halo=None, own_group=-1):
         self.grid = grid
         self.edges = edges if edges is not None else Edges.uniform(grid.w, grid.h, grid.layers, 1, 0)
         self.pads = _u32(sorted(set(int(p) for p in pads)) + [0])
@@ -303,6 +309,13 @@ class Spec:
         self.s.own_halo = _p32(self._own_halo[0])
         self.s.own_halo_count = _p32(self._own_halo[1])
         self.s.n_own_halo = len(hc)
+        # route_legal_once hard blocks (copper owners, clearance halo)
+        self._legal = [np.ascontiguousarray(np.asarray(a, dtype=np.int64)) if a is not None else None
+                       for a in (legal_owner, legal_halo)]
+        P64 = C.POINTER(C.c_int64)
+        self.s.legal_owner = self._legal[0].ctypes.data_as(P64) if self._legal[0] is not None else None
+        self.s.legal_halo = self._legal[1].ctypes.data_as(P64) if self._legal[1] is not None else None
+        self.s.own_group = int(own_group)
 
 
 # ---- thin wrappers ---------------------------------------------------------
@@ -419,7 +432,8 @@ def _nets_array(nets: Sequence[Net]):
         hcnt = _u32(list(hn) + [0])
         keep += [p, vp, op, hcell, hcnt]
         arr[i] = NetS(n.src, n.dst, _p32(p), len(set(n.pads)), _p32(vp), len(set(n.via_pads)), n.own_id,
-                      1 if n.own_path is not None else 0, _p32(op), len(own), _p32(hcell), _p32(hcnt), len(hc))
+                      1 if n.own_path is not None else 0, _p32(op), len(own), _p32(hcell), _p32(hcnt), len(hc),
+                      int(n.own_group))
     return arr, keep
 
 
