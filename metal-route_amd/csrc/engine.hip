@@ -1281,7 +1281,6 @@ __global__ void walk_paths(WalkArgs W) {
     uint32_t hops = (uint32_t)lab;
     if (lane == 0) out[hops] = cur;
     if (hops == 0) return;
-    CellInfo ci = cell_info(B, F, W.pool, cur);
     uint32_t l = cur / B.plane, r = cur - l * B.plane, y = r / B.w, x = r - y * B.w;
     bool ok = false;
     uint32_t p = 0, e = 0;
@@ -1295,12 +1294,12 @@ __global__ void walk_paths(WalkArgs W) {
       case 5: ok = l + 1 < B.layers; p = cur + B.plane; e = ok ? B.ve[l] : 0; bit = K_UP; break;
       default: ok = false;
     }
-    ok = ok && (ci.mask & bit);
-    if (ok) {
-      uint64_t lp = field_label(W.R, B, F, p);
-      ok = (uint32_t)(lp >> 32) != kMax && extend(lp, step_price(e, ci.w, ci.a)) == lab;
-      if (ok) lab = lp;  // only meaningful in the chosen lane
-    }
+    // the neighbour's label (any on-board neighbour) and this cell's view are
+    // loaded together: one memory round trip per step of the walk
+    const uint64_t lp = ok ? field_label(W.R, B, F, p) : kInf;
+    CellInfo ci = cell_info(B, F, W.pool, cur);
+    ok = ok && (ci.mask & bit) && (uint32_t)(lp >> 32) != kMax && extend(lp, step_price(e, ci.w, ci.a)) == lab;
+    if (ok) lab = lp;  // only meaningful in the chosen lane
     uint64_t mask = __ballot(ok);
     if (mask == 0) {
       if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 16u);
