@@ -349,6 +349,11 @@ typedef struct {
     uint32_t max_layers;    /* largest layer count of any board (0 = 8) */
     uint32_t max_inflight;  /* requests on the device at once (0 = 256) */
     uint32_t tile_w, tile_h, threads; /* tile shape (0 = 16 x 16, 256 threads) */
+    /* Admission pacing: a queued request is admitted only while the running
+     * requests keep fewer tiles active than this (the device is not full),
+     * so a large request runs nearly alone and later ones fill its tail.
+     * 0 = six workgroups per CU; UINT32_MAX = admit whenever memory allows. */
+    uint32_t admit_below_tiles;
 } mrf_server_config;
 
 typedef struct {

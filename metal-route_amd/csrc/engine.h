@@ -359,6 +359,8 @@ void driver_remove_board(RoundDriver& d, int slot);
 uint64_t driver_group_words(const RoundDriver& d, const std::vector<FieldReq>& reqs);
 // Admit a group at the next round boundary; false when it does not fit now.
 bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& group);
+// Tiles the next round will visit: the active list plus admitted seeds.
+inline uint32_t driver_next_tiles(const RoundDriver& d) { return d.n_cur + (uint32_t)d.pending_seeds.size(); }
 // One relaxation round over every active item (new groups' seeds included).
 // Returns the number of items active for the next round.
 uint32_t driver_step(RoundDriver& d);

@@ -101,6 +101,7 @@ struct mrf_server {
   uint32_t next_board = 1;
   uint64_t next_req = 1;
   uint32_t max_inflight = 256;
+  uint32_t admit_below = 0;  // admission pacing: active tiles under which the device takes more work
   size_t label_bytes = 0;
   mrf_server_stats st{};
 };
@@ -203,12 +204,15 @@ void worker_loop(mrf_server* S) {
       if (S->stop) return;
     }
     try {
-      // Admission, in arrival order, while the budget and the slots allow.
+      // Admission, in arrival order, while the budget and the slots allow
+      // and the running requests leave the device room (active tiles of the
+      // next round, plus the seeds of requests admitted just now).
       for (;;) {
         ServeReq* r = nullptr;
         {
           std::lock_guard<std::mutex> lk(S->mu);
           if (S->queue.empty() || S->running.size() >= S->max_inflight) break;
+          if (!S->running.empty() && mrf::driver_next_tiles(S->d) >= S->admit_below) break;
           r = S->queue.front();
         }
         std::vector<mrf::FieldReq> reqs = field_reqs(*r);
@@ -332,6 +336,7 @@ int mrf_server_create(int device, const mrf_server_config* cfg, mrf_server** out
     if (lb > free_b / 10 * 9) throw Error(MRF_BACKEND_UNAVAILABLE, "label budget exceeds free device memory");
     S->label_bytes = lb;
     S->max_inflight = k.max_inflight ? k.max_inflight : 256;
+    S->admit_below = k.admit_below_tiles ? k.admit_below_tiles : 6u * (uint32_t)S->c.cus;
     mrf::driver_init(S->d, S->c, lb, k.max_boards ? k.max_boards : 64, k.max_layers ? k.max_layers : 8,
                      S->max_inflight);
     S->st.label_bytes = lb;
