@@ -300,7 +300,7 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import numpy as np

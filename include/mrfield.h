@@ -164,7 +164,7 @@ typedef struct {
  *   PLAIN       one launch per relaxation round; the host reads the number of
  *               activated tiles after each round.
  *   PERSISTENT  resident grid with a device barrier between rounds: disabled in
- *               this build, mrf_solve* return MRF_BACKEND_UNAVAILABLE for it.
+ *               this library, mrf_solve* return MRF_BACKEND_UNAVAILABLE for it.
  *   GRAPH       (default) grid-stride launches over a device work list, rounds
  *               captured in a hipGraph and replayed; one host read per replay. */
 enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1, MRF_MODE_GRAPH = 2 };

@@ -155,7 +155,7 @@ struct SolveStats {
 
 // MODE_PLAIN: one launch per round, host reads the activation count each round.
 // MODE_PERSISTENT: resident-grid kernel with a device grid barrier between
-//   rounds.  Disabled: it did not return on MI355X; requesting
+//   rounds.  Disabled: it did not return on MI355X (a device grid barrier that never completed); requesting
 //   it raises an error.
 // MODE_GRAPH: grid-stride kernel over a device work list per round, rounds
 //   separated by kernel boundaries, captured in a hipGraph and replayed; the

@@ -1906,7 +1906,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       local.replays = replays;
       local.launches = rounds;
     } else {
-      throw Error(1, "the persistent grid-barrier form is disabled on this build (it did not return on MI355X); "
+      throw Error(1, "the persistent grid-barrier form is disabled (it did not return on MI355X); "
                      "use the plain or graph form");
     }
     local.rounds = rounds;

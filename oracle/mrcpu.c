@@ -3,8 +3,8 @@
  *
  * Plain-C restatement of the reference CPU router's search core.  The product
  * engine (metal-route_amd/) never links or calls this file; tests/, smoke()
- * and bench.py's cpu_baseline leg load it as the checker and as the CPU
- * baseline.
+ * and bench.py's cpu_baseline leg load it as the reference the engine is
+ * compared against and as the CPU baseline.
  *
  * Sources restated (paths relative to the reference root):
  *   crates/mr-core/src/lib.rs     Dims mapping, Grid predicates   :90-164, :434-512

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * router's CPU search (crate mr-cpu) and of the mr-core grid predicates.  It is
- * the checker the HIP engine is compared against, and the CPU baseline that
+ * the reference the HIP engine is compared against, and the CPU baseline that
  * bench.py times beside the GPU.  Nothing in the product library links, loads
  * or calls it; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it.

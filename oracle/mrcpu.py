@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY: the product engine never imports this module.  Only
 tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg use it,
-as the checker and as the timed CPU baseline.
+as the reference results and as the timed CPU baseline.
 
 The C library restates the reference CPU router (crate mr-cpu); see the header
 of oracle/mrcpu.c for the file:line of every function and what pins it.

@@ -237,7 +237,7 @@ def test_astar_equals_brute_force_on_tiny_grids(seed):
 
 
 def test_brute_force_plants_catch_mistakes():
-    """The brute-force checker itself distinguishes the tie-break variants."""
+    """The brute-force enumeration itself distinguishes the tie-break variants."""
     g = mrcpu.Grid.filled(2, 2, 1, 1)
     spec = mrcpu.Spec(g, mrcpu.Edges([16], [16], []))
     want = brute_force(spec, g, 0, 3, lambda c: False)
