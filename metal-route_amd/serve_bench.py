@@ -111,7 +111,7 @@ def same_paths(a, b) -> bool:
 
 
 def run(seed: int = 1, n_boards: int = 8, n_req: int = 96, loads=(0.5, 1.0, 2.0, 4.0),
-        tiles=(16, 16, 256, 32), label_gib: int = 96):
+        tiles=(16, 16, 256, 32), label_gib: int = 96, admit_below_tiles: int = 0):
     boards, reqs = make_workload(seed, n_boards, n_req)
     total_nets = sum(len(n) for _, n in reqs)
     eng = mrfield.Engine(0)
@@ -121,6 +121,7 @@ def run(seed: int = 1, n_boards: int = 8, n_req: int = 96, loads=(0.5, 1.0, 2.0,
     eng.close()
     mu = 1.0 / float(np.mean(service))  # plain form's capacity, requests/s
     srv = mrfield.Server(0, label_bytes=label_gib << 30, max_layers=8, tile_w=tiles[0], tile_h=tiles[1],
+                         admit_below_tiles=admit_below_tiles,
                          threads=tiles[2])
     ids = [srv.add_board(s.grid, s.edges, s.pricing()) for s in boards]
     # warm-up: one request through the server
