@@ -2318,8 +2318,12 @@ uint32_t driver_step(RoundDriver& d) {
     d.staging.clear();
     return 0;
   }
+  // With one live group its fields are active exactly when the next list is
+  // not empty: the per-group marking and its readback are skipped.
+  uint32_t live = 0;
+  for (uint32_t g = 0; g < d.max_groups; ++g) live += d.groups[g].live ? 1u : 0u;
   launch_fill32(s, c.ws.counters, 0u, kCtrWords);
-  launch_fill32(s, d.group_active, 0u, d.max_groups);
+  if (live > 1) launch_fill32(s, d.group_active, 0u, d.max_groups);
   RelaxArgs A;
   memset(&A, 0, sizeof(A));
   A.boards = d.boards;
@@ -2355,12 +2359,13 @@ uint32_t driver_step(RoundDriver& d) {
   HIP_OK(hipEventRecord(c.ws.ek0, s));
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
   HIP_OK(hipEventRecord(c.ws.ek1, s));
-  hipLaunchKernelGGL(mark_groups, dim3(1024), dim3(256), 0, s, list_next, c.ws.counters, d.item_group,
-                     d.group_active);
+  if (live > 1) hipLaunchKernelGGL(mark_groups, dim3(1024), dim3(256), 0, s, list_next, c.ws.counters, d.item_group,
+                                   d.group_active);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(d.pinned, c.ws.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_OK(hipMemcpyAsync(d.pinned + kCtrWords, d.group_active, d.max_groups * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, s));
+  if (live > 1)
+    HIP_OK(hipMemcpyAsync(d.pinned + kCtrWords, d.group_active, d.max_groups * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   d.staging.clear();
   if (d.pinned[C_ERROR * kCtrStride])
@@ -2374,7 +2379,7 @@ uint32_t driver_step(RoundDriver& d) {
   d.n_cur = d.pinned[C_NEXTN * kCtrStride];
   d.cur = 1 - d.cur;
   for (uint32_t g = 0; g < d.max_groups; ++g)
-    if (d.groups[g].live) d.groups[g].active = d.pinned[kCtrWords + g] != 0;
+    if (d.groups[g].live) d.groups[g].active = live > 1 ? d.pinned[kCtrWords + g] != 0 : d.n_cur != 0;
   return d.n_cur;
 }
 

@@ -9,8 +9,9 @@
 // together with the fields of every other admitted request on any resident
 // board, and leave as soon as their own fields have converged
 // (driver_finished).  Each request's paths are re-validated on the host like
-// the provider's (path_ok), so a result is exactly what mrf_solve returns for
-// the same nets alone.
+// the provider's (path_ok), on a checker thread beside the worker so that the
+// device keeps relaxing meanwhile; a result is exactly what mrf_solve returns
+// for the same nets alone.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -81,6 +82,20 @@ struct ServeReq {
   Clock::time_point t_submit, t_admit, t_done;
   bool admitted_once = false;
   uint32_t rounds = 0, peers_max = 0;
+  uint32_t checks = 0;      // passes handed to the checker and not yet checked
+  bool check_failed = false;
+};
+
+// One finished pass of a request: its device paths, read back before the
+// pass's device memory is released, waiting for host re-validation.
+struct PassCheck {
+  ServeReq* r = nullptr;
+  std::vector<size_t> idx;
+  std::vector<mrf_window> wins;
+  std::vector<uint8_t> routed;
+  std::vector<uint32_t> cost, cells;
+  std::vector<uint64_t> off;
+  bool final = false;  // the request completes when this pass is checked
 };
 
 }  // namespace
@@ -88,9 +103,10 @@ struct ServeReq {
 struct mrf_server {
   mrf::Ctx c;
   mrf::RoundDriver d;
-  std::thread worker;
+  std::thread worker, checker;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_check;
+  std::deque<PassCheck*> checks;
   bool stop = false;
   bool broken = false;
   std::string broken_msg;
@@ -137,37 +153,72 @@ std::vector<mrf::FieldReq> field_reqs(const ServeReq& r) {
   return out;
 }
 
-// Paths of the request's current pass, re-validated; sets up the retry pass
-// when one is needed.  Returns true when the request is complete.
-bool finish_pass(mrf_server* S, ServeReq* r) {
-  std::vector<uint8_t> routed;
-  std::vector<uint32_t> cost, cells;
-  std::vector<uint64_t> off;
-  mrf::driver_paths(S->d, r->group, routed, cost, off, cells);
-  const HostGrid g = r->board->grid();
+// Reads back the paths of the request's current pass (the group's device
+// memory is still held) and sets up the retry pass when one is needed: the
+// nets the windowed pass left unrouted, searched again on the full board.
+PassCheck* collect_pass(mrf_server* S, ServeReq* r) {
+  PassCheck* pc = new PassCheck();
+  pc->r = r;
+  pc->idx = r->idx;
+  pc->wins = r->pass_wins;
+  mrf::driver_paths(S->d, r->group, pc->routed, pc->cost, pc->off, pc->cells);
   std::vector<size_t> unrouted;
-  for (size_t k = 0; k < r->idx.size(); ++k) {
-    const size_t i = r->idx[k];
-    if (!routed[k]) {
-      unrouted.push_back(i);
-      continue;
-    }
-    const uint32_t* p = cells.data() + off[k];
-    const size_t len = off[k + 1] - off[k];
-    const int32_t own = r->ids.empty() ? -1 : r->ids[i];
-    if (!path_ok(g, r->nets[i], r->plans[i], r->pass_wins[k], own, p, len, cost[k], false))
-      throw Error(MRF_BACKEND_UNAVAILABLE, "device path failed host re-validation");
-    r->result.routed[i] = 1;
-    r->result.cost[i] = cost[k];
-    r->result.paths[i].assign(p, p + len);
-  }
+  for (size_t k = 0; k < r->idx.size(); ++k)
+    if (!pc->routed[k]) unrouted.push_back(r->idx[k]);
+  pc->final = true;
   if (!r->retry_pass && r->retry_full && !unrouted.empty()) {
     const mrf_window full{0, 0, r->board->dims.w - 1, r->board->dims.h - 1};
     r->idx = unrouted;
     r->pass_wins.assign(unrouted.size(), full);
     r->retry_pass = true;
-    return false;
+    pc->final = false;
   }
+  return pc;
+}
+
+// body(begin, end) over [0, n) split across up to 8 host threads.
+template <typename F>
+void parallel_for(size_t n, size_t min_chunk, F&& body) {
+  const size_t nt =
+      std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), (n + min_chunk - 1) / min_chunk});
+  if (nt <= 1) {
+    body(size_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const size_t per = (n + nt - 1) / nt;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t a = t * per, b = std::min(n, a + per);
+    if (a < b) pool.emplace_back([&body, a, b] { body(a, b); });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// Host re-validation of one pass (path_ok, as mrf_solve does), on up to 8
+// threads; stores the checked paths in the request's result.  Returns false
+// when any path fails.
+bool check_pass(PassCheck& pc) {
+  ServeReq* r = pc.r;
+  const HostGrid g = r->board->grid();
+  std::vector<uint8_t> bad(pc.idx.size(), 0);
+  parallel_for(pc.idx.size(), 32, [&](size_t a, size_t b) {
+    for (size_t k = a; k < b; ++k) {
+      if (!pc.routed[k]) continue;
+      const size_t i = pc.idx[k];
+      const uint32_t* p = pc.cells.data() + pc.off[k];
+      const size_t len = pc.off[k + 1] - pc.off[k];
+      const int32_t own = r->ids.empty() ? -1 : r->ids[i];
+      if (!path_ok(g, r->nets[i], r->plans[i], pc.wins[k], own, p, len, pc.cost[k], false)) {
+        bad[k] = 1;
+        continue;
+      }
+      r->result.routed[i] = 1;
+      r->result.cost[i] = pc.cost[k];
+      r->result.paths[i].assign(p, p + len);
+    }
+  });
+  for (uint8_t v : bad)
+    if (v) return false;
   return true;
 }
 
@@ -193,6 +244,51 @@ void fail_all(mrf_server* S, const std::string& msg) {
   for (ServeReq* r : S->queue) complete(S, r, MRF_BACKEND_UNAVAILABLE, msg);
   S->queue.clear();
   S->cv_done.notify_all();
+}
+
+// Re-validates finished passes off the worker thread, so the device keeps
+// relaxing the other requests meanwhile, and completes each request when its
+// final pass is checked.  A request that failed elsewhere (state 2) is only
+// released: the waiter holds it until its passes here are done.
+void checker_loop(mrf_server* S) {
+  for (;;) {
+    PassCheck* pc = nullptr;
+    bool skip = false;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_check.wait(lk, [&] { return S->stop || !S->checks.empty(); });
+      if (S->stop) return;
+      pc = S->checks.front();
+      S->checks.pop_front();
+      skip = pc->r->state == 2;
+    }
+    ServeReq* r = pc->r;
+    bool ok = true;
+    std::string why;
+    if (!skip) {
+      try {
+        ok = check_pass(*pc);
+        if (!ok) why = "device path failed host re-validation";
+      } catch (const std::exception& e) {
+        ok = false;
+        why = e.what();
+      }
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!ok && !r->check_failed) {
+      r->check_failed = true;
+      r->msg = why;
+    }
+    if (pc->final && r->state != 2) {
+      if (r->check_failed)
+        complete(S, r, MRF_BACKEND_UNAVAILABLE, r->msg);
+      else
+        complete(S, r, MRF_OK, "ok");
+    }
+    r->checks--;
+    delete pc;
+    S->cv_done.notify_all();
+  }
 }
 
 void worker_loop(mrf_server* S) {
@@ -275,11 +371,11 @@ void worker_loop(mrf_server* S) {
           mrf::driver_release(S->d, g);
           continue;
         }
-        bool done = true;
+        PassCheck* pc = nullptr;
         int code = MRF_OK;
-        std::string msg = "ok";
+        std::string msg;
         try {
-          done = finish_pass(S, r);
+          pc = collect_pass(S, r);
         } catch (const Error& e) {
           code = e.code();
           msg = e.what();
@@ -288,14 +384,18 @@ void worker_loop(mrf_server* S) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->running.erase(std::find(S->running.begin(), S->running.end(), r));
         r->group = -1;
-        if (code != MRF_OK) {
+        if (!pc) {
           complete(S, r, code, msg);
-        } else if (done) {
-          complete(S, r, MRF_OK, msg);
-        } else {
+          S->cv_done.notify_all();
+          continue;
+        }
+        if (!pc->final) {
           r->state = 0;
           S->queue.push_front(r);  // the retry pass goes ahead of later arrivals
         }
+        r->checks++;
+        S->checks.push_back(pc);
+        S->cv_check.notify_one();
         S->cv_done.notify_all();
       }
     } catch (const std::exception& e) {
@@ -306,6 +406,18 @@ void worker_loop(mrf_server* S) {
       return;
     }
   }
+}
+
+// Stops and joins the worker and checker threads (either may not have started).
+void stop_threads(mrf_server* S) {
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  S->cv_check.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  if (S->checker.joinable()) S->checker.join();
 }
 
 }  // namespace
@@ -341,13 +453,16 @@ int mrf_server_create(int device, const mrf_server_config* cfg, mrf_server** out
                      S->max_inflight);
     S->st.label_bytes = lb;
     S->worker = std::thread(worker_loop, S);
+    S->checker = std::thread(checker_loop, S);
   } catch (const Error& e) {
+    stop_threads(S);
     mrf::driver_free(S->d);
     mrf::ctx_free(S->c);
     delete S;
     set_err(err, e.code(), e.what());
     return e.code();
   } catch (const std::exception& e) {
+    stop_threads(S);
     mrf::driver_free(S->d);
     mrf::ctx_free(S->c);
     delete S;
@@ -361,12 +476,8 @@ int mrf_server_create(int device, const mrf_server_config* cfg, mrf_server** out
 
 void mrf_server_destroy(mrf_server* S) {
   if (!S) return;
-  {
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->stop = true;
-  }
-  S->cv_work.notify_all();
-  if (S->worker.joinable()) S->worker.join();
+  stop_threads(S);
+  for (PassCheck* pc : S->checks) delete pc;
   for (auto& kv : S->reqs) delete kv.second;
   for (auto& kv : S->boards) {
     mrf::board_free(kv.second->dev);
@@ -536,7 +647,8 @@ int mrf_server_wait(mrf_server* S, uint64_t request_id, int64_t timeout_ms, mrf_
     return MRF_BACKEND_UNAVAILABLE;
   }
   ServeReq* r = it->second;
-  auto ready = [&] { return r->state == 2; };
+  // done, and no pass of it still with the checker
+  auto ready = [&] { return r->state == 2 && r->checks == 0; };
   if (timeout_ms < 0) {
     S->cv_done.wait(lk, ready);
   } else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
