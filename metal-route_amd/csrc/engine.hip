@@ -1266,51 +1266,131 @@ struct WalkArgs {
   uint32_t n;
 };
 
-// One wavefront per field: lanes 0..5 test the six candidate predecessors in
-// ascending CellIdx order (lower layer, up, left, right, down, upper layer).
+// Cells within Manhattan distance 3 of a walk's centre: 63 offsets, listed
+// in ascending (layer, y, x) order, one per lane (lane 63 idle).  On a board
+// every such offset order is the order of the cells' CellIdx, so the lowest
+// lane that passes a test is the lowest-CellIdx cell that passes it.
+constexpr int kWalkRadius = 3;
+constexpr int kWalkCells = 63;
+
+struct WalkOffsets {
+  int8_t d[64][3];  // (layer, y, x) offset of lane k
+};
+
+constexpr WalkOffsets make_walk_offsets() {
+  WalkOffsets t{};
+  int k = 0;
+  for (int a = -kWalkRadius; a <= kWalkRadius; ++a)
+    for (int b = -kWalkRadius; b <= kWalkRadius; ++b)
+      for (int c = -kWalkRadius; c <= kWalkRadius; ++c) {
+        const int m = (a < 0 ? -a : a) + (b < 0 ? -b : b) + (c < 0 ? -c : c);
+        if (m > kWalkRadius) continue;
+        t.d[k][0] = (int8_t)a;
+        t.d[k][1] = (int8_t)b;
+        t.d[k][2] = (int8_t)c;
+        ++k;
+      }
+  return t;
+}
+
+constexpr WalkOffsets kWalkHost = make_walk_offsets();
+// the centre (0, 0, 0) sits in the middle of the symmetric list
+constexpr int kWalkCentre = kWalkCells / 2;
+static_assert(kWalkHost.d[kWalkCentre][0] == 0 && kWalkHost.d[kWalkCentre][1] == 0 &&
+                  kWalkHost.d[kWalkCentre][2] == 0 && kWalkHost.d[kWalkCells - 1][0] == kWalkRadius,
+              "walk offsets");
+__constant__ WalkOffsets kWalkOff = make_walk_offsets();
+
+// One wavefront per field.  Each memory round trip loads, for every cell of
+// the distance-3 neighbourhood of the current cell, its label, its cell view
+// and the edge prices towards +x, +y and the layer above; the walk then takes
+// up to three steps inside the neighbourhood from registers.  A step from
+// cell c tests c's six neighbours (all inside the neighbourhood while c is at
+// most two steps from the centre): neighbour p is c's predecessor when c may
+// be entered from p's side and label(p) extended by the price of entering c
+// equals label(c); the lowest-CellIdx such p is taken (mr-metal
+// lib.rs:308-396, gpu.rs:569-714).
 __global__ void walk_paths(WalkArgs W) {
-  uint32_t f = blockIdx.x;
-  uint32_t lane = threadIdx.x;
+  const uint32_t f = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
   if (f >= W.n || W.plen[f] == 0) return;
   const FieldDesc F = W.R.fields[f];
   const BoardView B = W.R.boards[F.board];
   uint32_t* out = W.cells + W.poff[f];
+  const bool in_set = lane < (uint32_t)kWalkCells;
+  const int dl = kWalkOff.d[lane][0], dy = kWalkOff.d[lane][1], dx = kWalkOff.d[lane][2];
   uint32_t cur = F.dst;
   uint64_t lab = field_label(W.R, B, F, cur);
-  for (uint32_t guard = 0; guard <= W.plen[f]; ++guard) {
-    uint32_t hops = (uint32_t)lab;
-    if (lane == 0) out[hops] = cur;
-    if (hops == 0) return;
-    uint32_t l = cur / B.plane, r = cur - l * B.plane, y = r / B.w, x = r - y * B.w;
-    bool ok = false;
-    uint32_t p = 0, e = 0;
-    uint8_t bit = 0;
-    switch (lane) {
-      case 0: ok = l > 0; p = cur - B.plane; e = ok ? B.ve[l - 1] : 0; bit = K_DN; break;
-      case 1: ok = y > 0; p = cur - B.w; e = ok ? B.ye[y - 1] : 0; bit = K_NY; break;
-      case 2: ok = x > 0; p = cur - 1; e = ok ? B.xe[x - 1] : 0; bit = K_NX; break;
-      case 3: ok = x + 1 < B.w; p = cur + 1; e = ok ? B.xe[x] : 0; bit = K_PX; break;
-      case 4: ok = y + 1 < B.h; p = cur + B.w; e = ok ? B.ye[y] : 0; bit = K_PY; break;
-      case 5: ok = l + 1 < B.layers; p = cur + B.plane; e = ok ? B.ve[l] : 0; bit = K_UP; break;
-      default: ok = false;
+  uint32_t steps = 0;
+  const uint32_t limit = W.plen[f];
+  for (;;) {
+    const uint32_t cl = cur / B.plane, rr = cur - cl * B.plane, cy = rr / B.w, cx = rr - cy * B.w;
+    const int pl = (int)cl + dl, py = (int)cy + dy, px = (int)cx + dx;
+    const bool on = in_set && pl >= 0 && py >= 0 && px >= 0 && pl < (int)B.layers && py < (int)B.h && px < (int)B.w;
+    const uint32_t g = on ? (uint32_t)(((uint32_t)pl * B.h + (uint32_t)py) * B.w + (uint32_t)px) : cur;
+    // every load below depends only on the centre: one round trip
+    const uint64_t plab = on ? field_label(W.R, B, F, g) : kInf;
+    CellInfo ci;
+    ci.mask = 0;
+    ci.w = 0;
+    ci.a = 0;
+    if (on) ci = cell_info(B, F, W.pool, g);
+    const uint32_t ex = on && px + 1 < (int)B.w ? B.xe[px] : 0u;
+    const uint32_t ey = on && py + 1 < (int)B.h ? B.ye[py] : 0u;
+    const uint32_t ev = on && pl + 1 < (int)B.layers ? B.ve[pl] : 0u;
+    int c = kWalkCentre;
+    for (int k = 0; k < kWalkRadius; ++k) {
+      const uint32_t hops = (uint32_t)lab;
+      const uint32_t cg = (uint32_t)__builtin_amdgcn_readlane((int)g, c);
+      if (lane == 0) out[hops] = cg;
+      if (hops == 0) return;
+      if (++steps > limit) {
+        if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 32u);
+        return;
+      }
+      const int cdl = __builtin_amdgcn_readlane(dl, c), cdy = __builtin_amdgcn_readlane(dy, c),
+                cdx = __builtin_amdgcn_readlane(dx, c);
+      const uint32_t cmask = (uint32_t)__builtin_amdgcn_readlane((int)ci.mask, c);
+      const uint32_t cw = (uint32_t)__builtin_amdgcn_readlane((int)ci.w, c);
+      const uint32_t ca = (uint32_t)__builtin_amdgcn_readlane((int)ci.a, c);
+      const uint32_t cex = (uint32_t)__builtin_amdgcn_readlane((int)ex, c);
+      const uint32_t cey = (uint32_t)__builtin_amdgcn_readlane((int)ey, c);
+      const uint32_t cev = (uint32_t)__builtin_amdgcn_readlane((int)ev, c);
+      const int ol = dl - cdl, oy = dy - cdy, ox = dx - cdx;
+      uint32_t bit = 0, e = 0;
+      if (oy == 0 && ox == 0 && ol == -1) {
+        bit = K_DN;
+        e = ev;  // edge between this layer and the one above: its own
+      } else if (oy == 0 && ox == 0 && ol == 1) {
+        bit = K_UP;
+        e = cev;
+      } else if (ol == 0 && ox == 0 && oy == -1) {
+        bit = K_NY;
+        e = ey;
+      } else if (ol == 0 && ox == 0 && oy == 1) {
+        bit = K_PY;
+        e = cey;
+      } else if (ol == 0 && oy == 0 && ox == -1) {
+        bit = K_NX;
+        e = ex;
+      } else if (ol == 0 && oy == 0 && ox == 1) {
+        bit = K_PX;
+        e = cex;
+      }
+      const bool ok = on && bit && (cmask & bit) && (uint32_t)(plab >> 32) != kMax &&
+                      extend(plab, step_price(e, cw, ca)) == lab;
+      const uint64_t mask = __ballot(ok);
+      if (mask == 0) {
+        if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 16u);
+        return;
+      }
+      c = __ffsll((long long)mask) - 1;
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)plab, c);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(plab >> 32), c);
+      lab = ((uint64_t)hi << 32) | lo;
     }
-    // the neighbour's label (any on-board neighbour) and this cell's view are
-    // loaded together: one memory round trip per step of the walk
-    const uint64_t lp = ok ? field_label(W.R, B, F, p) : kInf;
-    CellInfo ci = cell_info(B, F, W.pool, cur);
-    ok = ok && (ci.mask & bit) && (uint32_t)(lp >> 32) != kMax && extend(lp, step_price(e, ci.w, ci.a)) == lab;
-    if (ok) lab = lp;  // only meaningful in the chosen lane
-    uint64_t mask = __ballot(ok);
-    if (mask == 0) {
-      if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 16u);
-      return;
-    }
-    uint32_t win = __ffsll((long long)mask) - 1;
-    cur = __shfl(p, win);
-    uint32_t lo = __shfl((uint32_t)lab, win), hi = __shfl((uint32_t)(lab >> 32), win);
-    lab = ((uint64_t)hi << 32) | lo;
+    cur = (uint32_t)__builtin_amdgcn_readlane((int)g, c);
   }
-  if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 32u);
 }
 
 // Tile-major labels of one field -> window-major u32 dist (and hops).
