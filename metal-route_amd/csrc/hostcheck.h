@@ -130,34 +130,47 @@ inline bool path_ok(const HostGrid& g, const mrf_net& n, const NetPlan& p, const
   size_t total = dims_len(g.d);
   if (len == 0 || len > total || path[0] != n.src || path[len - 1] != n.dst) return false;
   uint32_t plane = g.d.w * g.d.h;
-  // simple path: no cell twice
-  thread_local std::vector<uint32_t> sorted;
-  sorted.assign(path, path + len);
-  std::sort(sorted.begin(), sorted.end());
-  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return false;
+  // simple path: no cell twice (open addressing over at least twice the
+  // path's length; the empty slot value is no cell, as total < 2^32)
+  thread_local std::vector<uint32_t> seen;
+  size_t cap = 16;
+  while (cap < 2 * len) cap <<= 1;
+  seen.assign(cap, UINT32_MAX);
   uint64_t cost = 0;
+  uint32_t al = 0, ar = 0, ax = 0, ay = 0;  // the previous cell's layer, in-plane index, x, y
   for (size_t i = 0; i < len; ++i) {
     uint32_t c = path[i];
     if (c >= total || g.trace(c)) return false;
+    for (size_t h = (c * 0x9E3779B1u) & (cap - 1);; h = (h + 1) & (cap - 1)) {
+      if (seen[h] == c) return false;
+      if (seen[h] == UINT32_MAX) {
+        seen[h] = c;
+        break;
+      }
+    }
     const bool endpoint = c == n.src || c == n.dst;
     if (g.owner && g.owner[c] >= 0 && g.owner[c] != p.own_group) return false;  // foreign copper
     if (!endpoint && g.lhalo && g.owner_free_of(c, p.own_group) &&
         (g.lhalo[c] == -2 || (g.lhalo[c] >= 0 && g.lhalo[c] != p.own_group)))
       return false;  // foreign halo
     if (g.obst(c) && !sorted_has(p.pads, c)) return false;
-    uint32_t r = c % plane, x = r % g.d.w, y = r / g.d.w;
+    const uint32_t cl = c / plane, r = c - cl * plane, y = r / g.d.w, x = r - y * g.d.w;
     if (x < w.x0 || x > w.x1 || y < w.y0 || y > w.y1) return false;
+    const uint32_t a = i ? path[i - 1] : 0u;
+    const uint32_t pl = al, pr = ar, px = ax, py = ay;
+    al = cl;
+    ar = r;
+    ax = x;
+    ay = y;
     if (i == 0) continue;
-    uint32_t a = path[i - 1];
-    uint32_t al = a / plane, ar = a % plane, ax = ar % g.d.w, ay = ar / g.d.w, cl = c / plane;
     uint32_t e;
-    if (al == cl) {
-      bool adj = (ax == x && (ay + 1 == y || y + 1 == ay)) || (ay == y && (ax + 1 == x || x + 1 == ax));
+    if (pl == cl) {
+      bool adj = (px == x && (py + 1 == y || y + 1 == py)) || (py == y && (px + 1 == x || x + 1 == px));
       if (!adj || g.planar_forbidden(a, c)) return false;
-      e = lee ? 1u : (ax != x ? g.xe[std::min(ax, x)] : g.ye[std::min(ay, y)]);
+      e = lee ? 1u : (px != x ? g.xe[std::min(px, x)] : g.ye[std::min(py, y)]);
     } else {
-      uint32_t lo = std::min(al, cl), hi = std::max(al, cl);
-      if (lee || ar != r || hi != lo + 1 || !g.va[lo] || g.vianode(a) || g.vianode(c)) return false;
+      uint32_t lo = std::min(pl, cl), hi = std::max(pl, cl);
+      if (lee || pr != r || hi != lo + 1 || !g.va[lo] || g.vianode(a) || g.vianode(c)) return false;
       if ((g.viaforb(a) && !sorted_has(p.vpads, a)) || (g.viaforb(c) && !sorted_has(p.vpads, c))) return false;
       e = g.ve[lo];
     }

@@ -349,6 +349,32 @@ def test_fail_closed_contracts(eng):
     assert eng.route_isolated_batch(g, [], [], e) == []
 
 
+@pytest.mark.parametrize("mode,message", [(1, "injected backend failure"), (2, "re-validation")])
+def test_host_revalidation_fails_closed(monkeypatch, mode, message):
+    """A device result the host cannot re-validate fails the whole call (the
+    provider's fail-closed contract, mr-metal lib.rs:902-974).  The library's
+    MRF_INJECT_FAULT hook, read when a context opens, either raises after the
+    device solve (1) or flips one returned cell on the host before
+    re-validation (2)."""
+    s = mrfield.synth_config(1, seed=3, scale_nets=8)
+    monkeypatch.setenv("MRF_INJECT_FAULT", str(mode))
+    e = mrfield.Engine(0)
+    monkeypatch.delenv("MRF_INJECT_FAULT")
+    try:
+        with pytest.raises(mrfield.MrfError) as x:
+            e.route_isolated_batch(s.grid, s.nets(), [(0, 0, s.grid.w - 1, s.grid.h - 1)] * len(s.nets()), s.edges)
+        assert x.value.code == mrfield.BACKEND_UNAVAILABLE and message in str(x.value)
+    finally:
+        e.close()
+    # a context opened without the hook routes the same nets
+    e = mrfield.Engine(0)
+    try:
+        got = e.route_isolated_batch(s.grid, s.nets(), [(0, 0, s.grid.w - 1, s.grid.h - 1)] * len(s.nets()), s.edges)
+        assert sum(r is not None for r in got) > 0
+    finally:
+        e.close()
+
+
 def test_autotune_keeps_results(eng):
     """The autotuner only changes time: paths after tuning equal the oracle's."""
     s = mrfield.synth_config(3, seed=2, scale_nets=48)
