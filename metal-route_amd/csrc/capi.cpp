@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <iterator>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -739,11 +740,17 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
     // history into the board's additive plane: what every net pays on cells
     // outside its own footprint.
     std::vector<uint32_t> plane(n);
+    std::mutex max_mu;
+    uint32_t plane_max = 0;
     parallel_for(n, 1 << 16, [&](size_t a, size_t b) {
+      uint32_t m = 0;
       for (size_t c = a; c < b; ++c) {
         const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
         plane[c] = clamp32(sat_add64(neg->history[c], shared));
+        m = std::max(m, plane[c]);
       }
+      std::lock_guard<std::mutex> lk(max_mu);
+      plane_max = std::max(plane_max, m);
     });
     // Own footprint of each net: priced_with_base's self-subtraction
     // (negotiated.rs:2777-2797) as additive overrides.
@@ -785,7 +792,7 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
     });
     for (const std::string& m : bad)
       if (!m.empty()) throw Error(MRF_BACKEND_UNAVAILABLE, m.c_str());
-    mrf::board_set_additive(ctx->c, bd->b, plane.data());
+    mrf::board_set_additive(ctx->c, bd->b, plane.data(), plane_max);
     bd->add = std::move(plane);
     solve_board(ctx, bd, nets, n_nets, windows, retry_full, &prices, out);
     if (err) set_err(err, MRF_OK, "ok");

@@ -244,7 +244,9 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
                   uint32_t halo_pen);
 void board_free(DevBoard& b);
 // Replace (or add) the board's additive price plane; host array of w*h*layers.
-void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add);
+// max_add: the plane's largest value when the caller has it (UINT64_MAX:
+// computed here).
+void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_add = UINT64_MAX);
 // Upload the legal-search owner / halo planes (either may be null: none) and
 // make them apply to the board's solves until board_clear_legal.
 void board_set_legal(Ctx& c, DevBoard& b, const int32_t* owner, const int32_t* halo);

@@ -1672,10 +1672,10 @@ void board_clear_additive(DevBoard& b) {
   b.max_step = std::min<uint64_t>(b.max_ew + (b.halo ? b.halo_pen : 0u), kMax - 1);
 }
 
-void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add) {
+void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_add) {
   HIP_OK(hipSetDevice(c.device));
   const size_t n = (size_t)b.w * b.h * b.layers;
-  uint64_t max_a = *std::max_element(add, add + n);
+  uint64_t max_a = max_add != UINT64_MAX ? max_add : *std::max_element(add, add + n);
   if (b.halo) max_a += b.halo_pen;
   if (!b.add) {
     b.add = dev_copy(add, n, c.stream);

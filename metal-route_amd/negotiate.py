@@ -74,13 +74,14 @@ def _concat(paths: List[np.ndarray]):
     return np.concatenate([p for p in paths if len(p)]), np.repeat(np.arange(len(paths)), lens), lens
 
 
-def _overuse(paths: List[np.ndarray], groups: np.ndarray, present: Optional[np.ndarray] = None):
+def _overuse(paths: List[np.ndarray], groups: np.ndarray, present: Optional[np.ndarray] = None, flat=None):
     """Cells occupied by two or more distinct groups, ascending.  `present`
     (the per-cell path count) narrows the candidates to cells held twice;
-    when every net is its own group those are exactly the over-used cells."""
+    when every net is its own group those are exactly the over-used cells.
+    flat: _concat(paths) when the caller has it."""
     if present is not None and len(np.unique(groups)) == len(groups):
         return np.flatnonzero(present >= 2)
-    cells, net, _ = _concat(paths)
+    cells, net, _ = flat if flat is not None else _concat(paths)
     if not len(cells):
         return np.zeros(0, dtype=np.int64)
     if present is not None:
@@ -92,9 +93,10 @@ def _overuse(paths: List[np.ndarray], groups: np.ndarray, present: Optional[np.n
     return np.unique(c[1:][c[1:] == c[:-1]])
 
 
-def _touches(paths: List[np.ndarray], mask: np.ndarray) -> np.ndarray:
-    """Per path: whether any of its cells is set in mask."""
-    cells, net, lens = _concat(paths)
+def _touches(paths: List[np.ndarray], mask: np.ndarray, flat=None) -> np.ndarray:
+    """Per path: whether any of its cells is set in mask.  flat: _concat(paths)
+    when the caller has it."""
+    cells, net, lens = flat if flat is not None else _concat(paths)
     hit = np.zeros(len(paths), dtype=bool)
     if len(cells):
         hit[np.unique(net[mask[cells]])] = True
@@ -118,6 +120,7 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
     incremental = n_nets > 8
     use_parallel = n_nets > PARALLEL_NEGOTIATION_THRESHOLD
     out = Negotiated([], history, present, 0, False)
+    flat = None  # _concat(paths) of the settled paths, once per iteration
 
     for it in range(max_iters):
         pfac = 1 + it
@@ -127,7 +130,7 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
             if it == 0:
                 dirty = list(range(n_nets))
             else:
-                touch = _touches(paths, prev_overused)
+                touch = _touches(paths, prev_overused, flat)
                 dirty = [i for i in range(n_nets) if not len(paths[i]) or touch[i]]
             fused = len(dirty) >= FUSED_JACOBI_MIN_DIRTY_NETS
             if dirty:
@@ -138,17 +141,15 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
                     own_paths, own_halos = [paths[i] for i in dirty], None
                 got = route(dirty, history, present, pfac, fused, own_paths, own_halos)
                 out.routed_searches += len(dirty)
-                # Merge (:1779-1823).  Every old path is counted in present
-                # and a path holds each cell once, so removing all old paths
-                # and adding all new ones equals the reference's in-order
-                # saturating updates.
-                for i in dirty:
-                    if len(paths[i]):
-                        present[paths[i]] -= 1
+                # Merge (:1779-1823).  Every path is counted in present and
+                # holds each cell once, so present after the reference's
+                # in-order saturating updates (remove the old path, add the
+                # new one, net by net) is the per-cell count of the settled
+                # paths: one bincount over all of them.
                 for i, p in zip(dirty, got):
                     paths[i] = np.asarray(p, dtype=np.int64) if p is not None else empty
-                    if len(paths[i]):
-                        present[paths[i]] += 1
+            flat = _concat(paths)
+            np.copyto(present, np.bincount(flat[0], minlength=n_cells)[:n_cells], casting="unsafe")
         else:
             for i in range(n_nets):
                 if incremental and it > 0 and len(paths[i]) and not bool(prev_overused[paths[i]].any()):
@@ -162,7 +163,8 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
                 if got is not None:
                     paths[i] = np.asarray(got, dtype=np.int64)
                     present[paths[i]] += 1
-        over = _overuse(paths, grp, present)
+            flat = _concat(paths)
+        over = _overuse(paths, grp, present, flat)
         any_overuse = len(over) > 0
         if any_overuse:
             history[over] = np.minimum(history[over].astype(np.int64) + SCALE, U32MAX).astype(np.uint32)
@@ -220,7 +222,8 @@ def board_router(board, nets, windows=None) -> RouteBatch:
         sub = [nets[i] for i in idx]
         wins = [windows[i] for i in idx] if windows is not None else None
         res = board.route_negotiated(sub, history, present, pfac, fused=fused, own_paths=own_paths,
-                                     own_halos=own_halos, windows=wins, retry_full=True)
-        return [None if r is None else r[0] for r in res]
+                                     own_halos=own_halos, windows=wins, retry_full=True, arrays=True)
+        cells, off = res.cells, res.offsets
+        return [cells[off[k]:off[k + 1]] if res.routed[k] else None for k in range(len(sub))]
 
     return route
