@@ -43,8 +43,10 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 #include <type_traits>
 
 #include "engine.h"
@@ -1754,13 +1756,17 @@ static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_
 
 // Lay out a field's additive overrides inside its window in the pool (see
 // FieldDesc::ovr_off): by cell for point lookups, and by tile for the loader.
-static void put_overrides(const FieldReq& q, const DevBoard& b, const TileShape& S, FieldDesc& F,
-                          std::vector<uint32_t>& pool) {
-  F.ovr_off = (uint32_t)pool.size();
-  F.n_ovr = 0;
+// Override table of one field, independent of where it lands in the pool:
+// the n overrides inside the window (cells, then prices, in request order),
+// the per-tile offsets (tx * ty + 1 words), then cells and prices grouped by
+// tile.
+static void build_overrides(const FieldReq& q, const DevBoard& b, const TileShape& S, uint32_t tx, uint32_t ty,
+                            std::vector<uint32_t>& seg, uint32_t& n_out) {
+  seg.clear();
+  n_out = 0;
   if (q.ovr_cells.empty()) return;
   if (q.ovr_add.size() != q.ovr_cells.size()) throw Error(1, "override cells and prices differ in length");
-  const uint32_t plane = b.w * b.h, nt = F.tx * F.ty;
+  const uint32_t plane = b.w * b.h, nt = tx * ty;
   std::vector<uint32_t> cells, adds, tile;
   for (size_t k = 0; k < q.ovr_cells.size(); ++k) {
     const uint32_t g = q.ovr_cells[k];
@@ -1770,22 +1776,66 @@ static void put_overrides(const FieldReq& q, const DevBoard& b, const TileShape&
     if (gx < q.x0 || gx > q.x1 || gy < q.y0 || gy > q.y1) continue;
     cells.push_back(g);
     adds.push_back(q.ovr_add[k]);
-    tile.push_back(((gy - q.y0) / S.th) * F.tx + (gx - q.x0) / S.tw);
+    tile.push_back(((gy - q.y0) / S.th) * tx + (gx - q.x0) / S.tw);
   }
   const uint32_t n = (uint32_t)cells.size();
-  F.n_ovr = n;
+  n_out = n;
   if (!n) return;
   std::vector<uint32_t> order(n);
   for (uint32_t k = 0; k < n; ++k) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return tile[a] < tile[c]; });
-  pool.insert(pool.end(), cells.begin(), cells.end());
-  pool.insert(pool.end(), adds.begin(), adds.end());
+  seg.reserve((size_t)4 * n + nt + 1);
+  seg.insert(seg.end(), cells.begin(), cells.end());
+  seg.insert(seg.end(), adds.begin(), adds.end());
   std::vector<uint32_t> tab(nt + 1, 0);
   for (uint32_t k = 0; k < n; ++k) tab[tile[k] + 1]++;
   for (uint32_t t = 0; t < nt; ++t) tab[t + 1] += tab[t];
-  pool.insert(pool.end(), tab.begin(), tab.end());
-  for (uint32_t k : order) pool.push_back(cells[k]);
-  for (uint32_t k : order) pool.push_back(adds[k]);
+  seg.insert(seg.end(), tab.begin(), tab.end());
+  for (uint32_t k : order) seg.push_back(cells[k]);
+  for (uint32_t k : order) seg.push_back(adds[k]);
+}
+
+// Override tables of every field of a batch, on up to 8 host threads when the
+// batch carries many overrides (negotiation batches: a few hundred cells per
+// net); the first error of any field is rethrown here.
+static void build_all_overrides(const std::vector<const DevBoard*>& boards, const std::vector<FieldReq>& reqs,
+                                const TileShape& S, std::vector<std::vector<uint32_t>>& segs,
+                                std::vector<uint32_t>& ns) {
+  segs.assign(reqs.size(), {});
+  ns.assign(reqs.size(), 0);
+  size_t total = 0;
+  for (const FieldReq& q : reqs) total += q.ovr_cells.size();
+  if (!total) return;
+  auto work = [&](size_t a, size_t e) {
+    for (size_t i = a; i < e; ++i) {
+      const FieldReq& q = reqs[i];
+      const uint32_t tx = (q.x1 - q.x0 + S.tw) / S.tw, ty = (q.y1 - q.y0 + S.th) / S.th;
+      build_overrides(q, *boards[q.board], S, tx, ty, segs[i], ns[i]);
+    }
+  };
+  const size_t nt = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), total / 16384 + 1,
+                                      reqs.size()});
+  if (nt <= 1) {
+    work(0, reqs.size());
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> errs(nt);
+  const size_t per = (reqs.size() + nt - 1) / nt;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t a = t * per, e = std::min(reqs.size(), a + per);
+    if (a >= e) break;
+    pool.emplace_back([&, a, e, t] {
+      try {
+        work(a, e);
+      } catch (...) {
+        errs[t] = std::current_exception();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
 }
 
 // Whether a field's step prices all fit the narrow step planes: its board's
@@ -1822,6 +1872,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   if (lds > 160 * 1024 - 64) throw Error(1, "tile shape exceeds 160 KiB of LDS");
 
+  std::vector<std::vector<uint32_t>> ovr_seg;
+  std::vector<uint32_t> ovr_n;
+  build_all_overrides(boards, reqs, S, ovr_seg, ovr_n);
   std::vector<FieldDesc>& layout = c.layout;
   layout.clear();
   std::vector<uint32_t> pool, item_field, seeds;
@@ -1857,7 +1910,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     F.flags = q.flags;
     F.board = q.board;
     uint32_t nt = F.tx * F.ty;
-    put_overrides(q, b, S, F, pool);
+    F.ovr_off = (uint32_t)pool.size();
+    F.n_ovr = ovr_n[i];
+    pool.insert(pool.end(), ovr_seg[i].begin(), ovr_seg[i].end());
     for (uint32_t k = 0; k < nt; ++k) item_field.push_back((uint32_t)i);
     // seed tiles: tiles holding a source inside the window
     std::vector<uint32_t> st_items;
