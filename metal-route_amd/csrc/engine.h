@@ -210,6 +210,11 @@ struct Workspace {
   BoardView* boards = nullptr;   // device board table of the current batch
   size_t boards_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ek0 = nullptr, ek1 = nullptr;
+  // Plain form: one (start, end) event pair per round of a solve, read after
+  // the last round; and the stamp of the last launch whose counts the kernel
+  // published to hostctr (never 0).
+  std::vector<hipEvent_t> round_ev;
+  uint32_t stamp = 0;
 };
 
 struct GraphCache {

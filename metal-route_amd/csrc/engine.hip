@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -335,9 +336,11 @@ struct RelaxArgs {
   // Plain form: the round's activation count goes to counter next_ctr;
   // counter clear_ctr (the previous round's, already read) is zeroed for the
   // next round; the last workgroup to finish publishes the counts to the
-  // host-mapped words host_out (next count, error flags).
+  // host-mapped words host_out (next count, error flags), then the launch's
+  // stamp in host_out[2] with release semantics.
   uint32_t next_ctr, clear_ctr;
   uint32_t* host_out;
+  uint32_t stamp;
   // Delta stepping (key_cur non-null): every activation carries a key, the
   // smallest search cost among the border labels that changed; a round
   // processes the items whose key is at most counters[C_THRESH] and carries
@@ -1137,6 +1140,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
         const uint32_t er = __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&A.host_out[0], nn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(&A.host_out[1], er, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&A.host_out[2], A.stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         __threadfence_system();
       }
     }
@@ -1540,6 +1544,7 @@ void ctx_free(Ctx& c) {
   if (w.ek0) (void)hipEventDestroy(w.ek0);
   if (w.ek1) (void)hipEventDestroy(w.ek1);
   if (w.ev2) (void)hipEventDestroy(w.ev2);
+  for (hipEvent_t e : w.round_ev) (void)hipEventDestroy(e);
   w = Workspace();
   if (c.stream) (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
@@ -1968,27 +1973,55 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     // Rounds of relax_round over the active-item list until no item is
     // activated.  Every round starts at a kernel boundary, so it reads
     // coherent labels; an empty next list certifies the fixpoint.
-    // Per round: one launch and one stream synchronisation.  The activation
-    // count alternates between two counters (the kernel clears the one the
-    // host has read), and the launch's last workgroup publishes the counts to
-    // host-mapped memory, so no fill or copy is queued between rounds.
+    // Per round: one launch, then the host waits for the launch's stamp in
+    // host-mapped memory.  The activation count alternates between two
+    // counters (the kernel clears the one the host has read), and the
+    // launch's last workgroup publishes the counts and then the stamp, so no
+    // fill or copy is queued between rounds and the next launch follows as
+    // soon as the stamp lands.  A stamp that has not landed within 20 ms
+    // hands over to a stream synchronisation, which reports a failed launch;
+    // after it the stamp must be there.  Per-round kernel times are read from
+    // an event pair per round once the last round is done.
     auto plain_rounds = [&](uint32_t& rounds, uint32_t n_active) {
       A.host_out = W.hostctr_dev;
+      size_t pairs = 0;  // event pairs recorded and not yet read
+      auto harvest = [&] {
+        HIP_OK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < pairs; ++k) {
+          float kms = 0;
+          HIP_OK(hipEventElapsedTime(&kms, W.round_ev[2 * k], W.round_ev[2 * k + 1]));
+          local.kernel_ms += kms;
+        }
+        pairs = 0;
+      };
       while (n_active) {
         A.next_ctr = C_NEXT0 + (rounds & 1);
         A.clear_ctr = C_NEXT0 + ((rounds + 1) & 1);
         A.key_ctr = C_KEY0 + (rounds & 1);
         A.key_clear = C_KEY0 + ((rounds + 1) & 1);
-        HIP_OK(hipEventRecord(W.ek0, s));
+        if (++W.stamp == 0) W.stamp = 1;
+        A.stamp = W.stamp;
+        if (pairs == 512) harvest();  // bounds the event pool
+        const size_t ei = 2 * pairs++;
+        while (W.round_ev.size() < ei + 2) {
+          hipEvent_t e = nullptr;
+          HIP_OK(hipEventCreate(&e));
+          W.round_ev.push_back(e);
+        }
+        HIP_OK(hipEventRecord(W.round_ev[ei], s));
         HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
-        HIP_OK(hipEventRecord(W.ek1, s));
-        HIP_OK(hipStreamSynchronize(s));
-        float kms = 0;
-        HIP_OK(hipEventElapsedTime(&kms, W.ek0, W.ek1));
-        local.kernel_ms += kms;
+        HIP_OK(hipEventRecord(W.round_ev[ei + 1], s));
+        volatile uint32_t* hc = W.hostctr;
+        const auto spin0 = std::chrono::steady_clock::now();
+        while (hc[2] != A.stamp) {
+          if (std::chrono::steady_clock::now() - spin0 > std::chrono::milliseconds(20)) {
+            HIP_OK(hipStreamSynchronize(s));
+            if (hc[2] != A.stamp) throw Error(1, "relaxation round did not publish its counts");
+          }
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
         ++local.launches;
         ++rounds;
-        const volatile uint32_t* hc = W.hostctr;
         if (hc[1]) break;
         n_active = hc[0];
         std::swap(A.act_cur, A.act_next);
@@ -1998,6 +2031,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         A.list_next = lc;
         if (rounds > 1000000) throw Error(1, "relaxation did not converge");
       }
+      harvest();
     };
     uint32_t rounds = 0;
     launch_fill32(s, W.act, 0u, (size_t)item * 2);
