@@ -333,9 +333,36 @@ def _nets(nets: Sequence[Net]):
     return arr, keep
 
 
+def _csr(per_net, n: int):
+    """Per-net cell lists as (u64 offsets[n + 1], u32 cells): from a flat
+    (offsets, cells) pair, or from a sequence of per-net cell sequences."""
+    if isinstance(per_net, tuple):
+        off = np.ascontiguousarray(per_net[0], dtype=np.uint64)
+        cells = np.ascontiguousarray(per_net[1], dtype=np.uint32)
+        if off.size != n + 1 or (n and int(off[-1]) > cells.size):
+            raise MrfError(BACKEND_UNAVAILABLE, "own-footprint offsets do not match the nets")
+    else:
+        arrs = [np.asarray(p).ravel() for p in per_net]
+        off = np.zeros(n + 1, dtype=np.uint64)
+        np.cumsum([a.size for a in arrs], out=off[1:])
+        cells = np.concatenate(arrs).astype(np.uint32) if off[-1] else np.zeros(0, dtype=np.uint32)
+    if not cells.size:
+        cells = np.zeros(1, dtype=np.uint32)
+    return off, cells
+
+
 def _windows(wins, n):
+    """Windows as mrf_window records: from (x0, y0, x1, y1) tuples, or from an
+    (n, 4) integer array (passed without per-window conversion)."""
     if wins is None:
         return None
+    if isinstance(wins, np.ndarray):
+        a = np.ascontiguousarray(wins, dtype=np.uint32).reshape(-1, 4)
+        if a.shape[0] != n:
+            raise MrfError(BACKEND_UNAVAILABLE, "one window per net")
+        if n == 0:
+            a = np.zeros((1, 4), dtype=np.uint32)
+        return a.ctypes.data_as(C.POINTER(WindowS))
     return (WindowS * max(n, 1))(*[WindowS(*w) for w in wins])
 
 
@@ -571,7 +598,9 @@ class Board:
 
         own_paths: per net its own copper (sequence of cells), or None for the
         batch passing no own path.  own_halos: per net its own halo, a
-        {cell: count} dict or cells repeated by multiplicity, or None.
+        {cell: count} dict or cells repeated by multiplicity, or None.  Either
+        may also be given flat as a pair (offsets, cells): net i's cells are
+        cells[offsets[i]:offsets[i + 1]] (halo cells repeated by multiplicity).
         """
         arr, keep = _nets(nets)
         hist, pres = _u32(history), _u32(present)
@@ -580,14 +609,16 @@ class Board:
             raise MrfError(MALFORMED_GRID, "negotiation planes differ in length")
         ng = NegotiationS(_p32(hist), _p32(pres), _p32(halo), hist.size, int(pfac), 1 if fused else 0)
         if own_paths is not None:
-            arrs = [np.asarray(p, dtype=np.uint32).ravel() for p in own_paths]
-            poff = np.zeros(len(nets) + 1, dtype=np.uint64)
-            np.cumsum([a.size for a in arrs], out=poff[1:])
-            pc = np.concatenate(arrs) if poff[-1] else np.zeros(1, dtype=np.uint32)
+            poff, pc = _csr(own_paths, len(nets))
             keep += [poff, pc]
             ng.own_path_off = poff.ctypes.data_as(C.POINTER(C.c_uint64))
             ng.own_path = _p32(pc)
-        if own_halos is not None:
+        if isinstance(own_halos, tuple):
+            hoff, hc = _csr(own_halos, len(nets))
+            keep += [hoff, hc]
+            ng.own_halo_off = hoff.ctypes.data_as(C.POINTER(C.c_uint64))
+            ng.own_halo = _p32(hc)
+        elif own_halos is not None:
             cells, counts = [], []
             weighted = any(isinstance(h, dict) for h in own_halos)
             for hset in own_halos:

@@ -215,14 +215,26 @@ def negotiate_board(board, nets, groups=None, max_iters: int = MAX_ITERS, keep_t
                      on_iter)
 
 
+def _flat(per_net):
+    """Per-net cell sequences as one (offsets, cells) pair."""
+    off = np.zeros(len(per_net) + 1, dtype=np.uint64)
+    np.cumsum(np.fromiter((len(p) for p in per_net), dtype=np.uint64, count=len(per_net)), out=off[1:])
+    cells = (np.concatenate([np.asarray(p, dtype=np.int64) for p in per_net]) if len(per_net) and off[-1]
+             else np.zeros(0, dtype=np.int64))
+    return off, cells
+
+
 def board_router(board, nets, windows=None) -> RouteBatch:
     """RouteBatch over a resident board (mrfield.Board.route_negotiated)."""
 
+    wall = np.asarray(windows, dtype=np.uint32).reshape(-1, 4) if windows is not None else None
+
     def route(idx, history, present, pfac, fused, own_paths, own_halos):
         sub = [nets[i] for i in idx]
-        wins = [windows[i] for i in idx] if windows is not None else None
-        res = board.route_negotiated(sub, history, present, pfac, fused=fused, own_paths=own_paths,
-                                     own_halos=own_halos, windows=wins, retry_full=True, arrays=True)
+        wins = wall[np.asarray(idx, dtype=np.int64)] if wall is not None else None
+        flat = lambda per_net: None if per_net is None else _flat(per_net)
+        res = board.route_negotiated(sub, history, present, pfac, fused=fused, own_paths=flat(own_paths),
+                                     own_halos=flat(own_halos), windows=wins, retry_full=True, arrays=True)
         cells, off = res.cells, res.offsets
         return [cells[off[k]:off[k + 1]] if res.routed[k] else None for k in range(len(sub))]
 
