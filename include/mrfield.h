@@ -322,9 +322,12 @@ typedef struct {
 } mrf_tuning;
 
 /* Measures candidates on every k-th net (at most `sample` nets, windows as in
- * mrf_solve with the full-board retry), keeps the fastest on the context and
- * reports it.  The context's results are the same for every configuration;
- * only the time differs. */
+ * mrf_solve with the full-board retry), then times the three fastest and the
+ * context's own configuration again on every k-th net of a larger set (up to
+ * max(8 x sample, 1024) nets), keeps the fastest there on the context and
+ * reports it.  sample_ms is the kept configuration's time on the sample.  The
+ * context's results are the same for every configuration; only the time
+ * differs. */
 int mrf_autotune(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets, const mrf_window *windows,
                  size_t sample, mrf_tuning *out, mrf_error *err);
 

@@ -871,16 +871,25 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
     if (n_nets == 0 || !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "autotune needs nets");
     for (size_t i = 0; windows && i < n_nets; ++i) check_window(dims, windows[i], i);
     for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
-    // every k-th net
-    const size_t want = std::max<size_t>(1, std::min(sample ? sample : 64, n_nets));
-    const size_t step = n_nets / want;
-    std::vector<size_t> idx;
-    for (size_t i = 0; i < n_nets && idx.size() < want; i += std::max<size_t>(step, 1)) idx.push_back(i);
+    // every k-th net: the sample every candidate is timed on, and the larger
+    // set the fastest few are confirmed on (a small sample favours forms that
+    // save per-round host time, which a full batch may not)
     const mrf_window full{0, 0, dims.w - 1, dims.h - 1};
-    std::vector<mrf_window> wins;
-    for (size_t i : idx) wins.push_back(windows ? windows[i] : full);
+    auto every_kth = [&](size_t want, std::vector<size_t>& idx, std::vector<mrf_window>& wins) {
+      want = std::max<size_t>(1, std::min(want, n_nets));
+      const size_t step = std::max<size_t>(n_nets / want, 1);
+      idx.clear();
+      wins.clear();
+      for (size_t i = 0; i < n_nets && idx.size() < want; i += step) idx.push_back(i);
+      for (size_t i : idx) wins.push_back(windows ? windows[i] : full);
+    };
+    const size_t want = sample ? sample : 64;
+    std::vector<size_t> idx, cidx;
+    std::vector<mrf_window> wins, cwins;
+    every_kth(want, idx, wins);
+    every_kth(std::max<size_t>(8 * want, 1024), cidx, cwins);
     mrf::Ctx& c = ctx->c;
-    auto run = [&]() {
+    auto run_on = [&](const std::vector<size_t>& idx, const std::vector<mrf_window>& wins) {
       auto t0 = std::chrono::steady_clock::now();
       Batch b;
       solve_nets(c, bd->b, g, nets, idx, wins, false, nullptr, b);
@@ -895,6 +904,7 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
       }
       return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
+    auto run = [&]() { return run_on(idx, wins); };
     struct Cand {
       uint32_t tw, th, threads, max_local;
       int mode;
@@ -938,6 +948,7 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
     out->default_ms = best_ms;
     Cand best = base;
     uint32_t tried = 1;
+    std::vector<std::pair<double, Cand>> scored{{best_ms, base}};
     const uint32_t shapes[][2] = {{32, 16}, {16, 16}, {16, 8}, {8, 4}};
     for (auto& sh : shapes) {
       if (mrf::lds_bytes_for(sh[0], sh[1], dims.layers) > 160 * 1024 - 64) continue;
@@ -956,12 +967,31 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
               Cand k{sh[0], sh[1], threads, ml, mode, fm.first, fm.second};
               double ms = timed(k);
               ++tried;
-              if (ms < best_ms) {
-                best_ms = ms;
-                best = k;
-              }
+              scored.push_back({ms, k});
             }
           }
+    }
+    // Confirmation: the three fastest on the sample and the context's own
+    // configuration, timed on the larger set; the fastest there is kept.
+    std::stable_sort(scored.begin(), scored.end(),
+                     [](const std::pair<double, Cand>& a, const std::pair<double, Cand>& b) { return a.first < b.first; });
+    std::vector<std::pair<double, Cand>> finalists(scored.begin(), scored.begin() + std::min<size_t>(3, scored.size()));
+    bool has_base = false;
+    for (auto& f : finalists)
+      has_base = has_base || (f.second.tw == base.tw && f.second.th == base.th && f.second.threads == base.threads &&
+                              f.second.max_local == base.max_local && f.second.mode == base.mode &&
+                              f.second.graph_rounds == base.graph_rounds && f.second.delta == base.delta);
+    if (!has_base) finalists.push_back({out->default_ms, base});
+    double best_full = 0;
+    for (size_t k = 0; k < finalists.size(); ++k) {
+      apply(finalists[k].second);
+      run_on(cidx, cwins);  // warm-up
+      const double ms = std::min(run_on(cidx, cwins), run_on(cidx, cwins));
+      if (k == 0 || ms < best_full) {
+        best_full = ms;
+        best = finalists[k].second;
+        best_ms = finalists[k].first;
+      }
     }
     apply(best);
     out->tile_w = best.tw;
