@@ -76,6 +76,8 @@ enum : uint8_t {
   K_CHG = 0x80,
 };
 // per-item state bits
+// Item state: 0 until the tile's first visit, which stores S_TOUCHED with the
+// index of its round above it (state >> 1).
 enum : uint32_t { S_TOUCHED = 1u };
 // device counters
 enum : int {
@@ -341,6 +343,9 @@ struct RelaxArgs {
   uint32_t next_ctr, clear_ctr;
   uint32_t* host_out;
   uint32_t stamp;
+  // Plain form and round driver: index of this launch's round (the graph
+  // form reads it from counters[C_GROUND]).
+  uint32_t round;
   // Delta stepping (key_cur non-null): every activation carries a key, the
   // smallest search cost among the border labels that changed; a round
   // processes the items whose key is at most counters[C_THRESH] and carries
@@ -614,7 +619,7 @@ __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ?
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
 template <int TW, int TH, int LM, bool NARROW>
-__device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem) {
+__device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem, uint32_t round) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = const_load(A.item_field + item);
   // the item's own state travels with the descriptor loads: whether the tile
@@ -803,8 +808,16 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   for (uint32_t i = tid; i < L * TW; i += nt) s.cchg[i] = 0;
   for (uint32_t i = tid; i < PT; i += nt) s.vchg[i] = 0;
   __syncthreads();
-  const uint32_t nbv = ((uniform_u32(s.misc[1]) & S_TOUCHED) ? 1u : 0u) | ((uniform_u32(s.misc[2]) & S_TOUCHED) ? 2u : 0u) |
-                       ((uniform_u32(s.misc[3]) & S_TOUCHED) ? 4u : 0u) | ((uniform_u32(s.misc[4]) & S_TOUCHED) ? 8u : 0u);
+  // A neighbour's border labels count only if its first visit ran in an
+  // earlier round: those were stored before a kernel boundary.  A neighbour
+  // whose first visit runs in this round may have published its state before
+  // the labels this tile loaded alongside it (the two loads are not ordered),
+  // so its border is taken as unreached, an upper bound; that visit then
+  // offers its border to this tile, unreached to it, and activates it for the
+  // next round.
+  auto earlier = [&](uint32_t st) { return (st & S_TOUCHED) && (st >> 1) < round; };
+  const uint32_t nbv = (earlier(uniform_u32(s.misc[1])) ? 1u : 0u) | (earlier(uniform_u32(s.misc[2])) ? 2u : 0u) |
+                       (earlier(uniform_u32(s.misc[3])) ? 4u : 0u) | (earlier(uniform_u32(s.misc[4])) ? 8u : 0u);
   // halo labels into the ring (a neighbour never visited has none), and a
   // lower bound of the price of stepping from the tile into each halo cell:
   // the neighbour's gap price times the cell's weight (1 for an obstacle
@@ -1017,7 +1030,8 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   res.kb = uniform_u32(s.misc[13]);
   res.ka = uniform_u32(s.misc[14]);
   if (tid == 0) {
-    if (!touched) __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!touched)
+      __hip_atomic_fetch_or(&A.state[item], S_TOUCHED | (round << 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
     atomicAdd(&A.counters[(C_ROUNDS) * kCtrStride], rounds);
   }
@@ -1103,7 +1117,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
                                                 __HIP_MEMORY_SCOPE_AGENT));
   }
   TileResult res{};
-  if (!defer) res = process_tile<TW, TH, LM, NARROW>(A, item, smem);
+  if (!defer) res = process_tile<TW, TH, LM, NARROW>(A, item, smem, A.round);
   if (threadIdx.x >= 64) return;
   uint32_t kmin = kMax;
   if (defer) {
@@ -1174,7 +1188,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
   // the compiler so lanes other than 0 re-run the body without a new pop.
   for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
     uint32_t item = const_load(R.list_cur + k);
-    const TileResult res = process_tile<TW, TH, LM, NARROW>(R, item, smem);
+    const TileResult res = process_tile<TW, TH, LM, NARROW>(R, item, smem, r);
     if (threadIdx.x < 64) {
       if (threadIdx.x == 0) R.act_cur[item] = 0;
       activate_sides(R, item, res, res.ka);
@@ -2001,6 +2015,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         A.key_clear = C_KEY0 + ((rounds + 1) & 1);
         if (++W.stamp == 0) W.stamp = 1;
         A.stamp = W.stamp;
+        A.round = rounds;
         if (pairs == 512) harvest();  // bounds the event pool
         const size_t ei = 2 * pairs++;
         while (W.round_ev.size() < ei + 2) {
@@ -2526,6 +2541,7 @@ uint32_t driver_step(RoundDriver& d) {
   const size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   void* args[] = {&A};
   HIP_OK(hipEventRecord(c.ws.ek0, s));
+  A.round = (uint32_t)d.rounds;
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
   HIP_OK(hipEventRecord(c.ws.ek1, s));
   if (live > 1) hipLaunchKernelGGL(mark_groups, dim3(1024), dim3(256), 0, s, list_next, c.ws.counters, d.item_group,

@@ -349,6 +349,32 @@ def test_fail_closed_contracts(eng):
     assert eng.route_isolated_batch(g, [], [], e) == []
 
 
+def test_same_round_neighbours_after_a_layout_change(eng):
+    """A tile whose neighbour is first visited in the same round must not take
+    that neighbour's border labels: its state and labels are loaded together,
+    unordered, and the label memory still holds the previous solve's values
+    (here laid out for 16x16 tiles, so they index other cells).  Solving the
+    same batch at 16x16, then at 8x4 tiles with 128 threads and a round cap of
+    4 (short visits, many neighbours first visited in one round), gives the
+    same paths in both forms."""
+    s = mrfield.synth_config(2, seed=1)
+    board = mrfield.Board(eng, s.grid, s.edges, s.pricing())
+    nets = s.nets()
+    try:
+        configure(eng, (16, 16, 256, 32), mrfield.MODE_PLAIN)
+        ref = board.solve_arrays(nets)
+        for mode in (mrfield.MODE_PLAIN, mrfield.MODE_GRAPH):
+            configure(eng, (16, 16, 256, 32), mrfield.MODE_PLAIN)
+            board.solve_arrays(nets)
+            configure(eng, (8, 4, 128, 4), mode)
+            got = board.solve_arrays(nets)
+            for k in ("routed", "cost", "offsets", "cells"):
+                assert np.array_equal(getattr(got, k), getattr(ref, k)), (mode, k)
+    finally:
+        board.close()
+        configure(eng, (16, 16, 256, 32), mrfield.MODE_PLAIN)
+
+
 @pytest.mark.parametrize("mode,message", [(1, "injected backend failure"), (2, "re-validation")])
 def test_host_revalidation_fails_closed(monkeypatch, mode, message):
     """A device result the host cannot re-validate fails the whole call (the
@@ -381,9 +407,11 @@ def test_autotune_keeps_results(eng):
     g = s.grid
     board = mrfield.Board(eng, g, s.edges, s.pricing())
     try:
+        # the context's own configuration is a finalist too: start from a known one
+        configure(eng, (32, 32, 256, 32), mrfield.MODE_PLAIN)
         t = board.autotune(s.nets(), windows=s.windows, sample=16)
-        assert t["tried"] > 10 and t["sample_ms"] > 0
-        assert (t["tile_w"], t["tile_h"]) in [(32, 16), (16, 16), (16, 8), (8, 4)]
+        assert t["tried"] > 10 and t["sample_ms"] > 0 and t["default_ms"] > 0
+        assert (t["tile_w"], t["tile_h"]) in [(32, 32), (32, 16), (16, 16), (16, 8), (8, 4)]
         got = board.solve(s.nets(), windows=s.windows, retry_full=True)
     finally:
         board.close()
