@@ -351,6 +351,9 @@ struct RoundDriver {
   uint32_t n_cur = 0;
   uint32_t cur = 0;  // which half of act/list is the current round's
   uint64_t rounds = 0, tile_visits = 0, local_rounds = 0;
+  // Round index stored in item states at first visits (below 2^30; restarted
+  // at 1 after every item state is rewritten to round 0).
+  uint32_t stamp_round = 1;
   double kernel_ms = 0;
 };
 

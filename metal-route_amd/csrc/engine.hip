@@ -1207,6 +1207,15 @@ __global__ void advance_round(uint32_t* ctr) {
   ctr[C_GROUND * kCtrStride] += 1;
 }
 
+// Every touched item state to round 0 (the round driver restarting its
+// round index).
+__global__ void restamp_states(uint32_t* state, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride)
+    if (state[i]) state[i] = S_TOUCHED;
+}
+
 __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -2541,7 +2550,13 @@ uint32_t driver_step(RoundDriver& d) {
   const size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   void* args[] = {&A};
   HIP_OK(hipEventRecord(c.ws.ek0, s));
-  A.round = (uint32_t)d.rounds;
+  if (d.stamp_round >= (1u << 30)) {
+    // states stay below 2^31: every touched item now reads as first visited
+    // in round 0, before every later round
+    hipLaunchKernelGGL(restamp_states, dim3(1024), dim3(256), 0, s, d.state, (size_t)items);
+    d.stamp_round = 1;
+  }
+  A.round = d.stamp_round++;
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
   HIP_OK(hipEventRecord(c.ws.ek1, s));
   if (live > 1) hipLaunchKernelGGL(mark_groups, dim3(1024), dim3(256), 0, s, list_next, c.ws.counters, d.item_group,
