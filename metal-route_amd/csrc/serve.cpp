@@ -9,7 +9,7 @@
 // together with the fields of every other admitted request on any resident
 // board, and leave as soon as their own fields have converged
 // (driver_finished).  Each request's paths are re-validated on the host like
-// the provider's (path_ok), on a checker thread beside the worker so that the
+// the provider's (path_ok), on a validator thread beside the worker so that the
 // device keeps relaxing meanwhile; a result is exactly what mrf_solve returns
 // for the same nets alone.
 #include <chrono>
@@ -82,7 +82,7 @@ struct ServeReq {
   Clock::time_point t_submit, t_admit, t_done;
   bool admitted_once = false;
   uint32_t rounds = 0, peers_max = 0;
-  uint32_t checks = 0;      // passes handed to the checker and not yet checked
+  uint32_t checks = 0;      // passes handed to the validator and not yet checked
   bool check_failed = false;
 };
 
@@ -103,7 +103,7 @@ struct PassCheck {
 struct mrf_server {
   mrf::Ctx c;
   mrf::RoundDriver d;
-  std::thread worker, checker;
+  std::thread worker, validator;
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_check;
   std::deque<PassCheck*> checks;
@@ -250,7 +250,7 @@ void fail_all(mrf_server* S, const std::string& msg) {
 // relaxing the other requests meanwhile, and completes each request when its
 // final pass is checked.  A request that failed elsewhere (state 2) is only
 // released: the waiter holds it until its passes here are done.
-void checker_loop(mrf_server* S) {
+void validator_loop(mrf_server* S) {
   for (;;) {
     PassCheck* pc = nullptr;
     bool skip = false;
@@ -408,7 +408,7 @@ void worker_loop(mrf_server* S) {
   }
 }
 
-// Stops and joins the worker and checker threads (either may not have started).
+// Stops and joins the worker and validator threads (either may not have started).
 void stop_threads(mrf_server* S) {
   {
     std::lock_guard<std::mutex> lk(S->mu);
@@ -417,7 +417,7 @@ void stop_threads(mrf_server* S) {
   S->cv_work.notify_all();
   S->cv_check.notify_all();
   if (S->worker.joinable()) S->worker.join();
-  if (S->checker.joinable()) S->checker.join();
+  if (S->validator.joinable()) S->validator.join();
 }
 
 }  // namespace
@@ -453,7 +453,7 @@ int mrf_server_create(int device, const mrf_server_config* cfg, mrf_server** out
                      S->max_inflight);
     S->st.label_bytes = lb;
     S->worker = std::thread(worker_loop, S);
-    S->checker = std::thread(checker_loop, S);
+    S->validator = std::thread(validator_loop, S);
   } catch (const Error& e) {
     stop_threads(S);
     mrf::driver_free(S->d);
@@ -647,7 +647,7 @@ int mrf_server_wait(mrf_server* S, uint64_t request_id, int64_t timeout_ms, mrf_
     return MRF_BACKEND_UNAVAILABLE;
   }
   ServeReq* r = it->second;
-  // done, and no pass of it still with the checker
+  // done, and no pass of it still with the validator
   auto ready = [&] { return r->state == 2 && r->checks == 0; };
   if (timeout_ms < 0) {
     S->cv_done.wait(lk, ready);
