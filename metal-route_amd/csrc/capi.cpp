@@ -345,6 +345,8 @@ int mrf_open(int device, mrf_ctx** out, mrf_error* err) {
     mrf::ctx_init(c->c, device);
     const char* inj = getenv("MRF_INJECT_FAULT");
     if (inj) c->c.inject = atoi(inj);
+    const char* spin = getenv("MRF_STAMP_SPIN_US");
+    if (spin) c->c.stamp_spin_us = (uint32_t)strtoul(spin, nullptr, 10);
   } catch (const Error& e) {
     mrf::ctx_free(c->c);
     delete c;

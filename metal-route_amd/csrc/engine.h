@@ -239,6 +239,9 @@ struct Ctx {
   std::vector<BoardView> views;   // boards of the last solve
   // Fault injection for fail-closed tests (0 none, 1 backend error, 2 corrupt path).
   int inject = 0;
+  // Plain form: how long the host spins on a launch's stamp before it hands
+  // over to a stream synchronisation (microseconds).
+  uint32_t stamp_spin_us = 20000;
 };
 
 // ---- engine.hip -------------------------------------------------------------

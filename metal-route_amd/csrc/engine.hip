@@ -2001,9 +2001,10 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     // counters (the kernel clears the one the host has read), and the
     // launch's last workgroup publishes the counts and then the stamp, so no
     // fill or copy is queued between rounds and the next launch follows as
-    // soon as the stamp lands.  A stamp that has not landed within 20 ms
-    // hands over to a stream synchronisation, which reports a failed launch;
-    // after it the stamp must be there.  Per-round kernel times are read from
+    // soon as the stamp lands.  A stamp that has not landed within
+    // c.stamp_spin_us (20 ms unless set) hands over to a stream
+    // synchronisation, which reports a failed launch; after it the stamp must
+    // be there.  Per-round kernel times are read from
     // an event pair per round once the last round is done.
     auto plain_rounds = [&](uint32_t& rounds, uint32_t n_active) {
       A.host_out = W.hostctr_dev;
@@ -2038,7 +2039,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         volatile uint32_t* hc = W.hostctr;
         const auto spin0 = std::chrono::steady_clock::now();
         while (hc[2] != A.stamp) {
-          if (std::chrono::steady_clock::now() - spin0 > std::chrono::milliseconds(20)) {
+          if (std::chrono::steady_clock::now() - spin0 >= std::chrono::microseconds(c.stamp_spin_us)) {
             HIP_OK(hipStreamSynchronize(s));
             if (hc[2] != A.stamp) throw Error(1, "relaxation round did not publish its counts");
           }

@@ -375,6 +375,30 @@ def test_same_round_neighbours_after_a_layout_change(eng):
         configure(eng, (16, 16, 256, 32), mrfield.MODE_PLAIN)
 
 
+def test_stamp_wait_falls_back_to_a_stream_synchronisation(monkeypatch):
+    """The plain form waits for each launch's stamp in host-mapped memory and
+    hands over to a stream synchronisation after MRF_STAMP_SPIN_US (read when
+    a context opens); with 0 every round takes the hand-over, and the paths
+    are the same."""
+    s = mrfield.synth_config(1, seed=5)
+    wins = [(0, 0, s.grid.w - 1, s.grid.h - 1)] * len(s.nets())
+    e = mrfield.Engine(0)
+    try:
+        configure(e, (16, 16, 256, 32), mrfield.MODE_PLAIN)
+        ref = e.route_isolated_batch(s.grid, s.nets(), wins, s.edges)
+    finally:
+        e.close()
+    monkeypatch.setenv("MRF_STAMP_SPIN_US", "0")
+    e = mrfield.Engine(0)
+    monkeypatch.delenv("MRF_STAMP_SPIN_US")
+    try:
+        configure(e, (16, 16, 256, 32), mrfield.MODE_PLAIN)
+        got = e.route_isolated_batch(s.grid, s.nets(), wins, s.edges)
+        assert got == ref and e.stats()["relax_launches"] > 1
+    finally:
+        e.close()
+
+
 @pytest.mark.parametrize("mode,message", [(1, "injected backend failure"), (2, "re-validation")])
 def test_host_revalidation_fails_closed(monkeypatch, mode, message):
     """A device result the host cannot re-validate fails the whole call (the
