@@ -26,6 +26,27 @@ struct mrf_ctx {
   mrf::Ctx c;
 };
 
+// Page-locked host memory for the board's additive plane: the negotiated
+// batches rebuild and upload it every iteration, and a copy from pageable
+// memory would be staged through the driver first.
+template <typename T>
+struct PinnedAlloc {
+  using value_type = T;
+  PinnedAlloc() = default;
+  template <typename U>
+  PinnedAlloc(const PinnedAlloc<U>&) {}
+  T* allocate(size_t n) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    return (T*)p;
+  }
+  void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+  template <typename U>
+  bool operator==(const PinnedAlloc<U>&) const { return true; }
+  template <typename U>
+  bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+
 struct mrf_board {
   mrf_ctx* ctx = nullptr;
   mrf::DevBoard b;
@@ -33,7 +54,8 @@ struct mrf_board {
   // Host copies needed for request validation and path checks.
   std::vector<uint32_t> cost;
   std::vector<uint8_t> vf, board;
-  std::vector<uint32_t> xe, ye, ve, add;
+  std::vector<uint32_t> xe, ye, ve;
+  std::vector<uint32_t, PinnedAlloc<uint32_t>> add;
   std::vector<uint8_t> va;
   std::vector<int32_t> halo;
   uint32_t halo_pen = 0;
@@ -741,21 +763,13 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
     // Shared congestion snapshot (negotiated.rs:1654-1667), folded with
     // history into the board's additive plane: what every net pays on cells
     // outside its own footprint.
-    std::vector<uint32_t> plane(n);
-    std::mutex max_mu;
-    uint32_t plane_max = 0;
-    parallel_for(n, 1 << 16, [&](size_t a, size_t b) {
-      uint32_t m = 0;
-      for (size_t c = a; c < b; ++c) {
-        const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
-        plane[c] = clamp32(sat_add64(neg->history[c], shared));
-        m = std::max(m, plane[c]);
-      }
-      std::lock_guard<std::mutex> lk(max_mu);
-      plane_max = std::max(plane_max, m);
-    });
+    auto base_price = [&](size_t c) {
+      const uint64_t shared = sat_add64(sat_mul64(pf, neg->present[c]), sat_mul64(pf, halo_at(c)));
+      return clamp32(sat_add64(neg->history[c], shared));
+    };
     // Own footprint of each net: priced_with_base's self-subtraction
-    // (negotiated.rs:2777-2797) as additive overrides.
+    // (negotiated.rs:2777-2797) as additive overrides.  Built first: a
+    // malformed footprint fails the call before the board's plane changes.
     std::vector<NetPlan> prices(n_nets);
     std::vector<std::string> bad(n_nets);
     parallel_for(n_nets, 16, [&](size_t a, size_t b) {
@@ -785,7 +799,7 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
                                 sat_mul64(pf, sat_sub64(halo_at(c), self_halo)));
           }
           const uint32_t price = clamp32(sat_add64(neg->history[c], foreign));
-          if (price != plane[c]) {
+          if (price != base_price(c)) {
             p.ovr_cells.push_back(c);
             p.ovr_add.push_back(price);
           }
@@ -794,8 +808,22 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
     });
     for (const std::string& m : bad)
       if (!m.empty()) throw Error(MRF_BACKEND_UNAVAILABLE, m.c_str());
-    mrf::board_set_additive(ctx->c, bd->b, plane.data(), plane_max);
-    bd->add = std::move(plane);
+    // The plane, built in place in the board's page-locked host copy (kept
+    // for host re-validation), then uploaded.
+    bd->add.resize(n);
+    uint32_t* plane = bd->add.data();
+    std::mutex max_mu;
+    uint32_t plane_max = 0;
+    parallel_for(n, 1 << 16, [&](size_t a, size_t b) {
+      uint32_t m = 0;
+      for (size_t c = a; c < b; ++c) {
+        plane[c] = base_price(c);
+        m = std::max(m, plane[c]);
+      }
+      std::lock_guard<std::mutex> lk(max_mu);
+      plane_max = std::max(plane_max, m);
+    });
+    mrf::board_set_additive(ctx->c, bd->b, plane, plane_max);
     solve_board(ctx, bd, nets, n_nets, windows, retry_full, &prices, out);
     if (err) set_err(err, MRF_OK, "ok");
     return MRF_OK;
