@@ -1207,6 +1207,16 @@ __global__ void advance_round(uint32_t* ctr) {
   ctr[C_GROUND * kCtrStride] += 1;
 }
 
+// item_field of a batch from its field descriptors: block f writes field f's
+// tile items [item0, item0 + tx * ty).
+__global__ void fill_item_field(const FieldDesc* fields, uint32_t n_fields, uint32_t* item_field) {
+  for (uint32_t f = blockIdx.x; f < n_fields; f += gridDim.x) {
+    const FieldDesc F = fields[f];
+    const uint32_t nt = F.tx * F.ty;
+    for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) item_field[F.item0 + k] = f;
+  }
+}
+
 // Every touched item state to round 0 (the round driver restarting its
 // round index).
 __global__ void restamp_states(uint32_t* state, size_t n) {
@@ -1905,7 +1915,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   build_all_overrides(boards, reqs, S, ovr_seg, ovr_n);
   std::vector<FieldDesc>& layout = c.layout;
   layout.clear();
-  std::vector<uint32_t> pool, item_field, seeds;
+  std::vector<uint32_t> pool, seeds;
   uint64_t lab_off = 0, field_cells = 0;
   uint32_t item = 0;
   for (size_t i = 0; i < reqs.size(); ++i) {
@@ -1941,7 +1951,6 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     F.ovr_off = (uint32_t)pool.size();
     F.n_ovr = ovr_n[i];
     pool.insert(pool.end(), ovr_seg[i].begin(), ovr_seg[i].end());
-    for (uint32_t k = 0; k < nt; ++k) item_field.push_back((uint32_t)i);
     // seed tiles: tiles holding a source inside the window
     std::vector<uint32_t> st_items;
     for (uint32_t s : q.sources) {
@@ -1976,13 +1985,15 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   if (!layout.empty())
     HIP_OK(hipMemcpyAsync(W.fields, layout.data(), layout.size() * sizeof(FieldDesc),
                           hipMemcpyHostToDevice, s));
-  if (item)
-    HIP_OK(hipMemcpyAsync(W.item_field, item_field.data(), item * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  std::vector<uint32_t> pool_all = pool;
-  pool_all.insert(pool_all.end(), seeds.begin(), seeds.end());
-  if (!pool_all.empty())
-    HIP_OK(hipMemcpyAsync(W.pool, pool_all.data(), pool_all.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  const uint32_t* d_seeds = W.pool + pool.size();
+  if (item && !layout.empty())
+    hipLaunchKernelGGL(fill_item_field, dim3((uint32_t)std::min<size_t>(layout.size(), 4096)), dim3(256), 0, s,
+                       W.fields, (uint32_t)layout.size(), W.item_field);
+  // the seed items follow the pool (this call synchronises before it returns)
+  const size_t pool_n = pool.size();
+  pool.insert(pool.end(), seeds.begin(), seeds.end());
+  if (!pool.empty())
+    HIP_OK(hipMemcpyAsync(W.pool, pool.data(), pool.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  const uint32_t* d_seeds = W.pool + pool_n;
   launch_fill32(s, W.state, 0u, item);
   launch_fill32(s, W.counters, 0u, kCtrWords);
 
