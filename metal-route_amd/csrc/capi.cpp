@@ -17,6 +17,7 @@
 #include "../../include/mrfield.h"
 #include "engine.h"
 #include "hostcheck.h"
+#include "hostpool.h"
 
 using mrf::Error;
 using mrf::kMax;
@@ -140,7 +141,8 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
     mrf::reconstruct_paths(c, b, reqs.size(), routed, cost, off, cells, &st);
     if (c.inject == 1) throw Error(MRF_BACKEND_UNAVAILABLE, "injected backend failure");
     if (c.inject == 2 && !cells.empty()) cells[cells.size() / 2] ^= 1u;
-    // Re-validate on up to 8 host threads; any failure fails the whole call.
+    // Re-validate on up to 8 host threads (the shared workers); any failure
+    // fails the whole call.
     const auto tv0 = std::chrono::steady_clock::now();
     std::vector<uint8_t> bad(k1 - k0, 0);
     auto check = [&](size_t a, size_t b) {
@@ -159,18 +161,7 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
         out.paths[k].assign(p, p + len);
       }
     };
-    size_t nthreads = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), (k1 - k0 + 31) / 32});
-    if (nthreads <= 1) {
-      check(k0, k1);
-    } else {
-      std::vector<std::thread> pool;
-      size_t per = (k1 - k0 + nthreads - 1) / nthreads;
-      for (size_t t = 0; t < nthreads; ++t) {
-        size_t a = k0 + t * per, b = std::min(k1, a + per);
-        if (a < b) pool.emplace_back(check, a, b);
-      }
-      for (auto& th : pool) th.join();
-    }
+    mrf::host::parallel_for(k1 - k0, 32, [&](size_t a, size_t b) { check(k0 + a, k0 + b); });
     c.last.validate_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tv0).count();
     for (uint8_t v : bad)
@@ -332,22 +323,7 @@ void own_cells(const uint64_t* off, const uint32_t* cells, const uint32_t* weigh
   }
 }
 
-// body(begin, end) over [0, n) split across up to 8 host threads.
-template <typename F>
-void parallel_for(size_t n, size_t min_chunk, F&& body) {
-  const size_t nt = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), (n + min_chunk - 1) / min_chunk});
-  if (nt <= 1) {
-    body(size_t(0), n);
-    return;
-  }
-  std::vector<std::thread> pool;
-  const size_t per = (n + nt - 1) / nt;
-  for (size_t t = 0; t < nt; ++t) {
-    const size_t a = t * per, b = std::min(n, a + per);
-    if (a < b) pool.emplace_back([&body, a, b] { body(a, b); });
-  }
-  for (auto& th : pool) th.join();
-}
+using mrf::host::parallel_for;
 
 }  // namespace
 

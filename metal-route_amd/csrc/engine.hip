@@ -51,6 +51,7 @@
 #include <type_traits>
 
 #include "engine.h"
+#include "hostpool.h"
 
 namespace mrf {
 
@@ -1851,29 +1852,8 @@ static void build_all_overrides(const std::vector<const DevBoard*>& boards, cons
       build_overrides(q, *boards[q.board], S, tx, ty, segs[i], ns[i]);
     }
   };
-  const size_t nt = std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), total / 16384 + 1,
-                                      reqs.size()});
-  if (nt <= 1) {
-    work(0, reqs.size());
-    return;
-  }
-  std::vector<std::thread> pool;
-  std::vector<std::exception_ptr> errs(nt);
-  const size_t per = (reqs.size() + nt - 1) / nt;
-  for (size_t t = 0; t < nt; ++t) {
-    const size_t a = t * per, e = std::min(reqs.size(), a + per);
-    if (a >= e) break;
-    pool.emplace_back([&, a, e, t] {
-      try {
-        work(a, e);
-      } catch (...) {
-        errs[t] = std::current_exception();
-      }
-    });
-  }
-  for (auto& th : pool) th.join();
-  for (auto& e : errs)
-    if (e) std::rethrow_exception(e);
+  // one chunk per 16 K override cells on average, on the shared host workers
+  host::parallel_for(reqs.size(), std::max<size_t>(1, reqs.size() * 16384 / total), work);
 }
 
 // Whether a field's step prices all fit the narrow step planes: its board's

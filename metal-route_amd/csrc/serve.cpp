@@ -22,6 +22,7 @@
 #include "../../include/mrfield.h"
 #include "engine.h"
 #include "hostcheck.h"
+#include "hostpool.h"
 
 using mrf::Error;
 using mrf::kMax;
@@ -176,23 +177,7 @@ PassCheck* collect_pass(mrf_server* S, ServeReq* r) {
   return pc;
 }
 
-// body(begin, end) over [0, n) split across up to 8 host threads.
-template <typename F>
-void parallel_for(size_t n, size_t min_chunk, F&& body) {
-  const size_t nt =
-      std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), (n + min_chunk - 1) / min_chunk});
-  if (nt <= 1) {
-    body(size_t(0), n);
-    return;
-  }
-  std::vector<std::thread> pool;
-  const size_t per = (n + nt - 1) / nt;
-  for (size_t t = 0; t < nt; ++t) {
-    const size_t a = t * per, b = std::min(n, a + per);
-    if (a < b) pool.emplace_back([&body, a, b] { body(a, b); });
-  }
-  for (auto& th : pool) th.join();
-}
+using mrf::host::parallel_for;
 
 // Host re-validation of one pass (path_ok, as mrf_solve does), on up to 8
 // threads; stores the checked paths in the request's result.  Returns false
