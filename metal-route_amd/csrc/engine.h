@@ -342,8 +342,10 @@ struct RoundDriver {
   FieldDesc* fields = nullptr;
   uint32_t* pool = nullptr;
   BoardView* boards = nullptr;
-  uint32_t* group_active = nullptr;
-  uint32_t* pinned = nullptr;  // counters + group_active readback
+  // relaxation counters (kCtrWords) followed by one activity flag per group,
+  // cleared by one fill and read back by one copy per round
+  uint32_t* ctr = nullptr;
+  uint32_t* pinned = nullptr;  // readback of ctr
   RangeAlloc fa, ia, la, pa;   // fields, items, label words, pool words
   std::vector<BoardView> views;
   std::vector<const DevBoard*> devs;  // caller-owned boards, by slot

@@ -347,6 +347,11 @@ struct RelaxArgs {
   // Plain form and round driver: index of this launch's round (the graph
   // form reads it from counters[C_GROUND]).
   uint32_t round;
+  // Round driver with several groups live: a visit that activates any tile
+  // for the next round sets group_active[item_group[item]] (its neighbours
+  // are in its own group); null otherwise.
+  uint32_t* group_active;
+  const uint32_t* item_group;
   // Delta stepping (key_cur non-null): every activation carries a key, the
   // smallest search cost among the border labels that changed; a round
   // processes the items whose key is at most counters[C_THRESH] and carries
@@ -1110,6 +1115,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
       __hip_atomic_store(&A.counters[A.key_clear * kCtrStride], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   const uint32_t item = const_load(A.list_cur + blockIdx.x);
+  const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
   uint32_t key = 0;
   bool defer = false;
   if (A.key_cur) {
@@ -1129,6 +1135,9 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
   }
   if (threadIdx.x == 0) {
     A.act_cur[item] = 0;
+    // read back after the launch ends: a plain store is visible by then
+    if (A.group_active && (defer || (res.flags & (T_LEFT | T_RIGHT | T_TOP | T_BOTTOM | T_CAPPED))))
+      A.group_active[grp] = 1u;
     if (A.key_cur) A.key_cur[item] = kMax;
     if (!defer && (res.flags & (T_ANY | T_CAPPED))) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
     if (A.key_cur && kmin != kMax) atomicMin(&A.counters[A.key_ctr * kCtrStride], kmin);
@@ -2261,13 +2270,6 @@ void RangeAlloc::release(uint64_t off, uint64_t n) {
 }
 
 // Flag the groups that own an item of the next round's list.
-__global__ void mark_groups(const uint32_t* list, const uint32_t* counters, const uint32_t* item_group,
-                            uint32_t* group_active) {
-  const uint32_t n = counters[C_NEXTN * kCtrStride];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    group_active[item_group[list[i]]] = 1u;
-}
-
 namespace {
 uint64_t driver_items(const RoundDriver& d) { return d.ia.cap(); }
 
@@ -2310,10 +2312,10 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   dmalloc(d.fields, nfields);
   dmalloc(d.pool, pool_words);
   dmalloc(d.boards, max_boards);
-  dmalloc(d.group_active, max_groups);
+  dmalloc(d.ctr, kCtrWords + max_groups);
   HIP_OK(hipHostMalloc((void**)&d.pinned, (kCtrWords + max_groups) * sizeof(uint32_t), hipHostMallocDefault));
   launch_fill32(c.stream, d.act, 0u, 2 * items);
-  launch_fill32(c.stream, d.group_active, 0u, max_groups);
+  launch_fill32(c.stream, d.ctr, 0u, kCtrWords + max_groups);
   HIP_OK(hipStreamSynchronize(c.stream));
   d.fa.reset(nfields);
   d.ia.reset(items);
@@ -2332,7 +2334,7 @@ void driver_free(RoundDriver& d) {
   if (!d.c) return;
   (void)hipSetDevice(d.c->device);
   void* ptrs[] = {d.labels, d.state,  d.act,   d.list,   d.item_field,  d.item_group,
-                  d.seeds,  d.fields, d.pool,  d.boards, d.group_active};
+                  d.seeds,  d.fields, d.pool,  d.boards, d.ctr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (d.pinned) (void)hipHostFree(d.pinned);
@@ -2504,11 +2506,12 @@ uint32_t driver_step(RoundDriver& d) {
     return 0;
   }
   // With one live group its fields are active exactly when the next list is
-  // not empty: the per-group marking and its readback are skipped.
+  // not empty: the per-group flags are neither set nor read back.  One fill
+  // clears the counters and the flags, one copy reads them back.
   uint32_t live = 0;
   for (uint32_t g = 0; g < d.max_groups; ++g) live += d.groups[g].live ? 1u : 0u;
-  launch_fill32(s, c.ws.counters, 0u, kCtrWords);
-  if (live > 1) launch_fill32(s, d.group_active, 0u, d.max_groups);
+  const size_t ctr_words = kCtrWords + (live > 1 ? d.max_groups : 0u);
+  launch_fill32(s, d.ctr, 0u, ctr_words);
   RelaxArgs A;
   memset(&A, 0, sizeof(A));
   A.boards = d.boards;
@@ -2521,7 +2524,9 @@ uint32_t driver_step(RoundDriver& d) {
   A.act_next = act_next;
   A.list_cur = list_cur;
   A.list_next = list_next;
-  A.counters = c.ws.counters;
+  A.counters = d.ctr;
+  A.group_active = live > 1 ? d.ctr + kCtrWords : nullptr;
+  A.item_group = d.item_group;
   A.max_local = S.max_local;
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
@@ -2551,13 +2556,8 @@ uint32_t driver_step(RoundDriver& d) {
   A.round = d.stamp_round++;
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
   HIP_OK(hipEventRecord(c.ws.ek1, s));
-  if (live > 1) hipLaunchKernelGGL(mark_groups, dim3(1024), dim3(256), 0, s, list_next, c.ws.counters, d.item_group,
-                                   d.group_active);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(d.pinned, c.ws.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  if (live > 1)
-    HIP_OK(hipMemcpyAsync(d.pinned + kCtrWords, d.group_active, d.max_groups * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(d.pinned, d.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   d.staging.clear();
   if (d.pinned[C_ERROR * kCtrStride])
