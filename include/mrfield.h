@@ -369,8 +369,8 @@ typedef struct {
 
 typedef struct {
     uint64_t submitted, completed, failed;
-    uint64_t rounds, tile_visits;
-    double kernel_ms;         /* relaxation kernels, HIP events */
+    uint64_t rounds, tile_visits; /* tile_visits: up to the last request completion */
+    double kernel_ms;         /* relaxation kernels, HIP events (up to the last request completion) */
     size_t label_bytes;       /* admission budget */
     size_t label_bytes_peak;  /* most label bytes admitted at once */
     uint32_t inflight_peak;   /* most requests admitted at once */

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -342,10 +343,18 @@ struct RoundDriver {
   FieldDesc* fields = nullptr;
   uint32_t* pool = nullptr;
   BoardView* boards = nullptr;
-  // relaxation counters (kCtrWords) followed by one activity flag per group,
-  // cleared by one fill and read back by one copy per round
+  // relaxation counters (kCtrWords) followed by one activity flag per group
+  // (the last round in which one of its visits activated a tile, plus one)
   uint32_t* ctr = nullptr;
-  uint32_t* pinned = nullptr;  // readback of ctr
+  uint32_t* pinned = nullptr;  // readback of ctr (statistics)
+  // host-mapped words the relaxation publishes to: next count, error flags,
+  // stamp, then the group flags from kDriverFlagsAt
+  uint32_t* hostctr = nullptr;
+  uint32_t* hostctr_dev = nullptr;
+  uint32_t stamp = 0;
+  uint32_t seen_visits = 0, seen_lrounds = 0;
+  std::vector<hipEvent_t> ev_free;
+  std::deque<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
   RangeAlloc fa, ia, la, pa;   // fields, items, label words, pool words
   std::vector<BoardView> views;
   std::vector<const DevBoard*> devs;  // caller-owned boards, by slot
@@ -379,6 +388,9 @@ inline uint32_t driver_next_tiles(const RoundDriver& d) { return d.n_cur + (uint
 // One relaxation round over every active item (new groups' seeds included).
 // Returns the number of items active for the next round.
 uint32_t driver_step(RoundDriver& d);
+// Reads the device's visit and in-LDS round counters and the kernel times of
+// finished rounds into d's statistics (one copy and stream synchronisation).
+void driver_sync_stats(RoundDriver& d);
 // Live groups with no item on the active list: their fields are complete.
 void driver_finished(RoundDriver& d, std::vector<int>& out);
 void driver_paths(RoundDriver& d, int group, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,

@@ -101,6 +101,9 @@ enum : int {
 // behind the pollers.
 constexpr uint32_t kCtrStride = 32;
 constexpr uint32_t kCtrWords = C_NCOUNTERS * kCtrStride;
+// Round driver's host-mapped words: [0] next count, [1] error flags, [2] stamp,
+// group flags from here on.
+constexpr uint32_t kDriverFlagsAt = 16;
 enum : int { C_PAD_ = 0
 };
 
@@ -348,10 +351,14 @@ struct RelaxArgs {
   // form reads it from counters[C_GROUND]).
   uint32_t round;
   // Round driver with several groups live: a visit that activates any tile
-  // for the next round sets group_active[item_group[item]] (its neighbours
-  // are in its own group); null otherwise.
+  // for the next round stores round + 1 in group_active[item_group[item]]
+  // (its neighbours are in its own group), and the launch's last workgroup
+  // copies the n_groups words to host_flags before it publishes the stamp;
+  // null otherwise.
   uint32_t* group_active;
   const uint32_t* item_group;
+  uint32_t* host_flags;
+  uint32_t n_groups;
   // Delta stepping (key_cur non-null): every activation carries a key, the
   // smallest search cost among the border labels that changed; a round
   // processes the items whose key is at most counters[C_THRESH] and carries
@@ -1133,11 +1140,15 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
   } else {
     kmin = activate_sides(A, item, res, key);
   }
+  uint32_t last = 0;  // this workgroup is the launch's last (lane 0)
   if (threadIdx.x == 0) {
     A.act_cur[item] = 0;
-    // read back after the launch ends: a plain store is visible by then
+    // The group flag: an exchange whose result the done increment below
+    // depends on, so it has completed before the last workgroup reads it.
+    // Flags only hold round stamps up to this one, so the increment is 1.
+    uint32_t was = 0;
     if (A.group_active && (defer || (res.flags & (T_LEFT | T_RIGHT | T_TOP | T_BOTTOM | T_CAPPED))))
-      A.group_active[grp] = 1u;
+      was = __hip_atomic_exchange(&A.group_active[grp], A.round + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (A.key_cur) A.key_cur[item] = kMax;
     if (!defer && (res.flags & (T_ANY | T_CAPPED))) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
     if (A.key_cur && kmin != kMax) atomicMin(&A.counters[A.key_ctr * kCtrStride], kmin);
@@ -1148,9 +1159,10 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
       // at the device's coherence point, so the last one reads final counts;
       // a relaxed increment keeps the caches untouched (an acquire-release
       // one would write back and invalidate L2 in every workgroup).
-      const uint32_t prev = __hip_atomic_fetch_add(&A.counters[C_DONE * kCtrStride], 1u, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t prev = __hip_atomic_fetch_add(&A.counters[C_DONE * kCtrStride], was > A.round + 1u ? 2u : 1u,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (prev == gridDim.x - 1) {
+        last = 1;
         __hip_atomic_store(&A.counters[C_DONE * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (A.key_cur) {
           const uint64_t kn = __hip_atomic_load(&A.counters[A.key_ctr * kCtrStride], __ATOMIC_RELAXED,
@@ -1164,6 +1176,23 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
         const uint32_t er = __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&A.host_out[0], nn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(&A.host_out[1], er, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (!A.host_flags) {
+          __hip_atomic_store(&A.host_out[2], A.stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+          __threadfence_system();
+        }
+      }
+    }
+  }
+  if (A.host_flags) {
+    // the first wave of the last workgroup copies the group flags, then
+    // publishes the stamp
+    if (__shfl(last, 0)) {
+      for (uint32_t g = threadIdx.x; g < A.n_groups; g += 64)
+        __hip_atomic_store(&A.host_flags[g],
+                           __hip_atomic_load(&A.group_active[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __threadfence_system();
+      if (threadIdx.x == 0) {
         __hip_atomic_store(&A.host_out[2], A.stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         __threadfence_system();
       }
@@ -2314,6 +2343,10 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   dmalloc(d.boards, max_boards);
   dmalloc(d.ctr, kCtrWords + max_groups);
   HIP_OK(hipHostMalloc((void**)&d.pinned, (kCtrWords + max_groups) * sizeof(uint32_t), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&d.hostctr, (kDriverFlagsAt + max_groups) * sizeof(uint32_t),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  memset(d.hostctr, 0, (kDriverFlagsAt + max_groups) * sizeof(uint32_t));
+  HIP_OK(hipHostGetDevicePointer((void**)&d.hostctr_dev, d.hostctr, 0));
   launch_fill32(c.stream, d.act, 0u, 2 * items);
   launch_fill32(c.stream, d.ctr, 0u, kCtrWords + max_groups);
   HIP_OK(hipStreamSynchronize(c.stream));
@@ -2338,6 +2371,12 @@ void driver_free(RoundDriver& d) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (d.pinned) (void)hipHostFree(d.pinned);
+  if (d.hostctr) (void)hipHostFree(d.hostctr);
+  for (hipEvent_t e : d.ev_free) (void)hipEventDestroy(e);
+  for (auto& pr : d.ev_pending) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
   d = RoundDriver();
 }
 
@@ -2506,12 +2545,16 @@ uint32_t driver_step(RoundDriver& d) {
     return 0;
   }
   // With one live group its fields are active exactly when the next list is
-  // not empty: the per-group flags are neither set nor read back.  One fill
-  // clears the counters and the flags, one copy reads them back.
+  // not empty: no group flags are set or read.  Otherwise a visit stamps its
+  // group's flag with the round (flags need no clearing) and the launch's
+  // last workgroup copies the flags to host-mapped memory.  As in the plain
+  // form, the next-list count alternates between two counters (the kernel
+  // clears the other one), the launch publishes count, error flags and stamp,
+  // and the host spins on the stamp: no fill, copy or stream synchronisation
+  // per round.  Visit and in-LDS round counters accumulate on the device and
+  // are read by driver_sync_stats.
   uint32_t live = 0;
   for (uint32_t g = 0; g < d.max_groups; ++g) live += d.groups[g].live ? 1u : 0u;
-  const size_t ctr_words = kCtrWords + (live > 1 ? d.max_groups : 0u);
-  launch_fill32(s, d.ctr, 0u, ctr_words);
   RelaxArgs A;
   memset(&A, 0, sizeof(A));
   A.boards = d.boards;
@@ -2527,10 +2570,12 @@ uint32_t driver_step(RoundDriver& d) {
   A.counters = d.ctr;
   A.group_active = live > 1 ? d.ctr + kCtrWords : nullptr;
   A.item_group = d.item_group;
+  A.host_flags = live > 1 ? d.hostctr_dev + kDriverFlagsAt : nullptr;
+  A.n_groups = d.max_groups;
   A.max_local = S.max_local;
-  A.next_ctr = C_NEXTN;
-  A.clear_ctr = C_NEXTN;
-  A.host_out = nullptr;
+  A.next_ctr = C_NEXT0 + (uint32_t)(d.rounds & 1);
+  A.clear_ctr = C_NEXT0 + (uint32_t)((d.rounds + 1) & 1);
+  A.host_out = d.hostctr_dev;
   A.key_cur = A.key_next = nullptr;
   A.key_ctr = A.key_clear = C_KEY0;
   A.delta = kMax;
@@ -2546,33 +2591,78 @@ uint32_t driver_step(RoundDriver& d) {
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow);
   const size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   void* args[] = {&A};
-  HIP_OK(hipEventRecord(c.ws.ek0, s));
   if (d.stamp_round >= (1u << 30)) {
-    // states stay below 2^31: every touched item now reads as first visited
-    // in round 0, before every later round
+    // states and flags stay below 2^31: every touched item now reads as
+    // first visited in round 0, before every later round, and no group flag
+    // holds a later round than the next one
     hipLaunchKernelGGL(restamp_states, dim3(1024), dim3(256), 0, s, d.state, (size_t)items);
+    launch_fill32(s, d.ctr + kCtrWords, 0u, d.max_groups);
     d.stamp_round = 1;
   }
   A.round = d.stamp_round++;
+  if (++d.stamp == 0) d.stamp = 1;
+  A.stamp = d.stamp;
+  // kernel time from an event pair per round, read once the pair has passed
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  for (hipEvent_t* e : {&e0, &e1}) {
+    if (!d.ev_free.empty()) {
+      *e = d.ev_free.back();
+      d.ev_free.pop_back();
+    } else {
+      HIP_OK(hipEventCreate(e));
+    }
+  }
+  HIP_OK(hipEventRecord(e0, s));
   HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
-  HIP_OK(hipEventRecord(c.ws.ek1, s));
+  HIP_OK(hipEventRecord(e1, s));
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(d.pinned, d.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  d.staging.clear();
-  if (d.pinned[C_ERROR * kCtrStride])
-    throw Error(1, "device relaxation reported error flags " + std::to_string(d.pinned[C_ERROR * kCtrStride]));
-  float kms = 0;
-  HIP_OK(hipEventElapsedTime(&kms, c.ws.ek0, c.ws.ek1));
-  d.kernel_ms += kms;
+  d.ev_pending.push_back({e0, e1});
+  volatile uint32_t* hc = d.hostctr;
+  const auto spin0 = std::chrono::steady_clock::now();
+  while (hc[2] != A.stamp) {
+    if (std::chrono::steady_clock::now() - spin0 >= std::chrono::microseconds(c.stamp_spin_us)) {
+      HIP_OK(hipStreamSynchronize(s));
+      if (hc[2] != A.stamp) throw Error(1, "relaxation round did not publish its counts");
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  d.staging.clear();  // the launch's stamp follows the staged copies in the stream
+  if (hc[1]) throw Error(1, "device relaxation reported error flags " + std::to_string(hc[1]));
+  while (!d.ev_pending.empty() && hipEventQuery(d.ev_pending.front().second) == hipSuccess) {
+    float kms = 0;
+    HIP_OK(hipEventElapsedTime(&kms, d.ev_pending.front().first, d.ev_pending.front().second));
+    d.kernel_ms += kms;
+    d.ev_free.push_back(d.ev_pending.front().first);
+    d.ev_free.push_back(d.ev_pending.front().second);
+    d.ev_pending.pop_front();
+  }
   d.rounds += 1;
-  d.tile_visits += d.pinned[C_VISITS * kCtrStride];
-  d.local_rounds += d.pinned[C_ROUNDS * kCtrStride];
-  d.n_cur = d.pinned[C_NEXTN * kCtrStride];
+  d.n_cur = hc[0];
   d.cur = 1 - d.cur;
+  const volatile uint32_t* hf = d.hostctr + kDriverFlagsAt;
   for (uint32_t g = 0; g < d.max_groups; ++g)
-    if (d.groups[g].live) d.groups[g].active = live > 1 ? d.pinned[kCtrWords + g] != 0 : d.n_cur != 0;
+    if (d.groups[g].live) d.groups[g].active = live > 1 ? hf[g] == A.round + 1u : d.n_cur != 0;
   return d.n_cur;
+}
+
+void driver_sync_stats(RoundDriver& d) {
+  Ctx& c = *d.c;
+  HIP_OK(hipSetDevice(c.device));
+  HIP_OK(hipMemcpyAsync(d.pinned, d.ctr, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  const uint32_t v = d.pinned[C_VISITS * kCtrStride], r = d.pinned[C_ROUNDS * kCtrStride];
+  d.tile_visits += (uint32_t)(v - d.seen_visits);  // the device counters wrap at 2^32
+  d.local_rounds += (uint32_t)(r - d.seen_lrounds);
+  d.seen_visits = v;
+  d.seen_lrounds = r;
+  while (!d.ev_pending.empty()) {
+    float kms = 0;
+    HIP_OK(hipEventElapsedTime(&kms, d.ev_pending.front().first, d.ev_pending.front().second));
+    d.kernel_ms += kms;
+    d.ev_free.push_back(d.ev_pending.front().first);
+    d.ev_free.push_back(d.ev_pending.front().second);
+    d.ev_pending.pop_front();
+  }
 }
 
 void driver_finished(RoundDriver& d, std::vector<int>& out) {

@@ -334,6 +334,10 @@ void worker_loop(mrf_server* S) {
         n_running = S->running.size();
       }
       mrf::driver_step(S->d);
+      // Requests whose fields have all converged leave now; the device's
+      // visit counters and kernel times are read at these points only.
+      mrf::driver_finished(S->d, fin);
+      if (!fin.empty()) mrf::driver_sync_stats(S->d);
       std::vector<ServeReq*> run;
       {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -346,8 +350,6 @@ void worker_loop(mrf_server* S) {
         r->rounds++;
         r->peers_max = std::max<uint32_t>(r->peers_max, (uint32_t)n_running - 1);
       }
-      // Requests whose fields have all converged leave now.
-      mrf::driver_finished(S->d, fin);
       for (int g : fin) {
         ServeReq* r = nullptr;
         for (ServeReq* x : run)
