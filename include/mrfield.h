@@ -288,6 +288,21 @@ int mrf_route_negotiated_batch(mrf_ctx *ctx, mrf_board *board, const mrf_net *ne
                                const mrf_window *windows, int retry_full, const mrf_negotiation *neg,
                                mrf_paths *out, mrf_error *err);
 
+/* The settle step of one negotiation iteration over the settled paths
+ * (negotiated.rs:1779-1823 merge, :1924-1950 over-use and history), on the
+ * host: present[c] = number of paths holding cell c (each path holds a cell
+ * once); a cell is over-used when paths of two or more distinct groups hold
+ * it; history[c] += scale (saturating at UINT32_MAX) on every over-used cell;
+ * touches[i] = 1 when net i's path holds an over-used cell.  Path i is
+ * path_cells[path_off[i] .. path_off[i + 1]); groups[i] is net i's connection
+ * group.  The over-used cells are written ascending to `overused` (capacity
+ * n_cells) and counted in *n_overused.  No device is used.  It restates the
+ * part of NegotiatedRouter's loop body that follows each batch; the reference
+ * has no separate entry point for it. */
+int mrf_negotiation_settle(size_t n_cells, size_t n_nets, const uint64_t *path_off, const uint32_t *path_cells,
+                           const int64_t *groups, uint32_t scale, uint32_t *present, uint32_t *history,
+                           uint8_t *touches, uint32_t *overused, size_t *n_overused, mrf_error *err);
+
 /* ---- legalization searches ------------------------------------------------------
  * route_legal_once (negotiated.rs:2885-3052) for a batch of nets against one
  * committed snapshot, as the staged legalization's parallel phase runs them

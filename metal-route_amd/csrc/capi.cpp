@@ -813,6 +813,78 @@ int mrf_route_negotiated_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets,
   }
 }
 
+int mrf_negotiation_settle(size_t n_cells, size_t n_nets, const uint64_t* path_off, const uint32_t* path_cells,
+                           const int64_t* groups, uint32_t scale, uint32_t* present, uint32_t* history,
+                           uint8_t* touches, uint32_t* overused, size_t* n_overused, mrf_error* err) {
+  try {
+    if (!present || !history || !overused || !n_overused || (n_nets && (!path_off || !groups || !touches)))
+      throw Error(MRF_BACKEND_UNAVAILABLE, "null negotiation settle argument");
+    const uint64_t total = n_nets ? path_off[n_nets] : 0;
+    if (total && !path_cells) throw Error(MRF_BACKEND_UNAVAILABLE, "null path cells");
+    for (size_t i = 0; i < n_nets; ++i)
+      if (path_off[i] > path_off[i + 1]) throw Error(MRF_BACKEND_UNAVAILABLE, "path offsets must not decrease");
+    for (uint64_t k = 0; k < total; ++k)
+      if (path_cells[k] >= n_cells) throw Error(MRF_BACKEND_UNAVAILABLE, "path cell outside the board");
+    // present: paths per cell (merge, negotiated.rs:1779-1823)
+    memset(present, 0, n_cells * sizeof(uint32_t));
+    for (uint64_t k = 0; k < total; ++k) present[path_cells[k]]++;
+    // Over-use across groups (:1924-1945): cells held by two or more paths
+    // are the candidates; a candidate is over-used when two of its paths
+    // belong to different groups.  Two passes over the paths: the first
+    // records the first group seen at each candidate (first[c]), the second
+    // marks a candidate another group also holds.  The scratch planes stay
+    // zero between calls (only the candidates are reset).
+    thread_local std::vector<int64_t> first;
+    thread_local std::vector<uint8_t> over;  // 2 candidate, 1 over-used
+    if (first.size() < n_cells) {
+      first.assign(n_cells, 0);
+      over.assign(n_cells, 0);
+    }
+    std::vector<uint32_t> cells;
+    for (size_t i = 0; i < n_nets; ++i)
+      for (uint64_t k = path_off[i]; k < path_off[i + 1]; ++k) {
+        const uint32_t c = path_cells[k];
+        if (present[c] >= 2 && !over[c]) {
+          first[c] = groups[i];
+          over[c] = 2;  // candidate, not yet over-used
+          cells.push_back(c);
+        }
+      }
+    for (size_t i = 0; i < n_nets; ++i)
+      for (uint64_t k = path_off[i]; k < path_off[i + 1]; ++k) {
+        const uint32_t c = path_cells[k];
+        if (over[c] == 2 && first[c] != groups[i]) over[c] = 1;
+      }
+    size_t n_over = 0;
+    std::sort(cells.begin(), cells.end());
+    for (uint32_t c : cells) {
+      if (over[c] == 1) {
+        overused[n_over++] = c;
+        const uint64_t h = (uint64_t)history[c] + scale;
+        history[c] = h > UINT32_MAX ? UINT32_MAX : (uint32_t)h;
+      }
+    }
+    // per net: holds an over-used cell (the next iteration's dirty set)
+    for (size_t i = 0; i < n_nets; ++i) {
+      uint8_t t = 0;
+      for (uint64_t k = path_off[i]; k < path_off[i + 1] && !t; ++k) t = over[path_cells[k]] == 1;
+      touches[i] = t;
+    }
+    for (uint32_t c : cells) {
+      first[c] = 0;
+      over[c] = 0;
+    }
+    *n_overused = n_over;
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
 int mrf_route_legal_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets,
                           const mrf_window* windows, int retry_full, const int64_t* owner, const int64_t* halo,
                           size_t n_cells, const int64_t* own_groups, mrf_paths* out, mrf_error* err) {

@@ -159,6 +159,8 @@ def lib():
                                  C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, C.c_size_t, C.POINTER(WindowS), U32P,
                                  U32P, E]),
         "mrf_free_paths": (None, [C.POINTER(PathsS)]),
+        "mrf_negotiation_settle": (C.c_int, [C.c_size_t, C.c_size_t, U64P, U32P, C.POINTER(C.c_int64), C.c_uint32,
+                                             U32P, U32P, U8P, U32P, C.POINTER(C.c_size_t), E]),
         "mrf_board_create": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(EdgesS), C.POINTER(PricingS),
                                        C.POINTER(C.c_void_p), E]),
         "mrf_board_destroy": (None, [C.c_void_p]),
@@ -664,6 +666,32 @@ class Board:
         err = ErrorS()
         _check(lib().mrf_read_field(self.eng._h, self._h, i, _p32(dist), _p32(hops), C.byref(err)), err)
         return dist, hops
+
+
+def negotiation_settle(path_off, path_cells, groups, scale: int, present: np.ndarray, history: np.ndarray):
+    """mrf_negotiation_settle: recount `present` (uint32, in place) from the
+    settled paths (path i = path_cells[path_off[i]:path_off[i + 1]]), add
+    `scale` to `history` (uint32, in place, saturating) on every cell held by
+    two or more groups, and return (over-used cells ascending, per net whether
+    its path holds one)."""
+    if present.dtype != np.uint32 or history.dtype != np.uint32 or not present.flags.c_contiguous \
+            or not history.flags.c_contiguous or present.size != history.size:
+        raise MrfError(BACKEND_UNAVAILABLE, "present and history must be contiguous uint32 planes of one size")
+    off = np.ascontiguousarray(path_off, dtype=np.uint64)
+    cells = np.ascontiguousarray(path_cells, dtype=np.uint32)
+    grp = np.ascontiguousarray(groups, dtype=np.int64)
+    n = off.size - 1
+    if grp.size != n:
+        raise MrfError(BACKEND_UNAVAILABLE, "one group per path")
+    touches = np.zeros(max(n, 1), dtype=np.uint8)
+    over = np.zeros(max(present.size, 1), dtype=np.uint32)
+    n_over = C.c_size_t(0)
+    err = ErrorS()
+    _check(lib().mrf_negotiation_settle(present.size, n, off.ctypes.data_as(U64P), _p32(cells if cells.size else
+                                        np.zeros(1, dtype=np.uint32)), grp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                        int(scale), _p32(present), _p32(history),
+                                        touches.ctypes.data_as(U8P), _p32(over), C.byref(n_over), C.byref(err)), err)
+    return over[:n_over.value].astype(np.int64), touches[:n].astype(bool)
 
 
 NOT_READY = 4

@@ -30,6 +30,8 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
+import mrfield
+
 SCALE = 16                              # negotiated.rs:71 (COST_SCALE)
 MAX_ITERS = 60                          # negotiated.rs:108
 PARALLEL_NEGOTIATION_THRESHOLD = 16     # negotiated.rs:118
@@ -75,7 +77,10 @@ def _concat(paths: List[np.ndarray]):
 
 
 def _overuse(paths: List[np.ndarray], groups: np.ndarray, present: Optional[np.ndarray] = None, flat=None):
-    """Cells occupied by two or more distinct groups, ascending.  `present`
+    """The settle step's over-use in numpy (the loop runs the library's
+    mrf_negotiation_settle; tests hold the two equal).
+
+    Cells occupied by two or more distinct groups, ascending.  `present`
     (the per-cell path count) narrows the candidates to cells held twice;
     when every net is its own group those are exactly the over-used cells.
     flat: _concat(paths) when the caller has it."""
@@ -116,11 +121,10 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
     present = np.zeros(n_cells, dtype=np.uint32)
     empty = np.zeros(0, dtype=np.int64)
     paths: List[np.ndarray] = [empty for _ in range(n_nets)]
-    prev_overused = np.zeros(n_cells, dtype=bool)
+    touch = np.zeros(n_nets, dtype=bool)  # net's settled path holds an over-used cell
     incremental = n_nets > 8
     use_parallel = n_nets > PARALLEL_NEGOTIATION_THRESHOLD
     out = Negotiated([], history, present, 0, False)
-    flat = None  # _concat(paths) of the settled paths, once per iteration
 
     for it in range(max_iters):
         pfac = 1 + it
@@ -130,7 +134,6 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
             if it == 0:
                 dirty = list(range(n_nets))
             else:
-                touch = _touches(paths, prev_overused, flat)
                 dirty = [i for i in range(n_nets) if not len(paths[i]) or touch[i]]
             fused = len(dirty) >= FUSED_JACOBI_MIN_DIRTY_NETS
             if dirty:
@@ -145,14 +148,12 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
                 # holds each cell once, so present after the reference's
                 # in-order saturating updates (remove the old path, add the
                 # new one, net by net) is the per-cell count of the settled
-                # paths: one bincount over all of them.
+                # paths, which the settle step below recounts.
                 for i, p in zip(dirty, got):
                     paths[i] = np.asarray(p, dtype=np.int64) if p is not None else empty
-            flat = _concat(paths)
-            np.copyto(present, np.bincount(flat[0], minlength=n_cells)[:n_cells], casting="unsafe")
         else:
             for i in range(n_nets):
-                if incremental and it > 0 and len(paths[i]) and not bool(prev_overused[paths[i]].any()):
+                if incremental and it > 0 and len(paths[i]) and not touch[i]:
                     continue
                 dirty.append(i)
                 if len(paths[i]):
@@ -163,17 +164,17 @@ def negotiate(route: RouteBatch, n_cells: int, n_nets: int, groups: Optional[Seq
                 if got is not None:
                     paths[i] = np.asarray(got, dtype=np.int64)
                     present[paths[i]] += 1
-            flat = _concat(paths)
-        over = _overuse(paths, grp, present, flat)
+        # Settle (:1924-1950) in the library: present recounted, cells held by
+        # two or more groups, history += SCALE on them, and which nets touch
+        # one (the next iteration's dirty set).
+        off, cells = _flat(paths)
+        over, touched = mrfield.negotiation_settle(off, cells, grp, SCALE, present, history)
         any_overuse = len(over) > 0
-        if any_overuse:
-            history[over] = np.minimum(history[over].astype(np.int64) + SCALE, U32MAX).astype(np.uint32)
         if keep_trace:
             out.trace.append(IterTrace(it, pfac, dirty, fused, [p.tolist() for p in paths], over.tolist(),
                                        any_overuse))
         if incremental:
-            prev_overused[:] = False
-            prev_overused[over] = True
+            touch = touched
         out.iterations = it + 1
         if on_iter is not None:
             on_iter(it, len(dirty))

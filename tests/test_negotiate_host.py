@@ -103,3 +103,47 @@ def test_negotiation_converges_to_disjoint_paths():
     for p in res.paths:
         assert p and not (seen & set(p))
         seen |= set(p)
+
+
+def test_settle_matches_numpy_statement():
+    """mrf_negotiation_settle (host code of the library, no device) against
+    the numpy statement of the same step: present = paths per cell, over-use
+    across groups, saturating history bump, per-net touches."""
+    import mrfield
+
+    rng = np.random.default_rng(5)
+    for case in range(60):
+        n_cells = int(rng.integers(1, 400))
+        n_nets = int(rng.integers(0, 40))
+        paths = []
+        for _ in range(n_nets):
+            k = int(rng.integers(0, min(n_cells, 30) + 1))
+            paths.append(rng.choice(n_cells, size=k, replace=False).astype(np.int64))
+        groups = (rng.integers(0, max(1, n_nets // 2 + 1), size=n_nets) if case % 2
+                  else np.arange(n_nets)).astype(np.int64)
+        if case % 7 == 3 and n_nets:
+            groups[0] = -1  # any int64 group id
+        history = rng.integers(0, 2**32 - 1, size=n_cells, dtype=np.uint64).astype(np.uint32)
+        history[: n_cells // 3] = 2**32 - 1 - rng.integers(0, 20, size=n_cells // 3).astype(np.uint32)
+        present = rng.integers(0, 9, size=n_cells).astype(np.uint32)  # stale values are overwritten
+        off, cells = ng._flat(paths)
+        h2 = history.copy()
+        over, touches = mrfield.negotiation_settle(off, cells, groups, ng.SCALE, present, h2)
+        cnt = np.bincount(np.concatenate(paths) if paths and off[-1] else np.zeros(0, dtype=np.int64),
+                          minlength=n_cells)
+        assert np.array_equal(present, cnt.astype(np.uint32)), case
+        # the numpy statement packs (cell, group) pairs with non-negative
+        # groups: shifting the ids keeps which nets share a group
+        ref_over = ng._overuse(paths, groups - groups.min()) if n_nets else np.zeros(0, dtype=np.int64)
+        assert over.tolist() == ref_over.tolist(), case
+        want = history.astype(np.int64)
+        want[ref_over] = np.minimum(want[ref_over] + ng.SCALE, 2**32 - 1)
+        assert np.array_equal(h2, want.astype(np.uint32)), case
+        mask = np.zeros(n_cells, dtype=bool)
+        mask[ref_over] = True
+        ref_touch = ng._touches(paths, mask) if n_nets else np.zeros(0, dtype=bool)
+        assert touches.tolist() == ref_touch.tolist(), case
+    # malformed input fails closed
+    with pytest.raises(mrfield.MrfError):
+        mrfield.negotiation_settle(np.array([0, 2], dtype=np.uint64), np.array([1, 99], dtype=np.uint32),
+                                   np.array([0]), 16, np.zeros(10, dtype=np.uint32), np.zeros(10, dtype=np.uint32))
