@@ -14,8 +14,11 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
+
+#include <pthread.h>
 
 namespace mrf {
 namespace host {
@@ -54,18 +57,15 @@ class HostPool {
     if (job->error) std::rethrow_exception(job->error);
   }
 
-  ~HostPool() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
-    }
-    cv_work_.notify_all();
-    for (auto& t : workers_) t.join();
-  }
-
+  // One pool per process, never destroyed: parked workers end with the
+  // process, and nothing joins threads during static destruction.
   static HostPool& get() {
-    static HostPool pool;
-    return pool;
+    static HostPool* pool = [] {
+      HostPool* p = new HostPool();
+      pthread_atfork(nullptr, nullptr, [] { get().after_fork_in_child(); });
+      return p;
+    }();
+    return *pool;
   }
 
  private:
@@ -78,6 +78,21 @@ class HostPool {
     std::atomic<size_t> next{0};
     std::exception_ptr error;
   };
+
+  // A forked child has the parent's pool but only the forking thread: the
+  // locks and condition variables may record the parent's waiters, so they
+  // are made new, and the parent's thread handles are set aside (never
+  // joined); the child starts workers of its own on its first job.
+  void after_fork_in_child() {
+    new (&submit_mu_) std::mutex();
+    new (&mu_) std::mutex();
+    new (&cv_work_) std::condition_variable();
+    new (&cv_done_) std::condition_variable();
+    if (!workers_.empty()) new std::vector<std::thread>(std::move(workers_));
+    workers_.clear();
+    current_.reset();
+    stop_ = false;
+  }
 
   void work(Job& job) {
     const bool outer = in_job_;

@@ -1,13 +1,17 @@
 // CPU cases for the library's shared host workers (metal-route_amd/csrc/
 // hostpool.h): every index of a job is processed exactly once, jobs run back
 // to back and from two submitting threads at once, a chunk's exception reaches
-// the caller, and a call made inside a job runs inline.  Exit status 0 when
+// the caller, a call made inside a job runs inline, and a forked child starts
+// workers of its own.  Exit status 0 when
 // every case holds.
 #include <atomic>
 #include <cstdio>
 #include <stdexcept>
 #include <thread>
 #include <vector>
+
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include "../metal-route_amd/csrc/hostpool.h"
 
@@ -65,6 +69,15 @@ int main() {
   for (int rep = 0; rep < 200; ++rep) m_ok = m_ok && once_each(3000, 32);
   other.join();
   expect(m_ok && t_ok.load(), "two submitting threads");
+#if !defined(__SANITIZE_THREAD__)  // the thread sanitizer stops threads started after a fork
+  // a forked child (the parent's workers exist, the child's do not) runs jobs
+  const pid_t child = fork();
+  if (child == 0) _exit(once_each(10000, 16) && once_each(5000, 1) ? 0 : 1);
+  int status = 0;
+  waitpid(child, &status, 0);
+  expect(WIFEXITED(status) && WEXITSTATUS(status) == 0, "forked child");
+  expect(once_each(10000, 16), "parent after a fork");
+#endif
   printf("%s\n", fails ? "hostpool cases failed" : "hostpool cases ok");
   return fails ? 1 : 0;
 }
