@@ -17,8 +17,12 @@ kernel's average launch duration from HIP events recorded around every launch
 on the engine's stream.  ``traffic`` is the measured HBM bytes per launch
 from a committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated
 against a copy of known size).  ``cpu_baseline`` times the CPU oracle (a C
-restatement of mr-cpu, not the Rust reference) on a bounded sample of the
-same nets on the host cores.
+restatement of mr-cpu, not the Rust reference) on bounded samples of the same
+nets, on one thread and on every CPU of the process's affinity mask, in two
+modes (targeted A* with mr-cpu's Manhattan heuristic, and complete fields);
+``seam`` times the provider entry (mrf_route_isolated_batch) with its
+per-call uploads on a fresh context.  --config 0 times the reference's own
+GPU benchmark workload (bench_board) through the Router entry.
 
 --workload serve runs the serving comparison instead (serve_bench.py).
 --workload negotiate runs the negotiated-congestion loop (negotiate.py) on
@@ -51,7 +55,8 @@ def parse():
                     help="fields: one batch per step of --config (default, the headline); "
                          "serve: Poisson request mix through the serving loop vs one request at a time")
     ap.add_argument("--config", type=int, default=2,
-                    help="synthetic config: 1 = 128x128x2/64 nets, 2 = 512x512x4/256 nets (headline), "
+                    help="synthetic config: 0 = the reference's bench_board through the Router entry, "
+                         "1 = 128x128x2/64 nets, 2 = 512x512x4/256 nets (headline), "
                          "3 = 2048x2048x2/1024 windowed nets, 4 = 1024x1024x6/512 multi-pin fields")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
@@ -61,6 +66,7 @@ def parse():
                     help="pick tiles / round cap / form with the library's autotuner (always on for config 3)")
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-seam", action="store_true", help="skip the provider-seam measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (rocprof counter calibration)")
     return ap.parse_args()
@@ -101,79 +107,250 @@ def oracle_spec(s):
     return mrcpu, mrcpu.Spec(og, oe, additive=s.additive, halo_owner=s.halo_owner, halo_penalty=s.halo_penalty)
 
 
+def host_info():
+    """The host's CPUs: count, the affinity mask this process may use, model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = sorted(os.sched_getaffinity(0))
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(aff),
+            "affinity": f"{aff[0]}-{aff[-1]}" if aff and aff[-1] - aff[0] + 1 == len(aff) else ",".join(map(str, aff)),
+            "model": model}
+
+
 def cpu_threads():
-    return max(1, min(os.cpu_count() or 1, 16))
+    """Every CPU this process may run on."""
+    return max(1, len(os.sched_getaffinity(0)))
 
 
-def cpu_baseline_paths(s, got):
-    """The oracle's alone search (window, then full-board retry) over the
-    step's nets in chunks on the host cores until about CPU_SECONDS; the first
-    chunk's paths are compared with the GPU's."""
+def timed_chunks(fn, items, chunk, budget_s):
+    """Run fn over consecutive chunks of items (wrapping) until budget_s has
+    passed (at least one chunk); returns (items done, seconds)."""
+    done, k = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        idx = [(k + j) % len(items) for j in range(chunk)]
+        fn(idx)
+        done += len(idx)
+        k += chunk
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    return done, time.perf_counter() - t0
+
+
+def cpu_baseline_paths(s, got, board):
+    """The oracle on the step's nets, two modes, one thread and every core:
+    (ii) targeted: the alone search (astar_buf with the Manhattan heuristic
+    mr-cpu enables when no cell is free, negotiated.rs:3044-3050; window then
+    full-board retry) + canonical path, the work NegotiatedRouter runs; (i)
+    complete fields: the lexicographic label field of every cell from the
+    source (the bfs_distance_field analogue of sweep.rs:143-148).  Every
+    sampled net's path is compared with the GPU's, and the first complete
+    fields with the GPU's fields."""
+    import numpy as np
+
     mrcpu, spec = oracle_spec(s)
     g = s.grid
     nets = s.nets()
     wins = s.windows or [(0, 0, g.w - 1, g.h - 1)] * len(nets)
-    threads = cpu_threads()
-    chunk = max(threads, 16)
-    done, exact, k = 0, True, 0
-    t0 = time.perf_counter()
-    while True:
-        idx = [(k + j) % len(nets) for j in range(chunk)]
-        on = [mrcpu.Net(nets[i].src, nets[i].dst) for i in idx]
-        paths, _ = mrcpu.alone_batch(spec, on, [wins[i] for i in idx], retry_full=True, threads=threads)
-        if k == 0:
-            for i, p in zip(idx, paths):
+    via = [v for v in s.edges.via if v is not None]
+    has_zero = bool(np.any(np.asarray(g.cost) == 0))  # grid.cost.contains(&0), negotiated.rs:1501
+    tspec = mrcpu.Spec(spec.grid, spec.edges, additive=s.additive, halo_owner=s.halo_owner,
+                       halo_penalty=s.halo_penalty, use_heuristic=not has_zero, min_via_cost=min(via) if via else 0)
+    allc = cpu_threads()
+    modes = {}
+    checked, exact = 0, True
+
+    def targeted(threads):
+        nonlocal checked, exact
+
+        def run(idx):
+            nonlocal checked, exact
+            on = [mrcpu.Net(nets[i].src, nets[i].dst) for i in idx]
+            paths, costs = mrcpu.alone_batch(tspec, on, [wins[i] for i in idx], retry_full=True, threads=threads)
+            for i, p, c in zip(idx, paths, costs):
                 gp = got.path(i)
-                exact = exact and ((p is None and gp is None) or (gp is not None and p == gp[0]))
-        done += len(idx)
-        k += chunk
-        if time.perf_counter() - t0 >= CPU_SECONDS:
-            break
-    dt = time.perf_counter() - t0
+                exact = exact and ((p is None and gp is None) or (gp is not None and (p, c) == gp))
+                checked += 1
+        return timed_chunks(run, list(range(len(nets))), max(threads, 8), CPU_SECONDS / 4)
+
+    def fields(threads):
+        def run(idx):
+            on = [mrcpu.Net(nets[i].src, nets[i].dst) for i in idx]
+            mrcpu.fields_batch(spec, on, [[n.src] for n in on], [wins[i] for i in idx], threads=threads,
+                               want_hops=True)
+        return timed_chunks(run, list(range(len(nets))), threads, CPU_SECONDS / 4)
+
+    for name, fn, unit in (("targeted_astar", targeted, "nets/s"), ("complete_fields", fields, "fields/s")):
+        for t in (1, allc):
+            done, dt = fn(t)
+            modes[f"{name}_{t}t"] = {"value": round(done / dt, 3), "unit": unit, "threads": t,
+                                     "items": done, "seconds": round(dt, 2)}
+    # complete fields of the first nets against the GPU's
+    k = min(4, len(nets))
+    board.solve_fields([[nets[i].src] for i in range(k)], dsts=[nets[i].dst for i in range(k)],
+                       windows=[wins[i] for i in range(k)])
+    on = [mrcpu.Net(nets[i].src, nets[i].dst) for i in range(k)]
+    rd, rh = mrcpu.fields_batch(spec, on, [[n.src] for n in on], [wins[i] for i in range(k)], threads=allc)
+    fexact = True
+    for i in range(k):
+        gd, gh = board.read_field(i)
+        fexact = fexact and np.array_equal(gd, rd[i]) and np.array_equal(gh, rh[i])
+    head = modes[f"targeted_astar_{allc}t"]
     return {
-        "value": round(done / dt, 3),
+        "value": head["value"],
         "unit": "nets/s",
-        "cores": threads,
+        "cores": allc,
         "kind": "port",
-        "sample": f"{done} nets of the same workload in chunks of {chunk}: alone search (complete "
-                  f"lexicographic labels, window then full-board retry) + canonical path, {threads} threads, "
-                  f"{dt:.1f} s",
-        "paths_bit_exact_vs_gpu": bool(exact),
+        "sample": (f"{head['items']} nets of the same workload (step's nets in order, wrapping): oracle alone "
+                   f"search, A* with the Manhattan heuristic {'on' if not has_zero else 'off (zero-cost cells)'}, "
+                   f"window then full-board retry, canonical path, {allc} threads, {head['seconds']} s; "
+                   "modes below also at 1 thread and as complete fields"),
+        "host": host_info(),
+        "modes": modes,
+        "sample_paths_checked": checked,
+        "sample_paths_bit_exact_vs_gpu": bool(exact),
+        "complete_fields_bit_exact_vs_gpu": bool(fexact),
+        "complete_fields_checked": k,
     }
 
 
 def cpu_baseline_fields(s, board):
-    """The oracle's complete multi-source fields for nets in chunks until about
-    CPU_SECONDS; the first chunk's dist and hops planes are compared with the
-    GPU's fields."""
+    """The oracle's complete multi-source fields for the step's nets, on one
+    thread and on every core; the first fields' dist and hops planes are
+    compared with the GPU's."""
     import numpy as np
 
     mrcpu, spec = oracle_spec(s)
-    threads = cpu_threads()
+    allc = cpu_threads()
     pins = s.pins
-    done, exact, k = 0, True, 0
-    t0 = time.perf_counter()
-    while True:
-        idx = [(k + j) % len(pins) for j in range(threads)]
-        nets = [mrcpu.Net(0, 0xFFFFFFFF, own_id=i) for i in idx]  # no target: sources only
-        dist, hops = mrcpu.fields_batch(spec, nets, [pins[i] for i in idx], threads=threads)
-        if k == 0:
-            for j, i in enumerate(idx[:2]):
-                gd, gh = board.read_field(i)
-                exact = exact and np.array_equal(gd, dist[j]) and np.array_equal(gh, hops[j])
-        done += len(idx)
-        k += len(idx)
-        if time.perf_counter() - t0 >= CPU_SECONDS:
-            break
-    dt = time.perf_counter() - t0
+    modes = {}
+    for t in (1, allc):
+        def run(idx):
+            nets = [mrcpu.Net(0, 0xFFFFFFFF, own_id=i) for i in idx]
+            mrcpu.fields_batch(spec, nets, [pins[i] for i in idx], threads=t)
+        done, dt = timed_chunks(run, list(range(len(pins))), t, CPU_SECONDS / 2)
+        modes[f"complete_fields_{t}t"] = {"value": round(done / dt, 3), "unit": "fields/s", "threads": t,
+                                          "items": done, "seconds": round(dt, 2)}
+    k = min(2, len(pins))
+    nets = [mrcpu.Net(0, 0xFFFFFFFF, own_id=i) for i in range(k)]
+    dist, hops = mrcpu.fields_batch(spec, nets, [pins[i] for i in range(k)], threads=allc)
+    exact = True
+    for i in range(k):
+        gd, gh = board.read_field(i)
+        exact = exact and np.array_equal(gd, dist[i]) and np.array_equal(gh, hops[i])
+    head = modes[f"complete_fields_{allc}t"]
     return {
-        "value": round(done / dt, 3),
+        "value": head["value"],
         "unit": "fields/s",
-        "cores": threads,
+        "cores": allc,
         "kind": "port",
-        "sample": f"{done} multi-source fields of the same board, {threads} threads, {dt:.1f} s",
+        "sample": f"{head['items']} multi-source fields of the same board, {allc} threads, {head['seconds']} s",
+        "host": host_info(),
+        "modes": modes,
         "fields_bit_exact_vs_gpu": bool(exact),
+        "fields_checked": k,
     }
+
+
+def seam(s, steps=3):
+    """The provider seam as NegotiatedRouter calls it: mrf_route_isolated_batch
+    on a fresh context (its default form), the board's grid, masks and edge
+    prices passed per call (alone-path prices: the request carries no history
+    plane), full-board windows.  Per call the library builds the directed-edge
+    mask on the host, uploads every plane and frees them again."""
+    import mrfield
+
+    eng = mrfield.Engine(0)
+    g = s.grid
+    nets = s.nets()
+    wins = s.windows or [(0, 0, g.w - 1, g.h - 1)] * len(nets)
+    eng.route_isolated_batch(g, nets, wins, s.edges)
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        eng.route_isolated_batch(g, nets, wins, s.edges)
+        ts.append(time.perf_counter() - t0)
+    st = eng.stats()
+    eng.close()
+    ms = sorted(ts)[len(ts) // 2] * 1e3
+    return {"entry": "mrf_route_isolated_batch", "value": round(len(nets) / (ms * 1e-3), 3), "unit": "nets/s",
+            "ms_per_call_p50": round(ms, 3), "calls": steps, "relax_kernel_ms": round(st["relax_kernel_ms"], 3),
+            "note": "fresh context's default (plain) form; board planes, masks and prices uploaded per call "
+                    "(host-to-device copies included); alone-path pricing (no history plane)"}
+
+
+def config0_main(a):
+    """The reference's own GPU benchmark workload (mr-metal bench_board,
+    crates/mr-metal/src/lib.rs:2761-2906): MetalRouter::route on a 128x128
+    board with vertical walls and 64 nets, warm median of 7 calls.  Here the
+    Router entry, mrf_router_route (board uploaded per call like the
+    reference's), bit-exact against the oracle's Lee search."""
+    import numpy as np
+
+    import mrfield
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mrcpu
+
+    s = mrfield.synth_config(0)
+    g = s.grid
+    nets = s.nets()
+    eng = mrfield.Engine(0)
+    for _ in range(max(a.warmup, 1)):
+        got = eng.router_route(g, nets)
+    ts = []
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        got = eng.router_route(g, nets)
+        ts.append((time.perf_counter() - t0) * 1e3)
+    wall = time.perf_counter() - t_start
+    st = eng.stats()
+    eng.close()
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost)
+    lee = mrcpu.Spec(og, mode=mrcpu.MODE_LEE)
+    buf = mrcpu.SearchBuf(g.n)
+    exact = True
+    c0 = time.perf_counter()
+    for n, r in zip(nets, got):
+        ref = mrcpu.astar(lee, n.src, n.dst, buf)
+        exact = exact and ((ref is None and r is None) or (r is not None and ref is not None and r[0] == ref[0]))
+    cpu_s = time.perf_counter() - c0
+    p50 = float(np.percentile(ts, 50))
+    out = {
+        "metric": "ms per MetalRouter::route call, 128x128 bench_board, 64 nets (warm median)",
+        "value": round(p50, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "reference_m4_warm_median_ms": [3.42, 3.75],
+        "speedup_vs_reference_m4": round(3.42 / p50, 2),
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "reference bench_board: 128x128x1, side/8 single-column walls leaving row 0 "
+                               "open, unit cost, 64 nets from xorshift 0x9E3779B97F4A7C15 (config 0)"},
+        "nets_per_s": round(len(nets) / (p50 * 1e-3), 1),
+        "p99_ms": round(float(np.percentile(ts, 99)), 4),
+        "routed": int(sum(r is not None for r in got)),
+        "relax_kernel_ms": round(st["relax_kernel_ms"], 4),
+        "relax_launches": st["relax_launches"],
+        "paths_bit_exact_vs_oracle_lee": bool(exact),
+        "cpu_baseline": {"value": round(len(nets) / cpu_s, 1), "unit": "nets/s", "cores": 1, "kind": "port",
+                         "sample": f"all {len(nets)} nets: oracle Lee search (targeted, early stop), 1 thread, "
+                                   f"{cpu_s:.2f} s", "host": host_info()},
+    }
+    print(json.dumps(out))
 
 
 def serve_main(a, tiles):
@@ -308,6 +485,9 @@ def main():
     import mrfield
 
     tw, th, threads, ml = (int(v) for v in a.tiles.split(","))
+    if a.config == 0 and a.workload == "fields":
+        config0_main(a)
+        return
     if a.workload == "serve":
         serve_main(a, (tw, th, threads, ml))
         return
@@ -340,7 +520,7 @@ def main():
         return board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
-    kern_ms, launches, changed, valid_ms, call_ms = [], [], [], [], []
+    kern_ms, launches, changed, valid_ms, call_ms, fcells = [], [], [], [], [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
@@ -362,6 +542,7 @@ def main():
             valid_ms.append(st["validate_ms"])
             call_ms.append(st["total_ms"])
             launches.append(st["relax_launches"])
+            fcells.append(st["field_cells"])
     wall = time.perf_counter() - t_start
     n_nets = len(s.pins) if multi else len(nets)
     value = n_nets * a.steps / wall
@@ -374,6 +555,11 @@ def main():
     achieved = cells_per_launch * bpc / (avg_launch_ms * 1e-3) / 1e9
     workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
     kernel = "relax_list" if form == "graph" else "relax_round"
+    traffic = load_traffic(kernel, workload)
+    # compulsory floor of a step: every field cell's dist written once (4 B)
+    # and the board planes read once per field (bpc - 16 B per cell)
+    floor_bytes = float(np.mean(fcells)) * (4 + bpc - 16)
+    kern_s = float(np.mean(kern_ms)) * 1e-3
     if a.config == 2:
         metric, desc = METRIC, "two-pin nets, complete fields + canonical paths per step"
     elif multi:
@@ -424,19 +610,25 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(kernel, workload),
+            "traffic": traffic,
             "kernel": kernel,
             "bytes_per_cell_visit": bpc,
             "cell_visits_per_launch": int(cells_per_launch),
+            "achieved_from_traffic": (round(traffic / (avg_launch_ms * 1e-3) / 1e9, 2) if traffic else None),
+            "traffic_over_algorithmic": (round(traffic / (cells_per_launch * bpc), 3) if traffic else None),
+            "compulsory_bytes_per_step": int(floor_bytes),
+            "compulsory_GBs_over_kernel_time": round(floor_bytes / kern_s / 1e9, 2) if kern_s > 0 else None,
         },
         "cpu_baseline": None,
     }
     if tuning is not None:
         out["autotune"] = tuning
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_fields(s, board) if multi else cpu_baseline_paths(s, got)
+        out["cpu_baseline"] = cpu_baseline_fields(s, board) if multi else cpu_baseline_paths(s, got, board)
     board.close()
     eng.close()
+    if not multi and not a.no_seam:
+        out["seam"] = seam(s)
     print(json.dumps(out))
 
 

@@ -165,8 +165,9 @@ typedef struct {
  *               activated tiles after each round.
  *   PERSISTENT  resident grid with a device barrier between rounds: disabled in
  *               this library, mrf_solve* return MRF_BACKEND_UNAVAILABLE for it.
- *   GRAPH       (default) grid-stride launches over a device work list, rounds
- *               captured in a hipGraph and replayed; one host read per replay. */
+ *   GRAPH       grid-stride launches over a device work list, rounds captured
+ *               in a hipGraph and replayed; one host read per replay.
+ * A new context starts in PLAIN. */
 enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1, MRF_MODE_GRAPH = 2 };
 
 int mrf_abi_version(void);
@@ -186,6 +187,22 @@ int mrf_set_mode(mrf_ctx *ctx, int mode);
  * do not depend on it. */
 int mrf_set_delta(mrf_ctx *ctx, uint32_t delta);
 int mrf_last_stats(mrf_ctx *ctx, mrf_stats *out);
+
+/* Label memory (bytes) one device batch may hold; a call whose fields need
+ * more is split into batches that fit (mr-metal's MAX_BATCH_CELLS /
+ * fields_per_batch chunking, crates/mr-metal/src/lib.rs:251-296, :626-678).
+ * 0 restores the default, min(48 GiB, half of device memory); larger values
+ * are clamped to it.  A single field larger than the budget still runs
+ * alone in its own batch. */
+int mrf_set_label_budget(mrf_ctx *ctx, size_t bytes);
+size_t mrf_get_label_budget(mrf_ctx *ctx);
+
+/* BoardRoute::congestion_from (crates/mr-core/src/lib.rs:575-593), the
+ * congestion plane MetalRouter::route returns beside its paths
+ * (crates/mr-metal/src/lib.rs:1227): out[c] (dims.len() u32) = number of
+ * routed paths holding cell c, each path counting a cell once however often
+ * it visits it.  Host only. */
+int mrf_congestion_from(mrf_dims dims, const mrf_paths *paths, uint32_t *out, mrf_error *err);
 
 /* IsolatedRouteProvider::route_isolated_batch + MetalIsolatedProvider adapter
  * (negotiated.rs:554-560; mr-cli lib.rs:549-625): solve each net inside its

@@ -1131,4 +1131,43 @@ int mrf_read_field(mrf_ctx* ctx, mrf_board* bd, size_t field, uint32_t* dist_out
   }
 }
 
+int mrf_set_label_budget(mrf_ctx* ctx, size_t bytes) {
+  if (!ctx) return MRF_BACKEND_UNAVAILABLE;
+  std::lock_guard<std::mutex> g(ctx->c.lane);
+  ctx->c.label_budget = bytes ? std::min(bytes, ctx->c.label_budget_default) : ctx->c.label_budget_default;
+  return MRF_OK;
+}
+
+size_t mrf_get_label_budget(mrf_ctx* ctx) { return ctx ? ctx->c.label_budget : 0; }
+
+int mrf_congestion_from(mrf_dims dims, const mrf_paths* paths, uint32_t* out, mrf_error* err) {
+  try {
+    const size_t n = dims_len(dims);
+    if (!paths || (n && !out)) throw Error(MRF_BACKEND_UNAVAILABLE, "null paths or output");
+    std::fill(out, out + n, 0u);
+    // Per path, a cell counts once however often the path visits it: the
+    // stamp of the last path that counted the cell.
+    std::vector<uint32_t> seen(n, 0);
+    for (size_t i = 0; i < paths->n; ++i) {
+      if (!paths->routed[i]) continue;
+      const uint32_t stamp = (uint32_t)i + 1;
+      for (uint64_t k = paths->offsets[i]; k < paths->offsets[i + 1]; ++k) {
+        const uint32_t c = paths->cells[k];
+        if (c >= n) throw Error(MRF_BACKEND_UNAVAILABLE, "path cell outside the board");
+        if (seen[c] != stamp) {
+          seen[c] = stamp;
+          ++out[c];
+        }
+      }
+    }
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
 }  // extern "C"

@@ -230,10 +230,11 @@ struct Ctx {
   Workspace ws;
   GraphCache graph;
   TileShape shape;
-  int mode = MODE_GRAPH;
+  int mode = MODE_PLAIN;
   uint32_t graph_rounds = 8;  // rounds captured per hipGraph replay
   uint32_t delta = kMax;      // plain form: delta-stepping window in search cost (kMax = off)
-  size_t label_budget = size_t(48) << 30;  // bytes of labels per chunk
+  size_t label_budget = size_t(48) << 30;  // bytes of labels per chunk (set: mrf_set_label_budget)
+  size_t label_budget_default = size_t(48) << 30;  // min(48 GiB, half of device memory)
   int cus = 256;
   SolveStats last;
   std::vector<FieldDesc> layout;  // fields of the last solve (for paths / reads)

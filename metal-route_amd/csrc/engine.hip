@@ -1589,6 +1589,7 @@ void ctx_init(Ctx& c, int device) {
   HIP_OK(hipEventCreate(&c.ws.ev2));
   size_t total = prop.totalGlobalMem;
   c.label_budget = std::min<size_t>(c.label_budget, total / 2);
+  c.label_budget_default = c.label_budget;
   HIP_OK(hipMalloc((void**)&c.ws.counters, kCtrWords * sizeof(uint32_t)));
   HIP_OK(hipHostMalloc((void**)&c.ws.pinned, kCtrWords * sizeof(uint32_t), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&c.ws.hostctr, 64, hipHostMallocMapped | hipHostMallocCoherent));
@@ -1985,7 +1986,10 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     field_cells += (uint64_t)F.ww * F.wh * L;
     layout.push_back(F);
   }
-  if (lab_off * 8 > c.label_budget) throw Error(1, "label working set exceeds the device budget");
+  // A batch stays within the label budget; a single field larger than the
+  // budget runs alone, up to the default budget (half of device memory).
+  if ((reqs.size() > 1 && lab_off * 8 > c.label_budget) || lab_off * 8 > c.label_budget_default)
+    throw Error(1, "label working set exceeds the device budget");
 
   Workspace& W = c.ws;
   grow(W.labels, W.labels_cap, std::max<size_t>(lab_off, 1));

@@ -159,6 +159,9 @@ def lib():
                                  C.POINTER(U32P), C.POINTER(C.c_size_t), U32P, C.c_size_t, C.POINTER(WindowS), U32P,
                                  U32P, E]),
         "mrf_free_paths": (None, [C.POINTER(PathsS)]),
+        "mrf_set_label_budget": (C.c_int, [C.c_void_p, C.c_size_t]),
+        "mrf_get_label_budget": (C.c_size_t, [C.c_void_p]),
+        "mrf_congestion_from": (C.c_int, [Dims, C.POINTER(PathsS), U32P, E]),
         "mrf_negotiation_settle": (C.c_int, [C.c_size_t, C.c_size_t, U64P, U32P, C.POINTER(C.c_int64), C.c_uint32,
                                              U32P, U32P, U8P, U32P, C.POINTER(C.c_size_t), E]),
         "mrf_board_create": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(EdgesS), C.POINTER(PricingS),
@@ -449,6 +452,14 @@ class Engine:
         if lib().mrf_set_delta(self._h, delta) != OK:
             raise MrfError(BACKEND_UNAVAILABLE, "bad delta")
 
+    def set_label_budget(self, nbytes: int):
+        """Label bytes one device batch may hold (0 = the library default)."""
+        if lib().mrf_set_label_budget(self._h, int(nbytes)) != OK:
+            raise MrfError(BACKEND_UNAVAILABLE, "bad label budget")
+
+    def label_budget(self) -> int:
+        return int(lib().mrf_get_label_budget(self._h))
+
     def stats(self) -> dict:
         s = StatsS()
         lib().mrf_last_stats(self._h, C.byref(s))
@@ -506,6 +517,25 @@ class Engine:
         _check(lib().mrf_fields(self._h, grid.dims(), C.byref(g), C.byref(e), C.byref(pr) if pr else None, sp, ns,
                                 _p32(d), n, _windows(windows, n), _p32(dist), _p32(hops), C.byref(err)), err)
         return dist[:n], (hops[:n] if hops is not None else None)
+
+
+def congestion_from(dims: Tuple[int, int, int], paths) -> np.ndarray:
+    """mrf_congestion_from over a list of paths (each a list of cells, a
+    (cells, cost) pair, or None for an unrouted net)."""
+    w, h, L = dims
+    cells_l = [(list(p[0]) if isinstance(p, tuple) else list(p)) if p is not None else [] for p in paths]
+    n = len(cells_l)
+    routed = np.array([1 if p is not None else 0 for p in paths] + [0], dtype=np.uint8)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    for i, c in enumerate(cells_l):
+        off[i + 1] = off[i] + len(c)
+    flat = _u32([v for c in cells_l for v in c] + [0])
+    cost = np.zeros(n + 1, dtype=np.uint32)
+    ps = PathsS(n, routed.ctypes.data_as(U8P), _p32(cost), off.ctypes.data_as(U64P), _p32(flat))
+    out = np.zeros(max(w * h * L, 1), dtype=np.uint32)
+    err = ErrorS()
+    _check(lib().mrf_congestion_from(Dims(w, h, L), C.byref(ps), _p32(out), C.byref(err)), err)
+    return out[: w * h * L]
 
 
 class Board:

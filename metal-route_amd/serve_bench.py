@@ -134,8 +134,22 @@ def run(seed: int = 1, n_boards: int = 8, n_req: int = 96, loads=(0.5, 1.0, 2.0,
         p_lat, p_span = replay_fifo(arr, service)
         s_lat, s_span, s_res, peers = run_server(srv, ids, reqs, arr)
         equal = equal and all(same_paths(x, y) for x, y in zip(plain_res, s_res))
+        classes = {}
+        for name, sides in (("small_128_256", (128, 256)), ("large_512_1024", (512, 1024))):
+            sel = [i for i, (b, _) in enumerate(reqs) if boards[b].grid.w in sides]
+            if not sel:
+                continue
+            classes[name] = {
+                "requests": len(sel),
+                "nets": int(sum(len(reqs[i][1]) for i in sel)),
+                "plain_p50_ms": round(float(np.percentile(p_lat[sel], 50)) * 1e3, 2),
+                "plain_p99_ms": round(float(np.percentile(p_lat[sel], 99)) * 1e3, 2),
+                "serve_p50_ms": round(float(np.percentile(s_lat[sel], 50)) * 1e3, 2),
+                "serve_p99_ms": round(float(np.percentile(s_lat[sel], 99)) * 1e3, 2),
+            }
         rows.append({
             "load": load,
+            "by_board_size": classes,
             "lambda_req_s": round(lam, 3),
             "plain": {"nets_s": round(total_nets / p_span, 1),
                       "p50_ms": round(float(np.percentile(p_lat, 50)) * 1e3, 2),

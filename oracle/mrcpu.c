@@ -53,6 +53,10 @@
  *                                (randomized like :6665-6800); plus brute-force
  *                                simple-path enumeration under the restated
  *                                price on tiny grids.
+ *   mrc_congestion_from          mr-core's vectors [1,2,1] and, with a loopy
+ *                                path, [1,2,2,1] (mr-core/src/lib.rs:856-891),
+ *                                and equal to a brute count of distinct cells
+ *                                per path on random path sets.
  */
 #include "mrcpu.h"
 
@@ -1060,4 +1064,27 @@ void mrc_fields_batch(const mrc_spec *tmpl, const mrc_net *nets, size_t n_nets,
     j.dist_out = dist_out;
     j.hops_out = hops_out;
     run_workers(&j, n_threads, field_worker);
+}
+
+/* mr-core/src/lib.rs:575-593: a fresh seen-set per path (the reference's
+ * HashSet sized to the path), here a sorted copy of the path's cells. */
+static int cmp_u32(const void *a, const void *b) {
+    uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+void mrc_congestion_from(mrc_dims d, size_t n_paths, const uint64_t *off, const uint32_t *cells,
+                         uint32_t *out) {
+    size_t n = dims_len(d);
+    memset(out, 0, n * sizeof(uint32_t));
+    for (size_t i = 0; i < n_paths; ++i) {
+        size_t len = (size_t)(off[i + 1] - off[i]);
+        if (!len) continue;
+        uint32_t *tmp = (uint32_t *)malloc(len * sizeof(uint32_t));
+        memcpy(tmp, cells + off[i], len * sizeof(uint32_t));
+        qsort(tmp, len, sizeof(uint32_t), cmp_u32);
+        for (size_t k = 0; k < len; ++k)
+            if ((k == 0 || tmp[k] != tmp[k - 1]) && tmp[k] < n) out[tmp[k]] += 1;
+        free(tmp);
+    }
 }

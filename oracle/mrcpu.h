@@ -205,6 +205,12 @@ void mrc_fields_batch(const mrc_spec *tmpl, const mrc_net *nets, size_t n_nets,
                       const mrc_window *windows, int n_threads, uint32_t *dist_out,
                       uint32_t *hops_out);
 
+/* mr-core/src/lib.rs:575-593 BoardRoute::congestion_from: out[c] = number of
+ * paths that hold cell c, each path counting a cell once (a loopy path is one
+ * route).  Path i is cells[off[i] .. off[i+1]). */
+void mrc_congestion_from(mrc_dims d, size_t n_paths, const uint64_t *off, const uint32_t *cells,
+                         uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,6 +152,7 @@ def lib():
         L.mrc_fields_batch.argtypes = [
             C.POINTER(SpecS), C.POINTER(NetS), C.c_size_t, C.POINTER(U32P), C.POINTER(C.c_size_t),
             C.POINTER(WindowS), C.c_int, U32P, U32P]
+        L.mrc_congestion_from.argtypes = [Dims, C.c_size_t, C.POINTER(C.c_uint64), U32P, U32P]
         _lib = L
     return _lib
 
@@ -484,3 +485,18 @@ def fields_batch(tmpl: Spec, nets: Sequence[Net], sources: Sequence[Sequence[int
     lib().mrc_fields_batch(C.byref(tmpl.s), arr, nn, sp, ns, wins, threads, _p32(dist),
                            _p32(hops) if hops is not None else None)
     return dist[:nn], (hops[:nn] if hops is not None else None)
+
+
+def congestion_from(dims, paths) -> np.ndarray:
+    """BoardRoute::congestion_from (mr-core/src/lib.rs:575-593) over a list of
+    paths (lists of cells; None = unrouted)."""
+    w, h, L = dims
+    cells_l = [list(p) if p is not None else [] for p in paths]
+    off = np.zeros(len(cells_l) + 1, dtype=np.uint64)
+    for i, c in enumerate(cells_l):
+        off[i + 1] = off[i] + len(c)
+    flat = _u32([v for c in cells_l for v in c] + [0])
+    out = np.zeros(max(w * h * L, 1), dtype=np.uint32)
+    lib().mrc_congestion_from(Dims(w, h, L), len(cells_l), off.ctypes.data_as(C.POINTER(C.c_uint64)), _p32(flat),
+                              _p32(out))
+    return out[: w * h * L]

@@ -127,3 +127,35 @@ def test_synth_reference_bench_board():
         assert g.cost[g.w * 63 + x] == mrfield.MAX
         assert g.cost[g.w * 64 + x] == 1
     assert len(s.src) == 64
+
+
+# mr-core/src/lib.rs:856-891: congestion counts occupying routes, once per
+# route and cell, whatever the path's shape.
+CONGESTION_VECTORS = [
+    ((3, 1, 1), [[0, 1], [1, 2]], [1, 2, 1]),
+    ((4, 1, 1), [[0, 1, 2, 1, 0, 1], [1, 2, 3, 2]], [1, 2, 2, 1]),
+]
+
+
+@pytest.mark.parametrize("dims,paths,expect", CONGESTION_VECTORS)
+def test_congestion_from_reference_vectors(dims, paths, expect):
+    assert mrfield.congestion_from(dims, paths).tolist() == expect
+
+
+def test_congestion_from_skips_unrouted_and_matches_the_oracle():
+    import mrcpu
+
+    rng = np.random.default_rng(3)
+    dims = (7, 5, 2)
+    n = 7 * 5 * 2
+    for _ in range(50):
+        paths = [None if rng.random() < 0.2 else [int(c) for c in rng.integers(0, n, size=int(rng.integers(1, 30)))]
+                 for _ in range(int(rng.integers(0, 12)))]
+        want = np.zeros(n, dtype=np.uint32)
+        for p in paths:
+            for c in set(p or []):
+                want[c] += 1
+        assert np.array_equal(mrfield.congestion_from(dims, paths), want)
+        assert np.array_equal(mrcpu.congestion_from(dims, paths), want)
+    with pytest.raises(mrfield.MrfError):
+        mrfield.congestion_from((2, 1, 1), [[0, 5]])

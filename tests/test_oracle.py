@@ -360,3 +360,10 @@ def test_provider_validation_rejects_illegal_paths():
     assert mrcpu.provider_path_is_valid(g, net3, e, via_path)
     e_none = mrcpu.Edges([16, 16], [16], [None])
     assert not mrcpu.provider_path_is_valid(g, net3, e_none, via_path)
+
+
+def test_congestion_from_reference_vectors():
+    """mr-core/src/lib.rs:856-891 (congestion_counts_overlaps,
+    congestion_counts_each_route_once_per_cell)."""
+    assert mrcpu.congestion_from((3, 1, 1), [[0, 1], [1, 2]]).tolist() == [1, 2, 1]
+    assert mrcpu.congestion_from((4, 1, 1), [[0, 1, 2, 1, 0, 1], [1, 2, 3, 2]]).tolist() == [1, 2, 2, 1]
