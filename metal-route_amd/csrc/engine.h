@@ -36,8 +36,10 @@ enum : uint32_t {
 
 struct BoardView {
   uint32_t w, h, layers, plane;
-  const uint32_t* cost;     // enter weight; kMax = obstacle
-  const uint32_t* add;      // additive price plane or nullptr
+  const uint32_t* cost;     // enter weight; kMax = obstacle; nullptr when every
+                            // passable cell weighs 1 (obstacles: M_OBST)
+  const uint32_t* add;      // additive price plane (u32) or nullptr
+  const uint16_t* add16;    // additive price plane (u16) or nullptr; at most one of add / add16
   const int32_t* halo;      // halo owner plane (-1 free) or nullptr
   uint32_t halo_pen;
   const uint8_t* bmask;     // M_* bits
@@ -85,8 +87,9 @@ struct TileShape {
 
 struct DevBoard {
   uint32_t w = 0, h = 0, layers = 0;
-  uint32_t* cost = nullptr;
-  uint32_t* add = nullptr;
+  uint32_t* cost = nullptr;     // nullptr: every passable cell weighs 1
+  uint32_t* add = nullptr;      // additive prices as u32 ...
+  uint16_t* add16 = nullptr;    // ... or as u16 when all fit 16 bits
   int32_t* halo = nullptr;
   uint32_t halo_pen = 0;
   uint8_t* bmask = nullptr;
@@ -322,6 +325,7 @@ class RangeAlloc {
 struct Group {
   bool live = false;
   bool narrow = false;  // every step price fits the narrow step planes
+  bool lean = false;    // no halo-owner / legal planes, overrides, pads or via pads
   uint64_t field0 = 0, nfields = 0;
   uint64_t item0 = 0, nitems = 0;
   uint64_t lab0 = 0, labwords = 0;

@@ -115,6 +115,7 @@ BoardView DevBoard::view() const {
   v.plane = w * h;
   v.cost = cost;
   v.add = add;
+  v.add16 = add16;
   v.halo = halo;
   v.halo_pen = halo_pen;
   v.bmask = bmask;
@@ -206,8 +207,24 @@ typedef u8x4 u8x4_a1 __attribute__((aligned(1)));
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef u64x2 u64x2_a8 __attribute__((aligned(8)));
 
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+typedef u16x4 u16x4_a2 __attribute__((aligned(2)));
+
 __device__ __forceinline__ u32x4 load4(const uint32_t* p) { return *(const __attribute__((address_space(1))) u32x4_a4*)p; }
+__device__ __forceinline__ u16x4 load4(const uint16_t* p) { return *(const __attribute__((address_space(1))) u16x4_a2*)p; }
 __device__ __forceinline__ u8x4 load4(const uint8_t* p) { return *(const __attribute__((address_space(1))) u8x4_a1*)p; }
+
+// Enter weight of cell g whose board mask is bm: the weight plane's value,
+// or, on a board without one (every passable cell weighs 1), 1, and kMax for
+// an obstacle.
+__device__ __forceinline__ uint32_t weight_at(const BoardView& B, uint32_t g, uint8_t bm) {
+  return B.cost ? gbl(B.cost)[g] : ((bm & M_OBST) ? kMax : 1u);
+}
+
+// Additive price of cell g from the board's plane (u32 or u16), 0 without one.
+__device__ __forceinline__ uint32_t add_at(const BoardView& B, uint32_t g) {
+  return B.add ? gbl(B.add)[g] : B.add16 ? (uint32_t)gbl(B.add16)[g] : 0u;
+}
 
 __device__ __forceinline__ bool sorted_has(const uint32_t* a, uint32_t n, uint32_t v) {
   uint32_t lo = 0, hi = n;
@@ -253,7 +270,7 @@ __device__ __forceinline__ uint32_t override_index(const FieldDesc& F, const uin
 // penalty.
 __device__ __forceinline__ uint32_t add_price(const BoardView& b, const FieldDesc& F, const uint32_t* pool,
                                               uint32_t g) {
-  uint64_t a = b.add ? b.add[g] : 0u;
+  uint64_t a = b.add ? b.add[g] : b.add16 ? b.add16[g] : 0u;
   if (F.n_ovr) {
     const uint32_t k = override_index(F, pool, g);
     if (k != kMax) a = pool[F.ovr_off + F.n_ovr + k];
@@ -321,8 +338,8 @@ __device__ CellInfo cell_info(const BoardView& b, const FieldDesc& F, const uint
   if (pass) m |= K_PASS;
   if (F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(b, F, pool, g));
   ci.mask = m;
-  ci.w = (obst && pad) ? 1u : b.cost[g];
-  ci.a = (b.add || b.halo || F.n_ovr) ? add_price(b, F, pool, g) : 0u;
+  ci.w = obst ? (pad ? 1u : kMax) : (b.cost ? b.cost[g] : 1u);
+  ci.a = (b.add || b.add16 || b.halo || F.n_ovr) ? add_price(b, F, pool, g) : 0u;
   return ci;
 }
 
@@ -631,7 +648,11 @@ __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ?
 
 // Load, relax to an in-LDS fixpoint, and write back one (field, tile) item.
 // Returns T_* flags (valid in every thread).
-template <int TW, int TH, int LM, bool NARROW>
+// LEAN: the launch's boards carry no halo-owner or legal-search planes and its
+// fields no additive overrides, own pads or via-exempt pads (forced
+// endpoints remain), so none of that code or its descriptor fields exist in
+// the kernel: fewer scalar registers held across the visit, smaller code.
+template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem, uint32_t round) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = const_load(A.item_field + item);
@@ -721,8 +742,8 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     if (j < nhalo) halo_entry(j, src, hat[u], hside[u], hok[u], hgc[u]);
     hv[u] = __hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     hb[u] = gbl(B.bmask)[hgc[u]];
-    hw[u] = gbl(B.cost)[hgc[u]];
-    ha[u] = B.add ? gbl(B.add)[hgc[u]] : 0u;
+    hw[u] = weight_at(B, hgc[u], hb[u]);
+    ha[u] = add_at(B, hgc[u]);
   }
   // Interior planes by quads: four consecutive cells of one row per thread
   // per batch, loaded with one vector load per plane when the quad lies
@@ -738,7 +759,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   uint8_t lg[4];  // legal-search block codes
   bool inw[4];
   uint64_t lv[4];
-  const bool legal_planes = B.owner || B.lhalo;
+  const bool legal_planes = !LEAN && (B.owner || B.lhalo);
   auto load_planes = [&](uint32_t q) {
     qvalid = q < nquad;
     const uint32_t qq = qvalid ? q : 0;
@@ -755,13 +776,19 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     if (inw[3]) {
       const uint32_t g0 = qrowg + qwxb;
       const u8x4 b4 = load4(B.bmask + g0);
-      const u32x4 c4 = load4(B.cost + g0);
-      const u32x4 a4 = B.add ? load4(B.add + g0) : u32x4{0, 0, 0, 0};
-      const u32x4 o4 = B.halo ? load4((const uint32_t*)B.halo + g0) : u32x4{kMax, kMax, kMax, kMax};
+      const u32x4 c4 = B.cost ? load4(B.cost + g0) : u32x4{1, 1, 1, 1};
+      u32x4 a4 = u32x4{0, 0, 0, 0};
+      if (B.add) {
+        a4 = load4(B.add + g0);
+      } else if (B.add16) {
+        const u16x4 h4 = load4(B.add16 + g0);
+        a4 = u32x4{h4[0], h4[1], h4[2], h4[3]};
+      }
+      const u32x4 o4 = (!LEAN && B.halo) ? load4((const uint32_t*)B.halo + g0) : u32x4{kMax, kMax, kMax, kMax};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         bv[k] = b4[k];
-        wv[k] = c4[k];
+        wv[k] = (B.cost || !(bv[k] & M_OBST)) ? c4[k] : kMax;
         av[k] = a4[k];
         ov[k] = (int32_t)o4[k];
       }
@@ -770,9 +797,9 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
       for (int k = 0; k < 4; ++k) {
         const uint32_t g = qrowg + min(qwxb + k, F.ww - 1);
         bv[k] = gbl(B.bmask)[g];
-        wv[k] = gbl(B.cost)[g];
-        av[k] = B.add ? gbl(B.add)[g] : 0u;
-        ov[k] = B.halo ? gbl(B.halo)[g] : -1;
+        wv[k] = weight_at(B, g, bv[k]);
+        av[k] = add_at(B, g);
+        ov[k] = (!LEAN && B.halo) ? gbl(B.halo)[g] : -1;
       }
     }
     if (legal_planes) {
@@ -847,13 +874,13 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
       bool forced = false;
       if (F.flags & F_FORCE)
         for (uint32_t k = 0; k < F.n_forced; ++k) forced = forced || A.pool[F.forced_off + k] == gc;
-      if (F.n_pad && sorted_has(hpads, F.n_pad, gc))
+      if (!LEAN && F.n_pad && sorted_has(hpads, F.n_pad, gc))
         wgt = 1u;
       else if (!forced)
         return kMax;
     }
     const uint32_t e = side == 0 ? s.xg[0] : side == 1 ? s.xg[TW] : side == 2 ? s.yg[0] : s.yg[TH];
-    const uint32_t p = step_price(e, wgt, F.n_ovr ? 0u : a);
+    const uint32_t p = step_price(e, wgt, (!LEAN && F.n_ovr) ? 0u : a);
     return NARROW ? min(p, kNarrowMaxStep) : p;  // a smaller bound stays a bound
   };
 #pragma unroll
@@ -871,13 +898,15 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
                     ? lab_in(__hip_atomic_load(&A.labels[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     : kUnreached;
     s.hstep[j] =
-        to_step<ST>(ex ? entry_price(side, gc, gbl(B.bmask)[gc], gbl(B.cost)[gc], B.add ? gbl(B.add)[gc] : 0u) : kMax);
+        to_step<ST>(ex ? entry_price(side, gc, gbl(B.bmask)[gc], weight_at(B, gc, gbl(B.bmask)[gc]), add_at(B, gc))
+                       : kMax);
   }
 
   // Interior cells: labels (when the tile was visited before), the per-net
   // view of each cell (cell_info: own pads, forced endpoints, via exemptions,
   // additive and halo pricing) and the six step prices of entering it.
   const uint32_t* pads = A.pool + F.pad_off;
+  const bool unit_w = B.cost == nullptr;
   // The six step prices of entering cell (l, y, x) = global g with board mask
   // bm, cost weight w0, additive price a0 and halo owner o.
   auto cell_steps = [&](uint32_t l, uint32_t y, uint32_t x, uint32_t g, uint8_t bm, uint32_t w0, uint32_t a0,
@@ -886,7 +915,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     uint8_t m = (uint8_t)((bm >> 1) & 0x7E);
     bool pass = !trace && !obst;
     uint32_t wgt = w0;
-    if (obst && !trace && F.n_pad && sorted_has(pads, F.n_pad, g)) {
+    if (!LEAN && obst && !trace && F.n_pad && sorted_has(pads, F.n_pad, g)) {
       pass = true;
       wgt = 1u;
     }
@@ -896,15 +925,18 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
         if (A.pool[F.forced_off + j] == g) pass = true;
     }
     if (legal & 1u) pass = false;  // foreign copper: closed even at the endpoints
-    if (pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
+    if (!LEAN && pass && F.n_vpad) m = (uint8_t)((m & ~(K_DN | K_UP)) | via_bits(B, F, A.pool, g));
     uint64_t a64 = a0;
-    if (o != -1 && o != F.own_id) a64 += B.halo_pen;
+    if (!LEAN && o != -1 && o != F.own_id) a64 += B.halo_pen;
     const uint32_t ap = a64 < 0xFFFFFFFFull ? (uint32_t)a64 : 0xFFFFFFFFu;
     const uint32_t pc = l * STEP_PLANE + y * PS + x;
     // Narrow boards bound every open step price below 0x7FFF, so it is
-    // exact in 32-bit arithmetic without the saturation of step_price.
+    // exact in 32-bit arithmetic without the saturation of step_price; on a
+    // board without a weight plane every open cell of a narrow field weighs
+    // 1 (a forced endpoint on an obstacle makes the field wide), so the
+    // price is e + a.
     auto g6 = [&](bool open, uint32_t e) -> ST {
-      if (NARROW) return to_step<ST>(open ? e * wgt + ap : kMax);
+      if (NARROW) return to_step<ST>(open ? (unit_w ? e : e * wgt) + ap : kMax);
       return to_step<ST>(gate(open, e, wgt, ap));
     };
     s.sxl[pc] = g6(pass && (m & K_NX), s.xg[x]);
@@ -931,7 +963,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   __syncthreads();
   // The net's own additive overrides inside this tile (negotiated batches):
   // their cells are priced again over the board-plane prices just written.
-  if (F.n_ovr) {
+  if (!LEAN && F.n_ovr) {
     const uint32_t* tab = A.pool + F.ovr_off + 2 * F.n_ovr;
     const uint32_t o0 = uniform_u32(tab[t]), o1 = uniform_u32(tab[t + 1]);
     if (o1 > o0) {
@@ -940,7 +972,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
       for (uint32_t j = o0 + tid; j < o1; j += nt) {
         const uint32_t g = tc[j], a = tc[F.n_ovr + j];
         const uint32_t l = g / plane, r = g - l * plane, gy = r / B.w, gx = r - gy * B.w;
-        cell_steps(l, gy - F.y0 - wy0, gx - F.x0 - wx0, g, gbl(B.bmask)[g], gbl(B.cost)[g], a,
+        cell_steps(l, gy - F.y0 - wy0, gx - F.x0 - wx0, g, gbl(B.bmask)[g], weight_at(B, g, gbl(B.bmask)[g]), a,
                    B.halo ? gbl(B.halo)[g] : -1,
                    legal_planes ? legal_code(B.owner ? gbl(B.owner)[g] : -1, B.lhalo ? gbl(B.lhalo)[g] : -1,
                                              F.own_group)
@@ -1112,7 +1144,7 @@ struct MinWaves {
   static constexpr int list = TW >= 32 ? 2 : 4;
 };
 
-template <int TW, int TH, int LM, bool NARROW>
+template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1131,7 +1163,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
                                                 __HIP_MEMORY_SCOPE_AGENT));
   }
   TileResult res{};
-  if (!defer) res = process_tile<TW, TH, LM, NARROW>(A, item, smem, A.round);
+  if (!defer) res = process_tile<TW, TH, LM, NARROW, LEAN>(A, item, smem, A.round);
   if (threadIdx.x >= 64) return;
   uint32_t kmin = kMax;
   if (defer) {
@@ -1211,7 +1243,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
 // coherent labels; no workgroup ever waits on another.  A launch whose list is
 // empty returns at once, so a captured sequence can overshoot the fixpoint.
 
-template <int TW, int TH, int LM, bool NARROW>
+template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(RelaxArgs E, RelaxArgs O) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t r =
@@ -1227,7 +1259,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
   // the compiler so lanes other than 0 re-run the body without a new pop.
   for (uint32_t k = blockIdx.x; k < n_cur; k += gridDim.x) {
     uint32_t item = const_load(R.list_cur + k);
-    const TileResult res = process_tile<TW, TH, LM, NARROW>(R, item, smem, r);
+    const TileResult res = process_tile<TW, TH, LM, NARROW, LEAN>(R, item, smem, r);
     if (threadIdx.x < 64) {
       if (threadIdx.x == 0) R.act_cur[item] = 0;
       activate_sides(R, item, res, res.ka);
@@ -1509,31 +1541,36 @@ static T* dev_copy(const T* h, size_t n, hipStream_t s) {
   return d;
 }
 
-// Instantiated tile shapes (width x height), layer bounds and step widths.
+// Instantiated tile shapes (width x height), layer bounds, step widths and
+// lean variants.
 struct KernelSet {
   uint32_t tw, th, lm;
-  bool narrow;
+  bool narrow, lean;
   const void* round;
   const void* list;
 };
 
-template <int TW, int TH, int LM, bool NARROW>
+template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
-  return KernelSet{TW, TH, LM, NARROW, (const void*)relax_round<TW, TH, LM, NARROW>,
-                   (const void*)relax_list<TW, TH, LM, NARROW>};
+  return KernelSet{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
+                   (const void*)relax_list<TW, TH, LM, NARROW, LEAN>};
 }
 
-// Wide step planes for every layer bound; narrow ones up to 8 layers.
+// Wide step planes for every layer bound; narrow ones, full and lean, up to
+// 8 layers.
 template <int TW, int TH, bool WITH_NARROW>
 static void add_shape(std::vector<KernelSet>& v) {
-  v.push_back(make_set<TW, TH, 2, false>());
-  v.push_back(make_set<TW, TH, 4, false>());
-  v.push_back(make_set<TW, TH, 8, false>());
-  v.push_back(make_set<TW, TH, 16, false>());
+  v.push_back(make_set<TW, TH, 2, false, false>());
+  v.push_back(make_set<TW, TH, 4, false, false>());
+  v.push_back(make_set<TW, TH, 8, false, false>());
+  v.push_back(make_set<TW, TH, 16, false, false>());
   if (WITH_NARROW) {
-    v.push_back(make_set<TW, TH, 2, WITH_NARROW>());
-    v.push_back(make_set<TW, TH, 4, WITH_NARROW>());
-    v.push_back(make_set<TW, TH, 8, WITH_NARROW>());
+    v.push_back(make_set<TW, TH, 2, WITH_NARROW, false>());
+    v.push_back(make_set<TW, TH, 4, WITH_NARROW, false>());
+    v.push_back(make_set<TW, TH, 8, WITH_NARROW, false>());
+    v.push_back(make_set<TW, TH, 2, WITH_NARROW, WITH_NARROW>());
+    v.push_back(make_set<TW, TH, 4, WITH_NARROW, WITH_NARROW>());
+    v.push_back(make_set<TW, TH, 8, WITH_NARROW, WITH_NARROW>());
   }
 }
 
@@ -1559,17 +1596,27 @@ bool tile_shape_supported(uint32_t tw, uint32_t th) {
   return false;
 }
 
-// The narrow variant when asked for and instantiated, else the wide one.
-static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers, bool narrow) {
+// The narrow variant when asked for and instantiated (lean when asked for
+// and instantiated), else the wide one.
+static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers, bool narrow, bool lean) {
   const uint32_t lm = layers <= 2 ? 2 : layers <= 4 ? 4 : layers <= 8 ? 8 : 16;
-  const KernelSet* wide = nullptr;
+  const KernelSet *wide = nullptr, *full = nullptr;
   for (const KernelSet& k : kernel_sets()) {
     if (k.tw != tw || k.th != th || k.lm != lm) continue;
-    if (k.narrow == narrow) return k;
+    if (narrow && k.narrow && k.lean == lean) return k;
+    if (k.narrow && !k.lean) full = &k;
     if (!k.narrow) wide = &k;
   }
+  if (narrow && full) return *full;
   if (wide) return *wide;
   throw Error(1, "tile shape " + std::to_string(tw) + "x" + std::to_string(th) + " is not instantiated");
+}
+
+// Whether a field and its board need none of what the lean kernels leave
+// out: halo-owner and legal-search planes, additive overrides, own pads and
+// via-exempt pads.
+static bool field_lean(const BoardView& v, const FieldReq& q) {
+  return !v.halo && !v.owner && !v.lhalo && q.ovr_cells.empty() && q.pads.empty() && q.via_pads.empty();
 }
 
 void ctx_init(Ctx& c, int device) {
@@ -1624,7 +1671,7 @@ void ctx_free(Ctx& c) {
 }
 
 void board_free(DevBoard& b) {
-  void* ptrs[] = {b.cost, b.add, b.halo, b.bmask, b.raw, b.xe, b.ye, b.ve, b.owner, b.lhalo};
+  void* ptrs[] = {b.cost, b.add, b.add16, b.halo, b.bmask, b.raw, b.xe, b.ye, b.ve, b.owner, b.lhalo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b = DevBoard();
@@ -1675,16 +1722,21 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   // times the largest enter weight (obstacles count as weight 1, the price of
   // an own pad) plus the largest additive price.  Boards under the narrow
   // limit relax with 16-bit step planes in LDS.
-  uint64_t max_e = 0, max_w = 1, max_a = 0;
+  uint64_t max_e = 0, max_w = 1, max_a = 0, max_plane = 0;
   for (uint32_t k = 0; k + 1 < w; ++k) max_e = std::max<uint64_t>(max_e, xe[k]);
   for (uint32_t k = 0; k + 1 < h; ++k) max_e = std::max<uint64_t>(max_e, ye[k]);
   for (uint32_t k = 0; k + 1 < layers; ++k)
     if (via_allowed[k] && ve) max_e = std::max<uint64_t>(max_e, ve[k]);
   std::vector<uint8_t> obst(n);
+  bool unit_w = true;  // every passable cell weighs 1: no weight plane on the device
   for (size_t i = 0; i < n; ++i) {
     obst[i] = cost[i] == kMax;
-    if (!obst[i]) max_w = std::max<uint64_t>(max_w, cost[i]);
+    if (!obst[i]) {
+      max_w = std::max<uint64_t>(max_w, cost[i]);
+      unit_w = unit_w && cost[i] == 1u;
+    }
     uint64_t a = add ? add[i] : 0;
+    max_plane = std::max(max_plane, a);
     if (halo) a += halo_pen;
     max_a = std::max(max_a, a);
   }
@@ -1695,7 +1747,7 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   b.w = w;
   b.h = h;
   b.layers = layers;
-  b.cost = dev_copy(cost, n, c.stream);
+  if (!unit_w) b.cost = dev_copy(cost, n, c.stream);
   b.bmask = dev_copy(bm.data(), n, c.stream);
   b.raw = dev_copy(raw.data(), n, c.stream);
   std::vector<uint32_t> x(w, 0), y(h, 0), v(16, 0);
@@ -1709,8 +1761,14 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   b.xe = dev_copy(x.data(), x.size(), c.stream);
   b.ye = dev_copy(y.data(), y.size(), c.stream);
   b.ve = dev_copy(v.data(), v.size(), c.stream);
-  b.bytes = n * (4 + 1 + 1);
-  if (add) {
+  b.bytes = n * ((unit_w ? 0 : 4) + 1 + 1);
+  std::vector<uint16_t> add16;
+  if (add && max_plane <= 0xFFFFu) {
+    // additive prices that fit 16 bits travel as u16 (2 B per cell visit)
+    add16.assign(add, add + n);
+    b.add16 = dev_copy(add16.data(), n, c.stream);
+    b.bytes += n * 2;
+  } else if (add) {
     b.add = dev_copy(add, n, c.stream);
     b.bytes += n * 4;
   }
@@ -1749,6 +1807,11 @@ void board_clear_additive(DevBoard& b) {
     b.add = nullptr;
     b.bytes -= (size_t)b.w * b.h * b.layers * 4;
   }
+  if (b.add16) {
+    (void)hipFree(b.add16);
+    b.add16 = nullptr;
+    b.bytes -= (size_t)b.w * b.h * b.layers * 2;
+  }
   b.max_step = std::min<uint64_t>(b.max_ew + (b.halo ? b.halo_pen : 0u), kMax - 1);
 }
 
@@ -1757,6 +1820,13 @@ void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_a
   const size_t n = (size_t)b.w * b.h * b.layers;
   uint64_t max_a = max_add != UINT64_MAX ? max_add : *std::max_element(add, add + n);
   if (b.halo) max_a += b.halo_pen;
+  if (b.add16) {
+    // a replaced plane is written as u32 (negotiation prices grow past 16 bits)
+    HIP_OK(hipStreamSynchronize(c.stream));
+    (void)hipFree(b.add16);
+    b.add16 = nullptr;
+    b.bytes -= n * 2;
+  }
   if (!b.add) {
     b.add = dev_copy(add, n, c.stream);
     b.bytes += n * 4;
@@ -1920,12 +1990,13 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  bool narrow = true;
+  bool narrow = true, lean = true;
   for (const FieldReq& q : reqs) {
     if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
     narrow = narrow && field_narrow(*boards[q.board], q);
+    lean = lean && field_lean(c.views[q.board], q);
   }
-  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow);
+  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
   size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   if (lds > 160 * 1024 - 64) throw Error(1, "tile shape exceeds 160 KiB of LDS");
 
@@ -2458,9 +2529,11 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   G.nfields = reqs.size();
   G.nitems = nitems;
   G.narrow = true;
+  G.lean = true;
   for (const FieldReq& q : reqs) {
     G.layers = std::max(G.layers, d.views[q.board].layers);
     G.narrow = G.narrow && field_narrow(*d.devs[q.board], q);
+    G.lean = G.lean && field_lean(d.views[q.board], q);
   }
   G.labwords = words;
   G.npool = npool;
@@ -2586,13 +2659,14 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
-  bool narrow = true;
+  bool narrow = true, lean = true;
   for (uint32_t g = 0; g < d.max_groups; ++g)
     if (d.groups[g].live) {
       Lmax = std::max(Lmax, d.groups[g].layers);
       narrow = narrow && d.groups[g].narrow;
+      lean = lean && d.groups[g].lean;
     }
-  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow);
+  const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
   const size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
   void* args[] = {&A};
   if (d.stamp_round >= (1u << 30)) {
