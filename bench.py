@@ -43,6 +43,7 @@ sys.path.insert(0, os.path.join(ROOT, "metal-route_amd"))
 METRIC = "nets/sec (fields+paths, bit-exact) at 512²×4L; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_VISIT = {0: 17, 1: 17, 2: 19, 3: 19, 4: 21}
+FORMS = {"plain": 0, "graph": 2, "queue": 3}
 CPU_SECONDS = 10.0
 
 
@@ -59,7 +60,8 @@ def parse():
                          "1 = 128x128x2/64 nets, 2 = 512x512x4/256 nets (headline), "
                          "3 = 2048x2048x2/1024 windowed nets, 4 = 1024x1024x6/512 multi-pin fields")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["graph", "plain"], default="plain")
+    ap.add_argument("--mode", choices=["graph", "plain", "queue"], default="plain",
+                    help="solve form: plain rounds, hipGraph rounds, or one persistent work-queue launch")
     ap.add_argument("--tiles", default="16,16,256,32", help="tw,th,threads,max_local")
     ap.add_argument("--delta", type=int, default=0, help="delta-stepping window of the plain form (0 = off)")
     ap.add_argument("--autotune", action="store_true",
@@ -396,7 +398,7 @@ def negotiate_main(a, tiles):
 
     eng = mrfield.Engine(0)
     eng.set_tiles(*tiles)
-    eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)
+    eng.set_mode(FORMS[a.mode])
     s = mrfield.synth_config(a.config, seed=a.seed)
     g = s.grid
     board = mrfield.Board(eng, g, s.edges, None)
@@ -496,7 +498,7 @@ def main():
         return
     eng = mrfield.Engine(0)
     eng.set_tiles(tw, th, threads, ml)
-    eng.set_mode(mrfield.MODE_GRAPH if a.mode == "graph" else mrfield.MODE_PLAIN)
+    eng.set_mode(FORMS[a.mode])
     if a.delta:
         eng.set_delta(a.delta)
     s = mrfield.synth_config(a.config, seed=a.seed)
@@ -507,9 +509,9 @@ def main():
     wins = s.windows
     tuning = None
     form, tiles = a.mode, a.tiles
-    if (a.autotune or a.config == 3) and not multi:
+    if (a.autotune or (a.config == 3 and a.mode != "queue")) and not multi:
         tuning = board.autotune(nets, windows=wins, sample=64)
-        form = "graph" if tuning["mode"] == mrfield.MODE_GRAPH else "plain"
+        form = {v: k for k, v in FORMS.items()}[tuning["mode"]]
         tw, th = tuning["tile_w"], tuning["tile_h"]
         tiles = f"{tw},{th},{tuning['threads']},{tuning['max_local']}"
 
@@ -554,7 +556,7 @@ def main():
     avg_launch_ms = float(np.mean(kern_ms)) / max(n_launch, 1.0)
     achieved = cells_per_launch * bpc / (avg_launch_ms * 1e-3) / 1e9
     workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
-    kernel = "relax_list" if form == "graph" else "relax_round"
+    kernel = {"graph": "relax_list", "plain": "relax_round", "queue": "relax_queue"}[form]
     traffic = load_traffic(kernel, workload)
     # compulsory floor of a step: every field cell's dist written once (4 B)
     # and the board planes read once per field (bpc - 16 B per cell)

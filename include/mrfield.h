@@ -167,8 +167,17 @@ typedef struct {
  *               this library, mrf_solve* return MRF_BACKEND_UNAVAILABLE for it.
  *   GRAPH       grid-stride launches over a device work list, rounds captured
  *               in a hipGraph and replayed; one host read per replay.
- * A new context starts in PLAIN. */
-enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1, MRF_MODE_GRAPH = 2 };
+ *   QUEUE       one persistent launch: workgroups pop tiles from device work
+ *               queues (one per XCD, fields balanced over them by tiles) and
+ *               push the neighbours their visits improve; no rounds, no
+ *               barrier between workgroups.  Every wait inside the launch is
+ *               bounded (MRF_QUEUE_WAIT_MS, read when a context opens,
+ *               default 2000); a launch that exceeds a bound fails the call.
+ *               The persistent work-queue batching form of
+ *               crates/mr-metal/src/gpu.rs:1630-2039 (ragged batches), on
+ *               the device instead of a host round loop.
+ * A new context starts in PLAIN.  Results are the same in every form. */
+enum { MRF_MODE_PLAIN = 0, MRF_MODE_PERSISTENT = 1, MRF_MODE_GRAPH = 2, MRF_MODE_QUEUE = 3 };
 
 int mrf_abi_version(void);
 

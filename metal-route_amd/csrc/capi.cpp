@@ -345,6 +345,8 @@ int mrf_open(int device, mrf_ctx** out, mrf_error* err) {
     if (inj) c->c.inject = atoi(inj);
     const char* spin = getenv("MRF_STAMP_SPIN_US");
     if (spin) c->c.stamp_spin_us = (uint32_t)strtoul(spin, nullptr, 10);
+    const char* qwait = getenv("MRF_QUEUE_WAIT_MS");
+    if (qwait) c->c.queue_wait_ms = (uint32_t)strtoul(qwait, nullptr, 10);
   } catch (const Error& e) {
     mrf::ctx_free(c->c);
     delete c;
@@ -376,7 +378,8 @@ int mrf_set_tiles(mrf_ctx* ctx, uint32_t tw, uint32_t th, uint32_t threads, uint
 }
 
 int mrf_set_mode(mrf_ctx* ctx, int mode) {
-  if (!ctx || (mode != MRF_MODE_PLAIN && mode != MRF_MODE_PERSISTENT && mode != MRF_MODE_GRAPH))
+  if (!ctx || (mode != MRF_MODE_PLAIN && mode != MRF_MODE_PERSISTENT && mode != MRF_MODE_GRAPH &&
+               mode != MRF_MODE_QUEUE))
     return MRF_BACKEND_UNAVAILABLE;
   std::lock_guard<std::mutex> g(ctx->c.lane);
   ctx->c.mode = mode;

@@ -164,7 +164,10 @@ struct SolveStats {
 // MODE_GRAPH: grid-stride kernel over a device work list per round, rounds
 //   separated by kernel boundaries, captured in a hipGraph and replayed; the
 //   host reads the activation count once per replay.
-enum SolveMode : int { MODE_PLAIN = 0, MODE_PERSISTENT = 1, MODE_GRAPH = 2 };
+// MODE_QUEUE: one persistent launch; workgroups pop tiles from device work
+//   queues (one per XCD) and push the neighbours their visits improve, with
+//   no rounds and no barrier between workgroups.
+enum SolveMode : int { MODE_PLAIN = 0, MODE_PERSISTENT = 1, MODE_GRAPH = 2, MODE_QUEUE = 3 };
 
 class Error : public std::exception {
  public:
@@ -206,6 +209,10 @@ struct Workspace {
   size_t pcost_cap = 0;
   uint32_t* pcells = nullptr;    // path cells
   size_t pcells_cap = 0;
+  uint64_t* qslots = nullptr;    // queue form: every queue's ring
+  size_t qslots_cap = 0;
+  uint32_t* qctl = nullptr;      // queue form: head, tail, pending of each queue
+  size_t qctl_cap = 0;
   uint32_t* fout = nullptr;      // extracted field output staging
   size_t fout_cap = 0;
   uint32_t* pinned = nullptr;    // host-pinned counter readback
@@ -247,6 +254,9 @@ struct Ctx {
   // Plain form: how long the host spins on a launch's stamp before it hands
   // over to a stream synchronisation (microseconds).
   uint32_t stamp_spin_us = 20000;
+  // Queue form: the longest a workgroup waits for work or a queue slot, and
+  // the longest one runs, before the launch stops with an error (ms).
+  uint32_t queue_wait_ms = 2000, queue_run_ms = 20000;
 };
 
 // ---- engine.hip -------------------------------------------------------------

@@ -52,6 +52,7 @@
 
 #include "engine.h"
 #include "hostpool.h"
+#include "workqueue.h"
 
 namespace mrf {
 
@@ -652,15 +653,27 @@ __device__ __forceinline__ uint64_t lab_in(uint64_t v) { return v < kUnreached ?
 // fields no additive overrides, own pads or via-exempt pads (forced
 // endpoints remain), so none of that code or its descriptor fields exist in
 // the kernel: fewer scalar registers held across the visit, smaller code.
-template <int TW, int TH, int LM, bool NARROW, bool LEAN>
+// QUEUE: the visit runs inside the persistent work-queue kernel, where other
+// workgroups write labels and states during the same launch: every label of
+// the solve starts unreached (filled before the launch), so any label read is
+// an upper bound of its final value, whoever wrote it and when; labels, the
+// own state and neighbour borders are read and written with device-scope
+// atomics (never through a cache another workgroup's writes bypass); labels
+// are lowered with an atomic minimum, since two visits of one tile may
+// overlap; the first visit is the one whose atomic OR sets S_TOUCHED.
+template <int TW, int TH, int LM, bool NARROW, bool LEAN, bool QUEUE = false>
 __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem, uint32_t round) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t f = const_load(A.item_field + item);
   // the item's own state travels with the descriptor loads: whether the tile
   // holds labels is known before any plane load is issued
   // (an item is visited at most once per launch, so no workgroup of this
-  // launch has written its state yet)
-  const bool touched = (const_load(A.state + item) & S_TOUCHED) != 0;
+  // launch has written its state yet); in the queue form every visit loads
+  // the tile's labels, and the first visit is known after the first barrier
+  const bool touched = QUEUE ? true : (const_load(A.state + item) & S_TOUCHED) != 0;
+  uint32_t first_was = S_TOUCHED;
+  if (QUEUE && tid == 0)
+    first_was = __hip_atomic_fetch_or(&A.state[item], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const FieldDesc F = const_load(A.fields + f);
   const BoardView B = const_load(A.boards + F.board);
   const uint32_t t = item - F.item0;
@@ -683,7 +696,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   // issued before the barrier that publishes the states, so one memory round
   // trip covers them.
   uint32_t stv = 0;
-  if (tid >= 1 && tid < 5) {
+  if (!QUEUE && tid >= 1 && tid < 5) {
     uint32_t nb = item;
     bool ok = true;
     if (tid == 1) ok = txi > 0, nb = item - 1;
@@ -813,7 +826,11 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
 #pragma unroll
       for (int k = 0; k < 4; ++k) lg[k] = 0;
     }
-    if (touched) {
+    if (QUEUE) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        lv[k] = __hip_atomic_load(&A.labels[blockbase + qc0 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (touched) {
       // a quad is contiguous in the tile block whatever the window
       const auto* lp = (const __attribute__((address_space(1))) u64x2_a8*)(A.labels + blockbase + qc0);
       const u64x2 l01 = lp[0], l23 = lp[1];
@@ -828,6 +845,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   };
   load_planes(tid);
   if (tid >= 1 && tid < 5) s.misc[tid] = stv;
+  if (QUEUE && tid == 0) s.misc[6] = first_was;
   if (tid == 0) {
     s.misc[5] = 0;  // result flags
     s.misc[8] = 0;  // phase change stamps
@@ -855,8 +873,11 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   // so its border is taken as unreached, an upper bound; that visit then
   // offers its border to this tile, unreached to it, and activates it for the
   // next round.
+  // (the queue form takes every existing neighbour's border: its labels
+  // start unreached, so any value read is an upper bound)
   auto earlier = [&](uint32_t st) { return (st & S_TOUCHED) && (st >> 1) < round; };
-  const uint32_t nbv = (earlier(uniform_u32(s.misc[1])) ? 1u : 0u) | (earlier(uniform_u32(s.misc[2])) ? 2u : 0u) |
+  const bool first_visit = QUEUE ? (uniform_u32(s.misc[6]) & S_TOUCHED) == 0 : !touched;
+  const uint32_t nbv = QUEUE ? 15u : (earlier(uniform_u32(s.misc[1])) ? 1u : 0u) | (earlier(uniform_u32(s.misc[2])) ? 2u : 0u) |
                        (earlier(uniform_u32(s.misc[3])) ? 4u : 0u) | (earlier(uniform_u32(s.misc[4])) ? 8u : 0u);
   // halo labels into the ring (a neighbour never visited has none), and a
   // lower bound of the price of stepping from the tile into each halo cell:
@@ -982,7 +1003,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     }
   }
   // seeds (first visit only): passable sources start at label (0, 0)
-  if (!touched) {
+  if (first_visit) {
     const uint32_t plane = B.plane;
     for (uint32_t k = tid; k < F.n_src; k += nt) {
       uint32_t g = A.pool[F.src_off + k];
@@ -1020,12 +1041,15 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
       if (c >= ncell || !(chg[u] || !touched)) continue;
       const uint32_t l = c / PT, r = c - l * PT, y = r / TW, x = r - y * TW;
       const uint64_t v = s.lab[l * LAB_PLANE + (y + 1) * RS + x + 1];
-      __hip_atomic_store(&A.labels[blockbase + c], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (x == 0)
-        __hip_atomic_store(&A.labels[blockbase + ncell + l * TH + y], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (x == TW - 1)
-        __hip_atomic_store(&A.labels[blockbase + ncell + L * TH + l * TH + y], v, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      auto put = [&](uint64_t w) {
+        if (QUEUE)
+          __hip_atomic_fetch_min(&A.labels[w], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+          __hip_atomic_store(&A.labels[w], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      };
+      put(blockbase + c);
+      if (x == 0) put(blockbase + ncell + l * TH + y);
+      if (x == TW - 1) put(blockbase + ncell + L * TH + l * TH + y);
       if (chg[u]) {
         // A changed border cell wakes the neighbour across that side only if
         // it can improve the neighbour's adjacent cell: the candidate it
@@ -1063,6 +1087,9 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
     if (kt != kMax) atomicMin(&s.misc[12], kt);
     if (kb != kMax) atomicMin(&s.misc[13], kb);
   }
+  // queue form: this workgroup's label writes are performed before any of
+  // its activations can make another workgroup visit a neighbour
+  if (QUEUE) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();  // also orders this workgroup's label stores before the flag / queue updates
   TileResult res;
   const uint32_t sides = (txi > 0 ? T_LEFT : 0u) | (txi + 1 < F.tx ? T_RIGHT : 0u) | (tyi > 0 ? T_TOP : 0u) |
@@ -1075,7 +1102,7 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   res.kb = uniform_u32(s.misc[13]);
   res.ka = uniform_u32(s.misc[14]);
   if (tid == 0) {
-    if (!touched)
+    if (!QUEUE && !touched)
       __hip_atomic_fetch_or(&A.state[item], S_TOUCHED | (round << 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
     atomicAdd(&A.counters[(C_ROUNDS) * kCtrStride], rounds);
@@ -1263,6 +1290,111 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::list)) relax_list(Rela
     if (threadIdx.x < 64) {
       if (threadIdx.x == 0) R.act_cur[item] = 0;
       activate_sides(R, item, res, res.ka);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// persistent form: one launch, a device work queue per XCD
+// ---------------------------------------------------------------------------
+//
+// Every field belongs to one of kQueues queues (the host balances their
+// tiles); workgroup b serves queue b % kQueues, the XCD the hardware
+// dispatches it to, so a field's labels, states and queue traffic stay in
+// one XCD's L2.  Correctness does not depend on that placement: every access
+// the workgroups share is a device-scope atomic (workqueue.h, process_tile's
+// queue form).  A workgroup takes a ticket, waits for its item, visits it,
+// activates the neighbours its visit can improve (each a push unless already
+// queued) and marks the visit done; it stops when a wait finds nothing
+// pending, or, with an error flag, when a wait or the workgroup's whole run
+// outlasts its limit.  Control flow is uniform: lane 0 decides, the decision
+// reaches every lane through LDS and readfirstlane.
+
+constexpr uint32_t kQueues = 8;
+
+struct QueueArgs {
+  uint64_t* slots;             // every queue's ring
+  uint32_t* ctl;               // queue k: head, tail, pend at ctl[(3k + j) * kCtrStride]
+  uint32_t slot0[kQueues];     // first slot of each queue's ring
+  uint32_t cap[kQueues];
+  uint64_t wait_ticks;         // longest wait for a slot or for work (s_memrealtime ticks)
+  uint64_t run_ticks;          // longest run of one workgroup
+};
+
+struct DevOps {
+  __device__ static uint32_t load(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+  __device__ static uint64_t load64(uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static void store64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static uint32_t exch(uint32_t* p, uint32_t v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static uint32_t add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static uint64_t now() { return __builtin_amdgcn_s_memrealtime(); }
+  // about 2^k x 64 clocks
+  __device__ static void pause(uint32_t k) {
+    for (uint32_t i = 0; i < (1u << k); ++i) __builtin_amdgcn_s_sleep(1);
+  }
+  __device__ static void order() { __builtin_amdgcn_s_waitcnt(0); }
+};
+
+// Error flags of the persistent form (device counter C_ERROR).
+enum : uint32_t { E_QWAIT = 64u, E_QTAKE = 128u, E_QPUSH = 256u, E_QRUN = 512u };
+
+template <int TW, int TH, int LM, bool NARROW, bool LEAN>
+__global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(RelaxArgs A, QueueArgs Q) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ uint32_t cmd;
+  constexpr uint32_t kExit = 0xFFFFFFFFu;
+  const uint32_t qi = blockIdx.x % kQueues;
+  const wq::Queue q{Q.slots + Q.slot0[qi], Q.cap[qi], Q.ctl + (3 * qi + 0) * kCtrStride,
+                    Q.ctl + (3 * qi + 1) * kCtrStride, Q.ctl + (3 * qi + 2) * kCtrStride};
+  uint32_t* err = A.counters + C_ERROR * kCtrStride;
+  const uint64_t start = DevOps::now();
+  for (;;) {
+    if (threadIdx.x == 0) {
+      uint32_t d = kExit;
+      if (DevOps::now() - start > Q.run_ticks) {
+        atomicOr(err, E_QRUN);
+      } else {
+        uint32_t it = 0;
+        const uint32_t r = wq::wait_item<DevOps>(q, wq::take_ticket<DevOps>(q), Q.wait_ticks, it);
+        if (r == wq::GOT) {
+          if (!wq::take<DevOps>(A.act_cur, it)) atomicOr(err, E_QTAKE);
+          d = it;
+        } else if (r == wq::TIMEOUT) {
+          atomicOr(err, E_QWAIT);
+        }  // STOP: nothing pending, no item can come
+      }
+      cmd = d;
+    }
+    __syncthreads();
+    const uint32_t item = uniform_u32(cmd);
+    __syncthreads();
+    if (item == kExit) break;
+    const TileResult res = process_tile<TW, TH, LM, NARROW, LEAN, true>(A, item, smem, 0u);
+    if (threadIdx.x < 64) {
+      // lanes 0..3: left, right, top, bottom neighbour; lane 4: the tile
+      // itself when its visit hit the in-LDS round cap
+      const uint32_t lane = threadIdx.x, fl = res.flags;
+      bool want = false;
+      uint32_t nb = item;
+      if (lane == 0) want = (fl & T_LEFT) != 0, nb = item - 1;
+      if (lane == 1) want = (fl & T_RIGHT) != 0, nb = item + 1;
+      if (lane == 2) want = (fl & T_TOP) != 0, nb = item - res.tx;
+      if (lane == 3) want = (fl & T_BOTTOM) != 0, nb = item + res.tx;
+      if (lane == 4) want = (fl & T_CAPPED) != 0;
+      if (want && !wq::activate<DevOps>(q, A.act_cur, nb, Q.wait_ticks)) atomicOr(err, E_QPUSH);
+      if (lane == 0) {
+        if (fl & (T_ANY | T_CAPPED)) atomicAdd(A.counters + C_CHANGED * kCtrStride, 1u);
+        wq::done<DevOps>(q);  // after every lane's activations (order() waits for them)
+      }
     }
     __syncthreads();
   }
@@ -1548,12 +1680,13 @@ struct KernelSet {
   bool narrow, lean;
   const void* round;
   const void* list;
+  const void* queue;
 };
 
 template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
   return KernelSet{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
-                   (const void*)relax_list<TW, TH, LM, NARROW, LEAN>};
+                   (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>};
 }
 
 // Wide step planes for every layer bound; narrow ones, full and lean, up to
@@ -1645,13 +1778,14 @@ void ctx_init(Ctx& c, int device) {
   for (const KernelSet& k : kernel_sets()) {
     HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    HIP_OK(hipFuncSetAttribute(k.queue, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
   }
 }
 
 void ctx_free(Ctx& c) {
   Workspace& w = c.ws;
   void* ptrs[] = {w.labels, w.state, w.act, w.key, w.list, w.fields, w.item_field, w.pool, w.counters,
-                  w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards};
+                  w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards, w.qslots, w.qctl};
   (void)hipSetDevice(c.device);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -2005,7 +2139,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   build_all_overrides(boards, reqs, S, ovr_seg, ovr_n);
   std::vector<FieldDesc>& layout = c.layout;
   layout.clear();
-  std::vector<uint32_t> pool, seeds;
+  std::vector<uint32_t> pool, seeds, seed_field;
   uint64_t lab_off = 0, field_cells = 0;
   uint32_t item = 0;
   for (size_t i = 0; i < reqs.size(); ++i) {
@@ -2052,6 +2186,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     std::sort(st_items.begin(), st_items.end());
     st_items.erase(std::unique(st_items.begin(), st_items.end()), st_items.end());
     seeds.insert(seeds.end(), st_items.begin(), st_items.end());
+    seed_field.insert(seed_field.end(), st_items.size(), (uint32_t)i);
     item += nt;
     lab_off += (uint64_t)nt * tile_block_words(L, S.tw, S.th);
     field_cells += (uint64_t)F.ww * F.wh * L;
@@ -2203,9 +2338,67 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       rounds = pinned[C_PROUNDS * kCtrStride];
       local.replays = replays;
       local.launches = rounds;
+    } else if (c.mode == MODE_QUEUE) {
+      // fields over the queues, longest first onto the least loaded queue
+      std::vector<uint32_t> order(layout.size()), fq(layout.size());
+      std::iota(order.begin(), order.end(), 0u);
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (uint64_t)layout[a].tx * layout[a].ty > (uint64_t)layout[b].tx * layout[b].ty;
+      });
+      uint64_t qload[kQueues] = {};
+      for (uint32_t i : order) {
+        uint32_t k = 0;
+        for (uint32_t j = 1; j < kQueues; ++j)
+          if (qload[j] < qload[k]) k = j;
+        fq[i] = k;
+        qload[k] += (uint64_t)layout[i].tx * layout[i].ty;
+      }
+      QueueArgs Q;
+      memset(&Q, 0, sizeof(Q));
+      uint64_t nslots = 0;
+      for (uint32_t k = 0; k < kQueues; ++k) {
+        // an item is queued at most once at a time
+        Q.slot0[k] = (uint32_t)nslots;
+        Q.cap[k] = (uint32_t)(qload[k] + 64);
+        nslots += Q.cap[k];
+      }
+      if (nslots >= (1ull << 32)) throw Error(1, "too many tiles for the queue form");
+      std::vector<uint64_t> slots(nslots, 0ull);
+      std::vector<uint32_t> ctl((size_t)3 * kQueues * kCtrStride, 0u);
+      for (size_t j = 0; j < seeds.size(); ++j) {
+        const uint32_t k = fq[seed_field[j]];
+        uint32_t& tail = ctl[(3 * k + 1) * kCtrStride];
+        slots[Q.slot0[k] + tail] = wq::entry(tail, Q.cap[k], seeds[j] + 1u);  // ticket `tail`
+        ++tail;
+        ctl[(3 * k + 2) * kCtrStride] = tail;  // pending: one per queued seed
+      }
+      grow(W.qslots, W.qslots_cap, nslots);
+      grow(W.qctl, W.qctl_cap, ctl.size());
+      HIP_OK(hipMemcpyAsync(W.qslots, slots.data(), nslots * 8, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(W.qctl, ctl.data(), ctl.size() * 4, hipMemcpyHostToDevice, s));
+      Q.slots = W.qslots;
+      Q.ctl = W.qctl;
+      Q.wait_ticks = (uint64_t)c.queue_wait_ms * 100000ull;  // s_memrealtime runs at 100 MHz
+      Q.run_ticks = (uint64_t)c.queue_run_ms * 100000ull;
+      // every label of the solve starts unreached: any label a visit reads is
+      // an upper bound of its final value, whichever workgroup wrote it
+      launch_fill32(s, (uint32_t*)W.labels, 0xFFFFFFFFu, (size_t)lab_off * 2);
+      hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
+                         d_seeds, (uint32_t)seeds.size(), (uint32_t*)nullptr);
+      int occ = 0;
+      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.queue, S.threads, lds));
+      if (occ < 1) throw Error(1, "relaxation kernel does not fit on a CU");
+      const uint32_t grid = std::max<uint32_t>(kQueues, ((uint32_t)occ * (uint32_t)c.cus) / kQueues * kQueues);
+      void* qargs[] = {&A, &Q};
+      HIP_OK(hipEventRecord(W.ev0, s));
+      timed = true;
+      HIP_OK(hipLaunchKernel(K.queue, dim3(grid), dim3(S.threads), qargs, lds, s));
+      HIP_OK(hipGetLastError());
+      rounds = 1;
+      local.launches = 1;
     } else {
       throw Error(1, "the persistent grid-barrier form is disabled (it did not return on MI355X); "
-                     "use the plain or graph form");
+                     "use the plain, graph or queue form");
     }
     local.rounds = rounds;
   }

@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "libmrfield.so")
 
 MAX = 0xFFFFFFFF
 OK, BACKEND_UNAVAILABLE, INVALID_ENDPOINT, MALFORMED_GRID = 0, 1, 2, 3
-MODE_PLAIN, MODE_PERSISTENT, MODE_GRAPH = 0, 1, 2
+MODE_PLAIN, MODE_PERSISTENT, MODE_GRAPH, MODE_QUEUE = 0, 1, 2, 3
 
 U32P = C.POINTER(C.c_uint32)
 U8P = C.POINTER(C.c_uint8)

@@ -3,17 +3,21 @@
 // kernel uses, over a host operations policy.
 //
 // Worker threads play the kernel's workgroups: each serves one queue and
-// loops pop -> take -> visit -> activations -> done until its queue is empty
-// and pend is 0, exactly the control flow of the kernel's loop.  A visit is a
+// loops ticket -> wait for its item -> take -> visit -> activations -> done
+// until a wait finds nothing pending, exactly the control flow of the
+// kernel's loop.  A visit is a
 // label-correcting relaxation of a node of a random weighted graph (labels
 // lowered with an atomic minimum, so visits of one node may overlap, as tile
 // visits may); every improved neighbour is activated.  Checked:
 //   * every worker stops, no wait times out, and the labels equal Dijkstra's
 //     on every graph (one disjoint graph per queue, several workers per queue);
 //   * with pend raised by one that no item carries, every worker reports the
-//     idle timeout and stops instead of waiting forever;
-//   * every queue's head meets its tail (every push popped), every flag is
-//     clear and pend is 0 at the end.
+//     wait timeout and stops instead of waiting forever;
+//   * every push was received (each worker ends holding one unused ticket),
+//     every flag is clear and pend is 0 at the end;
+//   * with a ring of one slot more than the items (every slot reused many
+//     times over, pushes waiting for slots not yet received) the results
+//     still equal Dijkstra's.
 // Exit status 0 when every case holds.
 #include <atomic>
 #include <chrono>
@@ -30,11 +34,8 @@ namespace wq = mrf::wq;
 
 struct HostOps {
   static uint32_t load(uint32_t* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
-  static void store(uint32_t* p, uint32_t v) { __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
-  static uint32_t cas(uint32_t* p, uint32_t e, uint32_t d) {
-    __atomic_compare_exchange_n(p, &e, d, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
-    return e;
-  }
+  static uint64_t load64(uint64_t* p) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
+  static void store64(uint64_t* p, uint64_t v) { __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
   static uint32_t exch(uint32_t* p, uint32_t v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
   static uint32_t add(uint32_t* p, uint32_t v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
   static uint64_t now() {
@@ -42,7 +43,8 @@ struct HostOps {
                std::chrono::steady_clock::now().time_since_epoch())
         .count();
   }
-  static void pause() { std::this_thread::yield(); }
+  static void pause(uint32_t) { std::this_thread::yield(); }
+  static void order() {}  // every operation above is sequentially consistent
 };
 
 static int fails = 0;
@@ -66,7 +68,8 @@ static void atomic_min64(uint64_t* p, uint64_t v, uint64_t& old) {
 }
 
 struct Run {
-  std::vector<uint32_t> slots, act, ctl;  // ctl: per queue head, tail, pend (one line each)
+  std::vector<uint64_t> slots;
+  std::vector<uint32_t> act, ctl;  // ctl: per queue head, tail, pend (one line each)
   std::vector<uint64_t> dist;
   std::vector<wq::Queue> queues;
   std::atomic<uint32_t> errors{0}, pops{0}, exits{0};
@@ -76,15 +79,15 @@ struct Run {
 static void worker(Run& R, const std::vector<Graph>& gs, uint32_t q, uint64_t limit_us) {
   const wq::Queue& Q = R.queues[q];
   const Graph& G = gs[q];
-  uint64_t idle0 = HostOps::now();
   for (;;) {
-    const uint32_t v = wq::try_pop<HostOps>(Q, limit_us);
-    if (v == wq::POP_TIMEOUT) {
+    const uint32_t ticket = wq::take_ticket<HostOps>(Q);
+    uint32_t u = 0;
+    const uint32_t r = wq::wait_item<HostOps>(Q, ticket, limit_us, u);
+    if (r == wq::TIMEOUT) {
       R.errors++;
       break;
     }
-    if (v != wq::POP_EMPTY) {
-      const uint32_t u = v - 1u;
+    if (r == wq::GOT) {
       R.pops++;
       if (!wq::take<HostOps>(R.act.data(), u)) R.errors++;
       const uint64_t du = __atomic_load_n(&R.dist[u], __ATOMIC_SEQ_CST);
@@ -94,15 +97,9 @@ static void worker(Run& R, const std::vector<Graph>& gs, uint32_t q, uint64_t li
         if (du + e.second < old && !wq::activate<HostOps>(Q, R.act.data(), e.first, limit_us)) R.errors++;
       }
       wq::done<HostOps>(Q);
-      idle0 = HostOps::now();
       continue;
     }
-    if (HostOps::load(Q.pend) == 0u) break;
-    if (HostOps::now() - idle0 > limit_us) {
-      R.errors++;
-      break;
-    }
-    HostOps::pause();
+    break;  // STOP: nothing pending
   }
   R.exits++;
 }
@@ -126,7 +123,8 @@ static std::vector<uint64_t> dijkstra(const Graph& g) {
   return d;
 }
 
-static void setup(Run& R, std::vector<Graph>& gs, uint32_t nq, uint32_t nodes, uint32_t degree, uint64_t seed) {
+static void setup(Run& R, std::vector<Graph>& gs, uint32_t nq, uint32_t nodes, uint32_t degree, uint64_t seed,
+                  uint32_t spare = 8) {
   std::mt19937_64 rng(seed);
   gs.clear();
   uint32_t total = 0;
@@ -147,7 +145,7 @@ static void setup(Run& R, std::vector<Graph>& gs, uint32_t nq, uint32_t nodes, u
   R.ctl.assign(nq * 3 * 16, 0);
   uint32_t slot0 = 0;
   std::vector<uint32_t> caps;
-  for (uint32_t q = 0; q < nq; ++q) caps.push_back(nodes + 8);
+  for (uint32_t q = 0; q < nq; ++q) caps.push_back(nodes + spare);
   uint32_t ns = 0;
   for (uint32_t c : caps) ns += c;
   R.slots.assign(ns, 0);
@@ -161,19 +159,20 @@ static void setup(Run& R, std::vector<Graph>& gs, uint32_t nq, uint32_t nodes, u
     Q.pend = &R.ctl[(q * 3 + 2) * 16];
     slot0 += caps[q];
     R.queues.push_back(Q);
-    // seed: the source at label 0, activated by the host
+    // seed: the source at label 0, activated by the host (ticket 0)
     R.dist[gs[q].source] = 0;
     R.act[gs[q].source] = 1;
-    Q.slots[0] = gs[q].source + 1;
+    Q.slots[0] = wq::entry(0, Q.cap, gs[q].source + 1);
     *Q.tail = 1;
     *Q.pend = 1;
   }
 }
 
-static bool run_case(uint32_t nq, uint32_t per_q, uint32_t nodes, uint32_t degree, uint64_t seed) {
+static bool run_case(uint32_t nq, uint32_t per_q, uint32_t nodes, uint32_t degree, uint64_t seed,
+                     uint32_t spare = 8) {
   Run R;
   std::vector<Graph> gs;
-  setup(R, gs, nq, nodes, degree, seed);
+  setup(R, gs, nq, nodes, degree, seed, spare);
   std::vector<std::thread> th;
   for (uint32_t w = 0; w < nq * per_q; ++w) th.emplace_back(worker, std::ref(R), std::cref(gs), w % nq, 5000000ull);
   for (auto& t : th) t.join();
@@ -181,7 +180,8 @@ static bool run_case(uint32_t nq, uint32_t per_q, uint32_t nodes, uint32_t degre
   for (uint32_t q = 0; q < nq; ++q) {
     const std::vector<uint64_t> ref = dijkstra(gs[q]);
     for (uint32_t u = 0; u < nodes; ++u) ok = ok && R.dist[gs[q].n0 + u] == ref[u];
-    ok = ok && *R.queues[q].pend == 0 && *R.queues[q].head == *R.queues[q].tail;
+    // every push received: each worker holds one abandoned ticket at the end
+    ok = ok && *R.queues[q].pend == 0 && *R.queues[q].head == *R.queues[q].tail + per_q;
   }
   for (uint32_t a : R.act) ok = ok && a == 0;
   ok = ok && R.pops.load() > 0;
@@ -197,9 +197,10 @@ int main() {
   // more workers than host cores: preemption anywhere in the protocol
   expect(run_case(1, 32, 3000, 3, 5), "one queue, thirty-two workers");
   expect(run_case(8, 6, 800, 4, 6), "eight queues, six workers each");
+  for (uint64_t s = 40; s < 46; ++s) expect(run_case(2, 8, 60, 5, s, 1), "ring of one spare slot");
   {
     // pend raised with no queued item behind it: every worker must give up
-    // after the idle limit and report it, not wait forever
+    // after the wait limit and report it, not wait forever
     Run R;
     std::vector<Graph> gs;
     setup(R, gs, 2, 50, 2, 7);
@@ -207,7 +208,7 @@ int main() {
     std::vector<std::thread> th;
     for (uint32_t w = 0; w < 6; ++w) th.emplace_back(worker, std::ref(R), std::cref(gs), w % 2, 20000ull);
     for (auto& t : th) t.join();
-    expect(R.exits.load() == 6 && R.errors.load() == 6, "idle limit ends every worker with an error");
+    expect(R.exits.load() == 6 && R.errors.load() == 6, "wait limit ends every worker with an error");
   }
   printf("%s\n", fails ? "FAILED" : "ok");
   return fails ? 1 : 0;
