@@ -348,13 +348,14 @@ int mrf_route_legal_batch(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, s
 
 /* ---- autotuner -----------------------------------------------------------------
  * Picks the tile shape, workgroup size, in-LDS round cap (sweeps per tile
- * visit) and solve form (plain rounds, or hipGraph replays of `graph_rounds`
- * rounds: the work-list granularity) for a workload by measuring candidates
+ * visit) and solve form (plain rounds, hipGraph replays of `graph_rounds`
+ * rounds: the work-list granularity, or the persistent work-queue launch)
+ * for a workload by measuring candidates
  * on a sample of its nets on a resident board.  No reference counterpart: the
  * Metal engine uses fixed threadgroup sizes (crates/mr-metal/src/gpu.rs). */
 typedef struct {
     uint32_t tile_w, tile_h, threads, max_local;
-    int mode;               /* MRF_MODE_PLAIN or MRF_MODE_GRAPH */
+    int mode;               /* MRF_MODE_PLAIN, MRF_MODE_GRAPH or MRF_MODE_QUEUE */
     uint32_t graph_rounds;  /* rounds per hipGraph replay (graph form) */
     uint32_t delta;         /* delta-stepping window (plain form; UINT32_MAX = off) */
     double sample_ms;       /* wall time of the chosen configuration on the sample */

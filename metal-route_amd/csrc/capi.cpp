@@ -1035,10 +1035,12 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
       if (mrf::lds_bytes_for(sh[0], sh[1], dims.layers) > 160 * 1024 - 64) continue;
       for (uint32_t threads : {128u, 256u})
         for (uint32_t ml : {4u, 32u})
-          for (int mode : {(int)MRF_MODE_PLAIN, (int)MRF_MODE_GRAPH}) {
+          for (int mode : {(int)MRF_MODE_PLAIN, (int)MRF_MODE_GRAPH, (int)MRF_MODE_QUEUE}) {
             std::vector<std::pair<uint32_t, uint32_t>> forms;  // (graph rounds, delta)
             if (mode == MRF_MODE_GRAPH) {
               forms = {{4, kMax}, {16, kMax}};
+            } else if (mode == MRF_MODE_QUEUE) {
+              forms = {{8, kMax}};  // one launch: no rounds to group, no delta
             } else {
               forms = {{8, kMax}};
               for (double crossings : {2.0, 8.0})

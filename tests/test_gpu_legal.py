@@ -47,14 +47,16 @@ def snapshot(s, paths, groups, committed):
 CASES = [(64, 2, 24, 41, (16, 16, 256, 32)), (96, 4, 20, 42, (16, 16, 256, 32)), (72, 3, 16, 43, (8, 4, 64, 32))]
 
 
+@pytest.mark.parametrize("mode", [mrfield.MODE_PLAIN, mrfield.MODE_QUEUE])
 @pytest.mark.parametrize("side,layers,n,seed,tiles", CASES)
-def test_legal_batch_matches_oracle(side, layers, n, seed, tiles):
+def test_legal_batch_matches_oracle(mode, side, layers, n, seed, tiles):
     s = mrfield.synth_board(side, layers, n, seed)
     nets = s.nets()
     groups = [i // 2 for i in range(len(nets))]  # pairs of nets share a group
     eng = mrfield.Engine(0)
     try:
         eng.set_tiles(*tiles)
+        eng.set_mode(mode)
         board = mrfield.Board(eng, s.grid, s.edges, None)
         try:
             alone = board.solve(nets)

@@ -108,13 +108,15 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("mode", [mrfield.MODE_PLAIN, mrfield.MODE_QUEUE])
 @pytest.mark.parametrize("side,layers,n,seed,fused,with_halo,saturate,pfac,tiles", CASES)
-def test_negotiated_batch_matches_oracle(side, layers, n, seed, fused, with_halo, saturate, pfac, tiles):
+def test_negotiated_batch_matches_oracle(mode, side, layers, n, seed, fused, with_halo, saturate, pfac, tiles):
     s = mrfield.synth_board(side, layers, n, seed)
     nets = s.nets()
     eng = mrfield.Engine(0)
     try:
         eng.set_tiles(*tiles)
+        eng.set_mode(mode)
         history, present, halo, own_paths, own_halos = snapshot(s, eng, nets, seed, with_halo, saturate)
         # the fused batch passes no own path and counts the copper in the own halo
         if fused:

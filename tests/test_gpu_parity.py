@@ -2,8 +2,8 @@
 
 Integer work, so the bar is bit-exact: dist and hops labels of every cell,
 path cells, and search costs.  Tile shapes are varied so fields span many
-tiles (halo exchange, ragged edge tiles), and both solve forms (plain rounds
-and the persistent work queue) are checked.
+tiles (halo exchange, ragged edge tiles), and the solve forms (plain rounds,
+hipGraph rounds and the persistent work-queue launch) are checked.
 """
 from __future__ import annotations
 
@@ -22,7 +22,7 @@ SHAPES = [(32, 32, 256, 32), (8, 4, 64, 8), (16, 8, 128, 3), (4, 1, 64, 2)]
 # is shown.  The graph form replaces it.
 PERSISTENT_SKIP = pytest.param(mrfield.MODE_PERSISTENT, marks=pytest.mark.skip(
     reason="persistent grid-barrier form disabled: did not return on MI355X; cause not yet shown"))
-MODES = [mrfield.MODE_PLAIN, PERSISTENT_SKIP, mrfield.MODE_GRAPH]
+MODES = [mrfield.MODE_PLAIN, PERSISTENT_SKIP, mrfield.MODE_GRAPH, mrfield.MODE_QUEUE]
 
 
 @pytest.fixture(scope="module")
