@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="pick tiles / round cap / form with the library's autotuner (always on for config 3)")
     ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-sides", default="128,256,512,1024",
+                    help="board sides of the serving mix (boards cycle through them)")
+    ap.add_argument("--admit-small", type=int, default=0,
+                    help="serving: tiles up to which a request may go ahead (0 = library default, 1 = arrival order)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-seam", action="store_true", help="skip the provider-seam measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -359,7 +363,9 @@ def serve_main(a, tiles):
     import serve_bench
 
     t0 = time.perf_counter()
-    r = serve_bench.run(seed=a.seed, n_req=a.serve_requests, tiles=tiles)
+    sides = tuple(int(v) for v in a.serve_sides.split(","))
+    r = serve_bench.run(seed=a.seed, n_req=a.serve_requests, tiles=tiles, sides=sides,
+                        admit_small_tiles=a.admit_small)
     top = r["rows"][-1]
     out = {
         "metric": "serving nets/sec, Poisson request mix on resident boards (sides 128-1024, 2-8 layers, "
@@ -375,9 +381,10 @@ def serve_main(a, tiles):
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": f"{r['requests']} requests over {len(r['boards'])} resident boards, "
-                               f"{r['nets']} nets, Poisson arrivals at 0.5x-4x the one-request-at-a-time "
-                               f"capacity (seed {a.seed})", "boards": r["boards"], "tiles": a.tiles},
+        "config": {"workload": f"{r['requests']} requests over {len(r['boards'])} resident boards (sides "
+                               f"{a.serve_sides}), {r['nets']} nets, Poisson arrivals at 0.5x-4x the "
+                               f"one-request-at-a-time capacity (seed {a.seed})", "boards": r["boards"],
+                   "tiles": a.tiles, "admit_small_tiles": a.admit_small or "default"},
         "rows": r["rows"],
         "speedup_vs_plain_at_top_load": round(top["serve"]["nets_s"] / top["plain"]["nets_s"], 3),
         "paths_equal_plain": r["paths_equal_plain"],

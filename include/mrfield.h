@@ -399,6 +399,12 @@ typedef struct {
      * so a large request runs nearly alone and later ones fill its tail.
      * 0 = six workgroups per CU; UINT32_MAX = admit whenever memory allows. */
     uint32_t admit_below_tiles;
+    /* Size-aware admission: a queued request whose pass covers at most this
+     * many tiles is admitted even while the device is paced, and ahead of
+     * earlier requests that wait for memory or pacing, so small boards relax
+     * beside a large one instead of queueing behind it.  0 = 16 x the pacing
+     * threshold; 1 = no request goes ahead (arrival order only). */
+    uint32_t admit_small_tiles;
 } mrf_server_config;
 
 typedef struct {
@@ -416,6 +422,7 @@ typedef struct {
     size_t label_bytes;       /* admission budget */
     size_t label_bytes_peak;  /* most label bytes admitted at once */
     uint32_t inflight_peak;   /* most requests admitted at once */
+    uint64_t admitted_ahead;  /* requests admitted ahead of an earlier arrival */
 } mrf_server_stats;
 
 int mrf_server_create(int device, const mrf_server_config *cfg, mrf_server **out, mrf_error *err);

@@ -119,11 +119,22 @@ struct mrf_server {
   uint64_t next_req = 1;
   uint32_t max_inflight = 256;
   uint32_t admit_below = 0;  // admission pacing: active tiles under which the device takes more work
+  uint64_t admit_small = 0;  // requests of at most this many tiles are admitted despite the pacing
   size_t label_bytes = 0;
   mrf_server_stats st{};
 };
 
 namespace {
+
+uint64_t pass_tiles(const mrf_server* S, const ServeReq& r) {
+  const uint32_t tw = S->c.shape.tw, th = S->c.shape.th;
+  uint64_t t = 0;
+  for (size_t k = 0; k < r.idx.size(); ++k) {
+    const mrf_window& w = r.pass_wins[k];
+    t += (uint64_t)((w.x1 - w.x0 + tw) / tw) * ((w.y1 - w.y0 + th) / th);
+  }
+  return t;
+}
 
 void set_err(mrf_error* err, int code, const std::string& msg, uint32_t net = 0) {
   if (!err) return;
@@ -131,6 +142,10 @@ void set_err(mrf_error* err, int code, const std::string& msg, uint32_t net = 0)
   err->net = net;
   snprintf(err->message, sizeof(err->message), "%s", msg.c_str());
 }
+
+// Tiles of a request's current pass (its fields' windows at the server's
+// tile shape).
+uint64_t pass_tiles(const mrf_server* S, const ServeReq& r);
 
 std::vector<mrf::FieldReq> field_reqs(const ServeReq& r) {
   std::vector<mrf::FieldReq> out(r.idx.size());
@@ -285,22 +300,37 @@ void worker_loop(mrf_server* S) {
       if (S->stop) return;
     }
     try {
-      // Admission, in arrival order, while the budget and the slots allow
-      // and the running requests leave the device room (active tiles of the
-      // next round, plus the seeds of requests admitted just now).
+      // Admission at the round boundary.  Requests go in arrival order while
+      // the budget and the slots allow and the running requests leave the
+      // device room (active tiles of the next round, plus the seeds of
+      // requests admitted just now).  A small request (at most admit_small
+      // tiles over its pass) is also admitted when the device is paced or
+      // when the requests ahead of it wait for memory: it relaxes beside a
+      // large request instead of queueing behind it.
+      std::vector<ServeReq*> unfit;  // waiting for memory since this boundary
       for (;;) {
         ServeReq* r = nullptr;
         {
           std::lock_guard<std::mutex> lk(S->mu);
           if (S->queue.empty() || S->running.size() >= S->max_inflight) break;
-          if (!S->running.empty() && mrf::driver_next_tiles(S->d) >= S->admit_below) break;
-          r = S->queue.front();
+          const bool paced = !S->running.empty() && mrf::driver_next_tiles(S->d) >= S->admit_below;
+          bool head = true;  // the first request not waiting for memory
+          for (ServeReq* q : S->queue) {
+            if (std::find(unfit.begin(), unfit.end(), q) != unfit.end()) continue;
+            if ((head && !paced) || pass_tiles(S, *q) <= S->admit_small) {
+              r = q;
+              break;
+            }
+            head = false;
+          }
         }
+        if (!r) break;
+        auto unqueue = [&]() { S->queue.erase(std::find(S->queue.begin(), S->queue.end(), r)); };
         std::vector<mrf::FieldReq> reqs = field_reqs(*r);
         const uint64_t words = mrf::driver_group_words(S->d, reqs);
         if (words > S->d.la.cap()) {
           std::lock_guard<std::mutex> lk(S->mu);
-          S->queue.pop_front();
+          unqueue();
           complete(S, r, MRF_BACKEND_UNAVAILABLE, "request needs more label memory than the server budget");
           S->cv_done.notify_all();
           continue;
@@ -308,15 +338,19 @@ void worker_loop(mrf_server* S) {
         int g = -1;
         if (!mrf::driver_admit(S->d, reqs, g)) {
           std::lock_guard<std::mutex> lk(S->mu);
-          if (!S->running.empty()) break;  // wait for memory to free up
+          if (!S->running.empty()) {  // wait for memory to free up; a smaller one may fit
+            unfit.push_back(r);
+            continue;
+          }
           // nothing else holds device memory: the request can never fit
-          S->queue.pop_front();
+          unqueue();
           complete(S, r, MRF_BACKEND_UNAVAILABLE, "request exceeds the server's item, field or pool capacity");
           S->cv_done.notify_all();
           continue;
         }
         std::lock_guard<std::mutex> lk(S->mu);
-        S->queue.pop_front();
+        if (S->queue.front() != r) S->st.admitted_ahead++;
+        unqueue();
         r->group = g;
         r->state = 1;
         if (!r->admitted_once) {
@@ -436,6 +470,7 @@ int mrf_server_create(int device, const mrf_server_config* cfg, mrf_server** out
     S->label_bytes = lb;
     S->max_inflight = k.max_inflight ? k.max_inflight : 256;
     S->admit_below = k.admit_below_tiles ? k.admit_below_tiles : 6u * (uint32_t)S->c.cus;
+    S->admit_small = k.admit_small_tiles ? k.admit_small_tiles : 16ull * S->admit_below;
     mrf::driver_init(S->d, S->c, lb, k.max_boards ? k.max_boards : 64, k.max_layers ? k.max_layers : 8,
                      S->max_inflight);
     S->st.label_bytes = lb;

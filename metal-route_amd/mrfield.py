@@ -87,7 +87,7 @@ class TuningS(C.Structure):
 class ServerConfigS(C.Structure):
     _fields_ = [("label_bytes", C.c_size_t), ("max_boards", C.c_uint32), ("max_layers", C.c_uint32),
                 ("max_inflight", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
-                ("threads", C.c_uint32), ("admit_below_tiles", C.c_uint32)]
+                ("threads", C.c_uint32), ("admit_below_tiles", C.c_uint32), ("admit_small_tiles", C.c_uint32)]
 
 
 class RequestStatsS(C.Structure):
@@ -98,7 +98,8 @@ class RequestStatsS(C.Structure):
 class ServerStatsS(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("rounds", C.c_uint64), ("tile_visits", C.c_uint64), ("kernel_ms", C.c_double),
-                ("label_bytes", C.c_size_t), ("label_bytes_peak", C.c_size_t), ("inflight_peak", C.c_uint32)]
+                ("label_bytes", C.c_size_t), ("label_bytes_peak", C.c_size_t), ("inflight_peak", C.c_uint32),
+                ("admitted_ahead", C.c_uint64)]
 
 
 class SynthS(C.Structure):
@@ -732,10 +733,10 @@ class Server:
 
     def __init__(self, device: int = 0, label_bytes: int = 0, max_boards: int = 0, max_layers: int = 0,
                  max_inflight: int = 0, tile_w: int = 0, tile_h: int = 0, threads: int = 0,
-                 admit_below_tiles: int = 0):
+                 admit_below_tiles: int = 0, admit_small_tiles: int = 0):
         self._h = C.c_void_p()
         cfg = ServerConfigS(label_bytes, max_boards, max_layers, max_inflight, tile_w, tile_h, threads,
-                            admit_below_tiles)
+                            admit_below_tiles, admit_small_tiles)
         err = ErrorS()
         _check(lib().mrf_server_create(device, C.byref(cfg), C.byref(self._h), C.byref(err)), err)
         self._boards = {}

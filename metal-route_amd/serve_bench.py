@@ -29,11 +29,11 @@ import mrfield
 SIDES = (128, 256, 512, 1024)
 
 
-def make_workload(seed: int, n_boards: int, n_req: int, pool: int = 512):
+def make_workload(seed: int, n_boards: int, n_req: int, pool: int = 512, sides=SIDES):
     rng = np.random.default_rng(seed)
     boards = []
     for b in range(n_boards):
-        side = SIDES[b % len(SIDES)]
+        side = sides[b % len(sides)]
         layers = int(rng.integers(2, 9))
         boards.append(mrfield.synth_board(side, layers, pool, seed * 1000 + b))
     nets = [s.nets() for s in boards]
@@ -111,8 +111,9 @@ def same_paths(a, b) -> bool:
 
 
 def run(seed: int = 1, n_boards: int = 8, n_req: int = 96, loads=(0.5, 1.0, 2.0, 4.0),
-        tiles=(16, 16, 256, 32), label_gib: int = 96, admit_below_tiles: int = 0):
-    boards, reqs = make_workload(seed, n_boards, n_req)
+        tiles=(16, 16, 256, 32), label_gib: int = 96, admit_below_tiles: int = 0, admit_small_tiles: int = 0,
+        sides=SIDES):
+    boards, reqs = make_workload(seed, n_boards, n_req, sides=tuple(sides))
     total_nets = sum(len(n) for _, n in reqs)
     eng = mrfield.Engine(0)
     eng.set_tiles(*tiles)
@@ -121,7 +122,7 @@ def run(seed: int = 1, n_boards: int = 8, n_req: int = 96, loads=(0.5, 1.0, 2.0,
     eng.close()
     mu = 1.0 / float(np.mean(service))  # plain form's capacity, requests/s
     srv = mrfield.Server(0, label_bytes=label_gib << 30, max_layers=8, tile_w=tiles[0], tile_h=tiles[1],
-                         admit_below_tiles=admit_below_tiles,
+                         admit_below_tiles=admit_below_tiles, admit_small_tiles=admit_small_tiles,
                          threads=tiles[2])
     ids = [srv.add_board(s.grid, s.edges, s.pricing()) for s in boards]
     # warm-up: one request through the server
@@ -168,5 +169,6 @@ def run(seed: int = 1, n_boards: int = 8, n_req: int = 96, loads=(0.5, 1.0, 2.0,
         "plain_mean_service_ms": round(float(np.mean(service)) * 1e3, 3),
         "rows": rows,
         "paths_equal_plain": bool(equal),
-        "server": {k: st[k] for k in ("rounds", "tile_visits", "kernel_ms", "label_bytes_peak", "inflight_peak")},
+        "server": {k: st[k] for k in ("rounds", "tile_visits", "kernel_ms", "label_bytes_peak", "inflight_peak",
+                                      "admitted_ahead")},
     }

@@ -163,3 +163,41 @@ def test_errors_fail_closed(boards):
             srv.submit(ids[0], s.nets()[:1])
     finally:
         srv.close()
+
+
+def test_small_requests_go_ahead_of_a_large_one():
+    """Size-aware admission: with the device paced by a large request (pacing
+    threshold 1 tile), small requests that arrive after a large queued one
+    are admitted ahead of it and finish first; every result still equals the
+    oracle's.  With admit_small_tiles = 1 arrival order holds."""
+    big = mrfield.synth_board(384, 4, 48, 21)
+    small = mrfield.synth_board(64, 2, 32, 22)
+    big_nets = big.nets()
+    small_reqs = [small.nets()[k:k + 8] for k in range(0, 32, 8)]
+    exp_small = [oracle_paths(small, nets) for nets in small_reqs]
+    # small requests: 8 full 64x64 fields (at most 128 tiles); the large ones:
+    # 24 full 384x384 fields (thousands of tiles)
+    for admit_small, expect_ahead in ((1024, True), (1, False)):
+        # pacing threshold 1: while anything runs, only small requests enter
+        srv = mrfield.Server(0, label_bytes=8 << 30, admit_below_tiles=1, admit_small_tiles=admit_small)
+        try:
+            ib = srv.add_board(big.grid, big.edges, big.pricing())
+            isml = srv.add_board(small.grid, small.edges, small.pricing())
+            r0 = srv.submit(ib, big_nets[:24])
+            r1 = srv.submit(ib, big_nets[24:])  # queued behind the running request
+            rs = [srv.submit(isml, nets) for nets in small_reqs]
+            got_small = [srv.wait(r) for r in rs]
+            got_big = [srv.wait(r0), srv.wait(r1)]
+            st = srv.stats()
+        finally:
+            srv.close()
+        for (paths, _), exp in zip(got_small, exp_small):
+            assert paths == exp
+        if expect_ahead:
+            assert st["admitted_ahead"] >= 1
+            # the small requests did not wait for the second large one
+            assert max(s["total_ms"] for _, s in got_small) < got_big[1][1]["total_ms"]
+        else:
+            assert st["admitted_ahead"] == 0
+    ref_big = oracle_paths(big, big_nets[:24:4])
+    assert [got_big[0][0][i] for i in range(0, 24, 4)] == ref_big
