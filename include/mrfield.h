@@ -472,6 +472,109 @@ int mrf_synth_config(int config, uint64_t seed, uint32_t scale_nets, mrf_synth *
 int mrf_synth_board(uint32_t side, uint32_t layers, uint32_t nets, uint64_t seed, mrf_synth *out);
 void mrf_free_synth(mrf_synth *s);
 
+/* ---- SimpleRouteJson boards ---------------------------------------------------
+ * A tscircuit SimpleRouteJson problem rasterised into the engine's inputs: the
+ * Hanan grid, its obstacle / via / board-outline planes and the chained
+ * two-point nets with their own-pad cells (RasterizedProblem,
+ * crates/mr-srj/src/lib.rs:1303-1320).  Host code in
+ * metal-route_amd/csrc/srj.cpp; a board is built once, then solved many times
+ * (`cost`, `via_forbidden`, `board_constraint` go into mrf_grid as they are). */
+enum {
+    /* The CLI's rule selection (route_problem_impl, crates/mr-cli/src/lib.rs:751-834):
+     * the coherent typed physical rules when uniform_physical_rules resolves,
+     * else rasterize_with_layers with ceil(minClearance / resolution) cells,
+     * the minClearance rule (default 0.15 mm) and a 0.45 mm via pad. */
+    MRF_SRJ_ROUTE = 0,
+    /* rasterize_with_layers (crates/mr-srj/src/lib.rs:1364-1384) with the
+     * clearance_cells / min_clearance_mm / via_pad_mm given here; all zero is
+     * rasterize() (:1323-1334). */
+    MRF_SRJ_LAYERS = 1,
+    /* rasterize_with_uniform_physical_rules (:1400-1426); MRF_BACKEND_UNAVAILABLE
+     * when the problem's typed rules do not resolve. */
+    MRF_SRJ_PHYSICAL = 2
+};
+
+/* Resolved uniform physical rules (UniformPhysicalRules, crates/mr-srj/src/lib.rs:252-265);
+ * a negative optional value means None. */
+typedef struct mrf_srj_rules {
+    double trace_width_mm, obstacle_margin_mm, trace_to_pad_clearance_mm, via_to_pad_clearance_mm;
+    double pad_to_pad_clearance_mm; /* < 0: None */
+    double via_pad_diameter_mm, via_hole_diameter_mm;
+    double via_hole_to_hole_clearance_mm; /* < 0: None */
+} mrf_srj_rules;
+
+typedef struct {
+    int mode;                 /* MRF_SRJ_* */
+    double resolution;        /* <= 0: longer bounds span / 64 (default_resolution,
+                                 crates/mr-cli/src/lib.rs:462-473) */
+    uint32_t layers;          /* 0: the problem's layerCount */
+    uint32_t clearance_cells; /* MRF_SRJ_LAYERS only */
+    double min_clearance_mm;  /* MRF_SRJ_LAYERS only */
+    double via_pad_mm;        /* MRF_SRJ_LAYERS only */
+    uint64_t cell_budget;     /* planar cell ceiling of the fill lanes: 0 = the
+                                 MR_CELL_BUDGET rule (crates/mr-srj/src/lib.rs:728-792,
+                                 160,000 when unset), UINT64_MAX = no ceiling */
+    int model;                /* MRF_SRJ_LAYERS: 0 = select_clearance_raster_model
+                                 (lib.rs:689-726), 1 = legacy rounded, 2 = exact physical */
+    const struct mrf_srj_rules *rules; /* MRF_SRJ_PHYSICAL: the rules to project, as
+                                 rasterize_with_uniform_physical_rules takes them;
+                                 NULL = resolve the problem's own */
+} mrf_srj_params;
+
+/* NetEndpoints (crates/mr-core/src/lib.rs:520-537) with its name. */
+typedef struct {
+    const char *name; /* conn | conn#k | root#conn | root#conn#k (lib.rs:2138-2155) */
+    uint32_t src, dst;
+    uint32_t *pads; /* passable_pads, ascending */
+    size_t n_pads;
+    uint32_t *via_pads; /* via_passable_pads, ascending, a subset of pads */
+    size_t n_via_pads;
+} mrf_srj_net;
+
+
+typedef struct {
+    mrf_dims dims;
+    double *x_lines; /* dims.w line positions (Mapping::x_lines) */
+    double *y_lines; /* dims.h */
+    uint32_t *cost;  /* dims cells: 1, or UINT32_MAX (OBSTACLE) */
+    uint8_t *via_forbidden;    /* NULL: the empty mask */
+    uint8_t *board_constraint; /* NULL: the empty mask */
+    size_t n_nets;
+    mrf_srj_net *nets;
+    size_t n_pins;
+    uint32_t *pin_cells; /* pin_points keys, ascending */
+    double *pin_xy;      /* 2 * n_pins: the exact (x, y) of each */
+    /* The rules the raster was built with. */
+    double resolution, trace_width_mm, min_clearance_mm, via_pad_mm;
+    uint32_t clearance_cells;
+    int exact_model; /* ClearanceRasterModel::ExactPhysical (lib.rs:673-726) */
+    int physical;    /* 1 when the typed uniform rules were applied */
+    mrf_srj_rules rules;
+    uint32_t fill_dropped; /* fill lanes the cell budget removed */
+} mrf_srj_problem;
+
+/* Parse and rasterise one problem.  MRF_BACKEND_UNAVAILABLE with a message
+ * for malformed JSON, a missing required field or a bad resolution. */
+int mrf_srj_rasterize(const char *json, size_t len, const mrf_srj_params *params, mrf_srj_problem **out,
+                      mrf_error *err);
+void mrf_srj_free(mrf_srj_problem *p);
+/* SimpleRouteJson::uniform_physical_rules (crates/mr-srj/src/lib.rs:286-528):
+ * *ok = 1 and the rules when the typed profile is coherent, else *ok = 0. */
+int mrf_srj_uniform_rules(const char *json, size_t len, mrf_srj_rules *out, int *ok, mrf_error *err);
+/* The board-outline projection alone (BoardOutlineConstraint::from_srj and
+ * raster_mask_plane, crates/mr-srj/src/board_outline.rs:72-108, :223-229) over
+ * given sorted line arrays: *state = 0 when the problem declares no outline
+ * contract (out untouched), 1 when the nx*ny plane (row-major, y outer) is
+ * written, -1 when the contract is malformed (every plane bit set). */
+int mrf_srj_outline_plane(const char *json, size_t len, double trace_width_mm, double via_pad_mm,
+                          const double *x_lines, size_t nx, const double *y_lines, size_t ny, uint8_t *out,
+                          int *state, mrf_error *err);
+/* The fill-lane budget step alone (enforce_budget, crates/mr-srj/src/lib.rs:1047-1128)
+ * over sorted feature lines and fill candidates; the kept fill lanes are written
+ * back in place and their counts returned in *n_x_fill / *n_y_fill. */
+int mrf_srj_enforce_budget(const double *x_features, size_t nxf, const double *y_features, size_t nyf,
+                           double *x_fill, size_t *n_x_fill, double *y_fill, size_t *n_y_fill, uint64_t budget);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,34 @@ class NegotiationS(C.Structure):
     ]
 
 
+class SrjParamsS(C.Structure):
+    _fields_ = [("mode", C.c_int), ("resolution", C.c_double), ("layers", C.c_uint32),
+                ("clearance_cells", C.c_uint32), ("min_clearance_mm", C.c_double), ("via_pad_mm", C.c_double),
+                ("cell_budget", C.c_uint64), ("model", C.c_int), ("rules", C.c_void_p)]
+
+
+class SrjNetS(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("src", C.c_uint32), ("dst", C.c_uint32), ("pads", U32P),
+                ("n_pads", C.c_size_t), ("via_pads", U32P), ("n_via_pads", C.c_size_t)]
+
+
+class SrjRulesS(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("trace_width_mm", "obstacle_margin_mm", "trace_to_pad_clearance_mm",
+                                          "via_to_pad_clearance_mm", "pad_to_pad_clearance_mm",
+                                          "via_pad_diameter_mm", "via_hole_diameter_mm",
+                                          "via_hole_to_hole_clearance_mm")]
+
+
+class SrjProblemS(C.Structure):
+    _fields_ = [("dims", Dims), ("x_lines", C.POINTER(C.c_double)), ("y_lines", C.POINTER(C.c_double)),
+                ("cost", U32P), ("via_forbidden", U8P), ("board_constraint", U8P), ("n_nets", C.c_size_t),
+                ("nets", C.POINTER(SrjNetS)), ("n_pins", C.c_size_t), ("pin_cells", U32P),
+                ("pin_xy", C.POINTER(C.c_double)), ("resolution", C.c_double), ("trace_width_mm", C.c_double),
+                ("min_clearance_mm", C.c_double), ("via_pad_mm", C.c_double), ("clearance_cells", C.c_uint32),
+                ("exact_model", C.c_int), ("physical", C.c_int), ("rules", SrjRulesS),
+                ("fill_dropped", C.c_uint32)]
+
+
 class MrfError(RuntimeError):
     def __init__(self, code: int, message: str, net: int = 0):
         super().__init__(f"[{code}] {message}")
@@ -196,6 +224,16 @@ def lib():
         "mrf_synth_config": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(SynthS), E]),
         "mrf_synth_board": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(SynthS)]),
         "mrf_free_synth": (None, [C.POINTER(SynthS)]),
+        "mrf_srj_rasterize": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(SrjParamsS),
+                                        C.POINTER(C.POINTER(SrjProblemS)), E]),
+        "mrf_srj_free": (None, [C.POINTER(SrjProblemS)]),
+        "mrf_srj_uniform_rules": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(SrjRulesS), C.POINTER(C.c_int), E]),
+        "mrf_srj_outline_plane": (C.c_int, [C.c_char_p, C.c_size_t, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                            C.c_size_t, C.POINTER(C.c_double), C.c_size_t, U8P, C.POINTER(C.c_int),
+                                            E]),
+        "mrf_srj_enforce_budget": (C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_double), C.c_size_t,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_size_t), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_size_t), C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -860,3 +898,165 @@ def synth_board(side: int, layers: int, nets: int, seed: int) -> Synth:
     if rc != OK:
         raise MrfError(rc, "synth_board failed")
     return _from_synth(s)
+
+
+# ---- SimpleRouteJson boards -------------------------------------------------
+
+SRJ_ROUTE, SRJ_LAYERS, SRJ_PHYSICAL = 0, 1, 2
+BOARD_TRACE_NODE, BOARD_VIA_NODE = 1, 2
+BOARD_EDGE_NEG_Y, BOARD_EDGE_NEG_X, BOARD_EDGE_POS_X, BOARD_EDGE_POS_Y = 4, 8, 16, 32
+
+
+@dataclass
+class SrjNet:
+    """NetEndpoints with its name (crates/mr-core/src/lib.rs:520-537)."""
+
+    name: str
+    src: int
+    dst: int
+    pads: List[int]
+    via_pads: List[int]
+
+    def net(self) -> Net:
+        return Net(self.src, self.dst, self.pads, self.via_pads)
+
+
+@dataclass
+class SrjProblem:
+    """RasterizedProblem (crates/mr-srj/src/lib.rs:1303-1320) plus the rules it
+    was built with."""
+
+    w: int
+    h: int
+    layers: int
+    x_lines: np.ndarray
+    y_lines: np.ndarray
+    cost: np.ndarray
+    via_forbidden: Optional[np.ndarray]
+    board: Optional[np.ndarray]
+    nets: List[SrjNet]
+    pins: dict
+    resolution: float
+    trace_width_mm: float
+    min_clearance_mm: float
+    via_pad_mm: float
+    clearance_cells: int
+    exact_model: bool
+    physical: bool
+    rules: Optional[dict]
+    fill_dropped: int
+
+    def grid(self) -> Grid:
+        return Grid(self.w, self.h, self.layers, self.cost, self.via_forbidden, self.board)
+
+    def idx(self, x: int, y: int, layer: int = 0) -> int:
+        return (layer * self.h + y) * self.w + x
+
+    def is_obstacle(self, cell: int) -> bool:
+        return int(self.cost[cell]) == MAX
+
+
+def _rules_dict(r: SrjRulesS) -> dict:
+    d = {k: getattr(r, k) for k, _ in SrjRulesS._fields_}
+    for k in ("pad_to_pad_clearance_mm", "via_hole_to_hole_clearance_mm"):
+        if d[k] < 0:
+            d[k] = None
+    return d
+
+
+def srj_rasterize(doc, mode: int = SRJ_ROUTE, resolution: float = 0.0, layers: int = 0, clearance_cells: int = 0,
+                  min_clearance_mm: float = 0.0, via_pad_mm: float = 0.0, cell_budget: int = 0, model: int = 0,
+                  rules: Optional[dict] = None) -> SrjProblem:
+    """Rasterise a SimpleRouteJson document (str, bytes or a JSON-able object).
+    ``mode``: SRJ_ROUTE (the CLI's rule selection), SRJ_LAYERS
+    (rasterize_with_layers; all-zero rules = rasterize(); ``model`` 1 / 2
+    forces the legacy / exact clearance model), SRJ_PHYSICAL (``rules`` as
+    srj_uniform_rules returns them, or None to resolve the problem's own)."""
+    import json as _json
+
+    if isinstance(doc, (dict, list)):
+        doc = _json.dumps(doc)
+    data = doc.encode() if isinstance(doc, str) else bytes(doc)
+    rs = None
+    if rules is not None:
+        rs = SrjRulesS(*[(-1.0 if rules[k] is None else float(rules[k])) for k, _ in SrjRulesS._fields_])
+    prm = SrjParamsS(mode, resolution, layers, clearance_cells, min_clearance_mm, via_pad_mm, cell_budget, model,
+                     C.cast(C.pointer(rs), C.c_void_p) if rs is not None else None)
+    out = C.POINTER(SrjProblemS)()
+    err = ErrorS()
+    _check(lib().mrf_srj_rasterize(data, len(data), C.byref(prm), C.byref(out), C.byref(err)), err)
+    try:
+        p = out.contents
+        n = p.dims.w * p.dims.h * p.dims.layers
+        nets = []
+        for i in range(p.n_nets):
+            q = p.nets[i]
+            nets.append(SrjNet(q.name.decode(), int(q.src), int(q.dst),
+                               [int(v) for v in np.ctypeslib.as_array(q.pads, (q.n_pads,))] if q.n_pads else [],
+                               [int(v) for v in np.ctypeslib.as_array(q.via_pads, (q.n_via_pads,))]
+                               if q.n_via_pads else []))
+        pins = {}
+        if p.n_pins:
+            cells = np.ctypeslib.as_array(p.pin_cells, (p.n_pins,))
+            xy = np.ctypeslib.as_array(p.pin_xy, (2 * p.n_pins,))
+            pins = {int(c): (float(xy[2 * k]), float(xy[2 * k + 1])) for k, c in enumerate(cells)}
+        return SrjProblem(
+            int(p.dims.w), int(p.dims.h), int(p.dims.layers),
+            np.ctypeslib.as_array(p.x_lines, (p.dims.w,)).copy(), np.ctypeslib.as_array(p.y_lines, (p.dims.h,)).copy(),
+            np.ctypeslib.as_array(p.cost, (n,)).copy(),
+            np.ctypeslib.as_array(p.via_forbidden, (n,)).copy() if p.via_forbidden else None,
+            np.ctypeslib.as_array(p.board_constraint, (n,)).copy() if p.board_constraint else None,
+            nets, pins, p.resolution, p.trace_width_mm, p.min_clearance_mm, p.via_pad_mm, int(p.clearance_cells),
+            bool(p.exact_model), bool(p.physical), _rules_dict(p.rules) if p.physical else None,
+            int(p.fill_dropped))
+    finally:
+        lib().mrf_srj_free(out)
+
+
+def srj_uniform_rules(doc) -> Optional[dict]:
+    """SimpleRouteJson::uniform_physical_rules (crates/mr-srj/src/lib.rs:286-528)."""
+    import json as _json
+
+    if isinstance(doc, (dict, list)):
+        doc = _json.dumps(doc)
+    data = doc.encode() if isinstance(doc, str) else bytes(doc)
+    r = SrjRulesS()
+    ok = C.c_int()
+    err = ErrorS()
+    _check(lib().mrf_srj_uniform_rules(data, len(data), C.byref(r), C.byref(ok), C.byref(err)), err)
+    return _rules_dict(r) if ok.value else None
+
+
+def srj_enforce_budget(x_features, y_features, x_fill, y_fill, budget: int):
+    """enforce_budget (crates/mr-srj/src/lib.rs:1047-1128): the kept fill lanes."""
+    D = C.POINTER(C.c_double)
+    xf = np.ascontiguousarray(x_features, dtype=np.float64)
+    yf = np.ascontiguousarray(y_features, dtype=np.float64)
+    xs = np.ascontiguousarray(np.array(x_fill, dtype=np.float64))
+    ys = np.ascontiguousarray(np.array(y_fill, dtype=np.float64))
+    nx, ny = C.c_size_t(len(xs)), C.c_size_t(len(ys))
+    rc = lib().mrf_srj_enforce_budget(xf.ctypes.data_as(D), len(xf), yf.ctypes.data_as(D), len(yf),
+                                      xs.ctypes.data_as(D), C.byref(nx), ys.ctypes.data_as(D), C.byref(ny),
+                                      C.c_uint64(budget))
+    if rc != OK:
+        raise MrfError(rc, "enforce_budget failed")
+    return [float(v) for v in xs[:nx.value]], [float(v) for v in ys[:ny.value]]
+
+
+def srj_outline_plane(doc, trace_width_mm: float, via_pad_mm: float, x_lines, y_lines):
+    """The board-outline mask plane over given lines (board_outline.rs:72-108,
+    :223-229): None when the problem declares no outline contract."""
+    import json as _json
+
+    if isinstance(doc, (dict, list)):
+        doc = _json.dumps(doc)
+    data = doc.encode() if isinstance(doc, str) else bytes(doc)
+    D = C.POINTER(C.c_double)
+    xl = np.ascontiguousarray(x_lines, dtype=np.float64)
+    yl = np.ascontiguousarray(y_lines, dtype=np.float64)
+    out = np.zeros(len(xl) * len(yl), dtype=np.uint8)
+    state = C.c_int()
+    err = ErrorS()
+    _check(lib().mrf_srj_outline_plane(data, len(data), trace_width_mm, via_pad_mm, xl.ctypes.data_as(D), len(xl),
+                                       yl.ctypes.data_as(D), len(yl), _p8(out), C.byref(state), C.byref(err)), err)
+    return None if state.value == 0 else out
