@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["fields", "serve", "negotiate"], default="fields",
+    ap.add_argument("--workload", choices=["fields", "serve", "negotiate", "srj"], default="fields",
                     help="fields: one batch per step of --config (default, the headline); "
                          "serve: Poisson request mix through the serving loop vs one request at a time")
     ap.add_argument("--config", type=int, default=2,
@@ -359,6 +359,97 @@ def config0_main(a):
     print(json.dumps(out))
 
 
+def srj_main(a, tiles):
+    """Real boards end to end: every committed SimpleRouteJson fixture
+    (tests/golden/srj: the reference's CLI payloads and corpus boards) is
+    rasterised by the library's producer with the CLI's rule selection
+    (mrf_srj_rasterize) and all its nets routed through the provider seam
+    (mrf_route_isolated_batch, complete fields on the full board) with
+    geometric planar prices round(16 x line gap) and 160 per via.  A step is
+    one pass over every board; paths are checked against the oracle after the
+    timed region."""
+    import glob
+
+    import numpy as np
+
+    import mrfield
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mrcpu
+
+    docs = [(os.path.basename(f), open(f, "rb").read())
+            for f in sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "srj", "*.json")))]
+    eng = mrfield.Engine(0)
+    eng.set_tiles(*tiles)
+    eng.set_mode(FORMS[a.mode])
+
+    def prices(lines):
+        gaps = np.diff(np.asarray(lines, dtype=np.float64))
+        return [int(v) for v in np.floor(gaps * 16.0 + 0.5)]
+
+    def one(data):
+        t0 = time.perf_counter()
+        p = mrfield.srj_rasterize(data, mode=mrfield.SRJ_ROUTE)
+        t1 = time.perf_counter()
+        ed = mrfield.Edges(prices(p.x_lines), prices(p.y_lines), [160] * (p.layers - 1))
+        got = eng.route_isolated_batch(p.grid(), [n.net() for n in p.nets], [(0, 0, p.w - 1, p.h - 1)] * len(p.nets),
+                                       ed)
+        t2 = time.perf_counter()
+        return p, ed, got, t1 - t0, t2 - t1
+
+    for _ in range(max(a.warmup, 1)):
+        for _, d in docs:
+            one(d)
+    r_ms, s_ms, last = [], [], {}
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        for name, d in docs:
+            p, ed, got, tr, ts = one(d)
+            r_ms.append(tr * 1e3)
+            s_ms.append(ts * 1e3)
+            last[name] = (p, ed, got)
+    wall = time.perf_counter() - t_start
+    eng.close()
+    n_nets = sum(len(v[0].nets) for v in last.values())
+    exact, c_s = True, 0.0
+    for name, (p, ed, got) in last.items():
+        og = mrcpu.Grid(p.w, p.h, p.layers, p.cost, p.via_forbidden, p.board)
+        c0 = time.perf_counter()
+        ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(og, mrcpu.Edges(ed.x, ed.y, ed.via)),
+                                         [mrcpu.Net(n.src, n.dst, n.pads, n.via_pads) for n in p.nets], None)
+        c_s += time.perf_counter() - c0
+        for i, r in enumerate(got):
+            exact = exact and ((r is None) == (ref_p[i] is None)) and (r is None or r == (ref_p[i], ref_c[i]))
+    out = {
+        "metric": "boards/s (SimpleRouteJson rasterised + every net routed, bit-exact)",
+        "value": round(len(docs) * a.steps / wall, 3),
+        "unit": "boards/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "committed SimpleRouteJson fixtures (the reference's CLI payloads and corpus boards)",
+        "config": {"workload": f"{len(docs)} boards, {n_nets} two-pin nets, complete fields on the full board",
+                   "form": a.mode, "tiles": ",".join(str(t) for t in tiles)},
+        "nets_per_s": round(n_nets * a.steps / wall, 1),
+        "rasterise_ms_per_board_p50": round(float(np.percentile(r_ms, 50)), 3),
+        "rasterise_ms_per_board_max": round(float(np.max(r_ms)), 3),
+        "route_ms_per_board_p50": round(float(np.percentile(s_ms, 50)), 3),
+        "route_ms_per_board_max": round(float(np.max(s_ms)), 3),
+        "boards": {k: [v[0].w, v[0].h, v[0].layers, len(v[0].nets), sum(r is not None for r in v[2])]
+                   for k, v in last.items()},
+        "paths_bit_exact_vs_oracle": bool(exact),
+        "cpu_baseline": {"value": round(n_nets / c_s, 1) if c_s > 0 else None, "unit": "nets/s", "cores": 1,
+                         "kind": "port", "sample": f"all {n_nets} nets of one pass, oracle alone search "
+                                                   f"(complete fields, full board), 1 thread, {c_s:.2f} s",
+                         "host": host_info()},
+    }
+    print(json.dumps(out))
+
+
 def serve_main(a, tiles):
     import serve_bench
 
@@ -502,6 +593,9 @@ def main():
         return
     if a.workload == "negotiate":
         negotiate_main(a, (tw, th, threads, ml))
+        return
+    if a.workload == "srj":
+        srj_main(a, (tw, th, threads, ml))
         return
     eng = mrfield.Engine(0)
     eng.set_tiles(tw, th, threads, ml)

@@ -1692,7 +1692,7 @@ extern "C" int mrf_srj_enforce_budget(const double* x_features, size_t nxf, cons
 extern "C" int mrf_srj_outline_plane(const char* json, size_t len, double trace_width_mm, double via_pad_mm,
                                      const double* x_lines, size_t nx, const double* y_lines, size_t ny, uint8_t* out,
                                      int* state, mrf_error* err) {
-  if (!json || !state || (nx && !x_lines) || (ny && !y_lines) || (nx * ny && !out)) {
+  if (!json || !state || (nx && !x_lines) || (ny && !y_lines) || (nx && ny && !out)) {
     set_error(err, MRF_BACKEND_UNAVAILABLE, "null input or output");
     return MRF_BACKEND_UNAVAILABLE;
   }
