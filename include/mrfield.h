@@ -259,6 +259,26 @@ size_t mrf_board_bytes(const mrf_board *board);
  * Multi-pin nets: pass pins as sources via mrf_solve_fields. */
 int mrf_solve(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets,
               const mrf_window *windows, int retry_full, mrf_paths *out, mrf_error *err);
+/* Allocation-free form of mrf_solve.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during a call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The call makes no hipMalloc, hipFree, hipHostMalloc or hipEventCreate:
+ * every device buffer of the solve (labels, tile states, work lists,
+ * descriptors, path buffers) is carved from the library's fixed scratch
+ * array on the context's device, mrf_noalloc_scratch_bytes() bytes (64 MiB),
+ * which each device holds until the process ends.  Nets whose labels exceed
+ * 5/8 of the array run in several device batches; a call that still needs
+ * more (one field too large, or too many path cells at once) fails with
+ * MRF_BACKEND_UNAVAILABLE and a message naming the limit.  It runs the plain
+ * form whatever the context's mode; results equal mrf_solve's.  Paths go to
+ * caller arrays: routed[n_nets], search_cost[n_nets], offsets[n_nets + 1],
+ * cells[cells_cap]; *n_cells receives the number of path cells (also when
+ * cells_cap is too small, which fails the call).  Calls that overlap on one
+ * device, from any context, share the array and race: serialise them. */
+int mrf_solve_noalloc(mrf_ctx *ctx, mrf_board *board, const mrf_net *nets, size_t n_nets,
+                      const mrf_window *windows, int retry_full, uint8_t *routed, uint32_t *search_cost,
+                      uint64_t *offsets, uint32_t *cells, size_t cells_cap, size_t *n_cells, mrf_error *err);
+size_t mrf_noalloc_scratch_bytes(void);
 /* Fields on a resident board; labels stay on the device until the next call,
  * read them with mrf_read_field. */
 int mrf_solve_fields(mrf_ctx *ctx, mrf_board *board, const uint32_t *const *sources,

@@ -649,8 +649,9 @@ static HostGrid board_host(const mrf_board* bd) {
 
 // Nets on a resident board: windows (full board when NULL), then the
 // unreachable nets on the full board when retry_full.  Caller holds the lane.
-static void solve_board(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
-                        int retry_full, const std::vector<NetPlan>* prices, mrf_paths* out) {
+static void solve_board_batch(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets,
+                              const mrf_window* windows, int retry_full, const std::vector<NetPlan>* prices,
+                              Batch& all) {
   HostGrid g = board_host(bd);
   mrf_dims dims = bd->dims;
   if (n_nets && !nets) throw Error(MRF_BACKEND_UNAVAILABLE, "null nets");
@@ -661,7 +662,6 @@ static void solve_board(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t
   for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
   std::vector<mrf_window> wins(n_nets, full);
   if (windows) wins.assign(windows, windows + n_nets);
-  Batch all;
   solve_nets(ctx->c, bd->b, g, nets, idx, wins, false, nullptr, all, prices);
   if (retry_full && windows) {
     std::vector<size_t> retry_idx;
@@ -679,8 +679,34 @@ static void solve_board(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t
       }
     }
   }
+}
+
+static void solve_board(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
+                        int retry_full, const std::vector<NetPlan>* prices, mrf_paths* out) {
+  Batch all;
+  solve_board_batch(ctx, bd, nets, n_nets, windows, retry_full, prices, all);
   export_paths(all, out);
 }
+
+// The allocation-free mode of a context for one call: solves carve their
+// device buffers from the device's scratch array, run the plain form, and
+// chunk labels within a budget that leaves the rest of the array for the
+// other buffers of a solve and its paths.
+struct ArenaScope {
+  mrf::Ctx& c;
+  int mode;
+  size_t budget;
+  explicit ArenaScope(mrf::Ctx& cx) : c(cx), mode(cx.mode), budget(cx.label_budget) {
+    mrf::arena_begin(c);
+    c.mode = mrf::MODE_PLAIN;
+    c.label_budget = std::min(budget, mrf::scratch_bytes() / 8 * 5);
+  }
+  ~ArenaScope() {
+    c.mode = mode;
+    c.label_budget = budget;
+    mrf::arena_end(c);
+  }
+};
 
 int mrf_solve(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
               int retry_full, mrf_paths* out, mrf_error* err) {
@@ -1086,6 +1112,46 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
     out->delta = best.delta;
     out->sample_ms = best_ms;
     out->tried = tried;
+    if (err) set_err(err, MRF_OK, "ok");
+    return MRF_OK;
+  } catch (const Error& e) {
+    return fail(err, e);
+  } catch (const std::exception& e) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, e.what());
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+}
+
+size_t mrf_noalloc_scratch_bytes(void) { return mrf::scratch_bytes(); }
+
+int mrf_solve_noalloc(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets, const mrf_window* windows,
+                      int retry_full, uint8_t* routed, uint32_t* search_cost, uint64_t* offsets, uint32_t* cells,
+                      size_t cells_cap, size_t* n_cells, mrf_error* err) {
+  if (!ctx || !bd || bd->ctx != ctx || !offsets || !n_cells || (n_nets && (!routed || !search_cost)) ||
+      (cells_cap && !cells)) {
+    set_err(err, MRF_BACKEND_UNAVAILABLE, "null context, board or output");
+    return MRF_BACKEND_UNAVAILABLE;
+  }
+  try {
+    Lane lane(ctx->c);
+    Batch all;
+    {
+      ArenaScope arena(ctx->c);
+      solve_board_batch(ctx, bd, nets, n_nets, windows, retry_full, nullptr, all);
+    }
+    size_t total = 0;
+    for (const auto& p : all.paths) total += p.size();
+    *n_cells = total;
+    if (total > cells_cap)
+      throw Error(MRF_BACKEND_UNAVAILABLE, "path cells (" + std::to_string(total) + ") exceed cells_cap (" +
+                                               std::to_string(cells_cap) + ")");
+    offsets[0] = 0;
+    for (size_t i = 0; i < n_nets; ++i) {
+      routed[i] = all.routed[i];
+      search_cost[i] = all.routed[i] ? all.cost[i] : kMax;
+      std::copy(all.paths[i].begin(), all.paths[i].end(), cells + offsets[i]);
+      offsets[i + 1] = offsets[i] + all.paths[i].size();
+    }
     if (err) set_err(err, MRF_OK, "ok");
     return MRF_OK;
   } catch (const Error& e) {

@@ -257,9 +257,27 @@ struct Ctx {
   // Queue form: the longest a workgroup waits for work or a queue slot, and
   // the longest one runs, before the launch stops with an error (ms).
   uint32_t queue_wait_ms = 2000, queue_run_ms = 20000;
+  // Allocation-free calls (arena set, between arena_begin and arena_end):
+  // every per-solve device buffer is carved from `arena`, the library's
+  // scratch array on this device, instead of grown with hipMalloc; the
+  // buffers made when the context opened (device counters, host-mapped
+  // words, events) and the launch stamp are the context's own.
+  uint8_t* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  Workspace ws_arena;
+  Workspace& W() { return arena ? ws_arena : ws; }
 };
 
 // ---- engine.hip -------------------------------------------------------------
+// The library's scratch array (`scratch` in engine.hip): its device address on
+// `device` (resolved when the first context opens on it; nullptr before) and
+// its size in bytes.
+uint8_t* scratch_ptr(int device);
+size_t scratch_bytes();
+// Allocation-free mode of a context for the duration of one call; arena_begin
+// throws when the device has no scratch address.
+void arena_begin(Ctx& c);
+void arena_end(Ctx& c);
 void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
                   const uint32_t* cost, const uint8_t* via_forbidden, const uint8_t* board,
                   const uint32_t* xe, const uint32_t* ye, const uint32_t* ve,

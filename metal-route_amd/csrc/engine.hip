@@ -1665,6 +1665,105 @@ static void grow(T*& p, size_t& cap, size_t need) {
   cap = n;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free calls: the library's scratch array.  One copy lives on
+// every device the library's code object is loaded on; host code never uses
+// the array's name as an address: scratch_ptr() returns the device address
+// hipGetSymbolAddress gave for that device, resolved once, with the device
+// current, when the first context opens on it.  Calls that overlap on one
+// device share the array and race: the caller serialises them.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+uint8_t* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  return g_scratch[device];
+}
+
+size_t scratch_bytes() { return kScratchBytes; }
+
+// Once per device, with that device current.
+static void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error(1, "the scratch array has no device address");
+  g_scratch[device] = (uint8_t*)p;
+}
+
+void arena_begin(Ctx& c) {
+  uint8_t* p = scratch_ptr(c.device);
+  if (!p) throw Error(1, "no scratch array on this device");
+  Workspace& A = c.ws_arena;
+  A = Workspace();
+  // the context's own buffers, taken from its workspace (never from W(),
+  // which is the arena workspace once `arena` is set)
+  const Workspace& own = c.ws;
+  if (!own.counters || !own.pinned || !own.hostctr || !own.hostctr_dev)
+    throw Error(1, "context buffers missing for an allocation-free call");
+  A.counters = own.counters;
+  A.pinned = own.pinned;
+  A.hostctr = own.hostctr;
+  A.hostctr_dev = own.hostctr_dev;
+  A.ev0 = own.ev0;
+  A.ev1 = own.ev1;
+  A.ev2 = own.ev2;
+  A.ek0 = own.ek0;
+  A.ek1 = own.ek1;
+  A.stamp = own.stamp;
+  c.arena = p;
+  c.arena_cap = kScratchBytes;
+  c.arena_used = 0;
+}
+
+void arena_end(Ctx& c) {
+  if (!c.arena) return;
+  c.ws.stamp = c.ws_arena.stamp;  // launch stamps keep increasing across calls
+  c.ws_arena = Workspace();
+  c.arena = nullptr;
+  c.arena_cap = c.arena_used = 0;
+}
+
+// A device buffer of at least `need` elements: grown with hipMalloc in the
+// context's own workspace, or, in an allocation-free call, carved from the
+// scratch array (256-byte aligned; too little room is an error).
+template <typename T>
+static void take(Ctx& c, T*& p, size_t& cap, size_t need) {
+  if (!c.arena) {
+    grow(p, cap, need);
+    return;
+  }
+  const size_t bytes = (need * sizeof(T) + 255) & ~size_t(255);
+  if (bytes > c.arena_cap - c.arena_used)
+    throw Error(1, "the call needs more than the " + std::to_string(c.arena_cap >> 20) +
+                       " MiB scratch array of the allocation-free entry point");
+  p = (T*)(c.arena + c.arena_used);
+  c.arena_used += bytes;
+  cap = need;
+}
+
+// In an allocation-free call: every buffer a kernel of the solve will take is
+// set, and each carved one lies inside the scratch array.
+static void check_arena(const Ctx& c, std::initializer_list<std::pair<const void*, size_t>> carved) {
+  if (!c.arena) return;
+  if (!c.ws_arena.counters || !c.ws_arena.hostctr_dev) throw Error(1, "allocation-free call without counters");
+  for (const auto& b : carved) {
+    const uint8_t* q = (const uint8_t*)b.first;
+    if (!q || q < c.arena || q + b.second > c.arena + c.arena_cap)
+      throw Error(1, "allocation-free buffer outside the scratch array");
+  }
+}
+
 template <typename T>
 static T* dev_copy(const T* h, size_t n, hipStream_t s) {
   T* d = nullptr;
@@ -1775,6 +1874,7 @@ void ctx_init(Ctx& c, int device) {
   HIP_OK(hipHostMalloc((void**)&c.ws.hostctr, 64, hipHostMallocMapped | hipHostMallocCoherent));
   memset(c.ws.hostctr, 0, 64);
   HIP_OK(hipHostGetDevicePointer((void**)&c.ws.hostctr_dev, c.ws.hostctr, 0));
+  resolve_scratch(device);
   for (const KernelSet& k : kernel_sets()) {
     HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
@@ -1979,17 +2079,18 @@ size_t field_label_words(const TileShape& s, uint32_t ww, uint32_t wh, uint32_t 
 static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   RelaxArgs A;
   memset(&A, 0, sizeof(A));
-  A.boards = c.ws.boards;
-  A.fields = c.ws.fields;
-  A.item_field = c.ws.item_field;
-  A.pool = c.ws.pool;
-  A.labels = c.ws.labels;
-  A.state = c.ws.state;
-  A.act_cur = c.ws.act;
-  A.act_next = c.ws.act + n_items;
-  A.list_cur = c.ws.list;
-  A.list_next = c.ws.list + n_items;
-  A.counters = c.ws.counters;
+  Workspace& W = c.W();
+  A.boards = W.boards;
+  A.fields = W.fields;
+  A.item_field = W.item_field;
+  A.pool = W.pool;
+  A.labels = W.labels;
+  A.state = W.state;
+  A.act_cur = W.act;
+  A.act_next = W.act + n_items;
+  A.list_cur = W.list;
+  A.list_next = W.list + n_items;
+  A.counters = W.counters;
   A.max_local = c.shape.max_local;
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
@@ -2027,7 +2128,7 @@ static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_
   void* args[] = {&e, &o};
   for (uint32_t r = 0; r < c.graph_rounds; ++r) {
     HIP_OK(hipLaunchKernel(K.list, dim3(grid), dim3(c.shape.threads), args, lds, s));
-    hipLaunchKernelGGL(advance_round, dim3(1), dim3(64), 0, s, c.ws.counters);
+    hipLaunchKernelGGL(advance_round, dim3(1), dim3(64), 0, s, c.W().counters);
   }
   HIP_OK(hipStreamEndCapture(s, &graph));
   HIP_OK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
@@ -2197,17 +2298,28 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   if ((reqs.size() > 1 && lab_off * 8 > c.label_budget) || lab_off * 8 > c.label_budget_default)
     throw Error(1, "label working set exceeds the device budget");
 
-  Workspace& W = c.ws;
-  grow(W.labels, W.labels_cap, std::max<size_t>(lab_off, 1));
-  grow(W.state, W.state_cap, std::max<size_t>(item, 1));
-  grow(W.act, W.act_cap, std::max<size_t>((size_t)item * 2, 2));
+  if (c.arena && c.mode != MODE_PLAIN) throw Error(1, "allocation-free calls run the plain form");
+  if (c.arena) c.arena_used = 0;  // each solve carves the scratch array afresh
+  Workspace& W = c.W();
+  take(c, W.labels, W.labels_cap, std::max<size_t>(lab_off, 1));
+  take(c, W.state, W.state_cap, std::max<size_t>(item, 1));
+  take(c, W.act, W.act_cap, std::max<size_t>((size_t)item * 2, 2));
   const bool keyed = c.mode == MODE_PLAIN && c.delta != kMax;
-  if (keyed) grow(W.key, W.key_cap, std::max<size_t>((size_t)item * 2, 2));
-  grow(W.list, W.list_cap, std::max<size_t>((size_t)item * 2, 2));
-  grow(W.fields, W.fields_cap, std::max<size_t>(reqs.size(), 1));
-  grow(W.item_field, W.item_field_cap, std::max<size_t>(item, 1));
-  grow(W.pool, W.pool_cap, std::max<size_t>(pool.size() + seeds.size(), 1));
-  grow(W.boards, W.boards_cap, std::max<size_t>(c.views.size(), 1));
+  if (keyed) take(c, W.key, W.key_cap, std::max<size_t>((size_t)item * 2, 2));
+  take(c, W.list, W.list_cap, std::max<size_t>((size_t)item * 2, 2));
+  take(c, W.fields, W.fields_cap, std::max<size_t>(reqs.size(), 1));
+  take(c, W.item_field, W.item_field_cap, std::max<size_t>(item, 1));
+  take(c, W.pool, W.pool_cap, std::max<size_t>(pool.size() + seeds.size(), 1));
+  take(c, W.boards, W.boards_cap, std::max<size_t>(c.views.size(), 1));
+  check_arena(c, {{W.labels, std::max<size_t>(lab_off, 1) * 8},
+                  {W.state, std::max<size_t>(item, 1) * 4},
+                  {W.act, std::max<size_t>((size_t)item * 2, 2) * 4},
+                  {W.list, std::max<size_t>((size_t)item * 2, 2) * 4},
+                  {W.fields, std::max<size_t>(reqs.size(), 1) * sizeof(FieldDesc)},
+                  {W.item_field, std::max<size_t>(item, 1) * 4},
+                  {W.pool, std::max<size_t>(pool.size() + seeds.size(), 1) * 4},
+                  {W.boards, std::max<size_t>(c.views.size(), 1) * sizeof(BoardView)}});
+  if (keyed) check_arena(c, {{W.key, std::max<size_t>((size_t)item * 2, 2) * 4}});
   hipStream_t s = c.stream;
   HIP_OK(hipMemcpyAsync(W.boards, c.views.data(), c.views.size() * sizeof(BoardView), hipMemcpyHostToDevice, s));
   if (!layout.empty())
@@ -2265,16 +2377,22 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         if (++W.stamp == 0) W.stamp = 1;
         A.stamp = W.stamp;
         A.round = rounds;
-        if (pairs == 512) harvest();  // bounds the event pool
-        const size_t ei = 2 * pairs++;
-        while (W.round_ev.size() < ei + 2) {
-          hipEvent_t e = nullptr;
-          HIP_OK(hipEventCreate(&e));
-          W.round_ev.push_back(e);
+        // per-round kernel times: not in allocation-free calls, which create
+        // no events
+        const bool per_round = !c.arena;
+        if (per_round && pairs == 512) harvest();  // bounds the event pool
+        const size_t ei = 2 * pairs;
+        if (per_round) {
+          ++pairs;
+          while (W.round_ev.size() < ei + 2) {
+            hipEvent_t e = nullptr;
+            HIP_OK(hipEventCreate(&e));
+            W.round_ev.push_back(e);
+          }
+          HIP_OK(hipEventRecord(W.round_ev[ei], s));
         }
-        HIP_OK(hipEventRecord(W.round_ev[ei], s));
         HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
-        HIP_OK(hipEventRecord(W.round_ev[ei + 1], s));
+        if (per_round) HIP_OK(hipEventRecord(W.round_ev[ei + 1], s));
         volatile uint32_t* hc = W.hostctr;
         const auto spin0 = std::chrono::steady_clock::now();
         while (hc[2] != A.stamp) {
@@ -2372,8 +2490,8 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         ++tail;
         ctl[(3 * k + 2) * kCtrStride] = tail;  // pending: one per queued seed
       }
-      grow(W.qslots, W.qslots_cap, nslots);
-      grow(W.qctl, W.qctl_cap, ctl.size());
+      take(c, W.qslots, W.qslots_cap, nslots);
+      take(c, W.qctl, W.qctl_cap, ctl.size());
       HIP_OK(hipMemcpyAsync(W.qslots, slots.data(), nslots * 8, hipMemcpyHostToDevice, s));
       HIP_OK(hipMemcpyAsync(W.qctl, ctl.data(), ctl.size() * 4, hipMemcpyHostToDevice, s));
       Q.slots = W.qslots;
@@ -2422,10 +2540,11 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
 
 static FieldRead make_read(Ctx& c) {
   FieldRead R;
-  R.boards = c.ws.boards;
-  R.fields = c.ws.fields;
-  R.labels = c.ws.labels;
-  R.state = c.ws.state;
+  Workspace& W = c.W();
+  R.boards = W.boards;
+  R.fields = W.fields;
+  R.labels = W.labels;
+  R.state = W.state;
   R.tw = c.shape.tw;
   R.th = c.shape.th;
   return R;
@@ -2437,16 +2556,17 @@ static void reconstruct_with(Ctx& c, const FieldRead& R, const uint32_t* pool, s
                              std::vector<uint8_t>& routed, std::vector<uint32_t>& cost, std::vector<uint64_t>& off,
                              std::vector<uint32_t>& cells, SolveStats* st) {
   HIP_OK(hipSetDevice(c.device));
-  Workspace& W = c.ws;
+  Workspace& W = c.W();
   hipStream_t s = c.stream;
   routed.assign(n, 0);
   cost.assign(n, kMax);
   off.assign(n + 1, 0);
   cells.clear();
   if (n == 0) return;
-  grow(W.plen, W.plen_cap, n);
-  grow(W.pcost, W.pcost_cap, n);
-  grow(W.poff, W.poff_cap, n + 1);
+  take(c, W.plen, W.plen_cap, n);
+  take(c, W.pcost, W.pcost_cap, n);
+  take(c, W.poff, W.poff_cap, n + 1);
+  check_arena(c, {{W.plen, n * 4}, {W.pcost, n * 4}, {W.poff, (n + 1) * 8}});
   HIP_OK(hipEventRecord(W.ev1, s));
   hipLaunchKernelGGL(path_lengths, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, R, (uint32_t)n, W.plen,
                      W.pcost);
@@ -2461,7 +2581,8 @@ static void reconstruct_with(Ctx& c, const FieldRead& R, const uint32_t* pool, s
   size_t total = off[n];
   cells.resize(total);
   if (total) {
-    grow(W.pcells, W.pcells_cap, total);
+    take(c, W.pcells, W.pcells_cap, total);
+    check_arena(c, {{W.pcells, total * 4}});
     HIP_OK(hipMemcpyAsync(W.poff, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
     WalkArgs WA;
     memset(&WA, 0, sizeof(WA));
@@ -2490,7 +2611,7 @@ static void reconstruct_with(Ctx& c, const FieldRead& R, const uint32_t* pool, s
 
 void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
                        std::vector<uint64_t>& off, std::vector<uint32_t>& cells, SolveStats* st) {
-  reconstruct_with(c, make_read(c), c.ws.pool, n, routed, cost, off, cells, st);
+  reconstruct_with(c, make_read(c), c.W().pool, n, routed, cost, off, cells, st);
 }
 
 void read_field(Ctx& c, size_t f, uint32_t* dist, uint32_t* hops) {
@@ -2499,8 +2620,9 @@ void read_field(Ctx& c, size_t f, uint32_t* dist, uint32_t* hops) {
   const FieldDesc& F = c.layout[f];
   const BoardView& b = c.views[F.board];
   size_t n = (size_t)F.ww * F.wh * b.layers;
-  Workspace& W = c.ws;
-  grow(W.fout, W.fout_cap, 2 * n);
+  Workspace& W = c.W();
+  take(c, W.fout, W.fout_cap, 2 * n);
+  check_arena(c, {{W.fout, 2 * n * 4}});
   FieldRead R = make_read(c);
   uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(extract_field, dim3(blocks), dim3(256), 0, c.stream, R, (uint32_t)f, W.fout,

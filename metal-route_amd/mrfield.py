@@ -224,6 +224,9 @@ def lib():
         "mrf_synth_config": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(SynthS), E]),
         "mrf_synth_board": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(SynthS)]),
         "mrf_free_synth": (None, [C.POINTER(SynthS)]),
+        "mrf_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NetS), C.c_size_t, C.POINTER(WindowS),
+                                        C.c_int, U8P, U32P, U64P, U32P, C.c_size_t, C.POINTER(C.c_size_t), E]),
+        "mrf_noalloc_scratch_bytes": (C.c_size_t, []),
         "mrf_srj_rasterize": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(SrjParamsS),
                                         C.POINTER(C.POINTER(SrjProblemS)), E]),
         "mrf_srj_free": (None, [C.POINTER(SrjProblemS)]),
@@ -625,6 +628,22 @@ class Board:
         _check(lib().mrf_autotune(self.eng._h, self._h, arr, len(nets), _windows(windows, len(nets)), sample,
                                   C.byref(t), C.byref(err)), err)
         return {k: getattr(t, k) for k, _ in TuningS._fields_}
+
+    def solve_noalloc(self, nets, windows=None, retry_full=True, cells_cap=1 << 20) -> PathBatch:
+        """mrf_solve_noalloc: the same solve with every device buffer carved
+        from the library's scratch array; paths into arrays owned here."""
+        arr, keep = _nets(nets)
+        n = len(nets)
+        routed = np.zeros(max(n, 1), dtype=np.uint8)
+        cost = np.zeros(max(n, 1), dtype=np.uint32)
+        off = np.zeros(n + 1, dtype=np.uint64)
+        cells = np.zeros(max(cells_cap, 1), dtype=np.uint32)
+        n_cells = C.c_size_t(0)
+        err = ErrorS()
+        _check(lib().mrf_solve_noalloc(self.eng._h, self._h, arr, n, _windows(windows, n), 1 if retry_full else 0,
+                                       routed.ctypes.data_as(U8P), _p32(cost), off.ctypes.data_as(U64P), _p32(cells),
+                                       cells_cap, C.byref(n_cells), C.byref(err)), err)
+        return PathBatch(routed[:n], cost[:n], off, cells[:n_cells.value])
 
     def solve_arrays(self, nets, windows=None, retry_full=True) -> PathBatch:
         """Like solve(), with the paths left as flat numpy arrays."""
