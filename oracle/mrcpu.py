@@ -460,7 +460,8 @@ def alone_batch(tmpl: Spec, nets: Sequence[Net], windows: Optional[Sequence[tupl
     paths, costs = [], []
     for i in range(nn):
         if plen[i]:
-            paths.append([int(v) for v in cells[i * n: i * n + plen[i]]])
+            start = i * n
+            paths.append([int(v) for v in cells[start: start + int(plen[i])]])
             costs.append(int(cost[i]))
         else:
             paths.append(None)
