@@ -8,7 +8,7 @@ every path.  Inputs (board planes) are uploaded once before the timed region.
 --config 4 solves multi-source fields (one per multi-pin net, no paths).
 
 Output: one JSON line (see the keys below).  ``roofline`` is per launch of
-the relaxation kernel (relax_round in the plain form): algorithmic bytes =
+the relaxation kernel (relax_wave or relax_round in the plain form): algorithmic bytes =
 B per cell visit (8 B label read + 8 B label write + the per-cell planes of
 the workload: 1 B mask, + 2 B u16 history for configs 2 and 3, + 4 B u32
 history for config 4) times the cell visits of one launch (device counter of
@@ -62,7 +62,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["graph", "plain", "queue"], default="plain",
                     help="solve form: plain rounds, hipGraph rounds, or one persistent work-queue launch")
-    ap.add_argument("--tiles", default="16,16,256,32", help="tw,th,threads,max_local")
+    ap.add_argument("--tiles", default="16,16,64,32",
+                    help="tw,th,threads,max_local (16,16,64: one wavefront per tile where the batch qualifies)")
     ap.add_argument("--delta", type=int, default=0, help="delta-stepping window of the plain form (0 = off)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick tiles / round cap / form with the library's autotuner (always on for config 3)")
@@ -623,7 +624,7 @@ def main():
         return board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
-    kern_ms, launches, changed, valid_ms, call_ms, fcells = [], [], [], [], [], []
+    kern_ms, launches, changed, valid_ms, call_ms, fcells, wave = [], [], [], [], [], [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
@@ -645,6 +646,7 @@ def main():
             valid_ms.append(st["validate_ms"])
             call_ms.append(st["total_ms"])
             launches.append(st["relax_launches"])
+            wave.append(st["wave_launches"])
             fcells.append(st["field_cells"])
     wall = time.perf_counter() - t_start
     n_nets = len(s.pins) if multi else len(nets)
@@ -658,6 +660,8 @@ def main():
     achieved = cells_per_launch * bpc / (avg_launch_ms * 1e-3) / 1e9
     workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
     kernel = {"graph": "relax_list", "plain": "relax_round", "queue": "relax_queue"}[form]
+    if form == "plain" and sum(wave) == sum(launches) > 0:
+        kernel = "relax_wave"  # one wavefront per tile visit
     traffic = load_traffic(kernel, workload)
     # compulsory floor of a step: every field cell's dist written once (4 B)
     # and the board planes read once per field (bpc - 16 B per cell)

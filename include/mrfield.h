@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define MRF_ABI_VERSION 1
+#define MRF_ABI_VERSION 2
 
 /* RouterError (crates/mr-core/src/lib.rs:699-708) */
 enum {
@@ -158,6 +158,8 @@ typedef struct {
   uint32_t relax_launches; /* relaxation kernel launches */
   uint64_t changed_visits; /* tile visits that improved at least one label */
   double validate_ms;      /* host re-validation of the device paths */
+  uint32_t wave_launches;  /* relaxation launches that ran one wavefront per tile (tile width 16,
+                              64 threads; lean unit-weight narrow batches in the plain form) */
 } mrf_stats;
 
 /* Solve form of a context.
@@ -185,7 +187,13 @@ int mrf_abi_version(void);
 int mrf_open(int device, mrf_ctx **out, mrf_error *err);
 void mrf_close(mrf_ctx *ctx);
 /* Tile shape (tw x th cells, all layers), workgroup size (64..256), in-LDS
- * round cap.  Instantiated shapes: 32x32, 32x16, 16x16, 16x8, 8x4, 4x1. */
+ * round cap.  Instantiated shapes: 32x32, 32x16, 16x16, 16x8, 8x4, 4x1.
+ * 16 x 16 at 64 threads (a new context's shape, round cap 32) is the wave
+ * form: one wavefront per tile visit for plain-form launches of fields with
+ * no halo-owner or legal planes, overrides, own pads or via-exempt pads, on
+ * boards without a weight plane, up to 4 layers, with every step price below
+ * 0x7FFF and no delta stepping; every other launch of that shape runs the
+ * workgroup kernels at 256 threads.  Results are the same either way. */
 int mrf_set_tiles(mrf_ctx *ctx, uint32_t tw, uint32_t th, uint32_t threads, uint32_t max_local,
                   mrf_error *err);
 int mrf_set_mode(mrf_ctx *ctx, int mode);
@@ -413,7 +421,7 @@ typedef struct {
     uint32_t max_boards;    /* resident board slots (0 = 64) */
     uint32_t max_layers;    /* largest layer count of any board (0 = 8) */
     uint32_t max_inflight;  /* requests on the device at once (0 = 256) */
-    uint32_t tile_w, tile_h, threads; /* tile shape (0 = 16 x 16, 256 threads) */
+    uint32_t tile_w, tile_h, threads; /* tile shape (0 = 16 x 16, 64 threads: the wave form) */
     /* Admission pacing: a queued request is admitted only while the running
      * requests keep fewer tiles active than this (the device is not full),
      * so a large request runs nearly alone and later ones fill its tail.

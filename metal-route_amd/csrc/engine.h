@@ -80,8 +80,10 @@ struct FieldDesc {
 };
 
 struct TileShape {
-  uint32_t tw = 32, th = 32;  // tile width / height in cells (all layers per tile)
-  uint32_t threads = 256;     // workgroup size
+  uint32_t tw = 16, th = 16;  // tile width / height in cells (all layers per tile)
+  // Workgroup size.  64 with 16 x 16 tiles: one wavefront per tile visit
+  // (the wave form) where the launch qualifies, 256 for the other launches.
+  uint32_t threads = 64;
   uint32_t max_local = 32;    // in-LDS relaxation rounds per tile visit
 };
 
@@ -139,6 +141,7 @@ struct SolveStats {
   double kernel_ms = 0;        // relaxation kernels alone: sum of per-launch event times (plain form)
   double validate_ms = 0;      // host re-validation of device paths
   uint32_t launches = 0;       // relaxation kernel launches
+  uint32_t wave_launches = 0;  // of which ran the one-wavefront-per-tile kernel
   void add(const SolveStats& o) {
     relax_ms += o.relax_ms;
     path_ms += o.path_ms;
@@ -154,6 +157,7 @@ struct SolveStats {
     kernel_ms += o.kernel_ms;
     validate_ms += o.validate_ms;
     launches += o.launches;
+    wave_launches += o.wave_launches;
   }
 };
 
@@ -354,6 +358,7 @@ struct Group {
   bool live = false;
   bool narrow = false;  // every step price fits the narrow step planes
   bool lean = false;    // no halo-owner / legal planes, overrides, pads or via pads
+  bool unit = false;    // every board of its fields has no weight plane
   uint64_t field0 = 0, nfields = 0;
   uint64_t item0 = 0, nitems = 0;
   uint64_t lab0 = 0, labwords = 0;

@@ -1171,27 +1171,11 @@ struct MinWaves {
   static constexpr int list = TW >= 32 ? 2 : 4;
 };
 
-template <int TW, int TH, int LM, bool NARROW, bool LEAN>
-__global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(RelaxArgs A) {
-  extern __shared__ __align__(16) char smem[];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (A.clear_ctr != A.next_ctr)
-      __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (A.key_cur && A.key_clear != A.key_ctr)
-      __hip_atomic_store(&A.counters[A.key_clear * kCtrStride], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  const uint32_t item = const_load(A.list_cur + blockIdx.x);
-  const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
-  uint32_t key = 0;
-  bool defer = false;
-  if (A.key_cur) {
-    key = uniform_u32(A.key_cur[item]);
-    defer = key > uniform_u32(__hip_atomic_load(&A.counters[C_THRESH * kCtrStride], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT));
-  }
-  TileResult res{};
-  if (!defer) res = process_tile<TW, TH, LM, NARROW, LEAN>(A, item, smem, A.round);
-  if (threadIdx.x >= 64) return;
+// After a visit (or a deferred item): activations, the item's flags and
+// counters, and, in the launch's last workgroup, the counts, group flags and
+// stamp published to the host.  Run by the first wavefront of the workgroup.
+__device__ __forceinline__ void round_tail(const RelaxArgs& A, uint32_t item, uint32_t grp, const TileResult& res,
+                                           bool defer, uint32_t key) {
   uint32_t kmin = kMax;
   if (defer) {
     if (threadIdx.x == 0) activate(A, item, key);  // carried over to the next round with its key
@@ -1258,6 +1242,32 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(Re
     }
   }
 }
+
+template <int TW, int TH, int LM, bool NARROW, bool LEAN>
+__global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(RelaxArgs A) {
+  extern __shared__ __align__(16) char smem[];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (A.clear_ctr != A.next_ctr)
+      __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (A.key_cur && A.key_clear != A.key_ctr)
+      __hip_atomic_store(&A.counters[A.key_clear * kCtrStride], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const uint32_t item = const_load(A.list_cur + blockIdx.x);
+  const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
+  uint32_t key = 0;
+  bool defer = false;
+  if (A.key_cur) {
+    key = uniform_u32(A.key_cur[item]);
+    defer = key > uniform_u32(__hip_atomic_load(&A.counters[C_THRESH * kCtrStride], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT));
+  }
+  TileResult res{};
+  if (!defer) res = process_tile<TW, TH, LM, NARROW, LEAN>(A, item, smem, A.round);
+  if (threadIdx.x >= 64) return;
+  round_tail(A, item, grp, res, defer, key);
+}
+
+#include "wavetile.h"
 
 // ---------------------------------------------------------------------------
 // graph form: one grid-stride launch per round, rounds replayed from a hipGraph
@@ -1780,12 +1790,16 @@ struct KernelSet {
   const void* round;
   const void* list;
   const void* queue;
+  const void* wave;  // relax_wave for this shape and layer bound (lean narrow sets only), or nullptr
 };
 
 template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
+  const void* wave = nullptr;
+  if constexpr (NARROW && LEAN && TW == 16 && TH == 16 && LM <= 4) wave = (const void*)relax_wave<TW, TH, LM>;
   return KernelSet{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
-                   (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>};
+                   (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
+                   wave};
 }
 
 // Wide step planes for every layer bound; narrow ones, full and lean, up to
@@ -1844,6 +1858,25 @@ static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers, 
   throw Error(1, "tile shape " + std::to_string(tw) + "x" + std::to_string(th) + " is not instantiated");
 }
 
+// The round-form kernel of a launch: the wave form (one wavefront per tile)
+// when the shape asks for 64 threads and the launch qualifies -- a lean
+// narrow set with a wave kernel, boards without a weight plane, no delta
+// stepping -- else relax_round with the shape's workgroup size.
+// 16 x 16 tiles at 64 threads select the wave form; launches it does not
+// cover (and the graph and queue forms) run the workgroup kernels at 256.
+static bool wave_shape(const TileShape& S) { return S.threads == 64 && S.tw == 16 && S.th == 16; }
+static uint32_t workgroup_threads(const TileShape& S) { return wave_shape(S) ? 256u : S.threads; }
+struct RoundLaunch {
+  const void* fn;
+  uint32_t threads;
+  size_t lds;
+};
+static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool unit, bool keyed) {
+  if (wave_shape(S) && K.wave && K.narrow && K.lean && unit && !keyed && Lmax <= K.lm)
+    return RoundLaunch{K.wave, 64u, wave_lds_bytes(S.tw, S.th, K.lm)};
+  return RoundLaunch{K.round, workgroup_threads(S), lds_bytes(S.tw, S.th, Lmax, K.narrow)};
+}
+
 // Whether a field and its board need none of what the lean kernels leave
 // out: halo-owner and legal-search planes, additive overrides, own pads and
 // via-exempt pads.
@@ -1879,6 +1912,7 @@ void ctx_init(Ctx& c, int device) {
     HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.queue, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
+    if (k.wave) HIP_OK(hipFuncSetAttribute(k.wave, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
   }
 }
 
@@ -2112,7 +2146,8 @@ static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_
   GraphCache& g = c.graph;
   std::vector<uint8_t> key(sizeof(RelaxArgs) + 32);
   memcpy(key.data(), &A, sizeof(RelaxArgs));
-  uint64_t extra[4] = {(uint64_t)(uintptr_t)K.list, grid, lds, (uint64_t)c.shape.threads * 1000 + c.graph_rounds};
+  uint64_t extra[4] = {(uint64_t)(uintptr_t)K.list, grid, lds,
+                       (uint64_t)workgroup_threads(c.shape) * 1000 + c.graph_rounds};
   memcpy(key.data() + sizeof(RelaxArgs), extra, sizeof(extra));
   if (g.exec && g.key == key) return g.exec;
   if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -2127,7 +2162,7 @@ static hipGraphExec_t graph_for(Ctx& c, const KernelSet& K, uint32_t grid, size_
   o.list_next = const_cast<uint32_t*>(A.list_cur);
   void* args[] = {&e, &o};
   for (uint32_t r = 0; r < c.graph_rounds; ++r) {
-    HIP_OK(hipLaunchKernel(K.list, dim3(grid), dim3(c.shape.threads), args, lds, s));
+    HIP_OK(hipLaunchKernel(K.list, dim3(grid), dim3(workgroup_threads(c.shape)), args, lds, s));
     hipLaunchKernelGGL(advance_round, dim3(1), dim3(64), 0, s, c.W().counters);
   }
   HIP_OK(hipStreamEndCapture(s, &graph));
@@ -2225,11 +2260,12 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  bool narrow = true, lean = true;
+  bool narrow = true, lean = true, unit = true;
   for (const FieldReq& q : reqs) {
     if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
     narrow = narrow && field_narrow(*boards[q.board], q);
     lean = lean && field_lean(c.views[q.board], q);
+    unit = unit && c.views[q.board].cost == nullptr;
   }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
   size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
@@ -2338,6 +2374,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   launch_fill32(s, W.counters, 0u, kCtrWords);
 
   RelaxArgs A = make_args(c, item);
+  const RoundLaunch RL = round_launch(K, S, Lmax, unit, c.mode == MODE_PLAIN && c.delta != kMax);
   SolveStats local;
   local.field_cells = field_cells;
   bool timed = false;
@@ -2391,7 +2428,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
           }
           HIP_OK(hipEventRecord(W.round_ev[ei], s));
         }
-        HIP_OK(hipLaunchKernel(K.round, dim3(n_active), dim3(S.threads), args, lds, s));
+        HIP_OK(hipLaunchKernel(RL.fn, dim3(n_active), dim3(RL.threads), args, RL.lds, s));
         if (per_round) HIP_OK(hipEventRecord(W.round_ev[ei + 1], s));
         volatile uint32_t* hc = W.hostctr;
         const auto spin0 = std::chrono::steady_clock::now();
@@ -2403,6 +2440,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         }
         std::atomic_thread_fence(std::memory_order_acquire);
         ++local.launches;
+        if (RL.fn == K.wave) ++local.wave_launches;
         ++rounds;
         if (hc[1]) break;
         n_active = hc[0];
@@ -2438,7 +2476,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
                          d_seeds, (uint32_t)seeds.size(), (uint32_t*)nullptr);
       launch_fill32(s, W.counters + C_CURN * kCtrStride, (uint32_t)seeds.size(), 1);
       int occ = 0;
-      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.list, S.threads, lds));
+      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.list, workgroup_threads(S), lds));
       if (occ < 1) throw Error(1, "relaxation kernel does not fit on a CU");
       uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)occ * c.cus, item));
       hipGraphExec_t exec = graph_for(c, K, grid, lds, A);
@@ -2504,13 +2542,13 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
                          d_seeds, (uint32_t)seeds.size(), (uint32_t*)nullptr);
       int occ = 0;
-      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.queue, S.threads, lds));
+      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.queue, workgroup_threads(S), lds));
       if (occ < 1) throw Error(1, "relaxation kernel does not fit on a CU");
       const uint32_t grid = std::max<uint32_t>(kQueues, ((uint32_t)occ * (uint32_t)c.cus) / kQueues * kQueues);
       void* qargs[] = {&A, &Q};
       HIP_OK(hipEventRecord(W.ev0, s));
       timed = true;
-      HIP_OK(hipLaunchKernel(K.queue, dim3(grid), dim3(S.threads), qargs, lds, s));
+      HIP_OK(hipLaunchKernel(K.queue, dim3(grid), dim3(workgroup_threads(S)), qargs, lds, s));
       HIP_OK(hipGetLastError());
       rounds = 1;
       local.launches = 1;
@@ -2845,10 +2883,12 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   G.nitems = nitems;
   G.narrow = true;
   G.lean = true;
+  G.unit = true;
   for (const FieldReq& q : reqs) {
     G.layers = std::max(G.layers, d.views[q.board].layers);
     G.narrow = G.narrow && field_narrow(*d.devs[q.board], q);
     G.lean = G.lean && field_lean(d.views[q.board], q);
+    G.unit = G.unit && d.views[q.board].cost == nullptr;
   }
   G.labwords = words;
   G.npool = npool;
@@ -2974,15 +3014,16 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
-  bool narrow = true, lean = true;
+  bool narrow = true, lean = true, unit = true;
   for (uint32_t g = 0; g < d.max_groups; ++g)
     if (d.groups[g].live) {
       Lmax = std::max(Lmax, d.groups[g].layers);
       narrow = narrow && d.groups[g].narrow;
       lean = lean && d.groups[g].lean;
+      unit = unit && d.groups[g].unit;
     }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
-  const size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
+  const RoundLaunch RL = round_launch(K, S, Lmax, unit, false);
   void* args[] = {&A};
   if (d.stamp_round >= (1u << 30)) {
     // states and flags stay below 2^31: every touched item now reads as
@@ -3006,7 +3047,7 @@ uint32_t driver_step(RoundDriver& d) {
     }
   }
   HIP_OK(hipEventRecord(e0, s));
-  HIP_OK(hipLaunchKernel(K.round, dim3(d.n_cur), dim3(S.threads), args, lds, s));
+  HIP_OK(hipLaunchKernel(RL.fn, dim3(d.n_cur), dim3(RL.threads), args, RL.lds, s));
   HIP_OK(hipEventRecord(e1, s));
   HIP_OK(hipGetLastError());
   d.ev_pending.push_back({e0, e1});

@@ -459,7 +459,7 @@ int mrf_server_create(int device, const mrf_server_config* cfg, mrf_server** out
     if (cfg) k = *cfg;
     S->c.shape.tw = k.tile_w ? k.tile_w : 16;
     S->c.shape.th = k.tile_h ? k.tile_h : 16;
-    S->c.shape.threads = k.threads ? k.threads : 256;
+    S->c.shape.threads = k.threads ? k.threads : 64;
     S->c.shape.max_local = 32;
     if (S->c.shape.threads < 64 || S->c.shape.threads > 256 || S->c.shape.threads % 64)
       throw Error(MRF_BACKEND_UNAVAILABLE, "threads must be 64, 128, 192 or 256");

@@ -75,7 +75,7 @@ class StatsS(C.Structure):
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
                 ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double),
                 ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32), ("changed_visits", C.c_uint64),
-                ("validate_ms", C.c_double)]
+                ("validate_ms", C.c_double), ("wave_launches", C.c_uint32)]
 
 
 class TuningS(C.Structure):
@@ -481,7 +481,7 @@ class Engine:
         except Exception:
             pass
 
-    def set_tiles(self, tw=32, th=32, threads=256, max_local=32):
+    def set_tiles(self, tw=16, th=16, threads=64, max_local=32):
         err = ErrorS()
         _check(lib().mrf_set_tiles(self._h, tw, th, threads, max_local, C.byref(err)), err)
 

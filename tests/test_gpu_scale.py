@@ -36,13 +36,27 @@ pytestmark = pytest.mark.gpu
 THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
 
 
-@pytest.fixture(scope="module")
-def eng():
+# The workgroup form (four wavefronts per tile visit) and the wave form (one
+# wavefront per tile visit, 64 threads: lean unit-weight narrow batches).
+FORMS = {"workgroup": (16, 16, 256, 32), "wave": (16, 16, 64, 32)}
+
+
+@pytest.fixture(scope="module", params=list(FORMS))
+def eng(request):
     e = mrfield.Engine(0)
-    e.set_tiles(16, 16, 256, 32)
+    e.set_tiles(*FORMS[request.param])
     e.set_mode(mrfield.MODE_PLAIN)
+    e.form = request.param
     yield e
     e.close()
+
+
+def assert_form_ran(eng):
+    st = eng.stats()
+    if eng.form == "wave":
+        assert st["wave_launches"] == st["relax_launches"] > 0, st
+    else:
+        assert st["wave_launches"] == 0, st
 
 
 def spec_of(s, **kw):
@@ -70,6 +84,7 @@ def test_config2_full_board(eng):
     board = mrfield.Board(eng, g, s.edges, s.pricing())
     try:
         got = board.solve_arrays(s.nets())
+        assert_form_ran(eng)
         assert int(np.count_nonzero(got.routed)) > 200
         idx = list(range(256))
         full = [(0, 0, g.w - 1, g.h - 1)] * len(idx)
@@ -93,6 +108,8 @@ def test_config2_full_board(eng):
 
 
 def test_config4_multisource_fields_full_board(eng):
+    if eng.form == "wave":
+        pytest.skip("six layers and u32 costs with a halo-owner plane: the wave form does not apply")
     s = mrfield.synth_config(4, seed=1)
     g = s.grid
     assert (g.w, g.h, g.layers, len(s.pins)) == (1024, 1024, 6, 512)
@@ -123,6 +140,7 @@ def test_config3_windowed_batch(eng):
     board = mrfield.Board(eng, g, s.edges, s.pricing())
     try:
         got = board.solve_arrays(s.nets(), windows=s.windows, retry_full=True)
+        assert_form_ran(eng)
     finally:
         board.close()
     spec = spec_of(s)

@@ -1,0 +1,203 @@
+"""GPU parity of the one-wavefront-per-tile form against the CPU oracle.
+
+The wave form (tiles 16 x 16, 64 threads) runs for lean batches on boards
+without a weight plane whose step prices fit 15 bits; these cases are built
+to qualify, and every one asserts that the wave kernel is what ran.  Integer
+work: dist and hops of every cell, paths and search costs, bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mrcpu
+import mrfield
+from cases import MAX
+
+pytestmark = pytest.mark.gpu
+
+WAVE = (16, 16, 64, 32)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = mrfield.Engine(0)
+    e.set_tiles(*WAVE)
+    e.set_mode(mrfield.MODE_PLAIN)
+    yield e
+    e.close()
+
+
+def unit_board(seed, w, h, L):
+    """Unit enter weights (1 or obstacle), trace nodes, one-way planar edges
+    and via-forbidden cells."""
+    rng = np.random.default_rng(seed)
+    n = w * h * L
+    cost = np.ones(n, dtype=np.uint32)
+    cost[rng.random(n) < 0.2] = MAX
+    board = np.zeros(n, dtype=np.uint8)
+    r = rng.random(n)
+    board[r < 0.04] |= 1
+    board[(r >= 0.04) & (r < 0.09)] |= 2
+    for bit in (4, 8, 16, 32):
+        board[rng.random(n) < 0.04] |= bit
+    vf = (rng.random(n) < 0.1).astype(np.uint8)
+    return mrcpu.Grid(w, h, L, cost, via_forbidden=vf, board=board)
+
+
+def free_cells(g):
+    ok = (g.cost != MAX) & ((g.board & 3) == 0)
+    return np.flatnonzero(ok)
+
+
+def as_dev(g):
+    return mrfield.Grid(g.w, g.h, g.layers, g.cost, g.via_forbidden, g.board)
+
+
+def assert_wave_ran(eng):
+    st = eng.stats()
+    assert st["relax_launches"] > 0
+    assert st["wave_launches"] == st["relax_launches"], st
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_wave_random_fields(eng, L, seed):
+    """Multi-source fields on boards with every lean feature: trace nodes,
+    obstacles, one-way edges, via-forbidden cells, per-gap x / y prices
+    (zero included), vias off between some layers, an additive plane; full
+    and partial windows (ragged edge tiles)."""
+    rng = np.random.default_rng(1000 * L + seed)
+    w, h = int(rng.integers(20, 70)), int(rng.integers(20, 70))
+    g = unit_board(seed + 10 * L, w, h, L)
+    ex = [int(v) for v in rng.integers(0, 30, size=w - 1)]
+    ey = [int(v) for v in rng.integers(0, 30, size=h - 1)]
+    via = [None if rng.random() < 0.25 else int(rng.integers(0, 60)) for _ in range(L - 1)]
+    edges = mrcpu.Edges(ex, ey, via)
+    add = rng.integers(0, 40, size=g.n).astype(np.uint32)
+    free = free_cells(g)
+    sources, dsts, wins = [], [], []
+    for i in range(10):
+        k = 1 + int(rng.integers(0, 3))
+        sources.append([int(c) for c in rng.choice(free, size=k, replace=False)])
+        dsts.append(int(rng.choice(free)))
+        if i % 3 == 0:
+            wins.append((0, 0, w - 1, h - 1))
+        else:
+            x0, y0 = int(rng.integers(0, w // 2)), int(rng.integers(0, h // 2))
+            wins.append((x0, y0, int(rng.integers(x0, w)), int(rng.integers(y0, h))))
+    nets = [mrcpu.Net(s[0], d) for s, d in zip(sources, dsts)]
+    ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(g, edges, additive=add), nets, sources, wins)
+    dedges = mrfield.Edges(list(edges.x), list(edges.y), list(edges.via))
+    got_d, got_h = eng.fields(as_dev(g), dedges, sources, dsts=dsts, windows=wins,
+                              pricing=mrfield.Pricing(additive=add))
+    assert_wave_ran(eng)
+    for i in range(len(nets)):
+        assert np.array_equal(got_d[i], ref_d[i]), (i, np.flatnonzero(got_d[i] != ref_d[i])[:10])
+        assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_wave_random_paths(eng, seed):
+    """Two-pin nets through the provider seam: window pass with full-board
+    retry, paths and search costs."""
+    rng = np.random.default_rng(seed)
+    w, h, L = 61, 47, 4
+    g = unit_board(seed, w, h, L)
+    edges = mrcpu.Edges([int(v) for v in rng.integers(1, 25, size=w - 1)],
+                        [int(v) for v in rng.integers(1, 25, size=h - 1)], [30, None, 45])
+    free = free_cells(g)
+    nets = [mrcpu.Net(int(rng.choice(free)), int(rng.choice(free))) for _ in range(24)]
+    wins = [mrcpu.net_window(g, n.src, n.dst, n.pads) for n in nets]
+    ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(g, edges), nets, wins, retry_full=True)
+    got = eng.route_isolated_batch(as_dev(g), [mrfield.Net(n.src, n.dst) for n in nets], wins,
+                                   mrfield.Edges(list(edges.x), list(edges.y), list(edges.via)))
+    assert_wave_ran(eng)
+    for i, r in enumerate(got):
+        assert (r is None) == (ref_p[i] is None), i
+        if r is not None:
+            assert r[0] == ref_p[i] and r[1] == ref_c[i], i
+
+
+def test_wave_config1_complete_fields_and_paths(eng):
+    """128x128x2, 64 nets: every field cell and every path."""
+    s = mrfield.synth_config(1, seed=1)
+    g = s.grid
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    nets = [mrcpu.Net(int(a), int(b)) for a, b in zip(s.src, s.dst)]
+    ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(og, oe), nets, [[n.src] for n in nets], threads=8)
+    got_d, got_h = eng.fields(g, s.edges, [[int(a)] for a in s.src], dsts=[int(b) for b in s.dst])
+    assert_wave_ran(eng)
+    assert np.array_equal(got_d, ref_d)
+    assert np.array_equal(got_h, ref_h)
+    full = [(0, 0, g.w - 1, g.h - 1)] * len(nets)
+    ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(og, oe), nets, full, threads=8)
+    got = eng.route_isolated_batch(g, s.nets(), full, s.edges)
+    assert_wave_ran(eng)
+    for i, r in enumerate(got):
+        assert (r is None) == (ref_p[i] is None), i
+        if r is not None:
+            assert r[0] == ref_p[i] and r[1] == ref_c[i], i
+
+
+def test_wave_round_cap_reactivates(eng):
+    """A round cap of one in-LDS round: visits end before their tile's
+    fixpoint and re-activate themselves; the fields are unchanged."""
+    s = mrfield.synth_config(1, seed=2, scale_nets=16)
+    g = s.grid
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    nets = [mrcpu.Net(int(a), int(b)) for a, b in zip(s.src, s.dst)]
+    ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(og, oe), nets, [[n.src] for n in nets], threads=8)
+    eng.set_tiles(16, 16, 64, 1)
+    try:
+        got_d, got_h = eng.fields(g, s.edges, [[int(a)] for a in s.src], dsts=[int(b) for b in s.dst])
+        assert_wave_ran(eng)
+    finally:
+        eng.set_tiles(*WAVE)
+    assert np.array_equal(got_d, ref_d)
+    assert np.array_equal(got_h, ref_h)
+
+
+def test_wave_equals_workgroup_form_with_weights_falling_back(eng):
+    """A board with a weight plane does not qualify: the wave shape runs the
+    workgroup kernel (at 256 threads) and still matches the oracle."""
+    rng = np.random.default_rng(9)
+    w, h, L = 40, 33, 2
+    g = unit_board(9, w, h, L)
+    g.cost[g.cost != MAX] = rng.integers(1, 5, size=int((g.cost != MAX).sum())).astype(np.uint32)
+    edges = mrcpu.Edges([10] * (w - 1), [10] * (h - 1), [50])
+    free = free_cells(g)
+    sources = [[int(rng.choice(free))] for _ in range(6)]
+    nets = [mrcpu.Net(s[0], MAX) for s in sources]
+    ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(g, edges), nets, sources)
+    got_d, got_h = eng.fields(as_dev(g), mrfield.Edges(list(edges.x), list(edges.y), list(edges.via)), sources)
+    st = eng.stats()
+    assert st["relax_launches"] > 0 and st["wave_launches"] == 0
+    assert np.array_equal(got_d, ref_d)
+    assert np.array_equal(got_h, ref_h)
+
+
+def test_wave_same_round_neighbours_after_a_layout_change(eng):
+    """Label memory laid out by an 8 x 4 solve, then the wave form at 16 x 16
+    with a round cap of one (short visits: many neighbours first visited in
+    the same round, whose border words still hold the other layout's
+    labels): the same paths as the workgroup form."""
+    s = mrfield.synth_config(2, seed=1)
+    board = mrfield.Board(eng, s.grid, s.edges, s.pricing())
+    nets = s.nets()
+    try:
+        eng.set_tiles(16, 16, 256, 32)
+        ref = board.solve_arrays(nets)
+        for cap in (1, 32):
+            eng.set_tiles(8, 4, 128, 4)
+            board.solve_arrays(nets)
+            eng.set_tiles(16, 16, 64, cap)
+            got = board.solve_arrays(nets)
+            assert_wave_ran(eng)
+            for k in ("routed", "cost", "offsets", "cells"):
+                assert np.array_equal(getattr(got, k), getattr(ref, k)), (cap, k)
+    finally:
+        board.close()
+        eng.set_tiles(*WAVE)
