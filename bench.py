@@ -624,7 +624,7 @@ def main():
         return board.solve_arrays(nets, windows=wins, retry_full=wins is not None)
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
-    kern_ms, launches, changed, valid_ms, call_ms, fcells, wave = [], [], [], [], [], [], []
+    kern_ms, launches, changed, valid_ms, call_ms, fcells, wave, lbytes = [], [], [], [], [], [], [], []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
@@ -647,6 +647,7 @@ def main():
             call_ms.append(st["total_ms"])
             launches.append(st["relax_launches"])
             wave.append(st["wave_launches"])
+            lbytes.append(st["label_bytes"])
             fcells.append(st["field_cells"])
     wall = time.perf_counter() - t_start
     n_nets = len(s.pins) if multi else len(nets)
@@ -720,6 +721,11 @@ def main():
             "traffic": traffic,
             "kernel": kernel,
             "bytes_per_cell_visit": bpc,
+            # the kernel's own label width: 8 B labels move bpc bytes per cell
+            # visit; packed 4 B labels move 8 fewer (the survey's unit above
+            # counts u64 labels either way)
+            "label_bytes": int(max(lbytes)) if lbytes else 8,
+            "bytes_moved_per_cell_visit": bpc - 2 * (8 - (int(max(lbytes)) if lbytes else 8)),
             "cell_visits_per_launch": int(cells_per_launch),
             "achieved_from_traffic": (round(traffic / (avg_launch_ms * 1e-3) / 1e9, 2) if traffic else None),
             "traffic_over_algorithmic": (round(traffic / (cells_per_launch * bpc), 3) if traffic else None),

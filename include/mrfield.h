@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define MRF_ABI_VERSION 2
+#define MRF_ABI_VERSION 3
 
 /* RouterError (crates/mr-core/src/lib.rs:699-708) */
 enum {
@@ -160,6 +160,8 @@ typedef struct {
   double validate_ms;      /* host re-validation of the device paths */
   uint32_t wave_launches;  /* relaxation launches that ran one wavefront per tile (tile width 16,
                               64 threads; lean unit-weight narrow batches in the plain form) */
+  uint32_t label_bytes;    /* label width of the last solve: 8, or 4 for packed labels (wave form on
+                              boards whose every open step costs at least 1, certified per batch) */
 } mrf_stats;
 
 /* Solve form of a context.

@@ -410,6 +410,7 @@ int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   out->changed_visits = s.changed_visits;
   out->validate_ms = s.validate_ms;
   out->wave_launches = s.wave_launches;
+  out->label_bytes = s.label_bytes;
   return MRF_OK;
 }
 

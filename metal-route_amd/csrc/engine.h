@@ -103,6 +103,10 @@ struct DevBoard {
   std::vector<uint8_t> obst;         // host copy: cost == kMax per cell
   uint64_t max_step = 0;  // bound on any step price (obstacles priced at weight 1, as own pads)
   uint64_t max_ew = 0;    // largest edge price x largest enter weight (max_step without the additive part)
+  // Lower bound on every open step price: the smallest edge price (planar
+  // gaps, allowed vias) x the smallest enter weight of a passable cell plus
+  // the smallest additive price (min_step without the additive part: min_ew).
+  uint64_t min_step = 0, min_ew = 0;
   int32_t* owner = nullptr;  // legal-search planes (allocated on first use)
   int32_t* lhalo = nullptr;
   bool legal = false;        // the owner / halo planes apply to the current call
@@ -142,6 +146,7 @@ struct SolveStats {
   double validate_ms = 0;      // host re-validation of device paths
   uint32_t launches = 0;       // relaxation kernel launches
   uint32_t wave_launches = 0;  // of which ran the one-wavefront-per-tile kernel
+  uint32_t label_bytes = 8;    // label width of the solve (4: packed u32 labels)
   void add(const SolveStats& o) {
     relax_ms += o.relax_ms;
     path_ms += o.path_ms;
@@ -158,6 +163,7 @@ struct SolveStats {
     validate_ms += o.validate_ms;
     launches += o.launches;
     wave_launches += o.wave_launches;
+    label_bytes = o.label_bytes;
   }
 };
 
@@ -219,6 +225,8 @@ struct Workspace {
   size_t qctl_cap = 0;
   uint32_t* fout = nullptr;      // extracted field output staging
   size_t fout_cap = 0;
+  uint32_t* fmax = nullptr;      // packed labels: per-field running maximum of the stored labels
+  size_t fmax_cap = 0;
   uint32_t* pinned = nullptr;    // host-pinned counter readback
   uint32_t* hostctr = nullptr;   // host-mapped words the plain form's kernel publishes to
   uint32_t* hostctr_dev = nullptr;
@@ -252,6 +260,9 @@ struct Ctx {
   int cus = 256;
   SolveStats last;
   std::vector<FieldDesc> layout;  // fields of the last solve (for paths / reads)
+  bool packed = false;             // the last solve's labels are packed u32 (wave form)
+  bool no_pack = false;            // solve in wide labels (set for the re-solve of a packed batch
+                                   // whose certificate failed)
   std::vector<BoardView> views;   // boards of the last solve
   // Fault injection for fail-closed tests (0 none, 1 backend error, 2 corrupt path).
   int inject = 0;

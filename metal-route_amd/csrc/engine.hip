@@ -95,7 +95,8 @@ enum : int {
   C_DONE = 10,    // plain form: workgroups of the launch that finished
   C_KEY0 = 11,    // 11..12: smallest activation key of the next list (delta stepping)
   C_THRESH = 13,  // delta stepping: the round processes items whose key is at most this
-  C_NCOUNTERS = 14
+  C_PMAX = 14,    // packed labels: largest stored label with a real dist, over every item
+  C_NCOUNTERS = 15
 };
 // Each device counter sits on its own 128-byte line: polled words and the
 // work-list atomics must not share a line, or every pop and push queues
@@ -386,6 +387,9 @@ struct RelaxArgs {
   uint32_t* key_next;
   uint32_t key_ctr, key_clear;
   uint32_t delta;
+  // Packed labels (wave form): per item, the largest label its last visit
+  // stored with a real dist (the host's certificate); null otherwise.
+  uint32_t* item_max;
 };
 
 // Words of one tile block in the label buffer: the L x TH x TW labels
@@ -1411,6 +1415,16 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
 }
 
 // Single-thread round step between two relax_list launches.
+// Largest of n per-item maxima into counters[C_PMAX].
+__global__ void reduce_item_max(const uint32_t* m, size_t n, uint32_t* counters) {
+  uint32_t v = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v = max(v, m[i]);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  if ((threadIdx.x & 63) == 0 && v) atomicMax(&counters[C_PMAX * kCtrStride], v);
+}
+
 __global__ void advance_round(uint32_t* ctr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t n = ctr[C_NEXTN * kCtrStride];
@@ -1474,6 +1488,7 @@ struct FieldRead {
   const uint64_t* labels;
   const uint32_t* state;
   uint32_t tw, th;
+  uint32_t packed;  // labels are u32 (dist << 16) | hops (wave form), read back as u64
 };
 
 // Label of global cell g in field F (kInf outside the window or untouched tile).
@@ -1487,7 +1502,14 @@ __device__ uint64_t field_label(const FieldRead& R, const BoardView& B, const Fi
   // state and label are loaded together (the label slot exists for every
   // tile of the field); the label counts only if the tile was visited
   const uint32_t st = R.state[F.item0 + t];
-  const uint64_t lab = R.labels[F.lab_off + t * tile_block_words(B.layers, R.tw, R.th) + (l * R.th + y) * R.tw + x];
+  const uint64_t at = F.lab_off + t * tile_block_words(B.layers, R.tw, R.th) + (l * R.th + y) * R.tw + x;
+  uint64_t lab;
+  if (R.packed) {
+    const uint32_t v = ((const uint32_t*)R.labels)[at];
+    lab = v >= 0xFFFF0000u ? kUnreached : ((uint64_t)(v >> 16) << 32) | (v & 0xFFFFu);
+  } else {
+    lab = R.labels[at];
+  }
   return (st & S_TOUCHED) ? lab : kInf;
 }
 
@@ -1790,16 +1812,21 @@ struct KernelSet {
   const void* round;
   const void* list;
   const void* queue;
-  const void* wave;  // relax_wave for this shape and layer bound (lean narrow sets only), or nullptr
+  const void* wave;     // relax_wave (u64 labels) for this shape and layer bound (lean narrow sets only), or nullptr
+  const void* wave_pk;  // relax_wave with packed u32 labels, or nullptr
 };
 
 template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
   const void* wave = nullptr;
-  if constexpr (NARROW && LEAN && TW == 16 && TH == 16 && LM <= 4) wave = (const void*)relax_wave<TW, TH, LM>;
+  const void* wave_pk = nullptr;
+  if constexpr (NARROW && LEAN && TW == 16 && TH == 16 && LM <= 4) {
+    wave = (const void*)relax_wave<TW, TH, LM, false>;
+    wave_pk = (const void*)relax_wave<TW, TH, LM, true>;
+  }
   return KernelSet{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
                    (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
-                   wave};
+                   wave, wave_pk};
 }
 
 // Wide step planes for every layer bound; narrow ones, full and lean, up to
@@ -1866,15 +1893,21 @@ static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers, 
 // cover (and the graph and queue forms) run the workgroup kernels at 256.
 static bool wave_shape(const TileShape& S) { return S.threads == 64 && S.tw == 16 && S.th == 16; }
 static uint32_t workgroup_threads(const TileShape& S) { return wave_shape(S) ? 256u : S.threads; }
+// `pack`: the batch may use packed u32 labels (every open step of its boards
+// costs at least 1); the wave form then runs them.
 struct RoundLaunch {
   const void* fn;
   uint32_t threads;
   size_t lds;
+  bool wave, packed;
 };
-static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool unit, bool keyed) {
-  if (wave_shape(S) && K.wave && K.narrow && K.lean && unit && !keyed && Lmax <= K.lm)
-    return RoundLaunch{K.wave, 64u, wave_lds_bytes(S.tw, S.th, K.lm)};
-  return RoundLaunch{K.round, workgroup_threads(S), lds_bytes(S.tw, S.th, Lmax, K.narrow)};
+static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool unit, bool keyed,
+                                bool pack) {
+  if (wave_shape(S) && K.wave && K.narrow && K.lean && unit && !keyed && Lmax <= K.lm) {
+    if (pack) return RoundLaunch{K.wave_pk, 64u, wave_lds_bytes(S.tw, S.th, K.lm, true), true, true};
+    return RoundLaunch{K.wave, 64u, wave_lds_bytes(S.tw, S.th, K.lm, false), true, false};
+  }
+  return RoundLaunch{K.round, workgroup_threads(S), lds_bytes(S.tw, S.th, Lmax, K.narrow), false, false};
 }
 
 // Whether a field and its board need none of what the lean kernels leave
@@ -1913,13 +1946,14 @@ void ctx_init(Ctx& c, int device) {
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.queue, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
     if (k.wave) HIP_OK(hipFuncSetAttribute(k.wave, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    if (k.wave_pk) HIP_OK(hipFuncSetAttribute(k.wave_pk, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
   }
 }
 
 void ctx_free(Ctx& c) {
   Workspace& w = c.ws;
   void* ptrs[] = {w.labels, w.state, w.act, w.key, w.list, w.fields, w.item_field, w.pool, w.counters,
-                  w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards, w.qslots, w.qctl};
+                  w.plen, w.poff, w.pcost, w.pcells, w.fout, w.boards, w.qslots, w.qctl, w.fmax};
   (void)hipSetDevice(c.device);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1991,19 +2025,22 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   // an own pad) plus the largest additive price.  Boards under the narrow
   // limit relax with 16-bit step planes in LDS.
   uint64_t max_e = 0, max_w = 1, max_a = 0, max_plane = 0;
-  for (uint32_t k = 0; k + 1 < w; ++k) max_e = std::max<uint64_t>(max_e, xe[k]);
-  for (uint32_t k = 0; k + 1 < h; ++k) max_e = std::max<uint64_t>(max_e, ye[k]);
+  uint64_t min_e = UINT64_MAX, min_w = UINT64_MAX, min_a = UINT64_MAX;
+  for (uint32_t k = 0; k + 1 < w; ++k) max_e = std::max<uint64_t>(max_e, xe[k]), min_e = std::min<uint64_t>(min_e, xe[k]);
+  for (uint32_t k = 0; k + 1 < h; ++k) max_e = std::max<uint64_t>(max_e, ye[k]), min_e = std::min<uint64_t>(min_e, ye[k]);
   for (uint32_t k = 0; k + 1 < layers; ++k)
-    if (via_allowed[k] && ve) max_e = std::max<uint64_t>(max_e, ve[k]);
+    if (via_allowed[k] && ve) max_e = std::max<uint64_t>(max_e, ve[k]), min_e = std::min<uint64_t>(min_e, ve[k]);
   std::vector<uint8_t> obst(n);
   bool unit_w = true;  // every passable cell weighs 1: no weight plane on the device
   for (size_t i = 0; i < n; ++i) {
     obst[i] = cost[i] == kMax;
+    uint64_t a = add ? add[i] : 0;
     if (!obst[i]) {
       max_w = std::max<uint64_t>(max_w, cost[i]);
+      min_w = std::min<uint64_t>(min_w, cost[i]);
+      min_a = std::min(min_a, a);
       unit_w = unit_w && cost[i] == 1u;
     }
-    uint64_t a = add ? add[i] : 0;
     max_plane = std::max(max_plane, a);
     if (halo) a += halo_pen;
     max_a = std::max(max_a, a);
@@ -2012,6 +2049,9 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   b.obst = std::move(obst);
   b.max_ew = max_e * max_w;
   b.max_step = std::min<uint64_t>(b.max_ew + max_a, kMax - 1);
+  // no edge or no passable cell: no step exists, any bound holds
+  b.min_ew = (min_e == UINT64_MAX || min_w == UINT64_MAX) ? 0 : min_e * min_w;
+  b.min_step = b.min_ew + (min_a == UINT64_MAX ? 0 : min_a);
   b.w = w;
   b.h = h;
   b.layers = layers;
@@ -2081,6 +2121,7 @@ void board_clear_additive(DevBoard& b) {
     b.bytes -= (size_t)b.w * b.h * b.layers * 2;
   }
   b.max_step = std::min<uint64_t>(b.max_ew + (b.halo ? b.halo_pen : 0u), kMax - 1);
+  b.min_step = b.min_ew;
 }
 
 void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_add) {
@@ -2102,6 +2143,7 @@ void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_a
     HIP_OK(hipMemcpyAsync(b.add, add, n * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
   }
   b.max_step = std::min<uint64_t>(b.max_ew + max_a, kMax - 1);
+  b.min_step = b.min_ew + *std::min_element(add, add + n);
   HIP_OK(hipStreamSynchronize(c.stream));
 }
 
@@ -2373,8 +2415,24 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   launch_fill32(s, W.state, 0u, item);
   launch_fill32(s, W.counters, 0u, kCtrWords);
 
+  // Packed u32 labels when every open step of the batch's boards costs at
+  // least 1 (hops <= dist then) and the wave form runs; checked after the
+  // solve, re-solved in wide labels when a label came within a step of the
+  // packed dist limit.
+  bool pack = !c.no_pack && c.mode == MODE_PLAIN;
+  uint64_t pk_step = 0;
+  for (const DevBoard* b : boards) {
+    pack = pack && b->min_step >= 1 && b->max_step < 0xFFFFu;
+    pk_step = std::max<uint64_t>(pk_step, b->max_step);
+  }
   RelaxArgs A = make_args(c, item);
-  const RoundLaunch RL = round_launch(K, S, Lmax, unit, c.mode == MODE_PLAIN && c.delta != kMax);
+  const RoundLaunch RL = round_launch(K, S, Lmax, unit, c.mode == MODE_PLAIN && c.delta != kMax, pack);
+  if (RL.packed) {
+    take(c, W.fmax, W.fmax_cap, std::max<size_t>(item, 1));
+    check_arena(c, {{W.fmax, std::max<size_t>(item, 1) * 4}});
+    launch_fill32(s, W.fmax, 0u, item);
+    A.item_max = W.fmax;
+  }
   SolveStats local;
   local.field_cells = field_cells;
   bool timed = false;
@@ -2440,7 +2498,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         }
         std::atomic_thread_fence(std::memory_order_acquire);
         ++local.launches;
-        if (RL.fn == K.wave) ++local.wave_launches;
+        if (RL.wave) ++local.wave_launches;
         ++rounds;
         if (hc[1]) break;
         n_active = hc[0];
@@ -2559,15 +2617,33 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     local.rounds = rounds;
   }
   if (timed) HIP_OK(hipEventRecord(W.ev1, s));
+  if (RL.packed && item)
+    hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<size_t>((item + 255) / 256, 1024)), dim3(256), 0, s,
+                       W.fmax, (size_t)item, W.counters);
   uint32_t* cnt = W.pinned;
   HIP_OK(hipMemcpyAsync(cnt, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   if (cnt[C_ERROR * kCtrStride]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR * kCtrStride]));
+  if (RL.packed && (uint64_t)cnt[C_PMAX * kCtrStride] + ((pk_step << 16) | 1u) >= 0xFFFF0000ull) {
+    // a path came within one step of the packed dist limit: the labels may
+    // be clipped, so the batch is solved again in wide labels
+    c.no_pack = true;
+    try {
+      solve_fields(c, boards, reqs, st);
+    } catch (...) {
+      c.no_pack = false;
+      throw;
+    }
+    c.no_pack = false;
+    return;
+  }
+  c.packed = RL.packed;
   float ms = 0;
   if (timed) HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   local.relax_ms = ms;
   local.tile_visits = cnt[C_VISITS * kCtrStride];
   local.changed_visits = cnt[C_CHANGED * kCtrStride];
+  local.label_bytes = RL.packed ? 4u : 8u;
   local.local_rounds = cnt[C_ROUNDS * kCtrStride];
   local.cells_visited = (uint64_t)cnt[C_VISITS * kCtrStride] * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2585,6 +2661,7 @@ static FieldRead make_read(Ctx& c) {
   R.state = W.state;
   R.tw = c.shape.tw;
   R.th = c.shape.th;
+  R.packed = c.packed ? 1u : 0u;
   return R;
 }
 
@@ -3023,7 +3100,7 @@ uint32_t driver_step(RoundDriver& d) {
       unit = unit && d.groups[g].unit;
     }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
-  const RoundLaunch RL = round_launch(K, S, Lmax, unit, false);
+  const RoundLaunch RL = round_launch(K, S, Lmax, unit, false, false);
   void* args[] = {&A};
   if (d.stamp_round >= (1u << 30)) {
     // states and flags stay below 2^31: every touched item now reads as
@@ -3113,6 +3190,7 @@ void driver_paths(RoundDriver& d, int g, std::vector<uint8_t>& routed, std::vect
   R.fields = d.fields + G.field0;
   R.labels = d.labels;
   R.state = d.state;
+  R.packed = 0;  // the round driver keeps wide labels
   R.tw = d.c->shape.tw;
   R.th = d.c->shape.th;
   reconstruct_with(*d.c, R, d.pool, G.nfields, routed, cost, off, cells, nullptr);

@@ -27,6 +27,23 @@
 // loaded (to tell which border cells this visit improved): 10.5 KB for
 // 16 x 16 x 4, so the register file, not the LDS, bounds the visits per CU.
 //
+// Labels come in two forms (template parameter PK):
+//   * wide: u64 (dist << 32) | hops, the form every other kernel uses;
+//   * packed: u32 (dist << 16) | hops, 4 B per cell in HBM and LDS.  The
+//     step of price s is (s << 16) | 1, added with saturation, so the packed
+//     minimum over paths is min(dist * 2^16 + hops).  That equals the
+//     lexicographic (dist, hops) minimum at every cell whose dist is below
+//     0xFFFF and whose hops are below 2^16; on a board whose every open step
+//     costs at least 1, hops <= dist, so dist < 0xFFFF suffices.  A closed
+//     step is 0xFFFF << 16 | 1, which saturates from any label with dist >= 1
+//     and from a source (dist 0) gives a label with dist field 0xFFFF that
+//     never propagates and never beats a real label: dist field 0xFFFF reads
+//     as unreached.  Every visit stores the largest label it writes back
+//     (excluding those) for its tile; the host accepts the packed result
+//     only if the largest of them plus the largest packed step stays below
+//     0xFFFF << 16 (then no path to any cell reaches dist 0xFFFF), and
+//     solves the batch again in the wide form otherwise.
+//
 // Scope: lean fields (no halo-owner or legal planes, overrides, own pads or
 // via-exempt pads) on boards without a weight plane whose step prices fit the
 // narrow (15-bit) form, in the round form without delta stepping.  A forced
@@ -34,7 +51,7 @@
 // a cell that is neither a trace node nor an obstacle and every open step
 // price is e + a.  The host picks this kernel only for such launches.
 
-template <int TW, int TH, int LM>
+template <int TW, int TH, int LM, bool PK = false>
 struct WaveTile {
   static_assert(LM * TH <= 64 && LM * TW <= 64, "a lane per row and a lane per column");
   static_assert((TW * TH) % 64 == 0 && 64 % TW == 0, "whole via columns per lane; a fixed x per lane in block order");
@@ -48,11 +65,11 @@ struct WaveTile {
   static constexpr uint32_t LAB = 64 * RS;
   // border labels as loaded: left [LM*TH], right [LM*TH], top [LM*TW], bottom [LM*TW]
   static constexpr uint32_t BORDER = 2 * LM * TH + 2 * LM * TW;
-  static constexpr size_t lds = (size_t)(LAB + BORDER) * 8;
+  static constexpr size_t lds = (size_t)(LAB + BORDER) * (PK ? 4 : 8);
 };
 
-__host__ __device__ inline size_t wave_lds_bytes(uint32_t tw, uint32_t th, uint32_t lm) {
-  return (64ull * (tw + 1) + 2ull * lm * th + 2ull * lm * tw) * 8;
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t tw, uint32_t th, uint32_t lm, bool pk) {
+  return (64ull * (tw + 1) + 2ull * lm * th + 2ull * lm * tw) * (pk ? 4 : 8);
 }
 
 // 16-bit step prices, two to a register: lo = entering from the lower
@@ -78,6 +95,45 @@ __device__ __forceinline__ uint64_t offer(uint64_t v, uint32_t p) {
   return ((uint64_t)__builtin_elementwise_add_sat((uint32_t)(v >> 32), p) << 32) | (uint32_t)((uint32_t)v + 1u);
 }
 
+// The two label forms: type, unreached value, relaxation through the lower
+// or upper step of a packed pair, the candidate a label offers through a
+// price pair half, and the halo value taken from a neighbour.
+template <bool PK>
+struct WaveLab;
+
+template <>
+struct WaveLab<false> {
+  using T = uint64_t;
+  static constexpr uint64_t unr = kUnreached;
+  __device__ static bool relax_lo(uint64_t from, uint32_t pair, uint64_t& cur) { return relax_w(from, step_lo(pair), cur); }
+  __device__ static bool relax_hi(uint64_t from, uint32_t pair, uint64_t& cur) { return relax_w(from, step_hi(pair), cur); }
+  __device__ static uint64_t offer_lo(uint64_t v, uint32_t pair) { return offer(v, step_lo(pair)); }
+  __device__ static uint64_t offer_hi(uint64_t v, uint32_t pair) { return offer(v, step_hi(pair)); }
+  __device__ static uint64_t halo(uint64_t v) { return lab_in(v); }
+};
+
+// Packed labels whose dist field is 0xFFFF read as unreached.
+constexpr uint32_t kPackedUnreached = 0xFFFF0000u;
+
+template <>
+struct WaveLab<true> {
+  using T = uint32_t;
+  static constexpr uint32_t unr = 0xFFFFFFFFu;
+  __device__ static uint32_t w_lo(uint32_t pair) { return (pair << 16) | 1u; }
+  __device__ static uint32_t w_hi(uint32_t pair) { return (pair & 0xFFFF0000u) | 1u; }
+  __device__ static bool relax(uint32_t from, uint32_t w, uint32_t& cur) {
+    const uint32_t cand = __builtin_elementwise_add_sat(from, w);
+    const bool better = cand < cur;
+    cur = better ? cand : cur;
+    return better;
+  }
+  __device__ static bool relax_lo(uint32_t from, uint32_t pair, uint32_t& cur) { return relax(from, w_lo(pair), cur); }
+  __device__ static bool relax_hi(uint32_t from, uint32_t pair, uint32_t& cur) { return relax(from, w_hi(pair), cur); }
+  __device__ static uint32_t offer_lo(uint32_t v, uint32_t pair) { return __builtin_elementwise_add_sat(v, w_lo(pair)); }
+  __device__ static uint32_t offer_hi(uint32_t v, uint32_t pair) { return __builtin_elementwise_add_sat(v, w_hi(pair)); }
+  __device__ static uint32_t halo(uint32_t v) { return v; }
+};
+
 // Narrow step price of entering a cell with board mask bm and additive
 // price a through the edge whose open bit is `edge`, over gap price e; the
 // extra condition `ok` closes it too (layer bounds).
@@ -86,36 +142,32 @@ __device__ __forceinline__ uint32_t wave_step(uint8_t bm, uint8_t edge, bool ok,
   return open ? e + a : kMax;
 }
 
-// The plane elements of four consecutive cells (element alignment only).
-__device__ __forceinline__ void load_row_planes(const BoardView& B, uint32_t g0, uint8_t* bm, uint32_t* a, int k0) {
-  const u8x4 b4 = load4(B.bmask + g0);
-  u32x4 a4 = u32x4{0, 0, 0, 0};
-  if (B.add16) {
-    const u16x4 h4 = load4(B.add16 + g0);
-    a4 = u32x4{h4[0], h4[1], h4[2], h4[3]};
-  } else if (B.add) {
-    a4 = load4(B.add + g0);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    bm[k0 + k] = b4[k];
-    a[k0 + k] = a4[k];
-  }
+// Additive plane of a board: none, u16 or u32 (a template parameter of the
+// visit, so no load waits on a pointer test).
+enum AddKind : int { ADD_NONE = 0, ADD_U16 = 1, ADD_U32 = 2 };
+
+template <int AK>
+__device__ __forceinline__ uint32_t add_k(const BoardView& B, uint32_t g) {
+  if constexpr (AK == ADD_U16) return gbl(B.add16)[g];
+  if constexpr (AK == ADD_U32) return gbl(B.add)[g];
+  return 0u;
 }
 
-template <int TW, int TH, int LM>
-__device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, uint32_t round) {
-  using WT = WaveTile<TW, TH, LM>;
+template <int TW, int TH, int LM, int AK, bool PK>
+__device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched, const FieldDesc& F,
+                                const BoardView& B, char* smem, uint32_t round) {
+  using WT = WaveTile<TW, TH, LM, PK>;
+  using WL = WaveLab<PK>;
+  using LT = typename WL::T;
   constexpr uint32_t RS = WT::RS, PT = WT::PT, NV = WT::NV, NS = WT::NS, LAB = WT::LAB;
-  uint64_t* lab = (uint64_t*)smem;
-  uint32_t* lab32 = (uint32_t*)smem;  // step staging before the labels go in: y prices [LAB], via prices [LAB]
-  uint64_t* b0 = lab + LAB;
+  LT* lab = (LT*)smem;
+  // step staging before the labels go in: y prices, then via prices (packed
+  // labels: one after the other in the first LAB words; wide: side by side)
+  uint32_t* lab32 = (uint32_t*)smem;
+  constexpr uint32_t VOFF = PK ? 0u : LAB;
+  LT* b0 = lab + LAB;
   const uint32_t lane = threadIdx.x;
 
-  const uint32_t f = const_load(A.item_field + item);
-  const bool touched = (const_load(A.state + item) & S_TOUCHED) != 0;
-  const FieldDesc F = const_load(A.fields + f);
-  const BoardView B = const_load(A.boards + F.board);
   const uint32_t t = item - F.item0;
   const uint32_t txi = t % F.tx, tyi = t / F.tx;
   const uint32_t L = B.layers;
@@ -125,74 +177,67 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
   const uint64_t blockbase = F.lab_off + (uint64_t)t * TB;
   const bool has_l = txi > 0, has_r = txi + 1 < F.tx, has_t = tyi > 0, has_b = tyi + 1 < F.ty;
 
-  // neighbour states (lanes 1..4): whether their border labels exist
-  uint32_t stv = 0;
-  if (lane >= 1 && lane < 5) {
-    uint32_t nb = item;
-    bool ok = true;
-    if (lane == 1) ok = has_l, nb = item - 1;
-    if (lane == 2) ok = has_r, nb = item + 1;
-    if (lane == 3) ok = has_t, nb = item - F.tx;
-    if (lane == 4) ok = has_b, nb = item + F.tx;
-    stv = ok ? __hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-  }
-
-  // ---- loads, all issued before the first use ---------------------------------
-  // The tile's labels in block order: lane holds cells lane + 64 j, i.e.
-  // column x = lane % TW of row (64 / TW) * (j % NV) + lane / TW of layer
-  // j / NV -- the same x for every j, so each cell's LDS slot is a constant
-  // offset from one per-lane base (`sbase`, also the via layout's base).
-  constexpr uint32_t RPJ = 64 / TW;  // rows per 64 cells
-  const uint32_t sx_ = lane % TW, sy_ = lane / TW;
-  const uint32_t sbase = sy_ * RS + sx_;
-  uint64_t sl[NS];
-#pragma unroll
-  for (uint32_t j = 0; j < NS; ++j) sl[j] = kUnreached;
-  if (touched) {
-    const auto* lp = gbl(A.labels) + blockbase + lane;
-#pragma unroll
-    for (uint32_t j = 0; j < NS; ++j)
-      if (j / NV < L) sl[j] = lp[64u * j];
-  }
+  // ---- loads, all issued before the first use, in the order they are used ------
+  // (vector loads complete in issue order: the step prices wait only for the
+  // planes, the tile's labels are still in flight meanwhile)
   // row layout: lane (rl, ry) holds row ry of layer rl
   const uint32_t rl = lane / TH, ry = lane - rl * TH;
   const bool rrow = rl < L;
   const bool row_in = rrow && wy0 + ry < F.wh;
   const uint32_t rgy = F.y0 + min(wy0 + ry, F.wh - 1);
   const uint32_t rowg = ((rrow ? rl : 0u) * B.h + rgy) * B.w + F.x0;  // board cell of window column 0
-  uint8_t bm[TW];
-  uint32_t av[TW];
-  if (wx0 + TW <= F.ww) {
+  // The row's planes as packed words: mask bytes four to a word, u16
+  // additive prices two to a word, u32 ones one to a word.  Rows that start
+  // on a multiple of four cells (every row when the board width and the
+  // tile's first column are) load whole words; others, and tiles past the
+  // window's edge, load cell by cell (clamped to the window) and pack.
+  constexpr uint32_t AW = AK == ADD_U32 ? TW : AK == ADD_U16 ? TW / 2 : 1;  // additive words
+  uint32_t mw[TW / 4], aw[AW];
+  if ((B.w % 4) == 0 && ((F.x0 + wx0) % 4) == 0 && wx0 + TW <= F.ww) {
     const uint32_t g0 = rowg + wx0;
+    const auto* mp = (const __attribute__((address_space(1))) uint32_t*)(B.bmask + g0);
 #pragma unroll
-    for (int q = 0; q < TW / 4; ++q) load_row_planes(B, g0 + 4 * q, bm, av, 4 * q);
+    for (uint32_t q = 0; q < TW / 4; ++q) mw[q] = mp[q];
+    if constexpr (AK == ADD_U16) {
+      const auto* ap = (const __attribute__((address_space(1))) uint32_t*)(B.add16 + g0);
+#pragma unroll
+      for (uint32_t q = 0; q < AW; ++q) aw[q] = ap[q];
+    }
+    if constexpr (AK == ADD_U32) {
+#pragma unroll
+      for (uint32_t q = 0; q < AW; ++q) aw[q] = gbl(B.add)[g0 + q];
+    }
   } else {
+    uint32_t bq[TW], aq[TW];
 #pragma unroll
     for (int x = 0; x < TW; ++x) {
       const uint32_t g = rowg + min(wx0 + x, F.ww - 1);
-      bm[x] = gbl(B.bmask)[g];
-      av[x] = add_at(B, g);
+      bq[x] = gbl(B.bmask)[g];
+      aq[x] = add_k<AK>(B, g);
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < TW / 4; ++q)
+      mw[q] = bq[4 * q] | (bq[4 * q + 1] << 8) | (bq[4 * q + 2] << 16) | (bq[4 * q + 3] << 24);
+    if constexpr (AK == ADD_U16) {
+#pragma unroll
+      for (uint32_t q = 0; q < AW; ++q) aw[q] = aq[2 * q] | (aq[2 * q + 1] << 16);
+    }
+    if constexpr (AK == ADD_U32) {
+#pragma unroll
+      for (uint32_t q = 0; q < AW; ++q) aw[q] = aq[q];
     }
   }
+  if constexpr (AK == ADD_NONE) aw[0] = 0;
   // column layout: lane (cl, cx) holds column cx of layer cl
   const uint32_t cl = lane / TW, cx = lane - cl * TW;
   const bool ccol = cl < L;
   const bool col_in = ccol && wx0 + cx < F.ww;
-  // Halo labels (kept only if the neighbour's first visit ran in an earlier
-  // round) and the board cells of the halo (the price of stepping from the
-  // tile into them).  Every load is issued: where a neighbour or the cell
-  // does not exist the address falls back to one inside this tile, and the
-  // value is dropped below.
+  // Board cells of the halo (the price of stepping from the tile into them)
+  // and, further down, the halo labels (kept only if the neighbour's first
+  // visit ran in an earlier round).  Every load is issued: where a neighbour
+  // or the cell does not exist the address falls back to one inside this
+  // tile, and the value is dropped.
   const bool el = row_in && has_l, er = row_in && has_r, et = col_in && has_t, eb = col_in && has_b;
-  const uint64_t own = blockbase + lane % TW;
-  const uint64_t hla = el ? blockbase - TB + ncell + L * TH + rl * TH + ry : own;
-  const uint64_t hra = er ? blockbase + TB + ncell + rl * TH + ry : own;
-  const uint64_t hta = et ? blockbase - (uint64_t)F.tx * TB + cl * PT + (TH - 1) * TW + cx : own;
-  const uint64_t hba = eb ? blockbase + (uint64_t)F.tx * TB + cl * PT + cx : own;
-  uint64_t hl = __hip_atomic_load(&A.labels[hla], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t hr = __hip_atomic_load(&A.labels[hra], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t ht = __hip_atomic_load(&A.labels[hta], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t hb = __hip_atomic_load(&A.labels[hba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint8_t hbm[4];
   uint32_t ha[4];
   {
@@ -204,9 +249,8 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint8_t m = gbl(B.bmask)[g[k]];
-      const uint32_t a = add_at(B, g[k]);
+      ha[k] = add_k<AK>(B, g[k]);
       hbm[k] = e[k] ? m : (uint8_t)M_TRACE;
-      ha[k] = a;
     }
   }
   // gap prices: x gaps uniform (lane i <= TW holds the price of entering
@@ -224,16 +268,55 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
   const uint32_t gyt = F.y0 + wy0, gyb = F.y0 + wy0 + TH;
   const uint32_t ytop = (gyt >= 1 && gyt < B.h) ? gbl(B.ye)[gyt - 1] : 0u;
   const uint32_t ybot = (gyb >= 1 && gyb < B.h) ? gbl(B.ye)[gyb - 1] : 0u;
+  // neighbour states (lanes 1..4): whether their border labels exist
+  uint32_t stv = 0;
+  if (lane >= 1 && lane < 5) {
+    uint32_t nb = item;
+    bool ok = true;
+    if (lane == 1) ok = has_l, nb = item - 1;
+    if (lane == 2) ok = has_r, nb = item + 1;
+    if (lane == 3) ok = has_t, nb = item - F.tx;
+    if (lane == 4) ok = has_b, nb = item + F.tx;
+    stv = ok ? __hip_atomic_load(&A.state[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+  }
+  // Halo labels.  Labels are read and written with plain 8-byte accesses
+  // (never torn): every label this visit takes was stored in an earlier
+  // launch, and a neighbour that is visited again in this one only lowers
+  // its labels, so whichever value a load returns is an upper bound.
+  const uint64_t own = blockbase + lane % TW;
+  const uint64_t hla = el ? blockbase - TB + ncell + L * TH + rl * TH + ry : own;
+  const uint64_t hra = er ? blockbase + TB + ncell + rl * TH + ry : own;
+  const uint64_t hta = et ? blockbase - (uint64_t)F.tx * TB + cl * PT + (TH - 1) * TW + cx : own;
+  const uint64_t hba = eb ? blockbase + (uint64_t)F.tx * TB + cl * PT + cx : own;
+  const auto* glab = gbl((const LT*)A.labels);
+  LT hl = glab[hla], hr = glab[hra], ht = glab[hta], hb = glab[hba];
+  // The tile's labels in block order: lane holds cells lane + 64 j, i.e.
+  // column x = lane % TW of row (64 / TW) * (j % NV) + lane / TW of layer
+  // j / NV -- the same x for every j, so each cell's LDS slot is a constant
+  // offset from one per-lane base (`sbase`, also the via layout's base).
+  // Layers beyond the board's re-read its last layer (dropped below).
+  constexpr uint32_t RPJ = 64 / TW;  // rows per 64 cells
+  const uint32_t sx_ = lane % TW, sy_ = lane / TW;
+  const uint32_t sbase = sy_ * RS + sx_;
+  LT sl[NS];
+  if (touched) {
+    const auto* lp = glab + blockbase + lane;
+#pragma unroll
+    for (uint32_t j = 0; j < NS; ++j) sl[j] = lp[64u * (min(j / NV, L - 1) * NV + j % NV)];
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < NS; ++j) sl[j] = WL::unr;
+  }
 
   auto earlier = [&](uint32_t st) { return (st & S_TOUCHED) && (st >> 1) < round; };
   const uint32_t nbv = (earlier(__builtin_amdgcn_readlane(stv, 1)) ? 1u : 0u) |
                        (earlier(__builtin_amdgcn_readlane(stv, 2)) ? 2u : 0u) |
                        (earlier(__builtin_amdgcn_readlane(stv, 3)) ? 4u : 0u) |
                        (earlier(__builtin_amdgcn_readlane(stv, 4)) ? 8u : 0u);
-  hl = (nbv & 1u) ? lab_in(hl) : kUnreached;
-  hr = (nbv & 2u) ? lab_in(hr) : kUnreached;
-  ht = (nbv & 4u) ? lab_in(ht) : kUnreached;
-  hb = (nbv & 8u) ? lab_in(hb) : kUnreached;
+  hl = (nbv & 1u) ? WL::halo(hl) : WL::unr;
+  hr = (nbv & 2u) ? WL::halo(hr) : WL::unr;
+  ht = (nbv & 4u) ? WL::halo(ht) : WL::unr;
+  hb = (nbv & 8u) ? WL::halo(hb) : WL::unr;
   // prices of stepping out of the tile into the halo cells: the cell's edge
   // facing the tile must be open and the cell passable (narrow: at most 0x7FFE)
   const uint32_t xleft = __builtin_amdgcn_readlane(xgv, 0), xright = __builtin_amdgcn_readlane(xgv, TW);
@@ -242,20 +325,27 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
 
   // ---- step prices of every cell (row layout) ---------------------------------
   uint32_t sx[TW];  // row layout: from x-1 | from x+1
+  uint32_t sv_st[TW];  // via prices of the row, staged after the y prices
   {
     uint32_t xg[TW + 1];
 #pragma unroll
     for (int i = 0; i <= TW; ++i) xg[i] = __builtin_amdgcn_readlane(xgv, i);
 #pragma unroll
     for (int x = 0; x < TW; ++x) {
-      const uint8_t m = (row_in && wx0 + x < F.ww) ? bm[x] : (uint8_t)M_TRACE;  // outside the window: closed
-      const uint32_t a = av[x];
-      sx[x] = pack_steps(wave_step(m, M_NX, true, xg[x], a), wave_step(m, M_PX, true, xg[x + 1], a));
+      const uint8_t mb = (uint8_t)(mw[x / 4] >> (8 * (x % 4)));
+      const uint8_t m = (row_in && wx0 + x < F.ww) ? mb : (uint8_t)M_TRACE;  // outside the window: closed
+      const uint32_t a = AK == ADD_U32 ? aw[x % AW] : AK == ADD_U16 ? (aw[(x / 2) % AW] >> (16 * (x % 2))) & 0xFFFFu : 0u;
+      uint32_t px = pack_steps(wave_step(m, M_NX, true, xg[x], a), wave_step(m, M_PX, true, xg[x + 1], a));
+      // keep the packed form: the sweeps read each half with a sign-extending
+      // operand select, no per-step gate
+      asm("" : "+v"(px));
+      sx[x] = px;
       // staged for the column and via layouts; lanes without a row
       // (layers beyond the board's) stage closed prices in rows no lane reads
       const uint32_t at = lane * RS + x;  // lane = rl * TH + ry
       lab32[at] = pack_steps(wave_step(m, M_NY, true, yg0, a), wave_step(m, M_PY, true, yg1, a));
-      lab32[LAB + at] = pack_steps(wave_step(m, M_DN, rl > 0, vlo, a), wave_step(m, M_UP, rl + 1 < L, vhi, a));
+      sv_st[x] = pack_steps(wave_step(m, M_DN, rl > 0, vlo, a), wave_step(m, M_UP, rl + 1 < L, vhi, a));
+      if (!PK) lab32[VOFF + at] = sv_st[x];
     }
   }
   __syncthreads();
@@ -263,10 +353,16 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
   uint32_t sv[NV][LM];    // via layout: from layer-1 | from layer+1
 #pragma unroll
   for (int y = 0; y < TH; ++y) sy[y] = ccol ? lab32[(cl * TH + y) * RS + cx] : kMax;
+  if (PK) {
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < TW; ++x) lab32[lane * RS + x] = sv_st[x];
+    __syncthreads();
+  }
 #pragma unroll
   for (uint32_t k = 0; k < NV; ++k) {
 #pragma unroll
-    for (int l = 0; l < LM; ++l) sv[k][l] = (uint32_t)l < L ? lab32[LAB + sbase + (l * TH + RPJ * k) * RS] : kMax;
+    for (int l = 0; l < LM; ++l) sv[k][l] = (uint32_t)l < L ? lab32[VOFF + sbase + (l * TH + RPJ * k) * RS] : kMax;
   }
   __syncthreads();
   // ---- labels into the buffer -------------------------------------------------
@@ -274,8 +370,8 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
 #pragma unroll
   for (uint32_t j = 0; j < NS; ++j) {
     const uint32_t l = j / NV, y = RPJ * (j % NV);  // + sy_
-    const bool in = sx_in && wy0 + y + sy_ < F.wh;
-    lab[sbase + (l * TH + y) * RS] = in ? sl[j] : kUnreached;
+    const bool in = l < L && sx_in && wy0 + y + sy_ < F.wh;
+    lab[sbase + (l * TH + y) * RS] = in ? sl[j] : WL::unr;
   }
   __syncthreads();
   // the border labels as loaded, before seeds and sweeps change them
@@ -300,7 +396,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
       const uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0;
       if (x >= TW || y >= TH || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
       if (!(gbl(B.bmask)[g] & (M_TRACE | M_OBST))) {
-        lab[(l * TH + y) * RS + x] = 0ull;
+        lab[(l * TH + y) * RS + x] = LT(0);
         put = true;
       }
     }
@@ -316,21 +412,25 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
     // rows
     {
       bool ch = false;
+      // the price pairs stay packed across rounds: derived prices are formed
+      // inside each sweep, not kept for every cell
+#pragma unroll
+      for (int x = 0; x < TW; ++x) asm("" : "+v"(sx[x]));
       if (rrow) {
         const uint32_t base = (rl * TH + ry) * RS;
-        uint64_t v[TW];
+        LT v[TW];
 #pragma unroll
         for (int x = 0; x < TW; ++x) v[x] = lab[base + x];
-        uint64_t p = hl;
+        LT p = hl;
 #pragma unroll
         for (int x = 0; x < TW; ++x) {
-          ch |= relax_w(p, step_lo(sx[x]), v[x]);
+          ch |= WL::relax_lo(p, sx[x], v[x]);
           p = v[x];
         }
         p = hr;
 #pragma unroll
         for (int x = TW - 1; x >= 0; --x) {
-          ch |= relax_w(p, step_hi(sx[x]), v[x]);
+          ch |= WL::relax_hi(p, sx[x], v[x]);
           p = v[x];
         }
         if (ch) {
@@ -350,21 +450,23 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
     // columns
     {
       bool ch = false;
+#pragma unroll
+      for (int y = 0; y < TH; ++y) asm("" : "+v"(sy[y]));
       if (ccol) {
         const uint32_t base = (cl * TH) * RS + cx;
-        uint64_t v[TH];
+        LT v[TH];
 #pragma unroll
         for (int y = 0; y < TH; ++y) v[y] = lab[base + y * RS];
-        uint64_t p = ht;
+        LT p = ht;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-          ch |= relax_w(p, step_lo(sy[y]), v[y]);
+          ch |= WL::relax_lo(p, sy[y], v[y]);
           p = v[y];
         }
         p = hb;
 #pragma unroll
         for (int y = TH - 1; y >= 0; --y) {
-          ch |= relax_w(p, step_hi(sy[y]), v[y]);
+          ch |= WL::relax_hi(p, sy[y], v[y]);
           p = v[y];
         }
         if (ch) {
@@ -385,16 +487,20 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
     if (L > 1) {
       bool ch = false;
 #pragma unroll
+      for (uint32_t kk = 0; kk < NV; ++kk)
+#pragma unroll
+        for (int l = 0; l < LM; ++l) asm("" : "+v"(sv[kk][l]));
+#pragma unroll
       for (uint32_t kk = 0; kk < NV; ++kk) {
         const uint32_t base = sbase + RPJ * kk * RS;
-        uint64_t v[LM];
+        LT v[LM];
 #pragma unroll
-        for (int l = 0; l < LM; ++l) v[l] = (uint32_t)l < L ? lab[l * TH * RS + base] : kUnreached;
+        for (int l = 0; l < LM; ++l) v[l] = (uint32_t)l < L ? lab[l * TH * RS + base] : WL::unr;
         bool cc = false;
 #pragma unroll
-        for (int l = 1; l < LM; ++l) cc |= relax_w(v[l - 1], step_lo(sv[kk][l]), v[l]);
+        for (int l = 1; l < LM; ++l) cc |= WL::relax_lo(v[l - 1], sv[kk][l], v[l]);
 #pragma unroll
-        for (int l = LM - 2; l >= 0; --l) cc |= relax_w(v[l + 1], step_hi(sv[kk][l]), v[l]);
+        for (int l = LM - 2; l >= 0; --l) cc |= WL::relax_hi(v[l + 1], sv[kk][l], v[l]);
         if (cc) {
 #pragma unroll
           for (int l = 0; l < LM; ++l)
@@ -421,35 +527,48 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, char* smem, u
   uint32_t fl = 0;
   if (row_in) {
     const uint32_t base = (rl * TH + ry) * RS;
-    const uint64_t v0 = lab[base], v1 = lab[base + TW - 1];
-    if (has_l && v0 < b0[rl * TH + ry] && step_lo(sx[0]) != kMax && offer(v0, step_lo(p_lr)) < hl) fl |= T_LEFT;
-    if (has_r && v1 < b0[LM * TH + rl * TH + ry] && step_hi(sx[TW - 1]) != kMax && offer(v1, step_hi(p_lr)) < hr)
+    const LT v0 = lab[base], v1 = lab[base + TW - 1];
+    if (has_l && v0 < b0[rl * TH + ry] && step_lo(sx[0]) != kMax && WL::offer_lo(v0, p_lr) < hl) fl |= T_LEFT;
+    if (has_r && v1 < b0[LM * TH + rl * TH + ry] && step_hi(sx[TW - 1]) != kMax && WL::offer_hi(v1, p_lr) < hr)
       fl |= T_RIGHT;
   }
   if (col_in) {
     const uint32_t base = (cl * TH) * RS + cx;
-    const uint64_t v0 = lab[base], v1 = lab[base + (TH - 1) * RS];
-    if (has_t && v0 < b0[2 * LM * TH + cl * TW + cx] && step_lo(sy[0]) != kMax && offer(v0, step_lo(p_tb)) < ht)
+    const LT v0 = lab[base], v1 = lab[base + (TH - 1) * RS];
+    if (has_t && v0 < b0[2 * LM * TH + cl * TW + cx] && step_lo(sy[0]) != kMax && WL::offer_lo(v0, p_tb) < ht)
       fl |= T_TOP;
     if (has_b && v1 < b0[2 * LM * TH + LM * TW + cl * TW + cx] && step_hi(sy[TH - 1]) != kMax &&
-        offer(v1, step_hi(p_tb)) < hb)
+        WL::offer_hi(v1, p_tb) < hb)
       fl |= T_BOTTOM;
   }
   // ---- write back: every label of the tile, then the border-column copies ------
+  // (device-scope stores: written through as they are issued, not left
+  // dirty in the XCD's L2 for the end of the launch to write back)
+  uint32_t vmax = 0;  // packed: largest label stored with a real dist
   {
-    uint64_t* lp = A.labels + blockbase + lane;
+    LT* lp = (LT*)A.labels + blockbase + lane;
 #pragma unroll
     for (uint32_t j = 0; j < NS; ++j) {
       const uint32_t l = j / NV, y = RPJ * (j % NV);
-      if (l < L) __hip_atomic_store(lp + 64u * j, lab[sbase + (l * TH + y) * RS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (l < L) {
+        const LT v = lab[sbase + (l * TH + y) * RS];
+        __hip_atomic_store(lp + 64u * j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (PK) vmax = max(vmax, (uint32_t)v < kPackedUnreached ? (uint32_t)v : 0u);
+      }
     }
   }
   if (rrow) {
     const uint32_t base = (rl * TH + ry) * RS;
-    __hip_atomic_store(&A.labels[blockbase + ncell + rl * TH + ry], lab[base], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&A.labels[blockbase + ncell + L * TH + rl * TH + ry], lab[base + TW - 1], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    LT* bp = (LT*)A.labels + blockbase + ncell + rl * TH + ry;
+    __hip_atomic_store(bp, lab[base], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(bp + L * TH, lab[base + TW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (PK) {
+    // the tile's largest stored label: its last visit's value is what the
+    // host's certificate reads (labels only decrease)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, o));
+    if (lane == 0) A.item_max[item] = vmax;
   }
   TileResult res;
   res.flags = (__ballot(fl & T_LEFT) ? T_LEFT : 0u) | (__ballot(fl & T_RIGHT) ? T_RIGHT : 0u) |
@@ -472,7 +591,7 @@ struct WaveMin {
   static constexpr int waves = 4;
 };
 
-template <int TW, int TH, int LM>
+template <int TW, int TH, int LM, bool PK>
 __global__ void __launch_bounds__(64, (WaveMin<TW, LM>::waves)) relax_wave(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -481,6 +600,16 @@ __global__ void __launch_bounds__(64, (WaveMin<TW, LM>::waves)) relax_wave(Relax
   }
   const uint32_t item = const_load(A.list_cur + blockIdx.x);
   const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
-  const TileResult res = wave_tile<TW, TH, LM>(A, item, smem, A.round);
+  const uint32_t f = const_load(A.item_field + item);
+  const bool touched = (const_load(A.state + item) & S_TOUCHED) != 0;
+  const FieldDesc F = const_load(A.fields + f);
+  const BoardView B = const_load(A.boards + F.board);
+  TileResult res;
+  if (B.add16)
+    res = wave_tile<TW, TH, LM, ADD_U16, PK>(A, item, f, touched, F, B, smem, A.round);
+  else if (B.add)
+    res = wave_tile<TW, TH, LM, ADD_U32, PK>(A, item, f, touched, F, B, smem, A.round);
+  else
+    res = wave_tile<TW, TH, LM, ADD_NONE, PK>(A, item, f, touched, F, B, smem, A.round);
   round_tail(A, item, grp, res, false, 0u);
 }

@@ -75,7 +75,7 @@ class StatsS(C.Structure):
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
                 ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double),
                 ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32), ("changed_visits", C.c_uint64),
-                ("validate_ms", C.c_double), ("wave_launches", C.c_uint32)]
+                ("validate_ms", C.c_double), ("wave_launches", C.c_uint32), ("label_bytes", C.c_uint32)]
 
 
 class TuningS(C.Structure):

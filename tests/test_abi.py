@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     L = mrfield.lib()
     missing = [s for s in declared_symbols() if not hasattr(L, s)]
     assert not missing, missing
-    assert L.mrf_abi_version() == 2
+    assert L.mrf_abi_version() == 3
     assert len(declared_symbols()) >= 20
 
 

@@ -51,12 +51,14 @@ def eng(request):
     e.close()
 
 
-def assert_form_ran(eng):
+def assert_form_ran(eng, wave_label_bytes=None):
     st = eng.stats()
     if eng.form == "wave":
         assert st["wave_launches"] == st["relax_launches"] > 0, st
+        if wave_label_bytes is not None:
+            assert st["label_bytes"] == wave_label_bytes, st
     else:
-        assert st["wave_launches"] == 0, st
+        assert st["wave_launches"] == 0 and st["label_bytes"] == 8, st
 
 
 def spec_of(s, **kw):
@@ -84,7 +86,7 @@ def test_config2_full_board(eng):
     board = mrfield.Board(eng, g, s.edges, s.pricing())
     try:
         got = board.solve_arrays(s.nets())
-        assert_form_ran(eng)
+        assert_form_ran(eng, 4)  # packed labels: every C2 distance is far below the 16-bit limit
         assert int(np.count_nonzero(got.routed)) > 200
         idx = list(range(256))
         full = [(0, 0, g.w - 1, g.h - 1)] * len(idx)

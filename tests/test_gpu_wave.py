@@ -2,8 +2,12 @@
 
 The wave form (tiles 16 x 16, 64 threads) runs for lean batches on boards
 without a weight plane whose step prices fit 15 bits; these cases are built
-to qualify, and every one asserts that the wave kernel is what ran.  Integer
-work: dist and hops of every cell, paths and search costs, bit for bit.
+to qualify, and every one asserts that the wave kernel is what ran.  Boards
+whose every open step costs at least 1 run it on packed u32 labels
+(dist << 16 | hops), certified after the solve and solved again on u64
+labels when a distance comes near the 16-bit limit; the cases assert which
+label width the result came from.  Integer work: dist and hops of every
+cell, paths and search costs, bit for bit.
 """
 from __future__ import annotations
 
@@ -54,25 +58,32 @@ def as_dev(g):
     return mrfield.Grid(g.w, g.h, g.layers, g.cost, g.via_forbidden, g.board)
 
 
-def assert_wave_ran(eng):
+def assert_wave_ran(eng, label_bytes=None):
     st = eng.stats()
     assert st["relax_launches"] > 0
     assert st["wave_launches"] == st["relax_launches"], st
+    if label_bytes is not None:
+        assert st["label_bytes"] == label_bytes, st
 
 
+@pytest.mark.parametrize("packed", [True, False])
 @pytest.mark.parametrize("L", [1, 2, 3, 4])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_wave_random_fields(eng, L, seed):
+def test_wave_random_fields(eng, L, seed, packed):
     """Multi-source fields on boards with every lean feature: trace nodes,
-    obstacles, one-way edges, via-forbidden cells, per-gap x / y prices
-    (zero included), vias off between some layers, an additive plane; full
-    and partial windows (ragged edge tiles)."""
-    rng = np.random.default_rng(1000 * L + seed)
+    obstacles, one-way edges, via-forbidden cells, per-gap x / y prices,
+    vias off between some layers, an additive plane; full and partial
+    windows (ragged edge tiles).  Packed: every price at least 1; wide: one
+    zero-price gap (a zero-cost step rules the packed labels out)."""
+    rng = np.random.default_rng(1000 * L + seed + (7 if packed else 0))
     w, h = int(rng.integers(20, 70)), int(rng.integers(20, 70))
     g = unit_board(seed + 10 * L, w, h, L)
-    ex = [int(v) for v in rng.integers(0, 30, size=w - 1)]
-    ey = [int(v) for v in rng.integers(0, 30, size=h - 1)]
-    via = [None if rng.random() < 0.25 else int(rng.integers(0, 60)) for _ in range(L - 1)]
+    lo = 1 if packed else 0
+    ex = [int(v) for v in rng.integers(lo, 30, size=w - 1)]
+    ey = [int(v) for v in rng.integers(lo, 30, size=h - 1)]
+    if not packed:
+        ex[int(rng.integers(0, w - 1))] = 0
+    via = [None if rng.random() < 0.25 else int(rng.integers(lo, 60)) for _ in range(L - 1)]
     edges = mrcpu.Edges(ex, ey, via)
     add = rng.integers(0, 40, size=g.n).astype(np.uint32)
     free = free_cells(g)
@@ -91,7 +102,7 @@ def test_wave_random_fields(eng, L, seed):
     dedges = mrfield.Edges(list(edges.x), list(edges.y), list(edges.via))
     got_d, got_h = eng.fields(as_dev(g), dedges, sources, dsts=dsts, windows=wins,
                               pricing=mrfield.Pricing(additive=add))
-    assert_wave_ran(eng)
+    assert_wave_ran(eng, 4 if packed else 8)
     for i in range(len(nets)):
         assert np.array_equal(got_d[i], ref_d[i]), (i, np.flatnonzero(got_d[i] != ref_d[i])[:10])
         assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
@@ -128,7 +139,7 @@ def test_wave_config1_complete_fields_and_paths(eng):
     nets = [mrcpu.Net(int(a), int(b)) for a, b in zip(s.src, s.dst)]
     ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(og, oe), nets, [[n.src] for n in nets], threads=8)
     got_d, got_h = eng.fields(g, s.edges, [[int(a)] for a in s.src], dsts=[int(b) for b in s.dst])
-    assert_wave_ran(eng)
+    assert_wave_ran(eng, 4)
     assert np.array_equal(got_d, ref_d)
     assert np.array_equal(got_h, ref_h)
     full = [(0, 0, g.w - 1, g.h - 1)] * len(nets)
@@ -201,3 +212,36 @@ def test_wave_same_round_neighbours_after_a_layout_change(eng):
     finally:
         board.close()
         eng.set_tiles(*WAVE)
+
+
+def test_wave_packed_certificate_falls_back_to_wide_labels(eng):
+    """Distances past the 16-bit packed limit (gap prices of 700 over a
+    96 x 80 board): the certificate rejects the packed solve and the batch is
+    solved again on u64 labels; fields and paths equal the oracle's.  The
+    same board with prices of 7 stays packed."""
+    w, h, L = 96, 80, 2
+    g = unit_board(31, w, h, L)
+    free = free_cells(g)
+    rng = np.random.default_rng(31)
+    sources = [[int(rng.choice(free))] for _ in range(5)]
+    dsts = [int(rng.choice(free)) for _ in range(5)]
+    nets = [mrcpu.Net(s[0], d) for s, d in zip(sources, dsts)]
+    for price, width in ((700, 8), (7, 4)):
+        edges = mrcpu.Edges([price] * (w - 1), [price] * (h - 1), [3 * price])
+        ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(g, edges), nets, sources)
+        if width == 8:
+            assert ref_d[ref_d != MAX].max() > 0xFFFF  # the case is past the limit
+        got_d, got_h = eng.fields(as_dev(g), mrfield.Edges(list(edges.x), list(edges.y), list(edges.via)), sources,
+                                  dsts=dsts)
+        assert_wave_ran(eng, width)
+        assert np.array_equal(got_d, ref_d)
+        assert np.array_equal(got_h, ref_h)
+        full = [(0, 0, w - 1, h - 1)] * len(nets)
+        ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(g, edges), nets, full)
+        got = eng.route_isolated_batch(as_dev(g), [mrfield.Net(n.src, n.dst) for n in nets], full,
+                                       mrfield.Edges(list(edges.x), list(edges.y), list(edges.via)))
+        assert_wave_ran(eng, width)
+        for i, r in enumerate(got):
+            assert (r is None) == (ref_p[i] is None), (price, i)
+            if r is not None:
+                assert r[0] == ref_p[i] and r[1] == ref_c[i], (price, i)
