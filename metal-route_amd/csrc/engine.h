@@ -395,6 +395,7 @@ struct RoundDriver {
   // relaxation counters (kCtrWords) followed by one activity flag per group
   // (the last round in which one of its visits activated a tile, plus one)
   uint32_t* ctr = nullptr;
+  uint32_t* shards = nullptr;  // sharded visit / round / done counters (engine.hip kShardWords)
   uint32_t* pinned = nullptr;  // readback of ctr (statistics)
   // host-mapped words the relaxation publishes to: next count, error flags,
   // stamp, then the group flags from kDriverFlagsAt

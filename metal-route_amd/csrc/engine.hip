@@ -103,6 +103,19 @@ enum : int {
 // behind the pollers.
 constexpr uint32_t kCtrStride = 32;
 constexpr uint32_t kCtrWords = C_NCOUNTERS * kCtrStride;
+// Counters every visit touches are sharded: workgroup b counts into shard
+// b % kShards of each kind (one 128-byte line per shard), so the visits of a
+// launch spread their atomics over kShards addresses instead of queueing on
+// one.  The host sums the shards.  SH_DONE holds each shard's count of
+// finished workgroups of the running launch (see round_tail).
+constexpr uint32_t kShards = 64;
+enum : uint32_t { SH_VISITS = 0, SH_ROUNDS = 1, SH_CHANGED = 2, SH_DONE = 3, SH_KINDS = 4 };
+constexpr uint32_t kShardWords = SH_KINDS * kShards * kCtrStride;
+inline uint64_t shard_sum(const uint32_t* shards, uint32_t kind) {
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < kShards; ++k) s += shards[((size_t)kind * kShards + k) * kCtrStride];
+  return s;
+}
 // Round driver's host-mapped words: [0] next count, [1] error flags, [2] stamp,
 // group flags from here on.
 constexpr uint32_t kDriverFlagsAt = 16;
@@ -390,7 +403,14 @@ struct RelaxArgs {
   // Packed labels (wave form): per item, the largest label its last visit
   // stored with a real dist (the host's certificate); null otherwise.
   uint32_t* item_max;
+  // Sharded counters (kShardWords words).
+  uint32_t* shards;
 };
+
+// The workgroup's shard of counter kind `kind`.
+__device__ __forceinline__ uint32_t* shard_of(const RelaxArgs& A, uint32_t kind) {
+  return A.shards + ((size_t)kind * kShards + blockIdx.x % kShards) * kCtrStride;
+}
 
 // Words of one tile block in the label buffer: the L x TH x TW labels
 // ([layer][y][x]) followed by copies of the tile's first and last columns
@@ -1108,8 +1128,8 @@ __device__ TileResult process_tile(const RelaxArgs& A, uint32_t item, char* smem
   if (tid == 0) {
     if (!QUEUE && !touched)
       __hip_atomic_fetch_or(&A.state[item], S_TOUCHED | (round << 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
-    atomicAdd(&A.counters[(C_ROUNDS) * kCtrStride], rounds);
+    atomicAdd(shard_of(A, SH_VISITS), 1u);
+    atomicAdd(shard_of(A, SH_ROUNDS), rounds);
   }
   __syncthreads();
   return res;
@@ -1197,7 +1217,7 @@ __device__ __forceinline__ void round_tail(const RelaxArgs& A, uint32_t item, ui
     if (A.group_active && (defer || (res.flags & (T_LEFT | T_RIGHT | T_TOP | T_BOTTOM | T_CAPPED))))
       was = __hip_atomic_exchange(&A.group_active[grp], A.round + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (A.key_cur) A.key_cur[item] = kMax;
-    if (!defer && (res.flags & (T_ANY | T_CAPPED))) atomicAdd(&A.counters[(C_CHANGED) * kCtrStride], 1u);
+    if (!defer && (res.flags & (T_ANY | T_CAPPED))) atomicAdd(shard_of(A, SH_CHANGED), 1u);
     if (A.key_cur && kmin != kMax) atomicMin(&A.counters[A.key_ctr * kCtrStride], kmin);
     if (A.host_out) {
       // The last workgroup of the launch publishes the counts.  Every
@@ -1206,9 +1226,23 @@ __device__ __forceinline__ void round_tail(const RelaxArgs& A, uint32_t item, ui
       // at the device's coherence point, so the last one reads final counts;
       // a relaxed increment keeps the caches untouched (an acquire-release
       // one would write back and invalidate L2 in every workgroup).
-      const uint32_t prev = __hip_atomic_fetch_add(&A.counters[C_DONE * kCtrStride], was > A.round + 1u ? 2u : 1u,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (prev == gridDim.x - 1) {
+      // Two levels: a workgroup counts itself in its shard; the one that
+      // completes its shard (whose size follows from the grid) resets it and
+      // counts the shard in C_DONE; the one that completes C_DONE is last.
+      // Each count is issued after the counts it completes have returned.
+      const uint32_t slot = blockIdx.x % kShards;
+      const uint32_t in_slot = (gridDim.x - slot + kShards - 1) / kShards;
+      uint32_t* sd = shard_of(A, SH_DONE);
+      const uint32_t prev = __hip_atomic_fetch_add(sd, was > A.round + 1u ? 2u : 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+      bool top = false;
+      if (prev + 1 == in_slot) {
+        __hip_atomic_store(sd, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t slots = gridDim.x < kShards ? gridDim.x : kShards;
+        top = __hip_atomic_fetch_add(&A.counters[C_DONE * kCtrStride], 1u, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) + 1 == slots;
+      }
+      if (top) {
         last = 1;
         __hip_atomic_store(&A.counters[C_DONE * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (A.key_cur) {
@@ -1406,7 +1440,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
       if (lane == 4) want = (fl & T_CAPPED) != 0;
       if (want && !wq::activate<DevOps>(q, A.act_cur, nb, Q.wait_ticks)) atomicOr(err, E_QPUSH);
       if (lane == 0) {
-        if (fl & (T_ANY | T_CAPPED)) atomicAdd(A.counters + C_CHANGED * kCtrStride, 1u);
+        if (fl & (T_ANY | T_CAPPED)) atomicAdd(shard_of(A, SH_CHANGED), 1u);
         wq::done<DevOps>(q);  // after every lane's activations (order() waits for them)
       }
     }
@@ -1935,8 +1969,8 @@ void ctx_init(Ctx& c, int device) {
   size_t total = prop.totalGlobalMem;
   c.label_budget = std::min<size_t>(c.label_budget, total / 2);
   c.label_budget_default = c.label_budget;
-  HIP_OK(hipMalloc((void**)&c.ws.counters, kCtrWords * sizeof(uint32_t)));
-  HIP_OK(hipHostMalloc((void**)&c.ws.pinned, kCtrWords * sizeof(uint32_t), hipHostMallocDefault));
+  HIP_OK(hipMalloc((void**)&c.ws.counters, (kCtrWords + kShardWords) * sizeof(uint32_t)));
+  HIP_OK(hipHostMalloc((void**)&c.ws.pinned, (kCtrWords + kShardWords) * sizeof(uint32_t), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&c.ws.hostctr, 64, hipHostMallocMapped | hipHostMallocCoherent));
   memset(c.ws.hostctr, 0, 64);
   HIP_OK(hipHostGetDevicePointer((void**)&c.ws.hostctr_dev, c.ws.hostctr, 0));
@@ -2167,6 +2201,7 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   A.list_cur = W.list;
   A.list_next = W.list + n_items;
   A.counters = W.counters;
+  A.shards = W.counters + kCtrWords;
   A.max_local = c.shape.max_local;
   A.next_ctr = C_NEXTN;
   A.clear_ctr = C_NEXTN;
@@ -2413,7 +2448,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     HIP_OK(hipMemcpyAsync(W.pool, pool.data(), pool.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   const uint32_t* d_seeds = W.pool + pool_n;
   launch_fill32(s, W.state, 0u, item);
-  launch_fill32(s, W.counters, 0u, kCtrWords);
+  launch_fill32(s, W.counters, 0u, kCtrWords + kShardWords);
 
   // Packed u32 labels when every open step of the batch's boards costs at
   // least 1 (hops <= dist then) and the wave form runs; checked after the
@@ -2621,7 +2656,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<size_t>((item + 255) / 256, 1024)), dim3(256), 0, s,
                        W.fmax, (size_t)item, W.counters);
   uint32_t* cnt = W.pinned;
-  HIP_OK(hipMemcpyAsync(cnt, W.counters, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(cnt, W.counters, (kCtrWords + kShardWords) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   if (cnt[C_ERROR * kCtrStride]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR * kCtrStride]));
   if (RL.packed && (uint64_t)cnt[C_PMAX * kCtrStride] + ((pk_step << 16) | 1u) >= 0xFFFF0000ull) {
@@ -2641,11 +2676,11 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   float ms = 0;
   if (timed) HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   local.relax_ms = ms;
-  local.tile_visits = cnt[C_VISITS * kCtrStride];
-  local.changed_visits = cnt[C_CHANGED * kCtrStride];
+  local.tile_visits = shard_sum(cnt + kCtrWords, SH_VISITS);
+  local.changed_visits = shard_sum(cnt + kCtrWords, SH_CHANGED);
   local.label_bytes = RL.packed ? 4u : 8u;
-  local.local_rounds = cnt[C_ROUNDS * kCtrStride];
-  local.cells_visited = (uint64_t)cnt[C_VISITS * kCtrStride] * Lmax * S.th * S.tw;
+  local.local_rounds = shard_sum(cnt + kCtrWords, SH_ROUNDS);
+  local.cells_visited = local.tile_visits * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (local.launches && local.kernel_ms == 0) local.kernel_ms = ms;  // graph form: replays only
   if (st) *st = local;
@@ -2847,13 +2882,16 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   dmalloc(d.pool, pool_words);
   dmalloc(d.boards, max_boards);
   dmalloc(d.ctr, kCtrWords + max_groups);
-  HIP_OK(hipHostMalloc((void**)&d.pinned, (kCtrWords + max_groups) * sizeof(uint32_t), hipHostMallocDefault));
+  dmalloc(d.shards, kShardWords);
+  HIP_OK(hipHostMalloc((void**)&d.pinned, (kCtrWords + max_groups + kShardWords) * sizeof(uint32_t),
+                       hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&d.hostctr, (kDriverFlagsAt + max_groups) * sizeof(uint32_t),
                        hipHostMallocMapped | hipHostMallocCoherent));
   memset(d.hostctr, 0, (kDriverFlagsAt + max_groups) * sizeof(uint32_t));
   HIP_OK(hipHostGetDevicePointer((void**)&d.hostctr_dev, d.hostctr, 0));
   launch_fill32(c.stream, d.act, 0u, 2 * items);
   launch_fill32(c.stream, d.ctr, 0u, kCtrWords + max_groups);
+  launch_fill32(c.stream, d.shards, 0u, kShardWords);
   HIP_OK(hipStreamSynchronize(c.stream));
   d.fa.reset(nfields);
   d.ia.reset(items);
@@ -2872,7 +2910,7 @@ void driver_free(RoundDriver& d) {
   if (!d.c) return;
   (void)hipSetDevice(d.c->device);
   void* ptrs[] = {d.labels, d.state,  d.act,   d.list,   d.item_field,  d.item_group,
-                  d.seeds,  d.fields, d.pool,  d.boards, d.ctr};
+                  d.seeds,  d.fields, d.pool,  d.boards, d.ctr, d.shards};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (d.pinned) (void)hipHostFree(d.pinned);
@@ -3078,6 +3116,7 @@ uint32_t driver_step(RoundDriver& d) {
   A.list_next = list_next;
   A.counters = d.ctr;
   A.group_active = live > 1 ? d.ctr + kCtrWords : nullptr;
+  A.shards = d.shards;
   A.item_group = d.item_group;
   A.host_flags = live > 1 ? d.hostctr_dev + kDriverFlagsAt : nullptr;
   A.n_groups = d.max_groups;
@@ -3159,9 +3198,9 @@ uint32_t driver_step(RoundDriver& d) {
 void driver_sync_stats(RoundDriver& d) {
   Ctx& c = *d.c;
   HIP_OK(hipSetDevice(c.device));
-  HIP_OK(hipMemcpyAsync(d.pinned, d.ctr, kCtrWords * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipMemcpyAsync(d.pinned, d.shards, kShardWords * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
-  const uint32_t v = d.pinned[C_VISITS * kCtrStride], r = d.pinned[C_ROUNDS * kCtrStride];
+  const uint32_t v = (uint32_t)shard_sum(d.pinned, SH_VISITS), r = (uint32_t)shard_sum(d.pinned, SH_ROUNDS);
   d.tile_visits += (uint32_t)(v - d.seen_visits);  // the device counters wrap at 2^32
   d.local_rounds += (uint32_t)(r - d.seen_lrounds);
   d.seen_visits = v;

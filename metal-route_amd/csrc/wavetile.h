@@ -579,8 +579,8 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   if (lane == 0) {
     if (!touched)
       __hip_atomic_fetch_or(&A.state[item], S_TOUCHED | (round << 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    atomicAdd(&A.counters[(C_VISITS) * kCtrStride], 1u);
-    atomicAdd(&A.counters[(C_ROUNDS) * kCtrStride], rounds);
+    atomicAdd(shard_of(A, SH_VISITS), 1u);
+    atomicAdd(shard_of(A, SH_ROUNDS), rounds);
   }
   return res;
 }
