@@ -412,7 +412,7 @@ struct RoundDriver {
   std::vector<Group> groups;
   std::vector<uint32_t> pending_seeds;
   std::vector<std::vector<uint8_t>> staging;  // host buffers of uploads in flight
-  uint32_t n_cur = 0;
+  uint32_t n_cur = 0;  // the round's list length
   uint32_t cur = 0;  // which half of act/list is the current round's
   uint64_t rounds = 0, tile_visits = 0, local_rounds = 0;
   // Round index stored in item states at first visits (below 2^30; restarted

@@ -108,16 +108,32 @@ constexpr uint32_t kCtrWords = C_NCOUNTERS * kCtrStride;
 // launch spread their atomics over kShards addresses instead of queueing on
 // one.  The host sums the shards.  SH_DONE holds each shard's count of
 // finished workgroups of the running launch (see round_tail).
+//
+// The plain form's active lists are sharded the same way: item i goes to
+// sub-list i % kShards (capacity `lcap` each), whose length is counted in
+// shard i % kShards of one of three count sets that rotate with the round
+// (appended to in round r: set r % 3; read by round r + 1; cleared by round
+// r + 2's first workgroup).  The launch's last workgroup writes the running
+// sums of the next list's sub-list lengths (kShards words after the shards,
+// the list table); workgroup b of the next launch takes entry b of the
+// sub-lists laid end to end.
 constexpr uint32_t kShards = 64;
-enum : uint32_t { SH_VISITS = 0, SH_ROUNDS = 1, SH_CHANGED = 2, SH_DONE = 3, SH_KINDS = 4 };
-constexpr uint32_t kShardWords = SH_KINDS * kShards * kCtrStride;
+enum : uint32_t { SH_VISITS = 0, SH_ROUNDS = 1, SH_CHANGED = 2, SH_DONE = 3, SH_L0 = 4, SH_KINDS = 7 };
+constexpr uint32_t kShardCounterWords = SH_KINDS * kShards * kCtrStride;
+constexpr uint32_t kShardWords = kShardCounterWords + kShards;  // + the list table
+static_assert(kShards == 64, "a launch's last workgroup reads the sub-list counts one per lane of a wavefront");
+// Capacity of each sub-list of a list over n items (item i goes to sub-list
+// i % kShards) and the words of one sharded list.
+inline uint32_t lshard_cap(uint64_t n) { return (uint32_t)((n + kShards - 1) / kShards); }
+inline uint64_t lshard_words(uint64_t n) { return (uint64_t)kShards * lshard_cap(n); }
 inline uint64_t shard_sum(const uint32_t* shards, uint32_t kind) {
   uint64_t s = 0;
   for (uint32_t k = 0; k < kShards; ++k) s += shards[((size_t)kind * kShards + k) * kCtrStride];
   return s;
 }
-// Round driver's host-mapped words: [0] next count, [1] error flags, [2] stamp,
-// group flags from here on.
+// Host-mapped words a plain-form launch publishes: [0] the next list's
+// length, [1] error flags, [2] stamp; the round driver's group flags from
+// here on.
 constexpr uint32_t kDriverFlagsAt = 16;
 enum : int { C_PAD_ = 0
 };
@@ -405,7 +421,38 @@ struct RelaxArgs {
   uint32_t* item_max;
   // Sharded counters (kShardWords words).
   uint32_t* shards;
+  // Plain form: sharded lists of lcap entries per shard; the count sets of
+  // the round's list, of the next list and of the set to clear.
+  uint32_t lcap;
+  uint32_t lset_cur, lset_next, lset_clear;
 };
+
+// Count of sub-list s in count set `set`.
+__device__ __forceinline__ uint32_t* lcount(const RelaxArgs& A, uint32_t set, uint32_t s) {
+  return A.shards + ((size_t)(SH_L0 + set) * kShards + s) * kCtrStride;
+}
+
+// Append item nb to sub-list nb % kShards of the next list.
+__device__ __forceinline__ void list_append(const RelaxArgs& A, uint32_t nb) {
+  const uint32_t s = nb % kShards;
+  const uint32_t pos = atomicAdd(lcount(A, A.lset_next, s), 1u);
+  A.list_next[(size_t)s * A.lcap + pos] = nb;
+}
+
+// The item of this workgroup in a plain-form round (the grid is the list's
+// length): entry blockIdx of the sub-lists laid end to end, found from the
+// list table (one running sum per lane; the sub-list is the number of sums
+// at or below blockIdx).  Workgroup 0 clears the count set two rounds old.
+__device__ __forceinline__ uint32_t round_item(const RelaxArgs& A) {
+  const uint32_t lane = threadIdx.x & 63u;
+  if (blockIdx.x == 0 && threadIdx.x < kShards)
+    __hip_atomic_store(lcount(A, A.lset_clear, threadIdx.x), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t b = blockIdx.x;
+  const uint32_t run = gbl(A.shards + kShardCounterWords)[lane];
+  const uint32_t s = (uint32_t)__popcll(__ballot(run <= b));
+  const uint32_t start = s ? (uint32_t)__builtin_amdgcn_readlane((int)run, (int)(s - 1)) : 0u;
+  return const_load(A.list_cur + (size_t)s * A.lcap + (b - start));
+}
 
 // The workgroup's shard of counter kind `kind`.
 __device__ __forceinline__ uint32_t* shard_of(const RelaxArgs& A, uint32_t kind) {
@@ -1146,7 +1193,12 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 // Activate `nb` for the next round once (flag + append to the next list).
 __device__ __forceinline__ void activate(const RelaxArgs& A, uint32_t nb, uint32_t key) {
   if (A.key_next) atomicMin(&A.key_next[nb], key);
-  if (atomicExch(&A.act_next[nb], 1u) == 0u) A.list_next[atomicAdd(&A.counters[A.next_ctr * kCtrStride], 1u)] = nb;
+  if (atomicExch(&A.act_next[nb], 1u) == 0u) {
+    if (A.lcap)
+      list_append(A, nb);
+    else
+      A.list_next[atomicAdd(&A.counters[A.next_ctr * kCtrStride], 1u)] = nb;
+  }
 }
 
 // Activations after a visit, run by every lane of the first wave: lanes 0..3
@@ -1173,7 +1225,10 @@ __device__ __forceinline__ uint32_t activate_sides(const RelaxArgs& A, uint32_t 
     fresh = atomicExch(&A.act_next[nb], 1u) == 0u;
   }
   const uint64_t m = __ballot(fresh);
-  if (m) {
+  if (A.lcap) {
+    // sharded lists: each new entry reserves its slot in its own sub-list
+    if (fresh) list_append(A, nb);
+  } else if (m) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&A.counters[A.next_ctr * kCtrStride], (uint32_t)__popcll(m));
     base = uniform_u32(base);
@@ -1252,15 +1307,34 @@ __device__ __forceinline__ void round_tail(const RelaxArgs& A, uint32_t item, ui
           __hip_atomic_store(&A.counters[C_THRESH * kCtrStride], th < kMax ? (uint32_t)th : kMax, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
         }
-        const uint32_t nn =
-            __hip_atomic_load(&A.counters[A.next_ctr * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t er = __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&A.host_out[0], nn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(&A.host_out[1], er, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if (!A.host_flags) {
-          __hip_atomic_store(&A.host_out[2], A.stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-          __threadfence_system();
-        }
+      }
+    }
+  }
+  if (__shfl(last, 0)) {
+    // the launch's last workgroup publishes the next round's size (sharded
+    // lists: the longest sub-list, one lane per count), the error flags and,
+    // without group flags, the stamp
+    uint32_t nn;
+    if (A.lcap) {
+      // running sums of the sub-list lengths (one per lane) into the list table
+      uint32_t run = __hip_atomic_load(lcount(A, A.lset_next, threadIdx.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t up = (uint32_t)__shfl_up((int)run, o);
+        if (threadIdx.x >= (uint32_t)o) run += up;
+      }
+      A.shards[kShardCounterWords + threadIdx.x] = run;
+      nn = (uint32_t)__builtin_amdgcn_readlane((int)run, 63);
+    } else {
+      nn = __hip_atomic_load(&A.counters[A.next_ctr * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) {
+      const uint32_t er = __hip_atomic_load(&A.counters[C_ERROR * kCtrStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&A.host_out[0], nn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&A.host_out[1], er, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (!A.host_flags) {
+        __hip_atomic_store(&A.host_out[2], A.stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
       }
     }
   }
@@ -1285,12 +1359,10 @@ template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_round(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (A.clear_ctr != A.next_ctr)
-      __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (A.key_cur && A.key_clear != A.key_ctr)
       __hip_atomic_store(&A.counters[A.key_clear * kCtrStride], kMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  const uint32_t item = const_load(A.list_cur + blockIdx.x);
+  const uint32_t item = round_item(A);
   const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
   uint32_t key = 0;
   bool defer = false;
@@ -1501,6 +1573,32 @@ __global__ void seed_list(uint32_t* act, uint32_t* list, const uint32_t* idx, ui
     list[i] = idx[i];
     if (key) key[idx[i]] = 0u;  // sources start at search cost 0
   }
+}
+
+// Activate items idx[0..n) for a plain-form round: flags, and each item
+// appended to its sub-list (count set `set`) of the sharded list.
+__global__ void seed_list_sharded(uint32_t* act, uint32_t* list, uint32_t lcap, uint32_t* shards, uint32_t set,
+                                  const uint32_t* idx, uint32_t n, uint32_t* key) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t it = idx[i], s = it % kShards;
+    act[it] = 1u;
+    const uint32_t pos = atomicAdd(shards + ((size_t)(SH_L0 + set) * kShards + s) * kCtrStride, 1u);
+    list[(size_t)s * lcap + pos] = it;
+    if (key) key[it] = 0u;  // sources start at search cost 0
+  }
+}
+
+// The list table of count set `set` (after seeds were appended to it): one
+// workgroup of 64 threads.
+__global__ void list_table(uint32_t* shards, uint32_t set) {
+  uint32_t run = shards[((size_t)(SH_L0 + set) * kShards + threadIdx.x) * kCtrStride];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t up = (uint32_t)__shfl_up((int)run, o);
+    if (threadIdx.x >= (uint32_t)o) run += up;
+  }
+  shards[kShardCounterWords + threadIdx.x] = run;
 }
 
 // Activate every item: flags and the identity list.
@@ -2419,7 +2517,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   take(c, W.act, W.act_cap, std::max<size_t>((size_t)item * 2, 2));
   const bool keyed = c.mode == MODE_PLAIN && c.delta != kMax;
   if (keyed) take(c, W.key, W.key_cap, std::max<size_t>((size_t)item * 2, 2));
-  take(c, W.list, W.list_cap, std::max<size_t>((size_t)item * 2, 2));
+  // two lists (round form: sharded, lshard_words each; graph form: item each)
+  const size_t list_words = std::max<size_t>(2 * std::max<uint64_t>(lshard_words(item), item), 2);
+  take(c, W.list, W.list_cap, list_words);
   take(c, W.fields, W.fields_cap, std::max<size_t>(reqs.size(), 1));
   take(c, W.item_field, W.item_field_cap, std::max<size_t>(item, 1));
   take(c, W.pool, W.pool_cap, std::max<size_t>(pool.size() + seeds.size(), 1));
@@ -2427,7 +2527,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   check_arena(c, {{W.labels, std::max<size_t>(lab_off, 1) * 8},
                   {W.state, std::max<size_t>(item, 1) * 4},
                   {W.act, std::max<size_t>((size_t)item * 2, 2) * 4},
-                  {W.list, std::max<size_t>((size_t)item * 2, 2) * 4},
+                  {W.list, list_words * 4},
                   {W.fields, std::max<size_t>(reqs.size(), 1) * sizeof(FieldDesc)},
                   {W.item_field, std::max<size_t>(item, 1) * 4},
                   {W.pool, std::max<size_t>(pool.size() + seeds.size(), 1) * 4},
@@ -2500,6 +2600,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         pairs = 0;
       };
       while (n_active) {
+        A.lset_next = rounds % 3;
+        A.lset_cur = (rounds + 2) % 3;
+        A.lset_clear = (rounds + 1) % 3;
         A.next_ctr = C_NEXT0 + (rounds & 1);
         A.clear_ctr = C_NEXT0 + ((rounds + 1) & 1);
         A.key_ctr = C_KEY0 + (rounds & 1);
@@ -2559,8 +2662,13 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         A.key_next = W.key + item;
         A.delta = c.delta;
       }
-      hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
-                         d_seeds, (uint32_t)seeds.size(), keyed ? W.key : (uint32_t*)nullptr);
+      // sharded lists: the seeds fill round 0's list, counted in set 2
+      A.lcap = lshard_cap(item);
+      A.list_next = W.list + lshard_words(item);
+      hipLaunchKernelGGL(seed_list_sharded, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act,
+                         W.list, A.lcap, A.shards, 2u, d_seeds, (uint32_t)seeds.size(),
+                         keyed ? W.key : (uint32_t*)nullptr);
+      hipLaunchKernelGGL(list_table, dim3(1), dim3(kShards), 0, s, A.shards, 2u);
       HIP_OK(hipEventRecord(W.ev0, s));
       timed = true;
       plain_rounds(rounds, (uint32_t)seeds.size());
@@ -2874,7 +2982,7 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   dmalloc(d.labels, words);
   dmalloc(d.state, items);
   dmalloc(d.act, 2 * items);
-  dmalloc(d.list, 2 * items);
+  dmalloc(d.list, 2 * lshard_words(items));
   dmalloc(d.item_field, items);
   dmalloc(d.item_group, items);
   dmalloc(d.seeds, items);
@@ -3076,13 +3184,17 @@ uint32_t driver_step(RoundDriver& d) {
   const uint64_t items = driver_items(d);
   uint32_t* act_cur = d.act + (uint64_t)d.cur * items;
   uint32_t* act_next = d.act + (uint64_t)(1 - d.cur) * items;
-  uint32_t* list_cur = d.list + (uint64_t)d.cur * items;
-  uint32_t* list_next = d.list + (uint64_t)(1 - d.cur) * items;
+  const uint32_t lcap = lshard_cap(items);
+  uint32_t* list_cur = d.list + (uint64_t)d.cur * lshard_words(items);
+  uint32_t* list_next = d.list + (uint64_t)(1 - d.cur) * lshard_words(items);
   if (!d.pending_seeds.empty()) {
     const uint32_t k = (uint32_t)d.pending_seeds.size();
     stage_copy(d, d.seeds, d.pending_seeds);
-    hipLaunchKernelGGL(seed_list, dim3((k + 255) / 256), dim3(256), 0, s, act_cur, list_cur + d.n_cur, d.seeds, k,
-                       (uint32_t*)nullptr);
+    // appended to the round's sharded list (count set (rounds + 2) % 3); the
+    // grid covers the longest sub-list it can reach
+    hipLaunchKernelGGL(seed_list_sharded, dim3((k + 255) / 256), dim3(256), 0, s, act_cur, list_cur, lcap, d.shards,
+                       (uint32_t)((d.rounds + 2) % 3), d.seeds, k, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(list_table, dim3(1), dim3(kShards), 0, s, d.shards, (uint32_t)((d.rounds + 2) % 3));
     d.n_cur += k;
     d.pending_seeds.clear();
   }
@@ -3163,6 +3275,10 @@ uint32_t driver_step(RoundDriver& d) {
     }
   }
   HIP_OK(hipEventRecord(e0, s));
+  A.lcap = lcap;
+  A.lset_next = (uint32_t)(d.rounds % 3);
+  A.lset_cur = (uint32_t)((d.rounds + 2) % 3);
+  A.lset_clear = (uint32_t)((d.rounds + 1) % 3);
   HIP_OK(hipLaunchKernel(RL.fn, dim3(d.n_cur), dim3(RL.threads), args, RL.lds, s));
   HIP_OK(hipEventRecord(e1, s));
   HIP_OK(hipGetLastError());

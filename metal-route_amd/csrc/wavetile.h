@@ -594,11 +594,7 @@ struct WaveMin {
 template <int TW, int TH, int LM, bool PK>
 __global__ void __launch_bounds__(64, (WaveMin<TW, LM>::waves)) relax_wave(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (A.clear_ctr != A.next_ctr)
-      __hip_atomic_store(&A.counters[A.clear_ctr * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  const uint32_t item = const_load(A.list_cur + blockIdx.x);
+  const uint32_t item = round_item(A);
   const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
   const uint32_t f = const_load(A.item_field + item);
   const bool touched = (const_load(A.state + item) & S_TOUCHED) != 0;
