@@ -320,8 +320,10 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   // prices of stepping out of the tile into the halo cells: the cell's edge
   // facing the tile must be open and the cell passable (narrow: at most 0x7FFE)
   const uint32_t xleft = __builtin_amdgcn_readlane(xgv, 0), xright = __builtin_amdgcn_readlane(xgv, TW);
-  const uint32_t p_lr = pack_steps(wave_step(hbm[0], M_PX, true, xleft, ha[0]), wave_step(hbm[1], M_NX, true, xright, ha[1]));
-  const uint32_t p_tb = pack_steps(wave_step(hbm[2], M_PY, true, ytop, ha[2]), wave_step(hbm[3], M_NY, true, ybot, ha[3]));
+  uint32_t p_lr = pack_steps(wave_step(hbm[0], M_PX, true, xleft, ha[0]), wave_step(hbm[1], M_NX, true, xright, ha[1]));
+  uint32_t p_tb = pack_steps(wave_step(hbm[2], M_PY, true, ytop, ha[2]), wave_step(hbm[3], M_NY, true, ybot, ha[3]));
+  // formed here, so the halo cells' planes are not kept through the sweeps
+  asm("" : "+v"(p_lr), "+v"(p_tb));
 
   // ---- step prices of every cell (row layout) ---------------------------------
   uint32_t sx[TW];  // row layout: from x-1 | from x+1
@@ -585,14 +587,15 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   return res;
 }
 
-// Waves per SIMD the wave form is compiled for.
-template <int TW, int LM>
+// Waves per SIMD the wave form is compiled for: the packed form fits five
+// (at most 96 VGPRs), the wide form's u64 lines four.
+template <int TW, int LM, bool PK>
 struct WaveMin {
-  static constexpr int waves = 4;
+  static constexpr int waves = PK ? 5 : 4;
 };
 
 template <int TW, int TH, int LM, bool PK>
-__global__ void __launch_bounds__(64, (WaveMin<TW, LM>::waves)) relax_wave(RelaxArgs A) {
+__global__ void __launch_bounds__(64, (WaveMin<TW, LM, PK>::waves)) relax_wave(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t item = round_item(A);
   const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
