@@ -1521,14 +1521,21 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
 }
 
 // Single-thread round step between two relax_list launches.
-// Largest of n per-item maxima into counters[C_PMAX].
-__global__ void reduce_item_max(const uint32_t* m, size_t n, uint32_t* counters) {
+// Largest of n per-item maxima into counters[C_PMAX]: a grid-stride pass,
+// reduced per workgroup (256 threads), one atomic per workgroup.
+__global__ void __launch_bounds__(256) reduce_item_max(const uint32_t* m, size_t n, uint32_t* counters) {
+  __shared__ uint32_t wmax[4];
   uint32_t v = 0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     v = max(v, m[i]);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-  if ((threadIdx.x & 63) == 0 && v) atomicMax(&counters[C_PMAX * kCtrStride], v);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    if (v) atomicMax(&counters[C_PMAX * kCtrStride], v);
+  }
 }
 
 __global__ void advance_round(uint32_t* ctr) {
@@ -2761,7 +2768,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if (timed) HIP_OK(hipEventRecord(W.ev1, s));
   if (RL.packed && item)
-    hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<size_t>((item + 255) / 256, 1024)), dim3(256), 0, s,
+    hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<size_t>((item + 255) / 256, 256)), dim3(256), 0, s,
                        W.fmax, (size_t)item, W.counters);
   uint32_t* cnt = W.pinned;
   HIP_OK(hipMemcpyAsync(cnt, W.counters, (kCtrWords + kShardWords) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
