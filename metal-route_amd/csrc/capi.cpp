@@ -200,8 +200,9 @@ struct UnitEdges {
   }
 };
 
+// transient: the board is freed before the calling entry point returns.
 void upload(mrf::Ctx& c, mrf::DevBoard& b, mrf_dims d, const mrf_grid* g, const mrf_edges* e,
-            const mrf_pricing* pr) {
+            const mrf_pricing* pr, bool transient) {
   const uint32_t* add = nullptr;
   const int32_t* halo = nullptr;
   uint32_t pen = 0;
@@ -219,7 +220,7 @@ void upload(mrf::Ctx& c, mrf::DevBoard& b, mrf_dims d, const mrf_grid* g, const 
   }
   mrf::board_upload(c, b, d.w, d.h, d.layers, g->cost, g->n_via_forbidden ? g->via_forbidden : nullptr,
                     g->n_board_constraint ? g->board_constraint : nullptr, e->x, e->y, e->via, e->via_allowed,
-                    add, halo, pen);
+                    add, halo, pen, transient);
 }
 
 int isolated(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* nets, size_t n_nets,
@@ -243,7 +244,7 @@ int isolated(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_net* n
     all.cost.assign(n_nets, kMax);
     all.paths.assign(n_nets, {});
     if (n_nets) {
-      upload(ctx->c, b, dims, grid, edges, nullptr);
+      upload(ctx->c, b, dims, grid, edges, nullptr, true);
       std::vector<size_t> idx(n_nets);
       for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
       std::vector<mrf_window> wins(windows, windows + n_nets);
@@ -439,7 +440,7 @@ int mrf_router_route(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mr
     for (size_t i = 0; i < n_nets; ++i) check_net(g, nets[i], i);
     Batch res;
     if (n_nets) {
-      upload(ctx->c, b, dims, grid, &ue.e, nullptr);
+      upload(ctx->c, b, dims, grid, &ue.e, nullptr, true);
       std::vector<size_t> idx(n_nets);
       for (size_t i = 0; i < n_nets; ++i) idx[i] = i;
       std::vector<mrf_window> wins(n_nets, mrf_window{0, 0, dims.w - 1, dims.h - 1});
@@ -514,7 +515,7 @@ int mrf_sweep_fields(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const ui
     if (n_sources && (!sources || !dist_out)) throw Error(MRF_BACKEND_UNAVAILABLE, "null sources or output");
     UnitEdges ue(dims);
     if (n_sources) {
-      upload(ctx->c, b, dims, grid, &ue.e, nullptr);
+      upload(ctx->c, b, dims, grid, &ue.e, nullptr, true);
       std::vector<const uint32_t*> sp(n_sources);
       std::vector<size_t> ns(n_sources, 1);
       for (size_t i = 0; i < n_sources; ++i) sp[i] = sources + i;
@@ -546,7 +547,7 @@ int mrf_fields(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mrf_edge
     if (n_fields && (!sources || !n_sources || !dist_out)) throw Error(MRF_BACKEND_UNAVAILABLE, "null sources or output");
     for (size_t i = 0; windows && i < n_fields; ++i) check_window(dims, windows[i], i);
     if (n_fields) {
-      upload(ctx->c, b, dims, grid, edges, pricing);
+      upload(ctx->c, b, dims, grid, edges, pricing, true);
       fields_impl(ctx->c, b, dims, sources, n_sources, dsts, pricing ? pricing->net_ids : nullptr, n_fields, windows,
                   true, false, dist_out, hops_out);
     }
@@ -603,7 +604,7 @@ int mrf_board_create(mrf_ctx* ctx, mrf_dims dims, const mrf_grid* grid, const mr
       bd->halo.assign(pricing->halo_owner, pricing->halo_owner + pricing->n_halo_owner);
       bd->halo_pen = pricing->halo_penalty;
     }
-    upload(ctx->c, bd->b, dims, grid, edges, pricing);
+    upload(ctx->c, bd->b, dims, grid, edges, pricing, false);
   } catch (const Error& e) {
     mrf::board_free(bd->b);
     delete bd;

@@ -100,7 +100,15 @@ struct DevBoard {
   uint32_t* ye = nullptr;
   uint32_t* ve = nullptr;
   std::vector<uint8_t> via_allowed;  // host copy
-  std::vector<uint8_t> obst;         // host copy: cost == kMax per cell
+  // Host copy of the obstacle bits (cost == kMax), one bit per cell, built on
+  // the device with the masks.
+  std::vector<uint64_t> obst_bits;
+  bool obstacle(size_t g) const { return (g >> 6) < obst_bits.size() && ((obst_bits[g >> 6] >> (g & 63)) & 1u); }
+  // The static planes (bmask, raw, obstacle bits, edge prices, weights) share
+  // one allocation: the board's own, or, for a board that lives for one call,
+  // the context's borrowed block (lease points at its busy flag).
+  uint8_t* block = nullptr;
+  bool* lease = nullptr;
   uint64_t max_step = 0;  // bound on any step price (obstacles priced at weight 1, as own pads)
   uint64_t max_ew = 0;    // largest edge price x largest enter weight (max_step without the additive part)
   // Lower bound on every open step price: the smallest edge price (planar
@@ -281,6 +289,15 @@ struct Ctx {
   size_t arena_cap = 0, arena_used = 0;
   Workspace ws_arena;
   Workspace& W() { return arena ? ws_arena : ws; }
+  // Board uploads: the caller's planes are staged in `stage` (grown on
+  // demand) and the masks built from them on the device; a board that lives
+  // for one call borrows `tblock` for its static planes while it is free.
+  uint8_t* stage = nullptr;
+  size_t stage_cap = 0;
+  uint8_t* tblock = nullptr;
+  size_t tblock_cap = 0;
+  bool tblock_busy = false;
+  uint32_t* upstat = nullptr;  // device words of the upload's weight / price bounds
 };
 
 // ---- engine.hip -------------------------------------------------------------
@@ -297,7 +314,9 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers,
                   const uint32_t* cost, const uint8_t* via_forbidden, const uint8_t* board,
                   const uint32_t* xe, const uint32_t* ye, const uint32_t* ve,
                   const uint8_t* via_allowed, const uint32_t* add, const int32_t* halo,
-                  uint32_t halo_pen);
+                  uint32_t halo_pen, bool transient = false);
+// transient: the board is freed before the call that uploaded it returns, so
+// its static planes may borrow the context's block.
 void board_free(DevBoard& b);
 // Replace (or add) the board's additive price plane; host array of w*h*layers.
 // max_add: the plane's largest value when the caller has it (UINT64_MAX:

@@ -1572,6 +1572,85 @@ __global__ void fill_u32(uint32_t* p, uint32_t v, size_t n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Board upload on the device.  board_scan reduces the bounds the host needs
+// to size the planes (largest and smallest enter weight of a passable cell,
+// smallest additive price of a passable cell, largest additive price, whether
+// any cell is passable) into st[0..5), minima as maxima of the complement so
+// that every word starts at 0.  board_build then writes one cell per thread:
+// the directed planar-edge bits and via bits (mr-core lib.rs:466-496: a
+// planar step is forbidden when either end's constraint bit for that
+// direction is set; a via needs both ends free of BOARD_VIA_NODE and
+// via_forbidden and the layer pair's via allowed), the trace and obstacle
+// bits, the raw via flags, and the obstacle bitset (one ballot per wave).
+__global__ void __launch_bounds__(256) board_scan(const uint32_t* cost, const uint32_t* add, size_t n,
+                                                  uint32_t* st) {
+  __shared__ uint32_t red[4][5];
+  uint32_t v[5] = {0u, 0u, 0u, 0u, 0u};  // max_w, max_add, passable, ~min_w, ~min_a
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint32_t w = cost[i];
+    const uint32_t a = add ? add[i] : 0u;
+    v[1] = max(v[1], a);
+    if (w != kMax) {
+      v[0] = max(v[0], w);
+      v[2] = 1u;
+      v[3] = max(v[3], ~w);
+      v[4] = max(v[4], ~a);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    for (int o = 32; o >= 1; o >>= 1) v[k] = max(v[k], (uint32_t)__shfl_xor((int)v[k], o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int k = threadIdx.x;
+    const uint32_t m = max(max(red[0][k], red[1][k]), max(red[2][k], red[3][k]));
+    if (m) atomicMax(st + k, m);
+  }
+}
+
+__global__ void __launch_bounds__(256) board_build(const uint32_t* __restrict__ cost,
+                                                   const uint8_t* __restrict__ board,
+                                                   const uint8_t* __restrict__ vf, uint32_t w, uint32_t h,
+                                                   uint32_t layers, uint32_t va_bits, uint8_t* __restrict__ bm,
+                                                   uint8_t* __restrict__ raw, uint64_t* __restrict__ obits) {
+  const size_t n = (size_t)w * h * layers;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ob = false;
+  if (i < n) {
+    const size_t plane = (size_t)w * h;
+    const uint32_t x = (uint32_t)(i % w), y = (uint32_t)((i / w) % h), l = (uint32_t)(i / plane);
+    const uint8_t bc = board ? board[i] : 0u;
+    ob = cost[i] == kMax;
+    uint8_t m = 0;
+    if (bc & 0x01) m |= M_TRACE;
+    if (ob) m |= M_OBST;
+    auto open = [&](size_t v, uint8_t sb, uint8_t db) { return !board || !((bc & sb) || (board[v] & db)); };
+    if (y > 0 && open(i - w, 0x04, 0x20)) m |= M_NY;
+    if (x > 0 && open(i - 1, 0x08, 0x10)) m |= M_NX;
+    if (x + 1 < w && open(i + 1, 0x10, 0x08)) m |= M_PX;
+    if (y + 1 < h && open(i + w, 0x20, 0x04)) m |= M_PY;
+    auto free_of_via_block = [&](size_t q) { return !((board && (board[q] & 0x02)) || (vf && vf[q])); };
+    const bool here = free_of_via_block(i);
+    if (l > 0 && ((va_bits >> (l - 1)) & 1u) && here && free_of_via_block(i - plane)) m |= M_DN;
+    if (l + 1 < layers && ((va_bits >> l) & 1u) && here && free_of_via_block(i + plane)) m |= M_UP;
+    bm[i] = m;
+    raw[i] = (uint8_t)(((bc & 0x02) ? R_VIANODE : 0) | ((vf && vf[i]) ? R_VF : 0));
+  }
+  const uint64_t bits = __ballot(ob);
+  if ((threadIdx.x & 63) == 0 && i < n) obits[i >> 6] = bits;
+}
+
+__global__ void to_u16(const uint32_t* a, uint16_t* o, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) o[i] = (uint16_t)a[i];
+}
+
 // Activate items idx[0..n) for the first round: flags and list.
 __global__ void seed_list(uint32_t* act, uint32_t* list, const uint32_t* idx, uint32_t n, uint32_t* key) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2075,6 +2154,7 @@ void ctx_init(Ctx& c, int device) {
   c.label_budget = std::min<size_t>(c.label_budget, total / 2);
   c.label_budget_default = c.label_budget;
   HIP_OK(hipMalloc((void**)&c.ws.counters, (kCtrWords + kShardWords) * sizeof(uint32_t)));
+  HIP_OK(hipMalloc((void**)&c.upstat, 64));
   HIP_OK(hipHostMalloc((void**)&c.ws.pinned, (kCtrWords + kShardWords) * sizeof(uint32_t), hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&c.ws.hostctr, 64, hipHostMallocMapped | hipHostMallocCoherent));
   memset(c.ws.hostctr, 0, 64);
@@ -2096,6 +2176,11 @@ void ctx_free(Ctx& c) {
   (void)hipSetDevice(c.device);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (void* p : {(void*)c.stage, (void*)c.tblock, (void*)c.upstat})
+    if (p) (void)hipFree(p);
+  c.stage = c.tblock = nullptr;
+  c.upstat = nullptr;
+  c.stage_cap = c.tblock_cap = 0;
   if (w.pinned) (void)hipHostFree(w.pinned);
   if (w.hostctr) (void)hipHostFree(w.hostctr);
   if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
@@ -2111,81 +2196,77 @@ void ctx_free(Ctx& c) {
   c.stream = nullptr;
 }
 
+static void launch_fill32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
+  if (!n) return;
+  uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+}
+
 void board_free(DevBoard& b) {
-  void* ptrs[] = {b.cost, b.add, b.add16, b.halo, b.bmask, b.raw, b.xe, b.ye, b.ve, b.owner, b.lhalo};
+  // cost, bmask, raw and the edge prices live in the block
+  void* ptrs[] = {b.add, b.add16, b.halo, b.owner, b.lhalo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (b.lease)
+    *b.lease = false;
+  else if (b.block)
+    (void)hipFree(b.block);
   b = DevBoard();
 }
 
 void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, const uint32_t* cost,
                   const uint8_t* via_forbidden, const uint8_t* board, const uint32_t* xe,
                   const uint32_t* ye, const uint32_t* ve, const uint8_t* via_allowed,
-                  const uint32_t* add, const int32_t* halo, uint32_t halo_pen) {
+                  const uint32_t* add, const int32_t* halo, uint32_t halo_pen, bool transient) {
   if (layers > 16) throw Error(1, "at most 16 layers are supported");
   HIP_OK(hipSetDevice(c.device));
-  size_t n = (size_t)w * h * layers;
-  uint32_t plane = w * h;
-  std::vector<uint8_t> bm(n), raw(n);
-  // Directed-edge bitset: mr-core is_board_planar_step_forbidden (lib.rs:466-496)
-  // is symmetric, so each cell stores whether each of its four planar edges is
-  // open; via bits fold ViaModel legality, BOARD_VIA_NODE and via_forbidden.
-  for (uint32_t l = 0; l < layers; ++l)
-    for (uint32_t y = 0; y < h; ++y)
-      for (uint32_t x = 0; x < w; ++x) {
-        size_t i = ((size_t)l * h + y) * w + x;
-        uint8_t bc = board ? board[i] : 0;
-        uint8_t m = 0;
-        if (bc & 0x01) m |= M_TRACE;
-        if (cost[i] == kMax) m |= M_OBST;
-        auto open = [&](size_t u, size_t v, uint8_t sb, uint8_t db) {
-          if (!board) return true;
-          return !((board[u] & sb) || (board[v] & db));
-        };
-        if (y > 0 && open(i, i - w, 0x04, 0x20)) m |= M_NY;
-        if (x > 0 && open(i, i - 1, 0x08, 0x10)) m |= M_NX;
-        if (x + 1 < w && open(i, i + 1, 0x10, 0x08)) m |= M_PX;
-        if (y + 1 < h && open(i, i + w, 0x20, 0x04)) m |= M_PY;
-        uint8_t r = 0;
-        if (bc & 0x02) r |= R_VIANODE;
-        if (via_forbidden && via_forbidden[i]) r |= R_VF;
-        raw[i] = r;
-        bm[i] = m;
-      }
-  for (uint32_t l = 0; l < layers; ++l)
-    for (uint32_t k = 0; k < plane; ++k) {
-      size_t i = (size_t)l * plane + k;
-      auto ok = [&](size_t q) { return !(raw[q] & (R_VIANODE | R_VF)); };
-      if (l > 0 && via_allowed[l - 1] && ok(i) && ok(i - plane)) bm[i] |= M_DN;
-      if (l + 1 < layers && via_allowed[l] && ok(i) && ok(i + plane)) bm[i] |= M_UP;
-    }
+  board_free(b);
+  hipStream_t s = c.stream;
+  const size_t n = (size_t)w * h * layers;
+  // 1. the caller's planes into the staging buffer (16 B of slack after each
+  // piece: kernels read planes with aligned dword loads)
+  size_t off = 0;
+  auto carve = [](size_t& o, size_t bytes) {
+    const size_t at = o;
+    o += (bytes + 16 + 255) & ~size_t(255);
+    return at;
+  };
+  const size_t s_cost = carve(off, n * 4);
+  const size_t s_board = board ? carve(off, n) : 0;
+  const size_t s_vf = via_forbidden ? carve(off, n) : 0;
+  const size_t s_add = add ? carve(off, n * 4) : 0;
+  grow(c.stage, c.stage_cap, off);
+  const uint32_t* d_cost = (const uint32_t*)(c.stage + s_cost);
+  const uint8_t* d_board = board ? c.stage + s_board : nullptr;
+  const uint8_t* d_vf = via_forbidden ? c.stage + s_vf : nullptr;
+  const uint32_t* d_add = add ? (const uint32_t*)(c.stage + s_add) : nullptr;
+  HIP_OK(hipMemcpyAsync(c.stage + s_cost, cost, n * 4, hipMemcpyHostToDevice, s));
+  if (board) HIP_OK(hipMemcpyAsync(c.stage + s_board, board, n, hipMemcpyHostToDevice, s));
+  if (via_forbidden) HIP_OK(hipMemcpyAsync(c.stage + s_vf, via_forbidden, n, hipMemcpyHostToDevice, s));
+  if (add) HIP_OK(hipMemcpyAsync(c.stage + s_add, add, n * 4, hipMemcpyHostToDevice, s));
+  // 2. weight and price bounds
+  launch_fill32(s, c.upstat, 0u, 5);
+  hipLaunchKernelGGL(board_scan, dim3((uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1024))),
+                     dim3(256), 0, s, d_cost, d_add, n, c.upstat);
+  uint32_t* st = c.ws.pinned;
+  HIP_OK(hipMemcpyAsync(st, c.upstat, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  const bool passable = st[2] != 0;
+  const uint64_t max_w = std::max<uint64_t>(1, st[0]), max_plane = st[1];
+  const uint64_t min_w = passable ? (uint64_t)(uint32_t)~st[3] : UINT64_MAX;
+  const uint64_t min_a = passable ? (uint64_t)(uint32_t)~st[4] : UINT64_MAX;
+  // every passable cell weighs 1: no weight plane on the device
+  const bool unit_w = !passable || (st[0] == 1u && min_w == 1u);
+  const uint64_t max_a = n ? max_plane + (halo ? halo_pen : 0u) : 0u;
   // Bound on every step price min(E*w + a, MAX-1): the largest edge price
   // times the largest enter weight (obstacles count as weight 1, the price of
   // an own pad) plus the largest additive price.  Boards under the narrow
   // limit relax with 16-bit step planes in LDS.
-  uint64_t max_e = 0, max_w = 1, max_a = 0, max_plane = 0;
-  uint64_t min_e = UINT64_MAX, min_w = UINT64_MAX, min_a = UINT64_MAX;
+  uint64_t max_e = 0, min_e = UINT64_MAX;
   for (uint32_t k = 0; k + 1 < w; ++k) max_e = std::max<uint64_t>(max_e, xe[k]), min_e = std::min<uint64_t>(min_e, xe[k]);
   for (uint32_t k = 0; k + 1 < h; ++k) max_e = std::max<uint64_t>(max_e, ye[k]), min_e = std::min<uint64_t>(min_e, ye[k]);
   for (uint32_t k = 0; k + 1 < layers; ++k)
     if (via_allowed[k] && ve) max_e = std::max<uint64_t>(max_e, ve[k]), min_e = std::min<uint64_t>(min_e, ve[k]);
-  std::vector<uint8_t> obst(n);
-  bool unit_w = true;  // every passable cell weighs 1: no weight plane on the device
-  for (size_t i = 0; i < n; ++i) {
-    obst[i] = cost[i] == kMax;
-    uint64_t a = add ? add[i] : 0;
-    if (!obst[i]) {
-      max_w = std::max<uint64_t>(max_w, cost[i]);
-      min_w = std::min<uint64_t>(min_w, cost[i]);
-      min_a = std::min(min_a, a);
-      unit_w = unit_w && cost[i] == 1u;
-    }
-    max_plane = std::max(max_plane, a);
-    if (halo) a += halo_pen;
-    max_a = std::max(max_a, a);
-  }
-  board_free(b);
-  b.obst = std::move(obst);
   b.max_ew = max_e * max_w;
   b.max_step = std::min<uint64_t>(b.max_ew + max_a, kMax - 1);
   // no edge or no passable cell: no step exists, any bound holds
@@ -2194,37 +2275,66 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   b.w = w;
   b.h = h;
   b.layers = layers;
-  if (!unit_w) b.cost = dev_copy(cost, n, c.stream);
-  b.bmask = dev_copy(bm.data(), n, c.stream);
-  b.raw = dev_copy(raw.data(), n, c.stream);
-  std::vector<uint32_t> x(w, 0), y(h, 0), v(16, 0);
-  for (uint32_t k = 0; k + 1 < w; ++k) x[k] = xe[k];
-  for (uint32_t k = 0; k + 1 < h; ++k) y[k] = ye[k];
-  b.via_allowed.assign(16, 0);
-  for (uint32_t k = 0; k + 1 < layers; ++k) {
-    v[k] = ve ? ve[k] : 0;
-    b.via_allowed[k] = via_allowed[k] ? 1 : 0;
+  // 3. the static planes in one block
+  size_t boff = 0;
+  const size_t o_bm = carve(boff, n), o_raw = carve(boff, n);
+  const size_t words = (n + 63) / 64;
+  const size_t o_bits = carve(boff, words * 8);
+  const size_t o_edges = carve(boff, ((size_t)w + h + 16) * 4);
+  const size_t o_cost = unit_w ? 0 : carve(boff, n * 4);
+  if (transient && !c.tblock_busy) {
+    grow(c.tblock, c.tblock_cap, boff);
+    b.block = c.tblock;
+    b.lease = &c.tblock_busy;
+    c.tblock_busy = true;
+  } else {
+    HIP_OK(hipMalloc((void**)&b.block, boff));
   }
-  b.xe = dev_copy(x.data(), x.size(), c.stream);
-  b.ye = dev_copy(y.data(), y.size(), c.stream);
-  b.ve = dev_copy(v.data(), v.size(), c.stream);
+  b.bmask = b.block + o_bm;
+  b.raw = b.block + o_raw;
+  uint64_t* d_bits = (uint64_t*)(b.block + o_bits);
+  b.xe = (uint32_t*)(b.block + o_edges);
+  b.ye = b.xe + w;
+  b.ve = b.ye + h;
+  uint32_t va_bits = 0;
+  b.via_allowed.assign(16, 0);
+  std::vector<uint32_t> edges((size_t)w + h + 16, 0u);
+  for (uint32_t k = 0; k + 1 < w; ++k) edges[k] = xe[k];
+  for (uint32_t k = 0; k + 1 < h; ++k) edges[(size_t)w + k] = ye[k];
+  for (uint32_t k = 0; k + 1 < layers; ++k) {
+    edges[(size_t)w + h + k] = ve ? ve[k] : 0;
+    b.via_allowed[k] = via_allowed[k] ? 1 : 0;
+    if (via_allowed[k]) va_bits |= 1u << k;
+  }
+  HIP_OK(hipMemcpyAsync(b.xe, edges.data(), edges.size() * 4, hipMemcpyHostToDevice, s));
+  if (n)
+    hipLaunchKernelGGL(board_build, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, d_cost, d_board, d_vf, w, h,
+                       layers, va_bits, b.bmask, b.raw, d_bits);
+  if (!unit_w) {
+    b.cost = (uint32_t*)(b.block + o_cost);
+    HIP_OK(hipMemcpyAsync(b.cost, d_cost, n * 4, hipMemcpyDeviceToDevice, s));
+  }
   b.bytes = n * ((unit_w ? 0 : 4) + 1 + 1);
-  std::vector<uint16_t> add16;
+  const uint32_t cblocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 4096));
   if (add && max_plane <= 0xFFFFu) {
     // additive prices that fit 16 bits travel as u16 (2 B per cell visit)
-    add16.assign(add, add + n);
-    b.add16 = dev_copy(add16.data(), n, c.stream);
+    HIP_OK(hipMalloc((void**)&b.add16, n * 2 + 16));
+    hipLaunchKernelGGL(to_u16, dim3(cblocks), dim3(256), 0, s, d_add, b.add16, n);
     b.bytes += n * 2;
   } else if (add) {
-    b.add = dev_copy(add, n, c.stream);
+    HIP_OK(hipMalloc((void**)&b.add, n * 4 + 16));
+    HIP_OK(hipMemcpyAsync(b.add, d_add, n * 4, hipMemcpyDeviceToDevice, s));
     b.bytes += n * 4;
   }
   if (halo) {
-    b.halo = dev_copy(halo, n, c.stream);
+    b.halo = dev_copy(halo, n, s);
     b.bytes += n * 4;
   }
   b.halo_pen = halo_pen;
-  HIP_OK(hipStreamSynchronize(c.stream));
+  b.obst_bits.resize(words);
+  if (words) HIP_OK(hipMemcpyAsync(b.obst_bits.data(), d_bits, words * 8, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(s));
 }
 
 void board_set_legal(Ctx& c, DevBoard& b, const int32_t* owner, const int32_t* halo) {
@@ -2315,12 +2425,6 @@ static RelaxArgs make_args(Ctx& c, uint32_t n_items) {
   A.key_ctr = A.key_clear = C_KEY0;
   A.delta = kMax;
   return A;
-}
-
-static void launch_fill32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
-  if (!n) return;
-  uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
 }
 
 // Capture (or reuse) `graph_rounds` rounds of relax_list + advance_round.
@@ -2425,7 +2529,7 @@ static bool field_narrow(const DevBoard& b, const FieldReq& q) {
   for (uint32_t a : q.ovr_add)
     if (b.max_ew + a + (b.halo ? b.halo_pen : 0u) > kNarrowMaxStep) return false;
   for (uint32_t g : q.forced)
-    if (g < b.obst.size() && b.obst[g] && !std::binary_search(q.pads.begin(), q.pads.end(), g)) return false;
+    if (b.obstacle(g) && !std::binary_search(q.pads.begin(), q.pads.end(), g)) return false;
   return true;
 }
 
