@@ -266,31 +266,33 @@ def cpu_baseline_fields(s, board):
     }
 
 
-def seam(s, steps=3):
+def seam(s, steps=5):
     """The provider seam as NegotiatedRouter calls it: mrf_route_isolated_batch
     on a fresh context (its default form), the board's grid, masks and edge
     prices passed per call (alone-path prices: the request carries no history
-    plane), full-board windows.  Per call the library builds the directed-edge
-    mask on the host, uploads every plane and frees them again."""
+    plane), full-board windows.  Per call the library uploads the caller's
+    planes, builds the directed-edge and via masks on the device, solves, and
+    releases the board again; the paths come back as the ABI's flat arrays."""
     import mrfield
 
     eng = mrfield.Engine(0)
     g = s.grid
     nets = s.nets()
     wins = s.windows or [(0, 0, g.w - 1, g.h - 1)] * len(nets)
-    eng.route_isolated_batch(g, nets, wins, s.edges)
+    eng.route_isolated_batch_arrays(g, nets, wins, s.edges)
     ts = []
     for _ in range(steps):
         t0 = time.perf_counter()
-        eng.route_isolated_batch(g, nets, wins, s.edges)
+        eng.route_isolated_batch_arrays(g, nets, wins, s.edges)
         ts.append(time.perf_counter() - t0)
     st = eng.stats()
     eng.close()
     ms = sorted(ts)[len(ts) // 2] * 1e3
     return {"entry": "mrf_route_isolated_batch", "value": round(len(nets) / (ms * 1e-3), 3), "unit": "nets/s",
             "ms_per_call_p50": round(ms, 3), "calls": steps, "relax_kernel_ms": round(st["relax_kernel_ms"], 3),
-            "note": "fresh context's default (plain) form; board planes, masks and prices uploaded per call "
-                    "(host-to-device copies included); alone-path pricing (no history plane)"}
+            "note": "fresh context's default (plain) form; the caller's planes and prices uploaded and the "
+                    "masks built per call (host-to-device copies included); alone-path pricing (no history "
+                    "plane); paths returned as flat arrays"}
 
 
 def config0_main(a):

@@ -508,7 +508,7 @@ class Engine:
         return {k: getattr(s, k) for k, _ in StatsS._fields_}
 
     # -- provider seam -------------------------------------------------------
-    def _isolated(self, fn, grid: Grid, nets, windows, edges: Edges):
+    def _isolated(self, fn, grid: Grid, nets, windows, edges: Edges, take=None):
         arr, keep = _nets(nets)
         g = grid.cstruct()
         e = edges.cstruct()
@@ -517,10 +517,14 @@ class Engine:
         err = ErrorS()
         rc = fn(self._h, grid.dims(), C.byref(g), arr, len(nets), w, C.byref(e), C.byref(out), C.byref(err))
         _check(rc, err)
-        return _take_paths(out)
+        return (take or _take_paths)(out)
 
     def route_isolated_batch(self, grid, nets, windows, edges):
         return self._isolated(lib().mrf_route_isolated_batch, grid, nets, windows, edges)
+
+    def route_isolated_batch_arrays(self, grid, nets, windows, edges) -> PathBatch:
+        """Like route_isolated_batch(), with the paths left as flat numpy arrays."""
+        return self._isolated(lib().mrf_route_isolated_batch, grid, nets, windows, edges, _take_arrays)
 
     def route_isolated_pass(self, grid, nets, windows, edges):
         return self._isolated(lib().mrf_route_isolated_pass, grid, nets, windows, edges)

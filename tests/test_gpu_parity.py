@@ -252,6 +252,10 @@ def test_config1_complete_fields_and_paths(eng, mode):
             n_routed += 1
             assert r[0] == ref_p[i] and r[1] == ref_c[i], i
     assert n_routed > 32
+    flat = eng.route_isolated_batch_arrays(g, s.nets(), full, s.edges)
+    assert len(flat) == len(nets)
+    for i in range(len(nets)):
+        assert flat.path(i) == got[i], i
 
 
 @pytest.mark.parametrize("delta", [1, 60, 2000])
