@@ -388,7 +388,8 @@ struct Group {
   bool live = false;
   bool narrow = false;  // every step price fits the narrow step planes
   bool lean = false;    // no halo-owner / legal planes, overrides, pads or via pads
-  bool unit = false;    // every board of its fields has no weight plane
+  bool wave = false;    // the wave form covers every field (no weight plane, legal planes, overrides, pads)
+  bool halo = false;    // some board of its fields has a halo-owner plane
   uint64_t field0 = 0, nfields = 0;
   uint64_t item0 = 0, nitems = 0;
   uint64_t lab0 = 0, labwords = 0;

@@ -2030,21 +2030,24 @@ struct KernelSet {
   const void* round;
   const void* list;
   const void* queue;
-  const void* wave;     // relax_wave (u64 labels) for this shape and layer bound (lean narrow sets only), or nullptr
+  const void* wave;     // relax_wave (u64 labels) for this shape and layer bound (narrow sets only), or nullptr
   const void* wave_pk;  // relax_wave with packed u32 labels, or nullptr
+  const void* wave_ho;  // relax_wave (u64 labels) for launches with halo-owner planes, or nullptr
 };
 
 template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
   const void* wave = nullptr;
   const void* wave_pk = nullptr;
-  if constexpr (NARROW && LEAN && TW == 16 && TH == 16 && LM <= 4) {
-    wave = (const void*)relax_wave<TW, TH, LM, false>;
-    wave_pk = (const void*)relax_wave<TW, TH, LM, true>;
+  const void* wave_ho = nullptr;
+  if constexpr (NARROW && TW == 16 && TH == 16 && LM <= 8) {
+    wave = (const void*)relax_wave<TW, TH, LM, false, false>;
+    wave_pk = (const void*)relax_wave<TW, TH, LM, true, false>;
+    wave_ho = (const void*)relax_wave<TW, TH, LM, false, true>;
   }
   return KernelSet{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
                    (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
-                   wave, wave_pk};
+                   wave, wave_pk, wave_ho};
 }
 
 // Wide step planes for every layer bound; narrow ones, full and lean, up to
@@ -2103,27 +2106,31 @@ static const KernelSet& pick_kernels(uint32_t tw, uint32_t th, uint32_t layers, 
   throw Error(1, "tile shape " + std::to_string(tw) + "x" + std::to_string(th) + " is not instantiated");
 }
 
-// The round-form kernel of a launch: the wave form (one wavefront per tile)
-// when the shape asks for 64 threads and the launch qualifies -- a lean
-// narrow set with a wave kernel, boards without a weight plane, no delta
+// The round-form kernel of a launch: the wave form (one wavefront per tile
+// up to four layers, two up to eight) when the shape asks for 64 threads and
+// the launch qualifies -- a narrow set with a wave kernel, fields the wave
+// form covers (field_wave) on boards without a weight plane, no delta
 // stepping -- else relax_round with the shape's workgroup size.
 // 16 x 16 tiles at 64 threads select the wave form; launches it does not
 // cover (and the graph and queue forms) run the workgroup kernels at 256.
 static bool wave_shape(const TileShape& S) { return S.threads == 64 && S.tw == 16 && S.th == 16; }
 static uint32_t workgroup_threads(const TileShape& S) { return wave_shape(S) ? 256u : S.threads; }
 // `pack`: the batch may use packed u32 labels (every open step of its boards
-// costs at least 1); the wave form then runs them.
+// costs at least 1); the wave form then runs them.  `halo`: some board of the
+// launch has a halo-owner plane (never packed).
 struct RoundLaunch {
   const void* fn;
   uint32_t threads;
   size_t lds;
   bool wave, packed;
 };
-static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool unit, bool keyed,
-                                bool pack) {
-  if (wave_shape(S) && K.wave && K.narrow && K.lean && unit && !keyed && Lmax <= K.lm) {
-    if (pack) return RoundLaunch{K.wave_pk, 64u, wave_lds_bytes(S.tw, S.th, K.lm, true), true, true};
-    return RoundLaunch{K.wave, 64u, wave_lds_bytes(S.tw, S.th, K.lm, false), true, false};
+static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool wave_ok, bool keyed,
+                                bool pack, bool halo) {
+  if (wave_shape(S) && K.wave && K.narrow && wave_ok && !keyed && Lmax <= K.lm) {
+    const uint32_t nt = 64u * wave_count(K.lm, S.th);
+    if (halo) return RoundLaunch{K.wave_ho, nt, wave_lds_bytes(S.tw, S.th, K.lm, false), true, false};
+    if (pack) return RoundLaunch{K.wave_pk, nt, wave_lds_bytes(S.tw, S.th, K.lm, true), true, true};
+    return RoundLaunch{K.wave, nt, wave_lds_bytes(S.tw, S.th, K.lm, false), true, false};
   }
   return RoundLaunch{K.round, workgroup_threads(S), lds_bytes(S.tw, S.th, Lmax, K.narrow), false, false};
 }
@@ -2133,6 +2140,12 @@ static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t
 // via-exempt pads.
 static bool field_lean(const BoardView& v, const FieldReq& q) {
   return !v.halo && !v.owner && !v.lhalo && q.ovr_cells.empty() && q.pads.empty() && q.via_pads.empty();
+}
+
+// Whether the wave form covers a field: lean, or lean but for a halo-owner
+// plane; and its board has no weight plane.
+static bool field_wave(const BoardView& v, const FieldReq& q) {
+  return !v.cost && !v.owner && !v.lhalo && q.ovr_cells.empty() && q.pads.empty() && q.via_pads.empty();
 }
 
 void ctx_init(Ctx& c, int device) {
@@ -2166,6 +2179,7 @@ void ctx_init(Ctx& c, int device) {
     HIP_OK(hipFuncSetAttribute(k.queue, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
     if (k.wave) HIP_OK(hipFuncSetAttribute(k.wave, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     if (k.wave_pk) HIP_OK(hipFuncSetAttribute(k.wave_pk, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    if (k.wave_ho) HIP_OK(hipFuncSetAttribute(k.wave_ho, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
   }
 }
 
@@ -2546,12 +2560,13 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  bool narrow = true, lean = true, unit = true;
+  bool narrow = true, lean = true, wave_ok = true, halo = false;
   for (const FieldReq& q : reqs) {
     if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
     narrow = narrow && field_narrow(*boards[q.board], q);
     lean = lean && field_lean(c.views[q.board], q);
-    unit = unit && c.views[q.board].cost == nullptr;
+    wave_ok = wave_ok && field_wave(c.views[q.board], q);
+    halo = halo || c.views[q.board].halo != nullptr;
   }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
   size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
@@ -2665,14 +2680,14 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   // least 1 (hops <= dist then) and the wave form runs; checked after the
   // solve, re-solved in wide labels when a label came within a step of the
   // packed dist limit.
-  bool pack = !c.no_pack && c.mode == MODE_PLAIN;
+  bool pack = !c.no_pack && c.mode == MODE_PLAIN && !halo;
   uint64_t pk_step = 0;
   for (const DevBoard* b : boards) {
     pack = pack && b->min_step >= 1 && b->max_step < 0xFFFFu;
     pk_step = std::max<uint64_t>(pk_step, b->max_step);
   }
   RelaxArgs A = make_args(c, item);
-  const RoundLaunch RL = round_launch(K, S, Lmax, unit, c.mode == MODE_PLAIN && c.delta != kMax, pack);
+  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, c.mode == MODE_PLAIN && c.delta != kMax, pack, halo);
   if (RL.packed) {
     take(c, W.fmax, W.fmax_cap, std::max<size_t>(item, 1));
     check_arena(c, {{W.fmax, std::max<size_t>(item, 1) * 4}});
@@ -3217,12 +3232,14 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   G.nitems = nitems;
   G.narrow = true;
   G.lean = true;
-  G.unit = true;
+  G.wave = true;
+  G.halo = false;
   for (const FieldReq& q : reqs) {
     G.layers = std::max(G.layers, d.views[q.board].layers);
     G.narrow = G.narrow && field_narrow(*d.devs[q.board], q);
     G.lean = G.lean && field_lean(d.views[q.board], q);
-    G.unit = G.unit && d.views[q.board].cost == nullptr;
+    G.wave = G.wave && field_wave(d.views[q.board], q);
+    G.halo = G.halo || d.views[q.board].halo != nullptr;
   }
   G.labwords = words;
   G.npool = npool;
@@ -3353,16 +3370,17 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
-  bool narrow = true, lean = true, unit = true;
+  bool narrow = true, lean = true, wave_ok = true, halo = false;
   for (uint32_t g = 0; g < d.max_groups; ++g)
     if (d.groups[g].live) {
       Lmax = std::max(Lmax, d.groups[g].layers);
       narrow = narrow && d.groups[g].narrow;
       lean = lean && d.groups[g].lean;
-      unit = unit && d.groups[g].unit;
+      wave_ok = wave_ok && d.groups[g].wave;
+      halo = halo || d.groups[g].halo;
     }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
-  const RoundLaunch RL = round_launch(K, S, Lmax, unit, false, false);
+  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, false, halo);
   void* args[] = {&A};
   if (d.stamp_round >= (1u << 30)) {
     // states and flags stay below 2^31: every touched item now reads as

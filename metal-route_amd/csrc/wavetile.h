@@ -6,14 +6,16 @@
 // row and column phases a 16 x 16 x 4 tile has 64 lines, so one wavefront
 // sweeps while three wait at the barrier, and the tile's 25.8 KB of LDS
 // (labels with a halo ring plus six step planes) lets six visits share a CU.
-// Here one wavefront owns the tile:
+// Here one wavefront owns a 16 x 16 tile of up to four layers (two
+// wavefronts up to eight layers: NW below):
 //   * labels live in an LDS buffer [layer][y][x] (row stride TW + 1, odd, so
 //     a lane per row and a lane per column both hit distinct banks); each
 //     phase reads the lines it sweeps into registers in its own layout --
-//     a lane per row (layer, y), a lane per column (layer, x), or TW*TH/64
-//     via columns per lane -- sweeps them forward and backward and writes a
-//     changed line back.  With L*TH = L*TW = 64 every lane sweeps in every
-//     phase.
+//     a thread per row (layer, y), a thread per column (layer, x), or
+//     TW*TH/(64 NW) via columns per thread -- sweeps them forward and
+//     backward and writes a changed line back.  With L*TH = L*TW = 64 NW
+//     every thread sweeps in every phase.  Two wavefronts meet only at the
+//     phase barriers (the phase's change flag is their joint ballot).
 //   * the six step prices of every cell are computed once per visit (in the
 //     row layout, where the board planes are loaded) and kept in registers,
 //     two 16-bit prices per register, in the layout of the phase that reads
@@ -44,32 +46,62 @@
 //     0xFFFF << 16 (then no path to any cell reaches dist 0xFFFF), and
 //     solves the batch again in the wide form otherwise.
 //
-// Scope: lean fields (no halo-owner or legal planes, overrides, own pads or
-// via-exempt pads) on boards without a weight plane whose step prices fit the
-// narrow (15-bit) form, in the round form without delta stepping.  A forced
-// endpoint on an obstacle makes a field wide, so every passable cell here is
-// a cell that is neither a trace node nor an obstacle and every open step
-// price is e + a.  The host picks this kernel only for such launches.
+// Scope: fields without legal planes, overrides, own pads or via-exempt
+// pads (a halo-owner plane is allowed: its penalty joins the additive price)
+// on boards without a weight plane whose step prices fit the narrow (15-bit)
+// form, in the round form without delta stepping.  A forced endpoint on an
+// obstacle makes a field wide, so every passable cell here is a cell that is
+// neither a trace node nor an obstacle and every open step price is e + a.
+// The host picks this kernel only for such launches.
+
+// Wavefronts per tile visit: one up to 64 rows (4 layers of 16x16), two up
+// to 128 (8 layers).
+__host__ __device__ constexpr uint32_t wave_count(uint32_t lm, uint32_t th) { return (lm * th + 63) / 64; }
 
 template <int TW, int TH, int LM, bool PK = false>
 struct WaveTile {
-  static_assert(LM * TH <= 64 && LM * TW <= 64, "a lane per row and a lane per column");
-  static_assert((TW * TH) % 64 == 0 && 64 % TW == 0, "whole via columns per lane; a fixed x per lane in block order");
-  static_assert(TH == TW, "the column layout reads rows cl * TH + y of the 64-row buffer");
+  static constexpr uint32_t NW = wave_count(LM, TH);  // wavefronts
+  static constexpr uint32_t NT = 64 * NW;             // threads
+  static_assert(LM * TH <= NT && LM * TW <= NT, "a lane per row and a lane per column");
+  static_assert((TW * TH) % NT == 0 && NT % TW == 0, "whole via columns per lane; a fixed x per lane in block order");
+  static_assert(TH == TW, "the column layout reads rows cl * TH + y of the NT-row buffer");
   static constexpr uint32_t RS = TW + 1;         // label buffer row stride
   static constexpr uint32_t PT = TW * TH;        // cells of one layer of the tile
-  static constexpr uint32_t NV = PT / 64;        // via columns per lane
-  static constexpr uint32_t NS = LM * PT / 64;   // cells per lane in block order
-  // label slots (u64): 64 rows whatever the layer bound, so the row layout's
-  // lane index is a row of the buffer for every lane
-  static constexpr uint32_t LAB = 64 * RS;
+  static constexpr uint32_t NV = PT / NT;        // via columns per lane
+  static constexpr uint32_t NS = LM * PT / NT;   // cells per lane in block order
+  // label slots: NT rows whatever the layer bound, so the row layout's
+  // thread index is a row of the buffer for every thread
+  static constexpr uint32_t LAB = NT * RS;
   // border labels as loaded: left [LM*TH], right [LM*TH], top [LM*TW], bottom [LM*TW]
   static constexpr uint32_t BORDER = 2 * LM * TH + 2 * LM * TW;
-  static constexpr size_t lds = (size_t)(LAB + BORDER) * (PK ? 4 : 8);
+  // then 16 words for the wavefronts' flags and maxima (two waves)
+  static constexpr size_t lds = (size_t)(LAB + BORDER) * (PK ? 4 : 8) + 64;
 };
 
 __host__ __device__ inline size_t wave_lds_bytes(uint32_t tw, uint32_t th, uint32_t lm, bool pk) {
-  return (64ull * (tw + 1) + 2ull * lm * th + 2ull * lm * tw) * (pk ? 4 : 8);
+  return (64ull * wave_count(lm, th) * (tw + 1) + 2ull * lm * th + 2ull * lm * tw) * (pk ? 4 : 8) + 64;
+}
+
+// Whether x holds in any thread of a visit's NW wavefronts; every wave has
+// passed a barrier on return.  With two waves each writes its ballot to
+// slot[wave] of the pair the caller alternates between consecutive calls, so
+// a wave that runs ahead never overwrites a flag another has yet to read
+// (a wave writing the pair again has passed the barrier of the call between,
+// which every wave reaches only after its reads).
+template <uint32_t NW>
+__device__ __forceinline__ bool visit_any(bool x, volatile uint32_t* slot) {
+  const uint64_t b = __ballot(x);
+  if constexpr (NW == 1) {
+    __syncthreads();
+    return b != 0;
+  } else {
+    if ((threadIdx.x & 63u) == 0) slot[threadIdx.x >> 6] = b != 0 ? 1u : 0u;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) r |= slot[w];
+    return r != 0;
+  }
 }
 
 // 16-bit step prices, two to a register: lo = entering from the lower
@@ -153,20 +185,26 @@ __device__ __forceinline__ uint32_t add_k(const BoardView& B, uint32_t g) {
   return 0u;
 }
 
-template <int TW, int TH, int LM, int AK, bool PK>
+// HO: the board has a halo-owner plane; a cell whose owner is another net
+// costs the board's halo penalty on top of its additive price (add_price).
+template <int TW, int TH, int LM, int AK, bool PK, bool HO>
 __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched, const FieldDesc& F,
                                 const BoardView& B, char* smem, uint32_t round) {
   using WT = WaveTile<TW, TH, LM, PK>;
   using WL = WaveLab<PK>;
   using LT = typename WL::T;
-  constexpr uint32_t RS = WT::RS, PT = WT::PT, NV = WT::NV, NS = WT::NS, LAB = WT::LAB;
+  constexpr uint32_t RS = WT::RS, PT = WT::PT, NV = WT::NV, NS = WT::NS, LAB = WT::LAB, NT = WT::NT, NW = WT::NW;
   LT* lab = (LT*)smem;
   // step staging before the labels go in: y prices, then via prices (packed
   // labels: one after the other in the first LAB words; wide: side by side)
   uint32_t* lab32 = (uint32_t*)smem;
   constexpr uint32_t VOFF = PK ? 0u : LAB;
   LT* b0 = lab + LAB;
-  const uint32_t lane = threadIdx.x;
+  // the wavefronts' flags (two alternating pairs) and results
+  volatile uint32_t* xw = (volatile uint32_t*)(smem + (size_t)(LAB + WT::BORDER) * sizeof(LT));
+  uint32_t par = 0;  // which flag pair the next visit_any uses
+  // tid: the thread's place in the tile layouts; lane: its lane in its wave
+  const uint32_t tid = threadIdx.x, lane = NW == 1 ? tid : tid & 63u;
 
   const uint32_t t = item - F.item0;
   const uint32_t txi = t % F.tx, tyi = t / F.tx;
@@ -181,7 +219,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   // (vector loads complete in issue order: the step prices wait only for the
   // planes, the tile's labels are still in flight meanwhile)
   // row layout: lane (rl, ry) holds row ry of layer rl
-  const uint32_t rl = lane / TH, ry = lane - rl * TH;
+  const uint32_t rl = tid / TH, ry = tid - rl * TH;
   const bool rrow = rl < L;
   const bool row_in = rrow && wy0 + ry < F.wh;
   const uint32_t rgy = F.y0 + min(wy0 + ry, F.wh - 1);
@@ -228,8 +266,18 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
     }
   }
   if constexpr (AK == ADD_NONE) aw[0] = 0;
-  // column layout: lane (cl, cx) holds column cx of layer cl
-  const uint32_t cl = lane / TW, cx = lane - cl * TW;
+  // foreign-halo cells of the row, one bit per cell
+  uint32_t hom = 0;
+  if constexpr (HO) {
+    const int32_t own = F.own_id;
+#pragma unroll
+    for (int x = 0; x < TW; ++x) {
+      const int32_t o = gbl(B.halo)[rowg + min(wx0 + x, F.ww - 1)];
+      hom |= (o != -1 && o != own) ? 1u << x : 0u;
+    }
+  }
+  // column layout: thread (cl, cx) holds column cx of layer cl
+  const uint32_t cl = tid / TW, cx = tid - cl * TW;
   const bool ccol = cl < L;
   const bool col_in = ccol && wx0 + cx < F.ww;
   // Board cells of the halo (the price of stepping from the tile into them)
@@ -250,6 +298,10 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
     for (int k = 0; k < 4; ++k) {
       const uint8_t m = gbl(B.bmask)[g[k]];
       ha[k] = add_k<AK>(B, g[k]);
+      if constexpr (HO) {
+        const int32_t o = gbl(B.halo)[g[k]];
+        if (o != -1 && o != F.own_id) ha[k] += B.halo_pen;
+      }
       hbm[k] = e[k] ? m : (uint8_t)M_TRACE;
     }
   }
@@ -283,26 +335,26 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   // (never torn): every label this visit takes was stored in an earlier
   // launch, and a neighbour that is visited again in this one only lowers
   // its labels, so whichever value a load returns is an upper bound.
-  const uint64_t own = blockbase + lane % TW;
+  const uint64_t own = blockbase + tid % TW;
   const uint64_t hla = el ? blockbase - TB + ncell + L * TH + rl * TH + ry : own;
   const uint64_t hra = er ? blockbase + TB + ncell + rl * TH + ry : own;
   const uint64_t hta = et ? blockbase - (uint64_t)F.tx * TB + cl * PT + (TH - 1) * TW + cx : own;
   const uint64_t hba = eb ? blockbase + (uint64_t)F.tx * TB + cl * PT + cx : own;
   const auto* glab = gbl((const LT*)A.labels);
   LT hl = glab[hla], hr = glab[hra], ht = glab[hta], hb = glab[hba];
-  // The tile's labels in block order: lane holds cells lane + 64 j, i.e.
-  // column x = lane % TW of row (64 / TW) * (j % NV) + lane / TW of layer
+  // The tile's labels in block order: thread tid holds cells tid + NT j,
+  // i.e. column x = tid % TW of row (NT / TW) * (j % NV) + tid / TW of layer
   // j / NV -- the same x for every j, so each cell's LDS slot is a constant
-  // offset from one per-lane base (`sbase`, also the via layout's base).
+  // offset from one per-thread base (`sbase`, also the via layout's base).
   // Layers beyond the board's re-read its last layer (dropped below).
-  constexpr uint32_t RPJ = 64 / TW;  // rows per 64 cells
-  const uint32_t sx_ = lane % TW, sy_ = lane / TW;
+  constexpr uint32_t RPJ = NT / TW;  // rows per NT cells
+  const uint32_t sx_ = tid % TW, sy_ = tid / TW;
   const uint32_t sbase = sy_ * RS + sx_;
   LT sl[NS];
   if (touched) {
-    const auto* lp = glab + blockbase + lane;
+    const auto* lp = glab + blockbase + tid;
 #pragma unroll
-    for (uint32_t j = 0; j < NS; ++j) sl[j] = lp[64u * (min(j / NV, L - 1) * NV + j % NV)];
+    for (uint32_t j = 0; j < NS; ++j) sl[j] = lp[NT * (min(j / NV, L - 1) * NV + j % NV)];
   } else {
 #pragma unroll
     for (uint32_t j = 0; j < NS; ++j) sl[j] = WL::unr;
@@ -336,7 +388,8 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
     for (int x = 0; x < TW; ++x) {
       const uint8_t mb = (uint8_t)(mw[x / 4] >> (8 * (x % 4)));
       const uint8_t m = (row_in && wx0 + x < F.ww) ? mb : (uint8_t)M_TRACE;  // outside the window: closed
-      const uint32_t a = AK == ADD_U32 ? aw[x % AW] : AK == ADD_U16 ? (aw[(x / 2) % AW] >> (16 * (x % 2))) & 0xFFFFu : 0u;
+      uint32_t a = AK == ADD_U32 ? aw[x % AW] : AK == ADD_U16 ? (aw[(x / 2) % AW] >> (16 * (x % 2))) & 0xFFFFu : 0u;
+      if constexpr (HO) a += ((hom >> x) & 1u) ? B.halo_pen : 0u;
       uint32_t px = pack_steps(wave_step(m, M_NX, true, xg[x], a), wave_step(m, M_PX, true, xg[x + 1], a));
       // keep the packed form: the sweeps read each half with a sign-extending
       // operand select, no per-step gate
@@ -344,7 +397,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
       sx[x] = px;
       // staged for the column and via layouts; lanes without a row
       // (layers beyond the board's) stage closed prices in rows no lane reads
-      const uint32_t at = lane * RS + x;  // lane = rl * TH + ry
+      const uint32_t at = tid * RS + x;  // tid = rl * TH + ry
       lab32[at] = pack_steps(wave_step(m, M_NY, true, yg0, a), wave_step(m, M_PY, true, yg1, a));
       sv_st[x] = pack_steps(wave_step(m, M_DN, rl > 0, vlo, a), wave_step(m, M_UP, rl + 1 < L, vhi, a));
       if (!PK) lab32[VOFF + at] = sv_st[x];
@@ -358,7 +411,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   if (PK) {
     __syncthreads();
 #pragma unroll
-    for (int x = 0; x < TW; ++x) lab32[lane * RS + x] = sv_st[x];
+    for (int x = 0; x < TW; ++x) lab32[tid * RS + x] = sv_st[x];
     __syncthreads();
   }
 #pragma unroll
@@ -391,7 +444,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
     __syncthreads();
     const uint32_t plane = B.plane;
     bool put = false;
-    for (uint32_t k = lane; k < F.n_src; k += 64) {
+    for (uint32_t k = tid; k < F.n_src; k += NT) {
       const uint32_t g = A.pool[F.src_off + k];
       const uint32_t l = g / plane, r = g - l * plane, gy = r / B.w, gx = r - gy * B.w;
       if (l >= L || gx < F.x0 + wx0 || gy < F.y0 + wy0) continue;
@@ -402,7 +455,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
         put = true;
       }
     }
-    seeded = __ballot(put) != 0;
+    seeded = visit_any<NW>(put, xw + 2 * (par++ & 1u));
   }
   __syncthreads();
 
@@ -440,8 +493,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
           for (int x = 0; x < TW; ++x) lab[base + x] = v[x];
         }
       }
-      const bool c = __ballot(ch) != 0;
-      __syncthreads();
+      const bool c = visit_any<NW>(ch, xw + 2 * (par++ & 1u));
       any = any || c;
       quiet = c ? 0u : quiet + 1;
       if (++k >= nphase && quiet + 1 >= nphase) {
@@ -476,8 +528,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
           for (int y = 0; y < TH; ++y) lab[base + y * RS] = v[y];
         }
       }
-      const bool c = __ballot(ch) != 0;
-      __syncthreads();
+      const bool c = visit_any<NW>(ch, xw + 2 * (par++ & 1u));
       any = any || c;
       quiet = c ? 0u : quiet + 1;
       if (++k >= nphase && quiet + 1 >= nphase) {
@@ -510,8 +561,7 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
         }
         ch = ch || cc;
       }
-      const bool c = __ballot(ch) != 0;
-      __syncthreads();
+      const bool c = visit_any<NW>(ch, xw + 2 * (par++ & 1u));
       any = any || c;
       quiet = c ? 0u : quiet + 1;
       if (++k >= nphase && quiet + 1 >= nphase) {
@@ -548,13 +598,13 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   // dirty in the XCD's L2 for the end of the launch to write back)
   uint32_t vmax = 0;  // packed: largest label stored with a real dist
   {
-    LT* lp = (LT*)A.labels + blockbase + lane;
+    LT* lp = (LT*)A.labels + blockbase + tid;
 #pragma unroll
     for (uint32_t j = 0; j < NS; ++j) {
       const uint32_t l = j / NV, y = RPJ * (j % NV);
       if (l < L) {
         const LT v = lab[sbase + (l * TH + y) * RS];
-        __hip_atomic_store(lp + 64u * j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(lp + NT * j, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (PK) vmax = max(vmax, (uint32_t)v < kPackedUnreached ? (uint32_t)v : 0u);
       }
     }
@@ -566,19 +616,33 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
     __hip_atomic_store(bp + L * TH, lab[base + TW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (PK) {
-    // the tile's largest stored label: its last visit's value is what the
-    // host's certificate reads (labels only decrease)
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, o));
-    if (lane == 0) A.item_max[item] = vmax;
   }
+  uint32_t sides = (__ballot(fl & T_LEFT) ? T_LEFT : 0u) | (__ballot(fl & T_RIGHT) ? T_RIGHT : 0u) |
+                   (__ballot(fl & T_TOP) ? T_TOP : 0u) | (__ballot(fl & T_BOTTOM) ? T_BOTTOM : 0u);
+  if constexpr (NW > 1) {
+    // the waves' side flags and maxima, combined by every wave
+    volatile uint32_t* xr = xw + 4;
+    if (lane == 0) {
+      xr[tid >> 6] = sides;
+      xr[NW + (tid >> 6)] = vmax;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+      sides |= xr[w];
+      vmax = max(vmax, (uint32_t)xr[NW + w]);
+    }
+  }
+  // the tile's largest stored label: its last visit's value is what the
+  // host's certificate reads (labels only decrease)
+  if (PK && tid == 0) A.item_max[item] = vmax;
   TileResult res;
-  res.flags = (__ballot(fl & T_LEFT) ? T_LEFT : 0u) | (__ballot(fl & T_RIGHT) ? T_RIGHT : 0u) |
-              (__ballot(fl & T_TOP) ? T_TOP : 0u) | (__ballot(fl & T_BOTTOM) ? T_BOTTOM : 0u) |
-              (any ? T_ANY : 0u) | (capped ? T_CAPPED : 0u);
+  res.flags = sides | (any ? T_ANY : 0u) | (capped ? T_CAPPED : 0u);
   res.tx = F.tx;
   res.kl = res.kr = res.kt = res.kb = res.ka = kMax;  // keys: the wave form runs without delta stepping
-  if (lane == 0) {
+  if (tid == 0) {
     if (!touched)
       __hip_atomic_fetch_or(&A.state[item], S_TOUCHED | (round << 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(shard_of(A, SH_VISITS), 1u);
@@ -594,8 +658,17 @@ struct WaveMin {
   static constexpr int waves = PK ? 5 : 4;
 };
 
-template <int TW, int TH, int LM, bool PK>
-__global__ void __launch_bounds__(64, (WaveMin<TW, LM, PK>::waves)) relax_wave(RelaxArgs A) {
+// HO: the launch has boards with a halo-owner plane (a separate kernel, so
+// the others do not carry that code's registers).
+template <int TW, int TH, int LM, bool PK, bool HO, int AK>
+__device__ __forceinline__ TileResult wave_visit(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched,
+                                                 const FieldDesc& F, const BoardView& B, char* smem) {
+  if (HO && B.halo) return wave_tile<TW, TH, LM, AK, PK, HO>(A, item, f, touched, F, B, smem, A.round);
+  return wave_tile<TW, TH, LM, AK, PK, false>(A, item, f, touched, F, B, smem, A.round);
+}
+
+template <int TW, int TH, int LM, bool PK, bool HO>
+__global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>::waves)) relax_wave(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t item = round_item(A);
   const uint32_t grp = A.group_active ? const_load(A.item_group + item) : 0u;
@@ -605,10 +678,11 @@ __global__ void __launch_bounds__(64, (WaveMin<TW, LM, PK>::waves)) relax_wave(R
   const BoardView B = const_load(A.boards + F.board);
   TileResult res;
   if (B.add16)
-    res = wave_tile<TW, TH, LM, ADD_U16, PK>(A, item, f, touched, F, B, smem, A.round);
+    res = wave_visit<TW, TH, LM, PK, HO, ADD_U16>(A, item, f, touched, F, B, smem);
   else if (B.add)
-    res = wave_tile<TW, TH, LM, ADD_U32, PK>(A, item, f, touched, F, B, smem, A.round);
+    res = wave_visit<TW, TH, LM, PK, HO, ADD_U32>(A, item, f, touched, F, B, smem);
   else
-    res = wave_tile<TW, TH, LM, ADD_NONE, PK>(A, item, f, touched, F, B, smem, A.round);
+    res = wave_visit<TW, TH, LM, PK, HO, ADD_NONE>(A, item, f, touched, F, B, smem);
+  if (threadIdx.x >= 64) return;  // the first wave finishes the visit
   round_tail(A, item, grp, res, false, 0u);
 }

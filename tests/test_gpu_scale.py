@@ -37,7 +37,8 @@ THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
 
 
 # The workgroup form (four wavefronts per tile visit) and the wave form (one
-# wavefront per tile visit, 64 threads: lean unit-weight narrow batches).
+# wavefront per tile visit up to four layers, two up to eight: unit-weight
+# narrow batches without pads, overrides or legal planes).
 FORMS = {"workgroup": (16, 16, 256, 32), "wave": (16, 16, 64, 32)}
 
 
@@ -110,8 +111,8 @@ def test_config2_full_board(eng):
 
 
 def test_config4_multisource_fields_full_board(eng):
-    if eng.form == "wave":
-        pytest.skip("six layers and u32 costs with a halo-owner plane: the wave form does not apply")
+    """Six layers with a halo-owner plane: the wave form runs two wavefronts
+    per tile on u64 labels."""
     s = mrfield.synth_config(4, seed=1)
     g = s.grid
     assert (g.w, g.h, g.layers, len(s.pins)) == (1024, 1024, 6, 512)
@@ -120,6 +121,7 @@ def test_config4_multisource_fields_full_board(eng):
     try:
         fi = [0, 101, 333, 511]
         board.solve_fields([s.pins[i] for i in fi], net_ids=fi)
+        assert_form_ran(eng, 8)
         og, oe = oracle_parts(s)
         spec = mrcpu.Spec(og, oe, additive=s.additive, halo_owner=s.halo_owner, halo_penalty=s.halo_penalty)
         nets = [mrcpu.Net(0, MAX, own_id=i) for i in fi]

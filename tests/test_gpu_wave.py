@@ -1,8 +1,10 @@
 """GPU parity of the one-wavefront-per-tile form against the CPU oracle.
 
-The wave form (tiles 16 x 16, 64 threads) runs for lean batches on boards
-without a weight plane whose step prices fit 15 bits; these cases are built
-to qualify, and every one asserts that the wave kernel is what ran.  Boards
+The wave form (tiles 16 x 16, 64 threads; two wavefronts per tile on boards
+of five to eight layers) runs for batches without pads, overrides or legal
+planes on boards without a weight plane whose step prices fit 15 bits; these
+cases are built to qualify, and every one asserts that the wave kernel is
+what ran.  Boards
 whose every open step costs at least 1 run it on packed u32 labels
 (dist << 16 | hops), certified after the solve and solved again on u64
 labels when a distance comes near the 16-bit limit; the cases assert which
@@ -67,7 +69,7 @@ def assert_wave_ran(eng, label_bytes=None):
 
 
 @pytest.mark.parametrize("packed", [True, False])
-@pytest.mark.parametrize("L", [1, 2, 3, 4])
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 8])
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_wave_random_fields(eng, L, seed, packed):
     """Multi-source fields on boards with every lean feature: trace nodes,
@@ -103,6 +105,43 @@ def test_wave_random_fields(eng, L, seed, packed):
     got_d, got_h = eng.fields(as_dev(g), dedges, sources, dsts=dsts, windows=wins,
                               pricing=mrfield.Pricing(additive=add))
     assert_wave_ran(eng, 4 if packed else 8)
+    for i in range(len(nets)):
+        assert np.array_equal(got_d[i], ref_d[i]), (i, np.flatnonzero(got_d[i] != ref_d[i])[:10])
+        assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
+
+
+@pytest.mark.parametrize("L", [3, 6])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_wave_halo_owner_fields(eng, L, seed):
+    """Boards with a halo-owner plane: a cell owned by another net (or by
+    several, -2) costs the penalty on top of its additive price, the net's
+    own cells do not.  Never packed: u64 labels."""
+    rng = np.random.default_rng(500 + 10 * L + seed)
+    w, h = int(rng.integers(30, 70)), int(rng.integers(30, 70))
+    g = unit_board(seed + 40, w, h, L)
+    edges = mrcpu.Edges([int(v) for v in rng.integers(1, 30, size=w - 1)],
+                        [int(v) for v in rng.integers(1, 30, size=h - 1)],
+                        [int(rng.integers(1, 60)) for _ in range(L - 1)])
+    add = rng.integers(1, 40, size=g.n).astype(np.uint32)
+    halo = np.full(g.n, -1, dtype=np.int32)
+    sel = rng.random(g.n) < 0.15
+    halo[sel] = rng.integers(0, 4, size=int(sel.sum()))
+    halo[rng.random(g.n) < 0.03] = -2
+    free = free_cells(g)
+    nets, sources, dsts, wins = [], [], [], []
+    for i in range(8):
+        src = [int(c) for c in rng.choice(free, size=1 + i % 3, replace=False)]
+        d = int(rng.choice(free))
+        nets.append(mrcpu.Net(src[0], d, own_id=i % 4))
+        sources.append(src)
+        dsts.append(d)
+        wins.append((0, 0, w - 1, h - 1) if i % 2 else (2, 3, w - 4, h - 2))
+    ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(g, edges, additive=add, halo_owner=halo, halo_penalty=13),
+                                      nets, sources, wins)
+    got_d, got_h = eng.fields(as_dev(g), mrfield.Edges(list(edges.x), list(edges.y), list(edges.via)), sources,
+                              dsts=dsts, windows=wins,
+                              pricing=mrfield.Pricing(add, halo, 13, [n.own_id for n in nets]))
+    assert_wave_ran(eng, 8)
     for i in range(len(nets)):
         assert np.array_equal(got_d[i], ref_d[i]), (i, np.flatnonzero(got_d[i] != ref_d[i])[:10])
         assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
