@@ -390,6 +390,10 @@ struct Group {
   bool lean = false;    // no halo-owner / legal planes, overrides, pads or via pads
   bool wave = false;    // the wave form covers every field (no weight plane, legal planes, overrides, pads)
   bool halo = false;    // some board of its fields has a halo-owner plane
+  // Packed u32 labels (the wave form on boards whose every open step costs
+  // at least 1 and below 0xFFFF); live groups all share one label width.
+  bool pack = false;
+  uint64_t pk_step = 0;  // largest step price of its boards (the certificate)
   uint64_t field0 = 0, nfields = 0;
   uint64_t item0 = 0, nitems = 0;
   uint64_t lab0 = 0, labwords = 0;
@@ -416,6 +420,8 @@ struct RoundDriver {
   // (the last round in which one of its visits activated a tile, plus one)
   uint32_t* ctr = nullptr;
   uint32_t* shards = nullptr;  // sharded visit / round / done counters (engine.hip kShardWords)
+  uint32_t* imax = nullptr;    // per item: the largest packed label its visits stored
+  uint32_t* pmax = nullptr;    // a packed group's largest label (one word)
   uint32_t* pinned = nullptr;  // readback of ctr (statistics)
   // host-mapped words the relaxation publishes to: next count, error flags,
   // stamp, then the group flags from kDriverFlagsAt
@@ -451,8 +457,10 @@ int driver_add_board(RoundDriver& d, const DevBoard& b);  // slot, or -1 when th
 void driver_remove_board(RoundDriver& d, int slot);
 // Label words a group of fields needs (FieldReq::board is a driver slot).
 uint64_t driver_group_words(const RoundDriver& d, const std::vector<FieldReq>& reqs);
-// Admit a group at the next round boundary; false when it does not fit now.
-bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& group);
+// Admit a group at the next round boundary; false when it does not fit now
+// (memory, slots, or live groups of the other label width).  allow_pack:
+// the group may run on packed labels where its boards allow.
+bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& group, bool allow_pack = true);
 // Tiles the next round will visit: the active list plus admitted seeds.
 inline uint32_t driver_next_tiles(const RoundDriver& d) { return d.n_cur + (uint32_t)d.pending_seeds.size(); }
 // One relaxation round over every active item (new groups' seeds included).
@@ -463,7 +471,11 @@ uint32_t driver_step(RoundDriver& d);
 void driver_sync_stats(RoundDriver& d);
 // Live groups with no item on the active list: their fields are complete.
 void driver_finished(RoundDriver& d, std::vector<int>& out);
-void driver_paths(RoundDriver& d, int group, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
+// Paths of a finished group.  False (and no paths) when the group ran on
+// packed labels and its certificate failed: a label came within one step of
+// the packed limit, so the group must be solved again on wide labels
+// (driver_admit with allow_pack false).
+bool driver_paths(RoundDriver& d, int group, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
                   std::vector<uint64_t>& off, std::vector<uint32_t>& cells);
 void driver_release(RoundDriver& d, int group);
 

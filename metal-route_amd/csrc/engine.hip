@@ -1523,7 +1523,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
 // Single-thread round step between two relax_list launches.
 // Largest of n per-item maxima into counters[C_PMAX]: a grid-stride pass,
 // reduced per workgroup (256 threads), one atomic per workgroup.
-__global__ void __launch_bounds__(256) reduce_item_max(const uint32_t* m, size_t n, uint32_t* counters) {
+__global__ void __launch_bounds__(256) reduce_item_max(const uint32_t* m, size_t n, uint32_t* out) {
   __shared__ uint32_t wmax[4];
   uint32_t v = 0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1534,7 +1534,7 @@ __global__ void __launch_bounds__(256) reduce_item_max(const uint32_t* m, size_t
   __syncthreads();
   if (threadIdx.x == 0) {
     v = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-    if (v) atomicMax(&counters[C_PMAX * kCtrStride], v);
+    if (v) atomicMax(out, v);
   }
 }
 
@@ -2888,7 +2888,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   if (timed) HIP_OK(hipEventRecord(W.ev1, s));
   if (RL.packed && item)
     hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<size_t>((item + 255) / 256, 256)), dim3(256), 0, s,
-                       W.fmax, (size_t)item, W.counters);
+                       W.fmax, (size_t)item, W.counters + C_PMAX * kCtrStride);
   uint32_t* cnt = W.pinned;
   HIP_OK(hipMemcpyAsync(cnt, W.counters, (kCtrWords + kShardWords) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
@@ -3117,6 +3117,8 @@ void driver_init(RoundDriver& d, Ctx& c, size_t label_bytes, uint32_t max_boards
   dmalloc(d.boards, max_boards);
   dmalloc(d.ctr, kCtrWords + max_groups);
   dmalloc(d.shards, kShardWords);
+  dmalloc(d.imax, items);
+  dmalloc(d.pmax, 16);
   HIP_OK(hipHostMalloc((void**)&d.pinned, (kCtrWords + max_groups + kShardWords) * sizeof(uint32_t),
                        hipHostMallocDefault));
   HIP_OK(hipHostMalloc((void**)&d.hostctr, (kDriverFlagsAt + max_groups) * sizeof(uint32_t),
@@ -3144,7 +3146,7 @@ void driver_free(RoundDriver& d) {
   if (!d.c) return;
   (void)hipSetDevice(d.c->device);
   void* ptrs[] = {d.labels, d.state,  d.act,   d.list,   d.item_field,  d.item_group,
-                  d.seeds,  d.fields, d.pool,  d.boards, d.ctr, d.shards};
+                  d.seeds,  d.fields, d.pool,  d.boards, d.ctr, d.shards, d.imax, d.pmax};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (d.pinned) (void)hipHostFree(d.pinned);
@@ -3193,7 +3195,7 @@ uint64_t driver_group_words(const RoundDriver& d, const std::vector<FieldReq>& r
   return w;
 }
 
-bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
+bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid, bool allow_pack) {
   const TileShape& S = d.c->shape;
   int g = -1;
   for (uint32_t k = 0; k < d.max_groups; ++k)
@@ -3203,6 +3205,12 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
     }
   if (g < 0) return false;
   uint64_t nitems = 0, words = 0, npool = 0;
+  Group G;
+  G.narrow = true;
+  G.lean = true;
+  G.wave = true;
+  G.halo = false;
+  bool steps_ok = true;  // every open step of its boards costs 1 .. 0xFFFE
   for (const FieldReq& q : reqs) {
     if (q.board >= d.max_boards || !d.board_used[q.board]) throw Error(1, "field names an unknown board");
     if (!q.ovr_cells.empty()) throw Error(1, "the round driver takes no additive overrides");
@@ -3210,14 +3218,28 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
     nitems += (uint64_t)((ww + S.tw - 1) / S.tw) * ((wh + S.th - 1) / S.th);
     words += field_label_words(S, ww, wh, d.views[q.board].layers);
     npool += q.sources.size() + q.forced.size() + q.pads.size() + q.via_pads.size();
+    const DevBoard& b = *d.devs[q.board];
+    G.layers = std::max(G.layers, d.views[q.board].layers);
+    G.narrow = G.narrow && field_narrow(b, q);
+    G.lean = G.lean && field_lean(d.views[q.board], q);
+    G.wave = G.wave && field_wave(d.views[q.board], q);
+    G.halo = G.halo || d.views[q.board].halo != nullptr;
+    steps_ok = steps_ok && b.min_step >= 1 && b.max_step < 0xFFFFu;
+    G.pk_step = std::max<uint64_t>(G.pk_step, b.max_step);
   }
-  Group G;
+  // packed labels only where the wave form runs them: the wave shape, a
+  // narrow wave-form group of at most eight layers, no halo-owner plane
+  G.pack = allow_pack && steps_ok && wave_shape(S) && G.narrow && G.wave && !G.halo && G.layers <= 8;
+  for (uint32_t k = 0; k < d.max_groups; ++k)
+    if (d.groups[k].live && d.groups[k].pack != G.pack) return false;  // the other label width is running
+  // packed: u32 labels, two to a label word of the allocator
+  const uint64_t alloc_words = G.pack ? (words + 1) / 2 : words;
   if (!d.fa.alloc(reqs.size(), G.field0)) return false;
   if (!d.ia.alloc(nitems, G.item0)) {
     d.fa.release(G.field0, reqs.size());
     return false;
   }
-  if (!d.la.alloc(words, G.lab0)) {
+  if (!d.la.alloc(alloc_words, G.lab0)) {
     d.fa.release(G.field0, reqs.size());
     d.ia.release(G.item0, nitems);
     return false;
@@ -3225,29 +3247,19 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   if (!d.pa.alloc(npool, G.pool0)) {
     d.fa.release(G.field0, reqs.size());
     d.ia.release(G.item0, nitems);
-    d.la.release(G.lab0, words);
+    d.la.release(G.lab0, alloc_words);
     return false;
   }
   G.nfields = reqs.size();
   G.nitems = nitems;
-  G.narrow = true;
-  G.lean = true;
-  G.wave = true;
-  G.halo = false;
-  for (const FieldReq& q : reqs) {
-    G.layers = std::max(G.layers, d.views[q.board].layers);
-    G.narrow = G.narrow && field_narrow(*d.devs[q.board], q);
-    G.lean = G.lean && field_lean(d.views[q.board], q);
-    G.wave = G.wave && field_wave(d.views[q.board], q);
-    G.halo = G.halo || d.views[q.board].halo != nullptr;
-  }
-  G.labwords = words;
+  G.labwords = alloc_words;
   G.npool = npool;
   std::vector<FieldDesc> descs(reqs.size());
   std::vector<uint32_t> pool, item_field;
   pool.reserve(npool);
   item_field.reserve(nitems);
-  uint64_t item = G.item0, lab = G.lab0;
+  // label offsets in labels: u64 words, or u32 halves of the group's words
+  uint64_t item = G.item0, lab = G.pack ? 2 * G.lab0 : G.lab0;
   std::vector<uint32_t> seeds;
   for (size_t i = 0; i < reqs.size(); ++i) {
     const FieldReq& q = reqs[i];
@@ -3296,6 +3308,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid) {
   stage_copy(d, d.item_field + G.item0, item_field);
   launch_fill32(s, d.item_group + G.item0, (uint32_t)g, nitems);
   launch_fill32(s, d.state + G.item0, 0u, nitems);
+  if (G.pack) launch_fill32(s, d.imax + G.item0, 0u, nitems);
   G.live = true;
   G.active = !seeds.empty();
   d.pending_seeds.insert(d.pending_seeds.end(), seeds.begin(), seeds.end());
@@ -3370,7 +3383,7 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
-  bool narrow = true, lean = true, wave_ok = true, halo = false;
+  bool narrow = true, lean = true, wave_ok = true, halo = false, pack = false;
   for (uint32_t g = 0; g < d.max_groups; ++g)
     if (d.groups[g].live) {
       Lmax = std::max(Lmax, d.groups[g].layers);
@@ -3378,9 +3391,12 @@ uint32_t driver_step(RoundDriver& d) {
       lean = lean && d.groups[g].lean;
       wave_ok = wave_ok && d.groups[g].wave;
       halo = halo || d.groups[g].halo;
+      pack = d.groups[g].pack;  // the same for every live group
     }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
-  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, false, halo);
+  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, pack, halo);
+  if (RL.packed != pack) throw Error(1, "packed groups without the packed kernel");
+  A.item_max = d.imax;
   void* args[] = {&A};
   if (d.stamp_round >= (1u << 30)) {
     // states and flags stay below 2^31: every touched item now reads as
@@ -3466,18 +3482,35 @@ void driver_finished(RoundDriver& d, std::vector<int>& out) {
     if (d.groups[g].live && !d.groups[g].active) out.push_back((int)g);
 }
 
-void driver_paths(RoundDriver& d, int g, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
+bool driver_paths(RoundDriver& d, int g, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
                   std::vector<uint64_t>& off, std::vector<uint32_t>& cells) {
   const Group& G = d.groups.at(g);
+  hipStream_t s = d.c->stream;
+  if (G.pack) {
+    // the group's certificate: its largest stored label, read back with the
+    // paths (reconstruct_with synchronises the stream)
+    launch_fill32(s, d.pmax, 0u, 1);
+    hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<uint64_t>((G.nitems + 255) / 256, 256)), dim3(256),
+                       0, s, d.imax + G.item0, (size_t)G.nitems, d.pmax);
+    HIP_OK(hipMemcpyAsync(d.pinned, d.pmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  }
   FieldRead R;
   R.boards = d.boards;
   R.fields = d.fields + G.field0;
   R.labels = d.labels;
   R.state = d.state;
-  R.packed = 0;  // the round driver keeps wide labels
+  R.packed = G.pack ? 1 : 0;
   R.tw = d.c->shape.tw;
   R.th = d.c->shape.th;
   reconstruct_with(*d.c, R, d.pool, G.nfields, routed, cost, off, cells, nullptr);
+  if (G.pack && (uint64_t)d.pinned[0] + ((G.pk_step << 16) | 1u) >= 0xFFFF0000ull) {
+    routed.clear();
+    cost.clear();
+    off.clear();
+    cells.clear();
+    return false;
+  }
+  return true;
 }
 
 void driver_release(RoundDriver& d, int g) {

@@ -75,6 +75,9 @@ struct ServeReq {
   std::vector<size_t> idx;
   std::vector<mrf_window> pass_wins;
   bool retry_pass = false;
+  // the current pass runs on wide labels: its packed run failed the
+  // certificate (a label came within one step of the packed limit)
+  bool wide = false;
   int group = -1;
   int state = 0;  // 0 queued, 1 running, 2 done
   int code = MRF_OK;
@@ -97,6 +100,7 @@ struct PassCheck {
   std::vector<uint32_t> cost, cells;
   std::vector<uint64_t> off;
   bool final = false;  // the request completes when this pass is checked
+  bool again = false;  // the pass ran on packed labels that failed the certificate: solve it again, wide
 };
 
 }  // namespace
@@ -177,7 +181,12 @@ PassCheck* collect_pass(mrf_server* S, ServeReq* r) {
   pc->r = r;
   pc->idx = r->idx;
   pc->wins = r->pass_wins;
-  mrf::driver_paths(S->d, r->group, pc->routed, pc->cost, pc->off, pc->cells);
+  if (!mrf::driver_paths(S->d, r->group, pc->routed, pc->cost, pc->off, pc->cells)) {
+    r->wide = true;
+    pc->again = true;
+    return pc;
+  }
+  r->wide = false;  // a later pass (the full-board retry) may pack again
   std::vector<size_t> unrouted;
   for (size_t k = 0; k < r->idx.size(); ++k)
     if (!pc->routed[k]) unrouted.push_back(r->idx[k]);
@@ -336,9 +345,12 @@ void worker_loop(mrf_server* S) {
           continue;
         }
         int g = -1;
-        if (!mrf::driver_admit(S->d, reqs, g)) {
+        if (!mrf::driver_admit(S->d, reqs, g, !r->wide)) {
           std::lock_guard<std::mutex> lk(S->mu);
           if (!S->running.empty()) {  // wait for memory to free up; a smaller one may fit
+            // a pass solved again on wide labels waits for the packed ones
+            // to drain: nothing else is admitted before it
+            if (r->wide) break;
             unfit.push_back(r);
             continue;
           }
@@ -407,6 +419,14 @@ void worker_loop(mrf_server* S) {
         r->group = -1;
         if (!pc) {
           complete(S, r, code, msg);
+          S->cv_done.notify_all();
+          continue;
+        }
+        if (pc->again) {
+          // the same pass again on wide labels, ahead of later arrivals
+          delete pc;
+          r->state = 0;
+          S->queue.push_front(r);
           S->cv_done.notify_all();
           continue;
         }

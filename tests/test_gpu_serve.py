@@ -201,3 +201,41 @@ def test_small_requests_go_ahead_of_a_large_one():
             assert st["admitted_ahead"] == 0
     ref_big = oracle_paths(big, big_nets[:24:4])
     assert [got_big[0][0][i] for i in range(0, 24, 4)] == ref_big
+
+
+def test_packed_certificate_falls_back_to_wide_labels(boards):
+    """Requests run on packed u32 labels where their boards allow.  A board
+    whose distances pass the 16-bit packed limit (gap prices of 700) fails
+    the certificate: its request runs again on u64 labels, after the packed
+    requests beside it drain.  Every result equals the oracle's."""
+    w, h, L = 96, 80, 2
+    rng = np.random.default_rng(31)
+    cost = np.ones(w * h * L, dtype=np.uint32)
+    cost[rng.random(w * h * L) < 0.15] = 0xFFFFFFFF
+    free = np.flatnonzero(cost != 0xFFFFFFFF)
+    pairs = rng.choice(free, size=(8, 2), replace=False)
+    nets = [mrfield.Net(int(a), int(b)) for a, b in pairs]
+    full = [(0, 0, w - 1, h - 1)] * len(nets)
+    og = mrcpu.Grid(w, h, L, cost)
+    oe = mrcpu.Edges([700] * (w - 1), [700] * (h - 1), [2100])
+    ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(og, oe), [mrcpu.Net(n.src, n.dst) for n in nets], full,
+                                     retry_full=True)
+    assert max(c for c in ref_c if c != 0xFFFFFFFF) > 0xFFFF  # the case is past the packed limit
+    high = [None if p is None else (p, c) for p, c in zip(ref_p, ref_c)]
+    small = boards[0]
+    srv = mrfield.Server(0, label_bytes=1 << 28)
+    try:
+        hid = srv.add_board(mrfield.Grid(w, h, L, cost), mrfield.Edges([700] * (w - 1), [700] * (h - 1), [2100]))
+        sid = srv.add_board(small.grid, small.edges, small.pricing())
+        a = small.nets()[:8]
+        b = small.nets()[8:16]
+        r1 = srv.submit(sid, a)
+        r2 = srv.submit(hid, nets)
+        r3 = srv.submit(sid, b)
+        got1, got2, got3 = srv.wait(r1)[0], srv.wait(r2)[0], srv.wait(r3)[0]
+        assert got2 == high
+        assert got1 == oracle_paths(small, a)
+        assert got3 == oracle_paths(small, b)
+        assert srv.stats()["failed"] == 0
+    finally:
+        srv.close()
