@@ -76,7 +76,11 @@ struct FieldDesc {
   // prices [n].
   uint32_t ovr_off, n_ovr;
   int32_t own_group;  // legal searches: the net's group
-  uint32_t pad_;
+  // Own pads and via-exempt pads by tile (the wave form), in the pool at
+  // xtab_off when the field has either: own-pad starts [tx*ty+1], via-pad
+  // starts [tx*ty+1], then the entries of each kind ordered by tile, as the
+  // cell's index in its tile block ((layer * TH + y) * TW + x).
+  uint32_t xtab_off;
 };
 
 struct TileShape {
@@ -100,6 +104,9 @@ struct DevBoard {
   uint32_t* ye = nullptr;
   uint32_t* ve = nullptr;
   std::vector<uint8_t> via_allowed;  // host copy
+  // A packed-label solve on this board failed its certificate: later solves
+  // on it go straight to wide labels.
+  mutable bool pack_failed = false;
   // Host copy of the obstacle bits (cost == kMax), one bit per cell, built on
   // the device with the masks.
   std::vector<uint64_t> obst_bits;
@@ -390,6 +397,7 @@ struct Group {
   bool lean = false;    // no halo-owner / legal planes, overrides, pads or via pads
   bool wave = false;    // the wave form covers every field (no weight plane, legal planes, overrides, pads)
   bool halo = false;    // some board of its fields has a halo-owner plane
+  bool ex = false;      // some field has own pads or via-exempt pads
   // Packed u32 labels (the wave form on boards whose every open step costs
   // at least 1 and below 0xFFFF); live groups all share one label width.
   bool pack = false;

@@ -2030,24 +2030,28 @@ struct KernelSet {
   const void* round;
   const void* list;
   const void* queue;
-  const void* wave;     // relax_wave (u64 labels) for this shape and layer bound (narrow sets only), or nullptr
-  const void* wave_pk;  // relax_wave with packed u32 labels, or nullptr
-  const void* wave_ho;  // relax_wave (u64 labels) for launches with halo-owner planes, or nullptr
+  // relax_wave for this shape and layer bound (narrow sets only, else
+  // nullptr), by wave_index(packed labels, halo-owner planes, per-net
+  // extras); packed labels never run with halo-owner planes
+  const void* wave[8];
 };
+
+static constexpr int wave_index(bool pk, bool ho, bool ex) { return (pk ? 4 : 0) + (ho ? 2 : 0) + (ex ? 1 : 0); }
 
 template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
-  const void* wave = nullptr;
-  const void* wave_pk = nullptr;
-  const void* wave_ho = nullptr;
+  KernelSet k{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
+              (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
+              {}};
   if constexpr (NARROW && TW == 16 && TH == 16 && LM <= 8) {
-    wave = (const void*)relax_wave<TW, TH, LM, false, false>;
-    wave_pk = (const void*)relax_wave<TW, TH, LM, true, false>;
-    wave_ho = (const void*)relax_wave<TW, TH, LM, false, true>;
+    k.wave[wave_index(false, false, false)] = (const void*)relax_wave<TW, TH, LM, false, false, false>;
+    k.wave[wave_index(false, false, true)] = (const void*)relax_wave<TW, TH, LM, false, false, true>;
+    k.wave[wave_index(false, true, false)] = (const void*)relax_wave<TW, TH, LM, false, true, false>;
+    k.wave[wave_index(false, true, true)] = (const void*)relax_wave<TW, TH, LM, false, true, true>;
+    k.wave[wave_index(true, false, false)] = (const void*)relax_wave<TW, TH, LM, true, false, false>;
+    k.wave[wave_index(true, false, true)] = (const void*)relax_wave<TW, TH, LM, true, false, true>;
   }
-  return KernelSet{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
-                   (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
-                   wave, wave_pk, wave_ho};
+  return k;
 }
 
 // Wide step planes for every layer bound; narrow ones, full and lean, up to
@@ -2117,7 +2121,8 @@ static bool wave_shape(const TileShape& S) { return S.threads == 64 && S.tw == 1
 static uint32_t workgroup_threads(const TileShape& S) { return wave_shape(S) ? 256u : S.threads; }
 // `pack`: the batch may use packed u32 labels (every open step of its boards
 // costs at least 1); the wave form then runs them.  `halo`: some board of the
-// launch has a halo-owner plane (never packed).
+// launch has a halo-owner plane (never packed).  `ex`: some field carries own
+// pads, via-exempt pads or additive overrides.
 struct RoundLaunch {
   const void* fn;
   uint32_t threads;
@@ -2125,12 +2130,12 @@ struct RoundLaunch {
   bool wave, packed;
 };
 static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool wave_ok, bool keyed,
-                                bool pack, bool halo) {
-  if (wave_shape(S) && K.wave && K.narrow && wave_ok && !keyed && Lmax <= K.lm) {
+                                bool pack, bool halo, bool ex) {
+  if (halo) pack = false;
+  const void* w = K.wave[wave_index(pack, halo, ex)];
+  if (wave_shape(S) && w && K.narrow && wave_ok && !keyed && Lmax <= K.lm) {
     const uint32_t nt = 64u * wave_count(K.lm, S.th);
-    if (halo) return RoundLaunch{K.wave_ho, nt, wave_lds_bytes(S.tw, S.th, K.lm, false), true, false};
-    if (pack) return RoundLaunch{K.wave_pk, nt, wave_lds_bytes(S.tw, S.th, K.lm, true), true, true};
-    return RoundLaunch{K.wave, nt, wave_lds_bytes(S.tw, S.th, K.lm, false), true, false};
+    return RoundLaunch{w, nt, wave_lds_bytes(S.tw, S.th, K.lm, pack, ex), true, pack};
   }
   return RoundLaunch{K.round, workgroup_threads(S), lds_bytes(S.tw, S.th, Lmax, K.narrow), false, false};
 }
@@ -2142,10 +2147,45 @@ static bool field_lean(const BoardView& v, const FieldReq& q) {
   return !v.halo && !v.owner && !v.lhalo && q.ovr_cells.empty() && q.pads.empty() && q.via_pads.empty();
 }
 
-// Whether the wave form covers a field: lean, or lean but for a halo-owner
-// plane; and its board has no weight plane.
-static bool field_wave(const BoardView& v, const FieldReq& q) {
-  return !v.cost && !v.owner && !v.lhalo && q.ovr_cells.empty() && q.pads.empty() && q.via_pads.empty();
+// Whether the wave form covers a field: no legal-search planes, and its
+// board has no weight plane (halo-owner planes, own pads, via-exempt pads and
+// overrides it applies per tile).
+static bool field_wave(const BoardView& v, const FieldReq& q) { return !v.cost && !v.owner && !v.lhalo; }
+
+// Whether a field carries what the wave form applies per tile: own pads,
+// via-exempt pads, additive overrides.
+static bool field_extras(const FieldReq& q) {
+  return !q.pads.empty() || !q.via_pads.empty() || !q.ovr_cells.empty();
+}
+
+// Own pads and via-exempt pads of a field by tile, for the wave form:
+// own-pad starts per tile [tx*ty+1], via-pad starts [tx*ty+1], then each
+// kind's entries ordered by tile, as the cell's index in its tile block
+// ((layer * TH + y) * TW + x).  Cells outside the window are dropped.
+static void build_pad_tables(const FieldReq& q, const DevBoard& b, const TileShape& S, uint32_t tx, uint32_t ty,
+                             std::vector<uint32_t>& seg) {
+  seg.clear();
+  if (q.pads.empty() && q.via_pads.empty()) return;
+  const uint32_t plane = b.w * b.h, nt = tx * ty;
+  std::vector<uint32_t> starts[2], ent[2];
+  const std::vector<uint32_t>* src[2] = {&q.pads, &q.via_pads};
+  for (int kind = 0; kind < 2; ++kind) {
+    std::vector<std::pair<uint32_t, uint32_t>> v;  // (tile, index in the tile block)
+    for (uint32_t g : *src[kind]) {
+      if ((size_t)g >= (size_t)plane * b.layers) continue;
+      const uint32_t l = g / plane, r = g % plane, gy = r / b.w, gx = r % b.w;
+      if (gx < q.x0 || gx > q.x1 || gy < q.y0 || gy > q.y1) continue;
+      const uint32_t wx = gx - q.x0, wy = gy - q.y0;
+      v.push_back({(wy / S.th) * tx + wx / S.tw, (l * S.th + wy % S.th) * S.tw + wx % S.tw});
+    }
+    std::sort(v.begin(), v.end());
+    starts[kind].assign(nt + 1, 0);
+    for (const auto& e : v) starts[kind][e.first + 1]++;
+    for (uint32_t t = 0; t < nt; ++t) starts[kind][t + 1] += starts[kind][t];
+    for (const auto& e : v) ent[kind].push_back(e.second);
+  }
+  for (int kind = 0; kind < 2; ++kind) seg.insert(seg.end(), starts[kind].begin(), starts[kind].end());
+  for (int kind = 0; kind < 2; ++kind) seg.insert(seg.end(), ent[kind].begin(), ent[kind].end());
 }
 
 void ctx_init(Ctx& c, int device) {
@@ -2177,9 +2217,8 @@ void ctx_init(Ctx& c, int device) {
     HIP_OK(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.list, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     HIP_OK(hipFuncSetAttribute(k.queue, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
-    if (k.wave) HIP_OK(hipFuncSetAttribute(k.wave, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
-    if (k.wave_pk) HIP_OK(hipFuncSetAttribute(k.wave_pk, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
-    if (k.wave_ho) HIP_OK(hipFuncSetAttribute(k.wave_ho, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    for (const void* w : k.wave)
+      if (w) HIP_OK(hipFuncSetAttribute(w, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
   }
 }
 
@@ -2560,13 +2599,15 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  bool narrow = true, lean = true, wave_ok = true, halo = false;
+  bool narrow = true, lean = true, wave_ok = true, halo = false, ex = false, ovr = false;
   for (const FieldReq& q : reqs) {
     if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
     narrow = narrow && field_narrow(*boards[q.board], q);
     lean = lean && field_lean(c.views[q.board], q);
     wave_ok = wave_ok && field_wave(c.views[q.board], q);
     halo = halo || c.views[q.board].halo != nullptr;
+    ex = ex || field_extras(q);
+    ovr = ovr || !q.ovr_cells.empty();
   }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
   size_t lds = lds_bytes(S.tw, S.th, Lmax, K.narrow);
@@ -2613,6 +2654,10 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     F.ovr_off = (uint32_t)pool.size();
     F.n_ovr = ovr_n[i];
     pool.insert(pool.end(), ovr_seg[i].begin(), ovr_seg[i].end());
+    std::vector<uint32_t> xtab;
+    build_pad_tables(q, b, S, F.tx, F.ty, xtab);
+    F.xtab_off = (uint32_t)pool.size();
+    pool.insert(pool.end(), xtab.begin(), xtab.end());
     // seed tiles: tiles holding a source inside the window
     std::vector<uint32_t> st_items;
     for (uint32_t s : q.sources) {
@@ -2680,14 +2725,17 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   // least 1 (hops <= dist then) and the wave form runs; checked after the
   // solve, re-solved in wide labels when a label came within a step of the
   // packed dist limit.
+  // With overrides a step costs at least the edge price times the weight
+  // (the net's own prices may undercut the plane's) and at most the narrow
+  // limit (field_narrow checked the override prices).
   bool pack = !c.no_pack && c.mode == MODE_PLAIN && !halo;
-  uint64_t pk_step = 0;
+  uint64_t pk_step = ovr ? kNarrowMaxStep : 0;
   for (const DevBoard* b : boards) {
-    pack = pack && b->min_step >= 1 && b->max_step < 0xFFFFu;
+    pack = pack && (ovr ? b->min_ew : b->min_step) >= 1 && b->max_step < 0xFFFFu && !b->pack_failed;
     pk_step = std::max<uint64_t>(pk_step, b->max_step);
   }
   RelaxArgs A = make_args(c, item);
-  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, c.mode == MODE_PLAIN && c.delta != kMax, pack, halo);
+  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, c.mode == MODE_PLAIN && c.delta != kMax, pack, halo, ex);
   if (RL.packed) {
     take(c, W.fmax, W.fmax_cap, std::max<size_t>(item, 1));
     check_arena(c, {{W.fmax, std::max<size_t>(item, 1) * 4}});
@@ -2895,7 +2943,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   if (cnt[C_ERROR * kCtrStride]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR * kCtrStride]));
   if (RL.packed && (uint64_t)cnt[C_PMAX * kCtrStride] + ((pk_step << 16) | 1u) >= 0xFFFF0000ull) {
     // a path came within one step of the packed dist limit: the labels may
-    // be clipped, so the batch is solved again in wide labels
+    // be clipped, so the batch is solved again in wide labels, and so are
+    // later batches on these boards
+    for (const DevBoard* b : boards) b->pack_failed = true;
     c.no_pack = true;
     try {
       solve_fields(c, boards, reqs, st);
@@ -3218,13 +3268,16 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid, b
     nitems += (uint64_t)((ww + S.tw - 1) / S.tw) * ((wh + S.th - 1) / S.th);
     words += field_label_words(S, ww, wh, d.views[q.board].layers);
     npool += q.sources.size() + q.forced.size() + q.pads.size() + q.via_pads.size();
+    if (!q.pads.empty() || !q.via_pads.empty())  // the pad tables of build_pad_tables
+      npool += 2ull * (((ww + S.tw - 1) / S.tw) * ((wh + S.th - 1) / S.th) + 1) + q.pads.size() + q.via_pads.size();
     const DevBoard& b = *d.devs[q.board];
     G.layers = std::max(G.layers, d.views[q.board].layers);
     G.narrow = G.narrow && field_narrow(b, q);
     G.lean = G.lean && field_lean(d.views[q.board], q);
     G.wave = G.wave && field_wave(d.views[q.board], q);
     G.halo = G.halo || d.views[q.board].halo != nullptr;
-    steps_ok = steps_ok && b.min_step >= 1 && b.max_step < 0xFFFFu;
+    G.ex = G.ex || field_extras(q);
+    steps_ok = steps_ok && b.min_step >= 1 && b.max_step < 0xFFFFu && !b.pack_failed;
     G.pk_step = std::max<uint64_t>(G.pk_step, b.max_step);
   }
   // packed labels only where the wave form runs them: the wave shape, a
@@ -3283,6 +3336,10 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid, b
     put(q.forced, F.forced_off, F.n_forced);
     put(q.pads, F.pad_off, F.n_pad);
     put(q.via_pads, F.vpad_off, F.n_vpad);
+    std::vector<uint32_t> xtab;
+    build_pad_tables(q, *d.devs[q.board], S, F.tx, F.ty, xtab);
+    uint32_t n_xtab = 0;
+    put(xtab, F.xtab_off, n_xtab);
     F.dst = q.dst;
     F.own_id = q.own_id;
     F.own_group = q.own_group;
@@ -3383,7 +3440,7 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
-  bool narrow = true, lean = true, wave_ok = true, halo = false, pack = false;
+  bool narrow = true, lean = true, wave_ok = true, halo = false, pack = false, ex = false;
   for (uint32_t g = 0; g < d.max_groups; ++g)
     if (d.groups[g].live) {
       Lmax = std::max(Lmax, d.groups[g].layers);
@@ -3391,10 +3448,11 @@ uint32_t driver_step(RoundDriver& d) {
       lean = lean && d.groups[g].lean;
       wave_ok = wave_ok && d.groups[g].wave;
       halo = halo || d.groups[g].halo;
+      ex = ex || d.groups[g].ex;
       pack = d.groups[g].pack;  // the same for every live group
     }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
-  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, pack, halo);
+  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, pack, halo, ex);
   if (RL.packed != pack) throw Error(1, "packed groups without the packed kernel");
   A.item_max = d.imax;
   void* args[] = {&A};

@@ -46,9 +46,9 @@
 //     0xFFFF << 16 (then no path to any cell reaches dist 0xFFFF), and
 //     solves the batch again in the wide form otherwise.
 //
-// Scope: fields without legal planes, overrides, own pads or via-exempt
-// pads (a halo-owner plane is allowed: its penalty joins the additive price)
-// on boards without a weight plane whose step prices fit the narrow (15-bit)
+// Scope: fields without legal planes (halo-owner planes, own pads,
+// via-exempt pads and additive overrides are applied per tile: HO, EX) on
+// boards without a weight plane whose step prices fit the narrow (15-bit)
 // form, in the round form without delta stepping.  A forced endpoint on an
 // obstacle makes a field wide, so every passable cell here is a cell that is
 // neither a trace node nor an obstacle and every open step price is e + a.
@@ -76,10 +76,14 @@ struct WaveTile {
   static constexpr uint32_t BORDER = 2 * LM * TH + 2 * LM * TW;
   // then 16 words for the wavefronts' flags and maxima (two waves)
   static constexpr size_t lds = (size_t)(LAB + BORDER) * (PK ? 4 : 8) + 64;
+  // per-net extras (own pads, via-exempt pads, additive overrides): the
+  // tile's raw via flags, four to a word, and two bitmaps of its cells
+  static constexpr uint32_t EXW = LM * PT / 4 + 2 * (LM * PT / 32);
 };
 
-__host__ __device__ inline size_t wave_lds_bytes(uint32_t tw, uint32_t th, uint32_t lm, bool pk) {
-  return (64ull * wave_count(lm, th) * (tw + 1) + 2ull * lm * th + 2ull * lm * tw) * (pk ? 4 : 8) + 64;
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t tw, uint32_t th, uint32_t lm, bool pk, bool ex) {
+  return (64ull * wave_count(lm, th) * (tw + 1) + 2ull * lm * th + 2ull * lm * tw) * (pk ? 4 : 8) + 64 +
+         (ex ? 4ull * (lm * tw * th / 4 + 2 * (lm * tw * th / 32)) : 0);
 }
 
 // Whether x holds in any thread of a visit's NW wavefronts; every wave has
@@ -187,8 +191,11 @@ __device__ __forceinline__ uint32_t add_k(const BoardView& B, uint32_t g) {
 
 // HO: the board has a halo-owner plane; a cell whose owner is another net
 // costs the board's halo penalty on top of its additive price (add_price).
-template <int TW, int TH, int LM, int AK, bool PK, bool HO>
-__device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched, const FieldDesc& F,
+// EX: the field may carry own pads (obstacle cells passable at weight 1),
+// via-exempt pads (via_forbidden lifted) and additive overrides (the net's
+// own price of a cell), applied per tile from the host's tables.
+template <int TW, int TH, int LM, int AK, bool PK, bool HO, bool EX>
+__device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched, const FieldDesc& F,
                                 const BoardView& B, char* smem, uint32_t round) {
   using WT = WaveTile<TW, TH, LM, PK>;
   using WL = WaveLab<PK>;
@@ -276,6 +283,73 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
       hom |= (o != -1 && o != own) ? 1u << x : 0u;
     }
   }
+  // ---- per-net extras of the tile ------------------------------------------------
+  // Own-pad cells and override cells of each row as bit masks (padm, ovm),
+  // override prices staged at the row's slots of the label buffer, and the
+  // row's via bits made again where via-exempt pads lift via_forbidden.
+  // (before the tile's labels are loaded: their registers are not held
+  // through this pass's barriers)
+  uint32_t padm = 0, ovm = 0, v0 = 0, v1 = 0;
+  bool tile_pads = false;
+  uint32_t* raww = (uint32_t*)(smem + WT::lds);  // [LM*PT/4]
+  uint32_t* padb = raww + LM * PT / 4;           // [LM*PT/32]
+  if constexpr (EX) {
+    uint32_t* ovb = padb + LM * PT / 32;           // [LM*PT/32]
+    const uint32_t ntile = F.tx * F.ty;
+    const uint32_t* P = A.pool + F.xtab_off;
+    uint32_t p0 = 0, p1 = 0, o0 = 0, o1 = 0;
+    if (F.n_pad | F.n_vpad) {
+      p0 = const_load(P + t);
+      p1 = const_load(P + t + 1);
+      v0 = const_load(P + ntile + 1 + t);
+      v1 = const_load(P + ntile + 2 + t);
+    }
+    const uint32_t* OT = A.pool + F.ovr_off + 2 * F.n_ovr;  // per-tile starts, cells, prices
+    if (F.n_ovr) {
+      o0 = const_load(OT + t);
+      o1 = const_load(OT + t + 1);
+    }
+    tile_pads = p1 > p0;
+    if (p1 > p0 || v1 > v0 || o1 > o0) {
+      for (uint32_t i = tid; i < 2 * (LM * PT / 32); i += NT) padb[i] = 0;
+      if (v1 > v0 && rrow) {
+#pragma unroll
+        for (uint32_t q = 0; q < TW / 4; ++q) {
+          uint32_t wd = 0;
+#pragma unroll
+          for (uint32_t b = 0; b < 4; ++b) wd |= (uint32_t)gbl(B.raw)[rowg + min(wx0 + 4 * q + b, F.ww - 1)] << (8 * b);
+          raww[(rl * TH + ry) * (TW / 4) + q] = wd;
+        }
+      }
+      __syncthreads();
+      const uint32_t* PE = P + 2 * (ntile + 1);
+      for (uint32_t k = p0 + tid; k < p1; k += NT) {
+        const uint32_t loc = PE[k];
+        atomicOr(&padb[loc >> 5], 1u << (loc & 31));
+      }
+      const uint32_t* VE = PE + const_load(P + ntile);
+      for (uint32_t k = v0 + tid; k < v1; k += NT) {
+        const uint32_t loc = VE[k];
+        atomicAnd(&raww[loc >> 2], ~((uint32_t)R_VF << (8 * (loc & 3))));
+      }
+      const uint32_t* OC = OT + ntile + 1;
+      const uint32_t* OA = OC + F.n_ovr;
+      for (uint32_t k = o0 + tid; k < o1; k += NT) {
+        const uint32_t g = OC[k];
+        const uint32_t l = g / B.plane, r = g - l * B.plane, gy = r / B.w, gx = r - gy * B.w;
+        const uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0, loc = (l * TH + y) * TW + x;
+        lab32[(l * TH + y) * RS + x] = OA[k];
+        atomicOr(&ovb[loc >> 5], 1u << (loc & 31));
+      }
+      __syncthreads();
+      const uint32_t rloc = (rl * TH + ry) * TW;  // a multiple of 16
+      if (rrow) {
+        padm = (padb[rloc >> 5] >> (rloc & 31)) & 0xFFFFu;
+        ovm = (ovb[rloc >> 5] >> (rloc & 31)) & 0xFFFFu;
+      }
+    }
+  }
+
   // column layout: thread (cl, cx) holds column cx of layer cl
   const uint32_t cl = tid / TW, cx = tid - cl * TW;
   const bool ccol = cl < L;
@@ -303,6 +377,16 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
         if (o != -1 && o != F.own_id) ha[k] += B.halo_pen;
       }
       hbm[k] = e[k] ? m : (uint8_t)M_TRACE;
+    }
+    if constexpr (EX) {
+      // The halo prices only decide activations, so a lower bound is
+      // enough: with own pads any obstacle may be one (passable), with
+      // overrides any additive price may be lower (0).
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (F.n_pad) hbm[k] &= (uint8_t)~M_OBST;
+        if (F.n_ovr) ha[k] = 0;
+      }
     }
   }
   // gap prices: x gaps uniform (lane i <= TW holds the price of entering
@@ -377,6 +461,31 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
   // formed here, so the halo cells' planes are not kept through the sweeps
   asm("" : "+v"(p_lr), "+v"(p_tb));
 
+  // via bits of rows holding via-exempt pads, made again without the
+  // exempt cells' via_forbidden flags (lifted in the LDS copy above)
+  if constexpr (EX) {
+    if (v1 > v0 && rrow) {
+      // a via between layers l and l+1 at (x, y) is open when the pair
+      // allows vias and neither end is a via node or via-forbidden
+#pragma unroll
+      for (uint32_t q = 0; q < TW / 4; ++q) {
+        const uint32_t cur = raww[(rl * TH + ry) * (TW / 4) + q];
+        const uint32_t dn = rl > 0 ? raww[((rl - 1) * TH + ry) * (TW / 4) + q] : 0x03030303u;
+        const uint32_t up = rl + 1 < L ? raww[((rl + 1) * TH + ry) * (TW / 4) + q] : 0x03030303u;
+        uint32_t wd = mw[q];
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+          const bool okc = ((cur >> (8 * b)) & 3u) == 0;
+          const bool okd = rl > 0 && ((B.va >> (rl - 1)) & 1u) && okc && ((dn >> (8 * b)) & 3u) == 0;
+          const bool oku = rl + 1 < L && ((B.va >> rl) & 1u) && okc && ((up >> (8 * b)) & 3u) == 0;
+          wd = (wd & ~((uint32_t)(M_DN | M_UP) << (8 * b))) |
+               ((uint32_t)((okd ? M_DN : 0) | (oku ? M_UP : 0)) << (8 * b));
+        }
+        mw[q] = wd;
+      }
+    }
+  }
+
   // ---- step prices of every cell (row layout) ---------------------------------
   uint32_t sx[TW];  // row layout: from x-1 | from x+1
   uint32_t sv_st[TW];  // via prices of the row, staged after the y prices
@@ -386,9 +495,15 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
     for (int i = 0; i <= TW; ++i) xg[i] = __builtin_amdgcn_readlane(xgv, i);
 #pragma unroll
     for (int x = 0; x < TW; ++x) {
-      const uint8_t mb = (uint8_t)(mw[x / 4] >> (8 * (x % 4)));
+      uint8_t mb = (uint8_t)(mw[x / 4] >> (8 * (x % 4)));
+      if constexpr (EX) {
+        if ((padm >> x) & 1u) mb &= (uint8_t)~M_OBST;  // an own pad: passable, weight 1
+      }
       const uint8_t m = (row_in && wx0 + x < F.ww) ? mb : (uint8_t)M_TRACE;  // outside the window: closed
       uint32_t a = AK == ADD_U32 ? aw[x % AW] : AK == ADD_U16 ? (aw[(x / 2) % AW] >> (16 * (x % 2))) & 0xFFFFu : 0u;
+      if constexpr (EX) {
+        if ((ovm >> x) & 1u) a = lab32[tid * RS + x];  // the net's own price, staged above
+      }
       if constexpr (HO) a += ((hom >> x) & 1u) ? B.halo_pen : 0u;
       uint32_t px = pack_steps(wave_step(m, M_NX, true, xg[x], a), wave_step(m, M_PX, true, xg[x + 1], a));
       // keep the packed form: the sweeps read each half with a sign-extending
@@ -450,7 +565,13 @@ __device__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, b
       if (l >= L || gx < F.x0 + wx0 || gy < F.y0 + wy0) continue;
       const uint32_t x = gx - F.x0 - wx0, y = gy - F.y0 - wy0;
       if (x >= TW || y >= TH || wx0 + x >= F.ww || wy0 + y >= F.wh) continue;
-      if (!(gbl(B.bmask)[g] & (M_TRACE | M_OBST))) {
+      const uint8_t sm = gbl(B.bmask)[g];
+      bool pass = !(sm & (M_TRACE | M_OBST));
+      if constexpr (EX) {
+        const uint32_t loc = (l * TH + y) * TW + x;
+        pass = pass || (!(sm & M_TRACE) && tile_pads && ((padb[loc >> 5] >> (loc & 31)) & 1u));
+      }
+      if (pass) {
         lab[(l * TH + y) * RS + x] = LT(0);
         put = true;
       }
@@ -658,16 +779,17 @@ struct WaveMin {
   static constexpr int waves = PK ? 5 : 4;
 };
 
-// HO: the launch has boards with a halo-owner plane (a separate kernel, so
-// the others do not carry that code's registers).
-template <int TW, int TH, int LM, bool PK, bool HO, int AK>
+// HO: the launch has boards with a halo-owner plane; EX: fields with own
+// pads, via-exempt pads or overrides (separate kernels, so the others do not
+// carry that code's registers).
+template <int TW, int TH, int LM, bool PK, bool HO, bool EX, int AK>
 __device__ __forceinline__ TileResult wave_visit(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched,
                                                  const FieldDesc& F, const BoardView& B, char* smem) {
-  if (HO && B.halo) return wave_tile<TW, TH, LM, AK, PK, HO>(A, item, f, touched, F, B, smem, A.round);
-  return wave_tile<TW, TH, LM, AK, PK, false>(A, item, f, touched, F, B, smem, A.round);
+  if (HO && B.halo) return wave_tile<TW, TH, LM, AK, PK, HO, EX>(A, item, f, touched, F, B, smem, A.round);
+  return wave_tile<TW, TH, LM, AK, PK, false, EX>(A, item, f, touched, F, B, smem, A.round);
 }
 
-template <int TW, int TH, int LM, bool PK, bool HO>
+template <int TW, int TH, int LM, bool PK, bool HO, bool EX>
 __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>::waves)) relax_wave(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t item = round_item(A);
@@ -678,11 +800,11 @@ __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>:
   const BoardView B = const_load(A.boards + F.board);
   TileResult res;
   if (B.add16)
-    res = wave_visit<TW, TH, LM, PK, HO, ADD_U16>(A, item, f, touched, F, B, smem);
+    res = wave_visit<TW, TH, LM, PK, HO, EX, ADD_U16>(A, item, f, touched, F, B, smem);
   else if (B.add)
-    res = wave_visit<TW, TH, LM, PK, HO, ADD_U32>(A, item, f, touched, F, B, smem);
+    res = wave_visit<TW, TH, LM, PK, HO, EX, ADD_U32>(A, item, f, touched, F, B, smem);
   else
-    res = wave_visit<TW, TH, LM, PK, HO, ADD_NONE>(A, item, f, touched, F, B, smem);
+    res = wave_visit<TW, TH, LM, PK, HO, EX, ADD_NONE>(A, item, f, touched, F, B, smem);
   if (threadIdx.x >= 64) return;  // the first wave finishes the visit
   round_tail(A, item, grp, res, false, 0u);
 }

@@ -105,6 +105,10 @@ CASES = [
     (80, 3, 20, 33, False, True, False, 9, (8, 4, 64, 32)),
     (64, 2, 16, 34, True, True, True, 60, (16, 16, 256, 32)),
     (72, 4, 12, 35, False, False, True, 2, (32, 16, 256, 32)),
+    # the wave form (one wavefront per tile; overrides applied per tile)
+    (64, 2, 24, 31, False, False, False, 3, (16, 16, 64, 32)),
+    (96, 4, 16, 32, True, True, False, 5, (16, 16, 64, 32)),
+    (80, 6, 20, 33, False, True, False, 9, (16, 16, 64, 32)),
 ]
 
 
@@ -201,8 +205,9 @@ def test_negotiated_batch_rejects_bad_planes():
         eng.close()
 
 
+@pytest.mark.parametrize("threads", [256, 64], ids=["workgroup", "wave"])
 @pytest.mark.parametrize("side,layers,n,seed,iters", [(64, 4, 24, 11, 60), (40, 1, 40, 3, 6), (24, 2, 12, 5, 10)])
-def test_negotiation_loop_matches_oracle_loop(side, layers, n, seed, iters):
+def test_negotiation_loop_matches_oracle_loop(side, layers, n, seed, iters, threads):
     """The whole loop (negotiate.py) with device batches equals the loop with
     the oracle's searches, iteration by iteration: Jacobi with term-by-term
     snapshots (converging), Jacobi with fused snapshots, and Gauss-Seidel."""
@@ -215,7 +220,7 @@ def test_negotiation_loop_matches_oracle_loop(side, layers, n, seed, iters):
     wins = [ng.net_window(g.w, g.h, x.src, x.dst, x.pads) for x in nets]
     eng = mrfield.Engine(0)
     try:
-        eng.set_tiles(16, 16, 256, 32)
+        eng.set_tiles(16, 16, threads, 32)
         board = mrfield.Board(eng, g, s.edges, None)
         try:
             gpu = ng.negotiate(ng.board_router(board, nets, wins), g.n, len(nets), max_iters=iters, keep_trace=True)

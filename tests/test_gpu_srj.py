@@ -26,10 +26,13 @@ FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "srj", "*.json")))
 VIA = 160
 
 
-@pytest.fixture(scope="module")
-def eng():
+# the workgroup form, and the wave form (own pads and via-exempt pads applied
+# per tile)
+@pytest.fixture(scope="module", params=[256, 64], ids=["workgroup", "wave"])
+def eng(request):
     e = mrfield.Engine(0)
-    e.set_tiles(16, 16, 256, 32)
+    e.set_tiles(16, 16, request.param, 32)
+    e.form = "wave" if request.param == 64 else "workgroup"
     yield e
     e.close()
 
@@ -54,8 +57,11 @@ def test_srj_board_routes_match_oracle(eng, path, mode):
     try:
         got = eng.route_isolated_batch(p.grid(), [n.net() for n in p.nets], wins,
                                        mrfield.Edges(list(oe.x), list(oe.y), list(oe.via)))
+        st = eng.stats()
     finally:
         eng.set_mode(mrfield.MODE_PLAIN)
+    if eng.form == "wave" and mode == mrfield.MODE_PLAIN:
+        assert st["wave_launches"] == st["relax_launches"] > 0, st  # unit weights: the wave form runs
     routed = 0
     for i, r in enumerate(got):
         assert (r is None) == (ref_p[i] is None), (os.path.basename(path), p.nets[i].name)

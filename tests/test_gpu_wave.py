@@ -147,6 +147,59 @@ def test_wave_halo_owner_fields(eng, L, seed):
         assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
 
 
+@pytest.mark.parametrize("L", [2, 4, 6])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_wave_pads_and_via_pads(eng, L, seed):
+    """Endpoints on obstacle cells made passable by the net's own pads (a
+    through-hole footprint: the endpoint's cell and its planar neighbours on
+    every layer), every footprint cell also via-exempt (via_forbidden lifted
+    for this net only); through the provider seam, window then full-board
+    retry.  The wave form applies pads per tile and prices the halo with
+    lower bounds (a border next to a pad only activates more); paths and
+    search costs equal the oracle's."""
+    rng = np.random.default_rng(700 + 10 * L + seed)
+    w, h = int(rng.integers(30, 60)), int(rng.integers(30, 60))
+    g = unit_board(seed + 60, w, h, L)
+    plane = w * h
+    edges = mrcpu.Edges([int(v) for v in rng.integers(1, 25, size=w - 1)],
+                        [int(v) for v in rng.integers(1, 25, size=h - 1)],
+                        [int(rng.integers(1, 40)) for _ in range(L - 1)])
+    free = free_cells(g)
+    blocked = np.flatnonzero((g.cost == MAX) & ((g.board & 1) == 0))
+
+    def footprint(c):
+        x, y = c % w, (c % plane) // w
+        out = set()
+        for l in range(L):
+            for dx, dy in ((0, 0), (1, 0), (-1, 0), (0, 1), (0, -1)):
+                if 0 <= x + dx < w and 0 <= y + dy < h:
+                    out.add(l * plane + (y + dy) * w + x + dx)
+        return out
+
+    nets = []
+    for i in range(10):
+        src = int(rng.choice(free))
+        dst = int(rng.choice(blocked)) if i % 2 == 0 else int(rng.choice(free))
+        pads = sorted(footprint(src) | footprint(dst)) if i % 3 != 2 else []
+        vpads = pads if i % 4 != 1 else pads[: len(pads) // 2]
+        if i % 2 == 0 and not pads:
+            dst = int(rng.choice(free))
+        nets.append(mrcpu.Net(src, dst, pads, vpads))
+    og = g
+    wins = [mrcpu.net_window(og, n.src, n.dst, n.pads) for n in nets]
+    ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(og, edges), nets, wins, retry_full=True)
+    got = eng.route_isolated_batch(as_dev(g), [mrfield.Net(n.src, n.dst, n.pads, n.via_pads) for n in nets], wins,
+                                   mrfield.Edges(list(edges.x), list(edges.y), list(edges.via)))
+    assert_wave_ran(eng, 4)
+    routed = 0
+    for i, r in enumerate(got):
+        assert (r is None) == (ref_p[i] is None), i
+        if r is not None:
+            routed += 1
+            assert r[0] == ref_p[i] and r[1] == ref_c[i], i
+    assert routed >= 5
+
+
 @pytest.mark.parametrize("seed", [4, 5])
 def test_wave_random_paths(eng, seed):
     """Two-pin nets through the provider seam: window pass with full-board
