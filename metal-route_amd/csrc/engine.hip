@@ -32,6 +32,14 @@
 //   relax_list        the same visits as a grid-stride loop over a device
 //                     work list; rounds captured in a hipGraph with
 //                     advance_round between them (graph form).
+//   relax_queue       persistent workgroups over device work queues, one
+//                     launch per solve (queue form).
+//   relax_wave        (wavetile.h) one wavefront per 16 x 16 tile visit, two
+//                     for 5-8 layers, u64 or packed u32 labels: the round
+//                     kernel of the plain form and the serving loop where
+//                     the batch qualifies.
+//   board_scan / board_build
+//                     a board's masks and bounds from the caller's planes.
 //   path_lengths / walk_paths
 //                     canonical reconstruction: from dst, the lowest-CellIdx
 //                     predecessor p with label(p) + (step(p, c), 1) == label(c)
