@@ -141,7 +141,7 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
     mrf::reconstruct_paths(c, b, reqs.size(), routed, cost, off, cells, &st);
     if (c.inject == 1) throw Error(MRF_BACKEND_UNAVAILABLE, "injected backend failure");
     if (c.inject == 2 && !cells.empty()) cells[cells.size() / 2] ^= 1u;
-    // Re-validate on up to 8 host threads (the shared workers); any failure
+    // Re-validate on up to 16 host threads (the shared workers); any failure
     // fails the whole call.
     const auto tv0 = std::chrono::steady_clock::now();
     std::vector<uint8_t> bad(k1 - k0, 0);
@@ -161,7 +161,7 @@ void solve_nets(mrf::Ctx& c, const mrf::DevBoard& b, const HostGrid& g, const mr
         out.paths[k].assign(p, p + len);
       }
     };
-    mrf::host::parallel_for(k1 - k0, 32, [&](size_t a, size_t b) { check(k0 + a, k0 + b); });
+    mrf::host::parallel_for(k1 - k0, 4, [&](size_t a, size_t b) { check(k0 + a, k0 + b); });
     c.last.validate_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tv0).count();
     for (uint8_t v : bad)

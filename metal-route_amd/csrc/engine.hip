@@ -2560,7 +2560,7 @@ static void build_overrides(const FieldReq& q, const DevBoard& b, const TileShap
   for (uint32_t k : order) seg.push_back(adds[k]);
 }
 
-// Override tables of every field of a batch, on up to 8 host threads when the
+// Override tables of every field of a batch, on up to 16 host threads when the
 // batch carries many overrides (negotiation batches: a few hundred cells per
 // net); the first error of any field is rethrown here.
 static void build_all_overrides(const std::vector<const DevBoard*>& boards, const std::vector<FieldReq>& reqs,

@@ -1,8 +1,8 @@
 // Host worker threads shared by the library's parallel host loops (path
 // re-validation, negotiation planes and overrides, per-field override
-// tables).  Creating and joining eight threads per call costs 60-120 us on
-// the GPU hosts, several times per negotiation iteration; the workers here
-// are created once and parked on a condition variable between jobs.
+// tables).  Creating and joining threads per call costs 60-120 us on the GPU
+// hosts, several times per negotiation iteration; the workers here are
+// created once and parked on a condition variable between jobs.
 #pragma once
 
 #include <algorithm>
@@ -19,6 +19,7 @@
 #include <vector>
 
 #include <pthread.h>
+#include <sched.h>
 
 namespace mrf {
 namespace host {
@@ -26,9 +27,11 @@ namespace host {
 class HostPool {
  public:
   // body(begin, end) over [0, n) in chunks of at least min_chunk, on the
-  // calling thread and up to max_threads - 1 workers.  One job runs at a time
-  // (concurrent callers wait their turn); a call from inside a job runs
-  // inline.  The first exception of any chunk is rethrown here.
+  // calling thread and up to max_threads - 1 workers; about four chunks per
+  // thread, claimed in turn, so uneven items (paths of different lengths)
+  // even out.  One job runs at a time (concurrent callers wait their turn); a
+  // call from inside a job runs inline.  The first exception of any chunk is
+  // rethrown here.
   void run(size_t n, size_t min_chunk, size_t max_threads, const std::function<void(size_t, size_t)>& body) {
     if (n == 0) return;
     const size_t want = std::min<size_t>(max_threads, (n + min_chunk - 1) / std::max<size_t>(min_chunk, 1));
@@ -40,7 +43,7 @@ class HostPool {
     auto job = std::make_shared<Job>();
     job->body = &body;
     job->n = n;
-    job->per = (n + want - 1) / want;
+    job->per = std::max<size_t>(std::max<size_t>(min_chunk, 1), (n + 4 * want - 1) / (4 * want));
     job->chunks = (n + job->per - 1) / job->per;
     job->pending = job->chunks;
     {
@@ -140,12 +143,22 @@ class HostPool {
 
 inline thread_local bool HostPool::in_job_ = false;
 
-// body(begin, end) over [0, n) on up to 8 host threads (the caller's and the
-// shared workers).
+// CPUs this process may run on (its affinity mask), read once.
+inline size_t usable_cpus() {
+  static const size_t n = [] {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return (size_t)std::max(1, CPU_COUNT(&set));
+    return (size_t)std::max(1u, std::thread::hardware_concurrency());
+  }();
+  return n;
+}
+
+// body(begin, end) over [0, n) on up to 16 host threads (the caller's and
+// the shared workers), no more than the process's CPUs.
 template <typename F>
 void parallel_for(size_t n, size_t min_chunk, F&& body) {
-  const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-  HostPool::get().run(n, min_chunk, std::min<size_t>(8, hw), std::function<void(size_t, size_t)>(body));
+  HostPool::get().run(n, min_chunk, std::min<size_t>(16, usable_cpus()), std::function<void(size_t, size_t)>(body));
 }
 
 }  // namespace host

@@ -203,14 +203,14 @@ PassCheck* collect_pass(mrf_server* S, ServeReq* r) {
 
 using mrf::host::parallel_for;
 
-// Host re-validation of one pass (path_ok, as mrf_solve does), on up to 8
+// Host re-validation of one pass (path_ok, as mrf_solve does), on up to 16
 // threads; stores the checked paths in the request's result.  Returns false
 // when any path fails.
 bool check_pass(PassCheck& pc) {
   ServeReq* r = pc.r;
   const HostGrid g = r->board->grid();
   std::vector<uint8_t> bad(pc.idx.size(), 0);
-  parallel_for(pc.idx.size(), 32, [&](size_t a, size_t b) {
+  parallel_for(pc.idx.size(), 4, [&](size_t a, size_t b) {
     for (size_t k = a; k < b; ++k) {
       if (!pc.routed[k]) continue;
       const size_t i = pc.idx[k];
