@@ -266,20 +266,23 @@ def cpu_baseline_fields(s, board):
     }
 
 
-def seam(s, steps=5):
+def seam(s, steps=7, warmup=3):
     """The provider seam as NegotiatedRouter calls it: mrf_route_isolated_batch
     on a fresh context (its default form), the board's grid, masks and edge
     prices passed per call (alone-path prices: the request carries no history
     plane), full-board windows.  Per call the library uploads the caller's
     planes, builds the directed-edge and via masks on the device, solves, and
     releases the board again; the paths come back as the ABI's flat arrays."""
+    import numpy as np
+
     import mrfield
 
     eng = mrfield.Engine(0)
     g = s.grid
     nets = s.nets()
-    wins = s.windows or [(0, 0, g.w - 1, g.h - 1)] * len(nets)
-    eng.route_isolated_batch_arrays(g, nets, wins, s.edges)
+    wins = np.asarray(s.windows or [(0, 0, g.w - 1, g.h - 1)] * len(nets), dtype=np.uint32).reshape(-1, 4)
+    for _ in range(warmup):  # a fresh context's first calls size its buffers
+        eng.route_isolated_batch_arrays(g, nets, wins, s.edges)
     ts = []
     for _ in range(steps):
         t0 = time.perf_counter()
@@ -289,7 +292,8 @@ def seam(s, steps=5):
     eng.close()
     ms = sorted(ts)[len(ts) // 2] * 1e3
     return {"entry": "mrf_route_isolated_batch", "value": round(len(nets) / (ms * 1e-3), 3), "unit": "nets/s",
-            "ms_per_call_p50": round(ms, 3), "calls": steps, "relax_kernel_ms": round(st["relax_kernel_ms"], 3),
+            "ms_per_call_p50": round(ms, 3), "calls": steps, "warmup_calls": warmup,
+            "relax_kernel_ms": round(st["relax_kernel_ms"], 3),
             "note": "fresh context's default (plain) form; the caller's planes and prices uploaded and the "
                     "masks built per call (host-to-device copies included); alone-path pricing (no history "
                     "plane); paths returned as flat arrays"}
