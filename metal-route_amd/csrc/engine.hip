@@ -1739,6 +1739,27 @@ __device__ uint64_t field_label(const FieldRead& R, const BoardView& B, const Fi
   return (st & S_TOUCHED) ? lab : kInf;
 }
 
+// Label of the board cell (layer l, column gx, row gy) in field F, with no
+// division: tws / ths are log2 of the label layout's tile width and height
+// (every instantiated tile side is a power of two).
+__device__ __forceinline__ uint64_t field_label_at(const FieldRead& R, const BoardView& B, const FieldDesc& F,
+                                                   uint32_t l, uint32_t gx, uint32_t gy, uint32_t tws, uint32_t ths) {
+  const uint32_t wx = gx - F.x0, wy = gy - F.y0;  // wrap below the window: outside
+  if (wx >= F.ww || wy >= F.wh) return kInf;
+  const uint32_t txi = wx >> tws, tyi = wy >> ths, t = tyi * F.tx + txi;
+  const uint32_t x = wx - (txi << tws), y = wy - (tyi << ths);
+  const uint32_t st = R.state[F.item0 + t];
+  const uint64_t at = F.lab_off + t * tile_block_words(B.layers, R.tw, R.th) + ((l << ths) + y) * R.tw + x;
+  uint64_t lab;
+  if (R.packed) {
+    const uint32_t v = ((const uint32_t*)R.labels)[at];
+    lab = v >= 0xFFFF0000u ? kUnreached : ((uint64_t)(v >> 16) << 32) | (v & 0xFFFFu);
+  } else {
+    lab = R.labels[at];
+  }
+  return (st & S_TOUCHED) ? lab : kInf;
+}
+
 __global__ void path_lengths(FieldRead R, uint32_t n, uint32_t* plen, uint32_t* pcost) {
   uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
@@ -1819,17 +1840,19 @@ __global__ void walk_paths(WalkArgs W) {
   uint32_t* out = W.cells + W.poff[f];
   const bool in_set = lane < (uint32_t)kWalkCells;
   const int dl = kWalkOff.d[lane][0], dy = kWalkOff.d[lane][1], dx = kWalkOff.d[lane][2];
+  const uint32_t tws = __ffs(W.R.tw) - 1, ths = __ffs(W.R.th) - 1;
   uint32_t cur = F.dst;
   uint64_t lab = field_label(W.R, B, F, cur);
   uint32_t steps = 0;
   const uint32_t limit = W.plen[f];
+  // the centre's coordinates, kept along the walk (no division per round trip)
+  uint32_t cl = cur / B.plane, cy = (cur - cl * B.plane) / B.w, cx = cur - cl * B.plane - cy * B.w;
   for (;;) {
-    const uint32_t cl = cur / B.plane, rr = cur - cl * B.plane, cy = rr / B.w, cx = rr - cy * B.w;
     const int pl = (int)cl + dl, py = (int)cy + dy, px = (int)cx + dx;
     const bool on = in_set && pl >= 0 && py >= 0 && px >= 0 && pl < (int)B.layers && py < (int)B.h && px < (int)B.w;
     const uint32_t g = on ? (uint32_t)(((uint32_t)pl * B.h + (uint32_t)py) * B.w + (uint32_t)px) : cur;
     // every load below depends only on the centre: one round trip
-    const uint64_t plab = on ? field_label(W.R, B, F, g) : kInf;
+    const uint64_t plab = on ? field_label_at(W.R, B, F, (uint32_t)pl, (uint32_t)px, (uint32_t)py, tws, ths) : kInf;
     CellInfo ci;
     ci.mask = 0;
     ci.w = 0;
@@ -1890,6 +1913,9 @@ __global__ void walk_paths(WalkArgs W) {
       lab = ((uint64_t)hi << 32) | lo;
     }
     cur = (uint32_t)__builtin_amdgcn_readlane((int)g, c);
+    cl = (uint32_t)__builtin_amdgcn_readlane(pl, c);
+    cy = (uint32_t)__builtin_amdgcn_readlane(py, c);
+    cx = (uint32_t)__builtin_amdgcn_readlane(px, c);
   }
 }
 
