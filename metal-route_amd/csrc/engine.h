@@ -234,6 +234,11 @@ struct Workspace {
   size_t pcost_cap = 0;
   uint32_t* pcells = nullptr;    // path cells
   size_t pcells_cap = 0;
+  // page-locked read-back of path counts, offsets and cells, and the last
+  // call's total cells (how many to read back with the counts)
+  uint8_t* hpath = nullptr;
+  size_t hpath_cap = 0;
+  size_t path_guess = 0;
   uint64_t* qslots = nullptr;    // queue form: every queue's ring
   size_t qslots_cap = 0;
   uint32_t* qctl = nullptr;      // queue form: head, tail, pending of each queue

@@ -40,7 +40,7 @@
 //                     the batch qualifies.
 //   board_scan / board_build
 //                     a board's masks and bounds from the caller's planes.
-//   path_lengths / walk_paths
+//   path_scan / walk_paths
 //                     canonical reconstruction: from dst, the lowest-CellIdx
 //                     predecessor p with label(p) + (step(p, c), 1) == label(c)
 //                     (mr-metal/src/lib.rs:304-396 order; masks re-checked).
@@ -1760,23 +1760,6 @@ __device__ __forceinline__ uint64_t field_label_at(const FieldRead& R, const Boa
   return (st & S_TOUCHED) ? lab : kInf;
 }
 
-__global__ void path_lengths(FieldRead R, uint32_t n, uint32_t* plen, uint32_t* pcost) {
-  uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n) return;
-  const FieldDesc F = R.fields[f];
-  const BoardView B = R.boards[F.board];
-  uint32_t len = 0, cost = kMax;
-  if (F.dst != kMax) {
-    uint64_t lab = field_label(R, B, F, F.dst);
-    if ((uint32_t)(lab >> 32) != kMax) {
-      len = (uint32_t)lab + 1;
-      cost = (uint32_t)(lab >> 32);
-    }
-  }
-  plen[f] = len;
-  pcost[f] = cost;
-}
-
 struct WalkArgs {
   FieldRead R;
   const uint32_t* pool;
@@ -1785,7 +1768,52 @@ struct WalkArgs {
   uint32_t* cells;
   uint32_t* counters;
   uint32_t n;
+  uint64_t cap;  // cells the buffer holds; a path past it sets error bit 64 and is not walked
 };
+
+// Path lengths (hops + 1 of the dst's label, 0 when unreachable), search
+// costs and exclusive offsets of n fields; poff[n] = total.  One workgroup
+// of 1024 threads, 1024 fields per pass.
+__global__ void __launch_bounds__(1024) path_scan(FieldRead R, uint32_t n, uint32_t* plen, uint32_t* pcost,
+                                                  uint64_t* poff) {
+  __shared__ uint64_t wsum[16];
+  __shared__ uint64_t base_s;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+  if (tid == 0) base_s = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < n; c0 += 1024) {
+    const uint32_t f = c0 + tid;
+    uint32_t len = 0, cost = kMax;
+    if (f < n) {
+      const FieldDesc F = R.fields[f];
+      const BoardView B = R.boards[F.board];
+      if (F.dst != kMax) {
+        const uint64_t lab = field_label(R, B, F, F.dst);
+        if ((uint32_t)(lab >> 32) != kMax) {
+          len = (uint32_t)lab + 1;
+          cost = (uint32_t)(lab >> 32);
+        }
+      }
+      plen[f] = len;
+      pcost[f] = cost;
+    }
+    // inclusive sums within the wave, then across the waves
+    uint64_t v = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t up = __shfl_up(v, o);
+      if (lane >= (uint32_t)o) v += up;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    uint64_t before = base_s;
+    for (uint32_t w = 0; w < wv; ++w) before += wsum[w];
+    if (f < n) poff[f] = before + v - len;
+    __syncthreads();
+    if (tid == 1023) base_s = before + v;
+    __syncthreads();
+  }
+  if (tid == 0) poff[n] = base_s;
+}
 
 // Cells within Manhattan distance 3 of a walk's centre: 63 offsets, listed
 // in ascending (layer, y, x) order, one per lane (lane 63 idle).  On a board
@@ -1835,6 +1863,10 @@ __global__ void walk_paths(WalkArgs W) {
   const uint32_t f = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   if (f >= W.n || W.plen[f] == 0) return;
+  if (W.poff[f] + W.plen[f] > W.cap) {
+    if (lane == 0) atomicOr(&W.counters[(C_ERROR) * kCtrStride], 64u);  // the buffer is too small
+    return;
+  }
   const FieldDesc F = W.R.fields[f];
   const BoardView B = W.R.boards[F.board];
   uint32_t* out = W.cells + W.poff[f];
@@ -2269,6 +2301,7 @@ void ctx_free(Ctx& c) {
   c.upstat = nullptr;
   c.stage_cap = c.tblock_cap = 0;
   if (w.pinned) (void)hipHostFree(w.pinned);
+  if (w.hpath) (void)hipHostFree(w.hpath);
   if (w.hostctr) (void)hipHostFree(w.hostctr);
   if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
   c.graph = GraphCache();
@@ -3020,6 +3053,12 @@ static FieldRead make_read(Ctx& c) {
 
 // Canonical paths of fields R.fields[0..n) (pool: the cell pool their
 // FieldDescs index); path buffers come from the context's workspace.
+// Lengths, costs and offsets come from path_scan on the device, the walk
+// follows at once, and one synchronisation reads everything back through
+// page-locked memory: the counts and offsets, the error flags and the cells,
+// as many as the last call's paths held (the rest, when the paths grew, in a
+// second copy).  Allocation-free calls carve the cell buffer from the rest
+// of the scratch array and read back into the output vectors directly.
 static void reconstruct_with(Ctx& c, const FieldRead& R, const uint32_t* pool, size_t n,
                              std::vector<uint8_t>& routed, std::vector<uint32_t>& cost, std::vector<uint64_t>& off,
                              std::vector<uint32_t>& cells, SolveStats* st) {
@@ -3035,46 +3074,111 @@ static void reconstruct_with(Ctx& c, const FieldRead& R, const uint32_t* pool, s
   take(c, W.pcost, W.pcost_cap, n);
   take(c, W.poff, W.poff_cap, n + 1);
   check_arena(c, {{W.plen, n * 4}, {W.pcost, n * 4}, {W.poff, (n + 1) * 8}});
-  HIP_OK(hipEventRecord(W.ev1, s));
-  hipLaunchKernelGGL(path_lengths, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, R, (uint32_t)n, W.plen,
-                     W.pcost);
-  std::vector<uint32_t> plen(n);
-  HIP_OK(hipMemcpyAsync(plen.data(), W.plen, n * 4, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipMemcpyAsync(cost.data(), W.pcost, n * 4, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  for (size_t i = 0; i < n; ++i) {
-    off[i + 1] = off[i] + plen[i];
-    routed[i] = plen[i] ? 1 : 0;
+  // the cell buffer: grown to the last call's total (at least 1 Mi cells)
+  // outside allocation-free calls, the rest of the scratch array in them
+  uint64_t cap;
+  if (c.arena) {
+    cap = (c.arena_cap - c.arena_used) / 4;
+    if (cap == 0) throw Error(1, "the paths need more than the scratch array of the allocation-free entry point");
+    W.pcells = (uint32_t*)(c.arena + c.arena_used);
+    W.pcells_cap = cap;
+  } else {
+    grow(W.pcells, W.pcells_cap, std::max<size_t>(c.ws.path_guess, size_t(1) << 20));
+    cap = W.pcells_cap;
   }
-  size_t total = off[n];
-  cells.resize(total);
-  if (total) {
-    take(c, W.pcells, W.pcells_cap, total);
-    check_arena(c, {{W.pcells, total * 4}});
-    HIP_OK(hipMemcpyAsync(W.poff, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
-    WalkArgs WA;
-    memset(&WA, 0, sizeof(WA));
-    WA.R = R;
-    WA.pool = pool;
-    WA.poff = W.poff;
-    WA.plen = W.plen;
+  // page-locked read-back: plen, cost, err, off, then the cells
+  const size_t head = (2 * n + 2) * 4 + (n + 1) * 8;
+  const size_t guess = c.arena ? 0 : std::min<size_t>(c.ws.path_guess, cap);
+  uint8_t* hp = nullptr;
+  if (!c.arena) {
+    if (c.ws.hpath_cap < head + guess * 4) {
+      if (c.ws.hpath) HIP_OK(hipHostFree(c.ws.hpath));
+      c.ws.hpath = nullptr;
+      c.ws.hpath_cap = 0;
+      const size_t want = (head + guess * 4) * 5 / 4 + 4096;
+      HIP_OK(hipHostMalloc((void**)&c.ws.hpath, want, hipHostMallocDefault));
+      c.ws.hpath_cap = want;
+    }
+    hp = c.ws.hpath;
+  }
+  WalkArgs WA;
+  memset(&WA, 0, sizeof(WA));
+  WA.R = R;
+  WA.pool = pool;
+  WA.poff = W.poff;
+  WA.plen = W.plen;
+  WA.cells = W.pcells;
+  WA.counters = W.counters;
+  WA.n = (uint32_t)n;
+  WA.cap = cap;
+  HIP_OK(hipEventRecord(W.ev1, s));
+  hipLaunchKernelGGL(path_scan, dim3(1), dim3(1024), 0, s, R, (uint32_t)n, W.plen, W.pcost, W.poff);
+  launch_fill32(s, W.counters + C_ERROR * kCtrStride, 0u, 1);
+  hipLaunchKernelGGL(walk_paths, dim3((uint32_t)n), dim3(64), 0, s, WA);
+  HIP_OK(hipGetLastError());
+  uint32_t* h_len = hp ? (uint32_t*)hp : nullptr;
+  uint32_t* h_cost = hp ? h_len + n : nullptr;
+  uint32_t* h_err = hp ? h_cost + n : nullptr;
+  uint64_t* h_off = hp ? (uint64_t*)(hp + (2 * n + 2) * 4) : nullptr;
+  uint32_t* h_cells = hp ? (uint32_t*)(hp + head) : nullptr;
+  std::vector<uint32_t> plen_v;
+  uint32_t err_v = 0;
+  if (hp) {
+    HIP_OK(hipMemcpyAsync(h_len, W.plen, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(h_cost, W.pcost, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(h_err, W.counters + C_ERROR * kCtrStride, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(h_off, W.poff, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (guess) HIP_OK(hipMemcpyAsync(h_cells, W.pcells, guess * 4, hipMemcpyDeviceToHost, s));
+  } else {
+    plen_v.resize(n);
+    HIP_OK(hipMemcpyAsync(plen_v.data(), W.plen, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(cost.data(), W.pcost, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&err_v, W.counters + C_ERROR * kCtrStride, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(off.data(), W.poff, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+  }
+  HIP_OK(hipEventRecord(W.ev2, s));
+  HIP_OK(hipStreamSynchronize(s));
+  const uint64_t total = hp ? h_off[n] : off[n];
+  uint32_t err = hp ? *h_err : err_v;
+  bool rewalked = false;
+  if (err & 64u) {
+    // the paths outgrew the cell buffer (total > cap): walk again into one
+    // that holds them
+    if (c.arena) throw Error(1, "the paths need more than the scratch array of the allocation-free entry point");
+    grow(W.pcells, W.pcells_cap, total);
     WA.cells = W.pcells;
-    WA.counters = W.counters;
-    WA.n = (uint32_t)n;
+    WA.cap = W.pcells_cap;
     launch_fill32(s, W.counters + C_ERROR * kCtrStride, 0u, 1);
     hipLaunchKernelGGL(walk_paths, dim3((uint32_t)n), dim3(64), 0, s, WA);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(cells.data(), W.pcells, total * 4, hipMemcpyDeviceToHost, s));
-    uint32_t err = 0;
-    HIP_OK(hipMemcpyAsync(&err, W.counters + C_ERROR * kCtrStride, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&err_v, W.counters + C_ERROR * kCtrStride, 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipEventRecord(W.ev2, s));
     HIP_OK(hipStreamSynchronize(s));
-    if (err) throw Error(1, "device path reconstruction failed (flags " + std::to_string(err) + ")");
-    float ms = 0;
-    HIP_OK(hipEventElapsedTime(&ms, W.ev1, W.ev2));
-    if (st) st->path_ms = ms;
-    c.last.path_ms += ms;
+    err = err_v;
+    rewalked = true;
   }
+  if (err) throw Error(1, "device path reconstruction failed (flags " + std::to_string(err) + ")");
+  cells.resize(total);
+  if (hp) {
+    for (size_t i = 0; i < n; ++i) {
+      routed[i] = h_len[i] ? 1 : 0;
+      cost[i] = h_cost[i];
+    }
+    memcpy(off.data(), h_off, (n + 1) * 8);
+    // the page-locked copy holds the first `guess` cells of the first walk
+    const size_t copied = rewalked ? 0 : std::min<size_t>(guess, total);
+    if (copied) memcpy(cells.data(), h_cells, copied * 4);
+    if (total > copied)
+      HIP_OK(hipMemcpy(cells.data() + copied, W.pcells + copied, (total - copied) * 4, hipMemcpyDeviceToHost));
+    c.ws.path_guess = total;
+  } else {
+    for (size_t i = 0; i < n; ++i) routed[i] = plen_v[i] ? 1 : 0;
+    if (total) HIP_OK(hipMemcpy(cells.data(), W.pcells, total * 4, hipMemcpyDeviceToHost));
+  }
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, W.ev1, W.ev2));
+  if (st) st->path_ms = ms;
+  c.last.path_ms += ms;
 }
 
 void reconstruct_paths(Ctx& c, size_t n, std::vector<uint8_t>& routed, std::vector<uint32_t>& cost,
