@@ -2812,6 +2812,16 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   SolveStats local;
   local.field_cells = field_cells;
   bool timed = false;
+  size_t pairs = 0;  // per-round event pairs (plain form) recorded and not yet read
+  auto harvest = [&](bool sync) {
+    if (sync) HIP_OK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < pairs; ++k) {
+      float kms = 0;
+      HIP_OK(hipEventElapsedTime(&kms, W.round_ev[2 * k], W.round_ev[2 * k + 1]));
+      local.kernel_ms += kms;
+    }
+    pairs = 0;
+  };
   if (item && !seeds.empty()) {
     uint32_t* pinned = W.pinned;
     void* args[] = {&A};
@@ -2830,16 +2840,6 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     // an event pair per round once the last round is done.
     auto plain_rounds = [&](uint32_t& rounds, uint32_t n_active) {
       A.host_out = W.hostctr_dev;
-      size_t pairs = 0;  // event pairs recorded and not yet read
-      auto harvest = [&] {
-        HIP_OK(hipStreamSynchronize(s));
-        for (size_t k = 0; k < pairs; ++k) {
-          float kms = 0;
-          HIP_OK(hipEventElapsedTime(&kms, W.round_ev[2 * k], W.round_ev[2 * k + 1]));
-          local.kernel_ms += kms;
-        }
-        pairs = 0;
-      };
       while (n_active) {
         A.lset_next = rounds % 3;
         A.lset_cur = (rounds + 2) % 3;
@@ -2854,7 +2854,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         // per-round kernel times: not in allocation-free calls, which create
         // no events
         const bool per_round = !c.arena;
-        if (per_round && pairs == 512) harvest();  // bounds the event pool
+        if (per_round && pairs == 512) harvest(true);  // bounds the event pool
         const size_t ei = 2 * pairs;
         if (per_round) {
           ++pairs;
@@ -2888,7 +2888,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         A.list_next = lc;
         if (rounds > 1000000) throw Error(1, "relaxation did not converge");
       }
-      harvest();
+      // the event pairs are read after the solve's closing synchronisation
     };
     uint32_t rounds = 0;
     launch_fill32(s, W.act, 0u, (size_t)item * 2);
@@ -3007,6 +3007,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   uint32_t* cnt = W.pinned;
   HIP_OK(hipMemcpyAsync(cnt, W.counters, (kCtrWords + kShardWords) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
+  harvest(false);
   if (cnt[C_ERROR * kCtrStride]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR * kCtrStride]));
   if (RL.packed && (uint64_t)cnt[C_PMAX * kCtrStride] + ((pk_step << 16) | 1u) >= 0xFFFF0000ull) {
     // a path came within one step of the packed dist limit: the labels may
