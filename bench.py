@@ -667,8 +667,9 @@ def main():
     achieved = cells_per_launch * bpc / (avg_launch_ms * 1e-3) / 1e9
     workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
     kernel = {"graph": "relax_list", "plain": "relax_round", "queue": "relax_queue"}[form]
-    if form == "plain" and sum(wave) == sum(launches) > 0:
-        kernel = "relax_wave"  # one wavefront per tile visit
+    if sum(wave) == sum(launches) > 0:
+        # one wavefront per tile visit (the queue form's persistent variant)
+        kernel = {"plain": "relax_wave", "queue": "relax_wave_queue"}.get(form, kernel)
     traffic = load_traffic(kernel, workload)
     # compulsory floor of a step: every field cell's dist written once (4 B)
     # and the board planes read once per field (bpc - 16 B per cell)

@@ -1464,6 +1464,13 @@ struct DevOps {
   __device__ static uint32_t add(uint32_t* p, uint32_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  __device__ static uint32_t orr(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static bool cas(uint32_t* p, uint32_t expect, uint32_t want) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+  }
   __device__ static uint64_t now() { return __builtin_amdgcn_s_memrealtime(); }
   // about 2^k x 64 clocks
   __device__ static void pause(uint32_t k) {
@@ -1521,7 +1528,92 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
       if (want && !wq::activate<DevOps>(q, A.act_cur, nb, Q.wait_ticks)) atomicOr(err, E_QPUSH);
       if (lane == 0) {
         if (fl & (T_ANY | T_CAPPED)) atomicAdd(shard_of(A, SH_CHANGED), 1u);
-        wq::done<DevOps>(q);  // after every lane's activations (order() waits for them)
+        // after every lane's activations (finish's order() waits for them)
+        if (!wq::finish<DevOps>(q, A.act_cur, item, Q.wait_ticks)) atomicOr(err, E_QPUSH);
+        wq::done<DevOps>(q);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// The persistent work-queue form on the wave form's tile visits: one
+// workgroup (one wavefront, two for 5-8 layers) per visit slot, taking tiles
+// from the device queues like relax_queue.  Visits of one tile never overlap
+// (the queue protocol), so the write-back stays plain device-scope stores;
+// every label is read with a device-scope load.  Lean fields only (no
+// halo-owner planes or per-net extras).
+template <int TW, int TH, int LM, bool PK>
+__global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>::waves))
+    relax_wave_queue(RelaxArgs A, QueueArgs Q) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ uint32_t cmd[2];
+  __shared__ uint64_t start;
+  constexpr uint32_t kExit = 0xFFFFFFFFu;
+  if (threadIdx.x == 0) start = DevOps::now();
+  // The queue's words are formed where they are used from the kernel
+  // arguments (nothing of the protocol stays live through a visit).
+  auto queue = [&]() {
+    const uint32_t qi = blockIdx.x % kQueues;
+    return wq::Queue{Q.slots + Q.slot0[qi], Q.cap[qi], Q.ctl + (3 * qi + 0) * kCtrStride,
+                     Q.ctl + (3 * qi + 1) * kCtrStride, Q.ctl + (3 * qi + 2) * kCtrStride};
+  };
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const wq::Queue q = queue();
+      uint32_t* err = A.counters + C_ERROR * kCtrStride;
+      uint32_t d = kExit, first = 0;
+      if (DevOps::now() - start > Q.run_ticks) {
+        atomicOr(err, E_QRUN);
+      } else {
+        uint32_t it = 0;
+        const uint32_t r = wq::wait_item<DevOps>(q, wq::take_ticket<DevOps>(q), Q.wait_ticks, it);
+        if (r == wq::GOT) {
+          if (!wq::take<DevOps>(A.act_cur, it)) atomicOr(err, E_QTAKE);
+          d = it;
+          // the tile's first visit is the one that marks it touched
+          first = (__hip_atomic_fetch_or(&A.state[it], S_TOUCHED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                   S_TOUCHED) ? 0u : 1u;
+        } else if (r == wq::TIMEOUT) {
+          atomicOr(err, E_QWAIT);
+        }  // STOP: nothing pending, no item can come
+      }
+      cmd[0] = d;
+      cmd[1] = first;
+    }
+    __syncthreads();
+    const uint32_t item = uniform_u32(cmd[0]);
+    const bool first = uniform_u32(cmd[1]) != 0u;
+    __syncthreads();
+    if (item == kExit) break;
+    const uint32_t f = const_load(A.item_field + item);
+    const FieldDesc F = const_load(A.fields + f);
+    const BoardView B = const_load(A.boards + F.board);
+    TileResult res;
+    if (B.add16)
+      res = wave_tile<TW, TH, LM, ADD_U16, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
+    else if (B.add)
+      res = wave_tile<TW, TH, LM, ADD_U32, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
+    else
+      res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
+    if (threadIdx.x < 64) {
+      // lanes 0..3: left, right, top, bottom neighbour; lane 4: the tile
+      // itself when its visit hit the in-LDS round cap
+      const uint32_t lane = threadIdx.x, fl = res.flags;
+      bool want = false;
+      uint32_t nb = item;
+      if (lane == 0) want = (fl & T_LEFT) != 0, nb = item - 1;
+      if (lane == 1) want = (fl & T_RIGHT) != 0, nb = item + 1;
+      if (lane == 2) want = (fl & T_TOP) != 0, nb = item - res.tx;
+      if (lane == 3) want = (fl & T_BOTTOM) != 0, nb = item + res.tx;
+      if (lane == 4) want = (fl & T_CAPPED) != 0;
+      const wq::Queue q = queue();
+      uint32_t* err = A.counters + C_ERROR * kCtrStride;
+      if (want && !wq::activate<DevOps>(q, A.act_cur, nb, Q.wait_ticks)) atomicOr(err, E_QPUSH);
+      if (lane == 0) {
+        if (fl & (T_ANY | T_CAPPED)) atomicAdd(shard_of(A, SH_CHANGED), 1u);
+        if (!wq::finish<DevOps>(q, A.act_cur, item, Q.wait_ticks)) atomicOr(err, E_QPUSH);
+        wq::done<DevOps>(q);
       }
     }
     __syncthreads();
@@ -2100,6 +2192,9 @@ struct KernelSet {
   // nullptr), by wave_index(packed labels, halo-owner planes, per-net
   // extras); packed labels never run with halo-owner planes
   const void* wave[8];
+  // relax_wave_queue, u64 and packed labels (narrow sets up to 8 layers at
+  // 16 x 16, else nullptr)
+  const void* wave_queue[2];
 };
 
 static constexpr int wave_index(bool pk, bool ho, bool ex) { return (pk ? 4 : 0) + (ho ? 2 : 0) + (ex ? 1 : 0); }
@@ -2108,8 +2203,10 @@ template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
   KernelSet k{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
               (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
-              {}};
+              {}, {}};
   if constexpr (NARROW && TW == 16 && TH == 16 && LM <= 8) {
+    k.wave_queue[0] = (const void*)relax_wave_queue<TW, TH, LM, false>;
+    k.wave_queue[1] = (const void*)relax_wave_queue<TW, TH, LM, true>;
     k.wave[wave_index(false, false, false)] = (const void*)relax_wave<TW, TH, LM, false, false, false>;
     k.wave[wave_index(false, false, true)] = (const void*)relax_wave<TW, TH, LM, false, false, true>;
     k.wave[wave_index(false, true, false)] = (const void*)relax_wave<TW, TH, LM, false, true, false>;
@@ -2285,6 +2382,8 @@ void ctx_init(Ctx& c, int device) {
     HIP_OK(hipFuncSetAttribute(k.queue, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
     for (const void* w : k.wave)
       if (w) HIP_OK(hipFuncSetAttribute(w, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
+    for (const void* w : k.wave_queue)
+      if (w) HIP_OK(hipFuncSetAttribute(w, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
   }
 }
 
@@ -2795,7 +2894,11 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   // With overrides a step costs at least the edge price times the weight
   // (the net's own prices may undercut the plane's) and at most the narrow
   // limit (field_narrow checked the override prices).
-  bool pack = !c.no_pack && c.mode == MODE_PLAIN && !halo;
+  // The queue form runs the wave form's visits (relax_wave_queue) for lean
+  // batches of the wave shape; other batches run relax_queue.
+  const bool wave_queue = c.mode == MODE_QUEUE && wave_shape(S) && K.wave_queue[0] && K.narrow && wave_ok &&
+                          !halo && !ex && Lmax <= K.lm;
+  bool pack = !c.no_pack && (c.mode == MODE_PLAIN || wave_queue) && !halo;
   uint64_t pk_step = ovr ? kNarrowMaxStep : 0;
   for (const DevBoard* b : boards) {
     pack = pack && (ovr ? b->min_ew : b->min_step) >= 1 && b->max_step < 0xFFFFu && !b->pack_failed;
@@ -2803,7 +2906,9 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   RelaxArgs A = make_args(c, item);
   const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, c.mode == MODE_PLAIN && c.delta != kMax, pack, halo, ex);
-  if (RL.packed) {
+  // labels of this solve are packed u32
+  const bool packed_run = c.mode == MODE_PLAIN ? RL.packed : (wave_queue && pack);
+  if (packed_run) {
     take(c, W.fmax, W.fmax_cap, std::max<size_t>(item, 1));
     check_arena(c, {{W.fmax, std::max<size_t>(item, 1) * 4}});
     launch_fill32(s, W.fmax, 0u, item);
@@ -2983,17 +3088,21 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       launch_fill32(s, (uint32_t*)W.labels, 0xFFFFFFFFu, (size_t)lab_off * 2);
       hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
                          d_seeds, (uint32_t)seeds.size(), (uint32_t*)nullptr);
+      const void* qfn = wave_queue ? K.wave_queue[pack ? 1 : 0] : K.queue;
+      const uint32_t qthreads = wave_queue ? 64u * wave_count(K.lm, S.th) : workgroup_threads(S);
+      const size_t qlds = wave_queue ? wave_lds_bytes(S.tw, S.th, K.lm, pack, false) : lds;
       int occ = 0;
-      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.queue, workgroup_threads(S), lds));
+      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, qfn, qthreads, qlds));
       if (occ < 1) throw Error(1, "relaxation kernel does not fit on a CU");
       const uint32_t grid = std::max<uint32_t>(kQueues, ((uint32_t)occ * (uint32_t)c.cus) / kQueues * kQueues);
       void* qargs[] = {&A, &Q};
       HIP_OK(hipEventRecord(W.ev0, s));
       timed = true;
-      HIP_OK(hipLaunchKernel(K.queue, dim3(grid), dim3(workgroup_threads(S)), qargs, lds, s));
+      HIP_OK(hipLaunchKernel(qfn, dim3(grid), dim3(qthreads), qargs, qlds, s));
       HIP_OK(hipGetLastError());
       rounds = 1;
       local.launches = 1;
+      if (wave_queue) local.wave_launches = 1;
     } else {
       throw Error(1, "the persistent grid-barrier form is disabled (it did not return on MI355X); "
                      "use the plain, graph or queue form");
@@ -3001,7 +3110,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     local.rounds = rounds;
   }
   if (timed) HIP_OK(hipEventRecord(W.ev1, s));
-  if (RL.packed && item)
+  if (packed_run && item)
     hipLaunchKernelGGL(reduce_item_max, dim3((uint32_t)std::min<size_t>((item + 255) / 256, 256)), dim3(256), 0, s,
                        W.fmax, (size_t)item, W.counters + C_PMAX * kCtrStride);
   uint32_t* cnt = W.pinned;
@@ -3009,7 +3118,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   HIP_OK(hipStreamSynchronize(s));
   harvest(false);
   if (cnt[C_ERROR * kCtrStride]) throw Error(1, "device relaxation reported error flags " + std::to_string(cnt[C_ERROR * kCtrStride]));
-  if (RL.packed && (uint64_t)cnt[C_PMAX * kCtrStride] + ((pk_step << 16) | 1u) >= 0xFFFF0000ull) {
+  if (packed_run && (uint64_t)cnt[C_PMAX * kCtrStride] + ((pk_step << 16) | 1u) >= 0xFFFF0000ull) {
     // a path came within one step of the packed dist limit: the labels may
     // be clipped, so the batch is solved again in wide labels, and so are
     // later batches on these boards
@@ -3024,13 +3133,13 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     c.no_pack = false;
     return;
   }
-  c.packed = RL.packed;
+  c.packed = packed_run;
   float ms = 0;
   if (timed) HIP_OK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   local.relax_ms = ms;
   local.tile_visits = shard_sum(cnt + kCtrWords, SH_VISITS);
   local.changed_visits = shard_sum(cnt + kCtrWords, SH_CHANGED);
-  local.label_bytes = RL.packed ? 4u : 8u;
+  local.label_bytes = packed_run ? 4u : 8u;
   local.local_rounds = shard_sum(cnt + kCtrWords, SH_ROUNDS);
   local.cells_visited = local.tile_visits * Lmax * S.th * S.tw;
   local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

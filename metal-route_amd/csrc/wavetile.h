@@ -194,7 +194,12 @@ __device__ __forceinline__ uint32_t add_k(const BoardView& B, uint32_t g) {
 // EX: the field may carry own pads (obstacle cells passable at weight 1),
 // via-exempt pads (via_forbidden lifted) and additive overrides (the net's
 // own price of a cell), applied per tile from the host's tables.
-template <int TW, int TH, int LM, int AK, bool PK, bool HO, bool EX>
+// QU: a visit of the persistent queue form: labels start unreached (filled
+// before the launch) and other workgroups write them while this one runs, so
+// every label is read with a device-scope load (another XCD's L2 may hold a
+// stale copy) and every existing neighbour's border counts; `touched` is
+// false on the tile's first visit (seeds) only.
+template <int TW, int TH, int LM, int AK, bool PK, bool HO, bool EX, bool QU = false>
 __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched, const FieldDesc& F,
                                 const BoardView& B, char* smem, uint32_t round) {
   using WT = WaveTile<TW, TH, LM, PK>;
@@ -406,7 +411,7 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   const uint32_t ybot = (gyb >= 1 && gyb < B.h) ? gbl(B.ye)[gyb - 1] : 0u;
   // neighbour states (lanes 1..4): whether their border labels exist
   uint32_t stv = 0;
-  if (lane >= 1 && lane < 5) {
+  if (!QU && lane >= 1 && lane < 5) {
     uint32_t nb = item;
     bool ok = true;
     if (lane == 1) ok = has_l, nb = item - 1;
@@ -425,7 +430,13 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   const uint64_t hta = et ? blockbase - (uint64_t)F.tx * TB + cl * PT + (TH - 1) * TW + cx : own;
   const uint64_t hba = eb ? blockbase + (uint64_t)F.tx * TB + cl * PT + cx : own;
   const auto* glab = gbl((const LT*)A.labels);
-  LT hl = glab[hla], hr = glab[hra], ht = glab[hta], hb = glab[hba];
+  auto lab_at = [&](uint64_t at) -> LT {
+    if constexpr (QU)
+      return __hip_atomic_load((const LT*)A.labels + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      return glab[at];
+  };
+  LT hl = lab_at(hla), hr = lab_at(hra), ht = lab_at(hta), hb = lab_at(hba);
   // The tile's labels in block order: thread tid holds cells tid + NT j,
   // i.e. column x = tid % TW of row (NT / TW) * (j % NV) + tid / TW of layer
   // j / NV -- the same x for every j, so each cell's LDS slot is a constant
@@ -435,20 +446,20 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   const uint32_t sx_ = tid % TW, sy_ = tid / TW;
   const uint32_t sbase = sy_ * RS + sx_;
   LT sl[NS];
-  if (touched) {
-    const auto* lp = glab + blockbase + tid;
+  if (touched || QU) {
 #pragma unroll
-    for (uint32_t j = 0; j < NS; ++j) sl[j] = lp[NT * (min(j / NV, L - 1) * NV + j % NV)];
+    for (uint32_t j = 0; j < NS; ++j) sl[j] = lab_at(blockbase + tid + NT * (min(j / NV, L - 1) * NV + j % NV));
   } else {
 #pragma unroll
     for (uint32_t j = 0; j < NS; ++j) sl[j] = WL::unr;
   }
 
   auto earlier = [&](uint32_t st) { return (st & S_TOUCHED) && (st >> 1) < round; };
-  const uint32_t nbv = (earlier(__builtin_amdgcn_readlane(stv, 1)) ? 1u : 0u) |
-                       (earlier(__builtin_amdgcn_readlane(stv, 2)) ? 2u : 0u) |
-                       (earlier(__builtin_amdgcn_readlane(stv, 3)) ? 4u : 0u) |
-                       (earlier(__builtin_amdgcn_readlane(stv, 4)) ? 8u : 0u);
+  const uint32_t nbv = QU ? (has_l ? 1u : 0u) | (has_r ? 2u : 0u) | (has_t ? 4u : 0u) | (has_b ? 8u : 0u)
+                          : (earlier(__builtin_amdgcn_readlane(stv, 1)) ? 1u : 0u) |
+                                (earlier(__builtin_amdgcn_readlane(stv, 2)) ? 2u : 0u) |
+                                (earlier(__builtin_amdgcn_readlane(stv, 3)) ? 4u : 0u) |
+                                (earlier(__builtin_amdgcn_readlane(stv, 4)) ? 8u : 0u);
   hl = (nbv & 1u) ? WL::halo(hl) : WL::unr;
   hr = (nbv & 2u) ? WL::halo(hr) : WL::unr;
   ht = (nbv & 4u) ? WL::halo(ht) : WL::unr;
@@ -764,7 +775,7 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   res.tx = F.tx;
   res.kl = res.kr = res.kt = res.kb = res.ka = kMax;  // keys: the wave form runs without delta stepping
   if (tid == 0) {
-    if (!touched)
+    if (!touched && !QU)
       __hip_atomic_fetch_or(&A.state[item], S_TOUCHED | (round << 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(shard_of(A, SH_VISITS), 1u);
     atomicAdd(shard_of(A, SH_ROUNDS), rounds);

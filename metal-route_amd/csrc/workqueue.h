@@ -2,15 +2,23 @@
 // GPU kernel and a CPU model (tests/workqueue_cases.cpp): the protocol is a
 // template over an operations policy (atomics, clock, pause).
 //
-// Items are tiles; an item is on a queue at most once at a time, which the
-// per-item activation flag (0 or 1) guarantees: only a visit that turns a flag
-// from 0 to 1 pushes the item, and the worker that receives it turns it back
-// to 0 before the item is visited.  `pend` counts outstanding work, raised
-// before a flag is set and lowered after the visit that consumed it has made
-// its own activations:
+// Items are tiles.  Each item's activation flag has two bits: bit 0 "wanted"
+// (queued, or to be visited again once the running visit ends) and bit 1
+// "running":
 //
-//   pend = (activation flags set) + (visits in progress) + (raises not yet
-//          undone by an activation that found its flag already set)
+//   0 idle   1 queued   2 running   3 running and wanted again
+//
+// An item is pushed only when its flag goes 0 -> 1 (activate) or 3 -> 1 (the
+// end of the running visit), the worker that receives it turns 1 -> 2 before
+// the visit, and the visit ends with 2 -> 0 or 3 -> 1; so an item is on a
+// queue at most once and visited by at most one worker at a time (its labels
+// need no atomic minimum).  `pend` counts outstanding work, raised before a
+// flag is set and lowered after the visit that consumed it has made its own
+// activations and ended:
+//
+//   pend = (queued items) + (visits in progress) + (wanted-again marks) +
+//          (raises not yet undone by an activation that found the item
+//          already queued or marked)
 //
 // so pend never counts less than the outstanding work, and pend == 0 means
 // no item is queued and no visit can push one: a worker waiting for an item
@@ -28,8 +36,9 @@
 // received) is bounded by a clock limit after which the caller reports an
 // error and stops, so no worker loops forever whatever goes wrong.
 //
-// The policy O supplies load / add / exch on u32, load64 / store64 on u64,
-// now() (a clock), pause(k) (about 2^k idle steps) and order(): order()
+// The policy O supplies load / add / exch / orr / cas on u32, load64 /
+// store64 on u64, now() (a clock), pause(k) (about 2^k idle steps) and
+// order(): order()
 // returns once every memory operation the caller issued before it has been
 // performed, which keeps a raise of pend ahead of the flag it accounts for
 // and a visit's own activations ahead of its done().
@@ -101,25 +110,39 @@ __host__ __device__ inline bool push(const Queue& q, uint32_t item, uint64_t lim
   return true;
 }
 
-// Activate an item: raise pend, then set its flag; a flag that was clear puts
-// the item on the queue, one that was set already undoes the raise.
+// Activate an item: raise pend, then set its wanted bit.  An idle item goes
+// on the queue; a running one is marked, and its visit's end pushes it (the
+// raise counts the mark); a queued or marked one undoes the raise.
 template <class O>
 __host__ __device__ inline bool activate(const Queue& q, uint32_t* act, uint32_t item, uint64_t limit) {
   O::add(q.pend, 1u);
   O::order();
-  if (O::exch(act + item, 1u) == 0u) return push<O>(q, item, limit);
+  const uint32_t was = O::orr(act + item, 1u);
+  if (was == 0u) return push<O>(q, item, limit);
+  if (was == 2u) return true;
   O::add(q.pend, 0xFFFFFFFFu);
   return true;
 }
 
-// Take a received item's activation (its flag back to 0); false when the flag
-// was not set, which the protocol never allows.
+// Take a received item for its visit (queued -> running); false when it was
+// not queued, which the protocol never allows.
 template <class O>
 __host__ __device__ inline bool take(uint32_t* act, uint32_t item) {
-  return O::exch(act + item, 0u) == 1u;
+  return O::exch(act + item, 2u) == 1u;
 }
 
-// The visit of a taken item has made its activations.
+// End an item's visit, after its writes and its own activations: running ->
+// idle, or, when it was marked meanwhile, queued again (the mark's raise
+// counts the entry).  False when the push timed out.
+template <class O>
+__host__ __device__ inline bool finish(const Queue& q, uint32_t* act, uint32_t item, uint64_t limit) {
+  O::order();
+  if (O::cas(act + item, 2u, 0u)) return true;
+  O::exch(act + item, 1u);  // 3 -> 1: nothing else changes a marked flag
+  return push<O>(q, item, limit);
+}
+
+// The visit of a taken item has made its activations and ended.
 template <class O>
 __host__ __device__ inline void done(const Queue& q) {
   O::order();

@@ -38,6 +38,10 @@ struct HostOps {
   static void store64(uint64_t* p, uint64_t v) { __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
   static uint32_t exch(uint32_t* p, uint32_t v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
   static uint32_t add(uint32_t* p, uint32_t v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+  static uint32_t orr(uint32_t* p, uint32_t v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+  static bool cas(uint32_t* p, uint32_t expect, uint32_t want) {
+    return __atomic_compare_exchange_n(p, &expect, want, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+  }
   static uint64_t now() {
     return (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
                std::chrono::steady_clock::now().time_since_epoch())
@@ -72,7 +76,8 @@ struct Run {
   std::vector<uint32_t> act, ctl;  // ctl: per queue head, tail, pend (one line each)
   std::vector<uint64_t> dist;
   std::vector<wq::Queue> queues;
-  std::atomic<uint32_t> errors{0}, pops{0}, exits{0};
+  std::atomic<uint32_t> errors{0}, pops{0}, exits{0}, overlaps{0};
+  std::vector<uint32_t> in_visit;  // per node: workers inside its visit (at most one)
 };
 
 // The kernel's loop for one worker of queue q.
@@ -90,12 +95,15 @@ static void worker(Run& R, const std::vector<Graph>& gs, uint32_t q, uint64_t li
     if (r == wq::GOT) {
       R.pops++;
       if (!wq::take<HostOps>(R.act.data(), u)) R.errors++;
+      if (__atomic_fetch_add(&R.in_visit[u], 1u, __ATOMIC_SEQ_CST) != 0u) R.overlaps++;
       const uint64_t du = __atomic_load_n(&R.dist[u], __ATOMIC_SEQ_CST);
       for (auto& e : G.adj[u - G.n0]) {
         uint64_t old;
         atomic_min64(&R.dist[e.first], du + e.second, old);
         if (du + e.second < old && !wq::activate<HostOps>(Q, R.act.data(), e.first, limit_us)) R.errors++;
       }
+      __atomic_fetch_sub(&R.in_visit[u], 1u, __ATOMIC_SEQ_CST);
+      if (!wq::finish<HostOps>(Q, R.act.data(), u, limit_us)) R.errors++;
       wq::done<HostOps>(Q);
       continue;
     }
@@ -141,6 +149,7 @@ static void setup(Run& R, std::vector<Graph>& gs, uint32_t nq, uint32_t nodes, u
     gs.push_back(std::move(g));
   }
   R.act.assign(total, 0);
+  R.in_visit.assign(total, 0);
   R.dist.assign(total, UINT64_MAX);
   R.ctl.assign(nq * 3 * 16, 0);
   uint32_t slot0 = 0;
@@ -176,7 +185,8 @@ static bool run_case(uint32_t nq, uint32_t per_q, uint32_t nodes, uint32_t degre
   std::vector<std::thread> th;
   for (uint32_t w = 0; w < nq * per_q; ++w) th.emplace_back(worker, std::ref(R), std::cref(gs), w % nq, 5000000ull);
   for (auto& t : th) t.join();
-  bool ok = R.errors.load() == 0 && R.exits.load() == nq * per_q;
+  // no node is ever visited by two workers at once
+  bool ok = R.errors.load() == 0 && R.exits.load() == nq * per_q && R.overlaps.load() == 0;
   for (uint32_t q = 0; q < nq; ++q) {
     const std::vector<uint64_t> ref = dijkstra(gs[q]);
     for (uint32_t u = 0; u < nodes; ++u) ok = ok && R.dist[gs[q].n0 + u] == ref[u];
