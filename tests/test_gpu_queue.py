@@ -68,8 +68,19 @@ def test_queue_isolated_stress(eng, shape):
                 assert got[i] == (ref_p[i], ref_c[i]), (case, i)
 
 
-def test_queue_config1_fields_and_paths(eng):
-    eng.set_tiles(16, 16, 256, 32)
+# workgroup visits (relax_queue) and the wave visit's queue kernel
+QSHAPES = [(16, 16, 256, 32), (16, 16, 64, 32)]
+
+
+def assert_queue_kernel(eng, shape):
+    st = eng.stats()
+    assert st["relax_launches"] == 1
+    assert st["wave_launches"] == (1 if shape[2] == 64 else 0), st
+
+
+@pytest.mark.parametrize("shape", QSHAPES)
+def test_queue_config1_fields_and_paths(eng, shape):
+    eng.set_tiles(*shape)
     s = mrfield.synth_config(1, seed=1)
     g = s.grid
     og, oe = oracle_parts(s)
@@ -77,8 +88,8 @@ def test_queue_config1_fields_and_paths(eng):
     ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(og, oe), nets, [[n.src] for n in nets], threads=8)
     got_d, got_h = eng.fields(g, s.edges, [[int(a)] for a in s.src], dsts=[int(b) for b in s.dst])
     assert np.array_equal(got_d, ref_d) and np.array_equal(got_h, ref_h)
-    st = eng.stats()
-    assert st["relax_launches"] == 1 and st["tile_visits"] > 0
+    assert_queue_kernel(eng, shape)
+    assert eng.stats()["tile_visits"] > 0
     full = [(0, 0, g.w - 1, g.h - 1)] * len(nets)
     ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(og, oe), nets, full, threads=8)
     got = eng.route_isolated_batch(g, s.nets(), full, s.edges)
@@ -139,8 +150,9 @@ def test_queue_random_fields_all_features(eng, shape):
             assert r == (ref_p[i], ref_c[i]), i
 
 
-def test_queue_config3_windowed_batch(eng):
-    eng.set_tiles(16, 16, 256, 32)
+@pytest.mark.parametrize("shape", QSHAPES)
+def test_queue_config3_windowed_batch(eng, shape):
+    eng.set_tiles(*shape)
     s = mrfield.synth_config(3, seed=1)
     board = mrfield.Board(eng, s.grid, s.edges, s.pricing())
     try:
@@ -160,12 +172,14 @@ def test_queue_config3_windowed_batch(eng):
             assert gp == (ref_p[k], ref_c[k]), i
 
 
-def test_queue_config2_equals_plain(eng):
+@pytest.mark.parametrize("shape", QSHAPES)
+def test_queue_config2_equals_plain(eng, shape):
     s = mrfield.synth_config(2, seed=1)
     board = mrfield.Board(eng, s.grid, s.edges, s.pricing())
     try:
-        eng.set_tiles(16, 16, 256, 32)
+        eng.set_tiles(*shape)
         got = board.solve_arrays(s.nets())
+        assert_queue_kernel(eng, shape)
         eng.set_mode(mrfield.MODE_PLAIN)
         ref = board.solve_arrays(s.nets())
     finally:

@@ -68,15 +68,13 @@ def assert_wave_ran(eng, label_bytes=None):
         assert st["label_bytes"] == label_bytes, st
 
 
-@pytest.mark.parametrize("packed", [True, False])
-@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 8])
-@pytest.mark.parametrize("seed", [1, 2, 3])
-def test_wave_random_fields(eng, L, seed, packed):
+def lean_fields_case(L, seed, packed):
     """Multi-source fields on boards with every lean feature: trace nodes,
     obstacles, one-way edges, via-forbidden cells, per-gap x / y prices,
     vias off between some layers, an additive plane; full and partial
     windows (ragged edge tiles).  Packed: every price at least 1; wide: one
-    zero-price gap (a zero-cost step rules the packed labels out)."""
+    zero-price gap (a zero-cost step rules the packed labels out).  Returns
+    the engine call's arguments and the oracle's fields."""
     rng = np.random.default_rng(1000 * L + seed + (7 if packed else 0))
     w, h = int(rng.integers(20, 70)), int(rng.integers(20, 70))
     g = unit_board(seed + 10 * L, w, h, L)
@@ -102,10 +100,45 @@ def test_wave_random_fields(eng, L, seed, packed):
     nets = [mrcpu.Net(s[0], d) for s, d in zip(sources, dsts)]
     ref_d, ref_h = mrcpu.fields_batch(mrcpu.Spec(g, edges, additive=add), nets, sources, wins)
     dedges = mrfield.Edges(list(edges.x), list(edges.y), list(edges.via))
-    got_d, got_h = eng.fields(as_dev(g), dedges, sources, dsts=dsts, windows=wins,
-                              pricing=mrfield.Pricing(additive=add))
+    args = (as_dev(g), dedges, sources)
+    kw = dict(dsts=dsts, windows=wins, pricing=mrfield.Pricing(additive=add))
+    return args, kw, ref_d, ref_h
+
+
+@pytest.mark.parametrize("packed", [True, False])
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 8])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_wave_random_fields(eng, L, seed, packed):
+    """The lean cases in in the plain form (one launch per round)."""
+    args, kw, ref_d, ref_h = lean_fields_case(L, seed, packed)
+    got_d, got_h = eng.fields(*args, **kw)
     assert_wave_ran(eng, 4 if packed else 8)
-    for i in range(len(nets)):
+    for i in range(len(ref_d)):
+        assert np.array_equal(got_d[i], ref_d[i]), (i, np.flatnonzero(got_d[i] != ref_d[i])[:10])
+        assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
+
+
+@pytest.fixture(scope="module")
+def qeng():
+    e = mrfield.Engine(0)
+    e.set_tiles(*WAVE)
+    e.set_mode(mrfield.MODE_QUEUE)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("packed", [True, False])
+@pytest.mark.parametrize("L", [1, 2, 4, 5, 8])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_wave_queue_random_fields(qeng, L, seed, packed):
+    """The same cases in the persistent queue form: one launch of the wave
+    visit's queue kernel (two wavefronts per visit from five layers), on
+    packed and on u64 labels."""
+    args, kw, ref_d, ref_h = lean_fields_case(L, seed, packed)
+    got_d, got_h = qeng.fields(*args, **kw)
+    assert_wave_ran(qeng, 4 if packed else 8)
+    assert qeng.stats()["relax_launches"] == 1
+    for i in range(len(ref_d)):
         assert np.array_equal(got_d[i], ref_d[i]), (i, np.flatnonzero(got_d[i] != ref_d[i])[:10])
         assert np.array_equal(got_h[i], ref_h[i]), (i, np.flatnonzero(got_h[i] != ref_h[i])[:10])
 
