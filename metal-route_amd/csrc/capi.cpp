@@ -735,7 +735,7 @@ int mrf_board_set_additive(mrf_board* bd, const uint32_t* additive, size_t n, mr
   try {
     Lane lane(bd->ctx->c, false);
     if (!additive && n == 0) {
-      mrf::board_clear_additive(bd->b);
+      mrf::board_clear_additive(bd->ctx->c, bd->b);
       bd->add.clear();
     } else {
       if (n != dims_len(bd->dims) || !additive)

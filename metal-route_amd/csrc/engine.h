@@ -53,6 +53,11 @@ struct BoardView {
   // free, halo -2 mixed, else a group), or nullptr.
   const int32_t* owner;
   const int32_t* lhalo;
+  // Step prices of the lean wave visit, precomputed per cell: three planes of
+  // n u32 (x, y, via), each a pair of narrow prices (lo: entering from the
+  // lower neighbour, hi: from the upper; 0xFFFF closed).  nullptr when the
+  // board has a weight or halo-owner plane.
+  const uint32_t* steps;
 };
 
 // One complete field: a window (all layers), its sources, overrides.
@@ -110,6 +115,10 @@ struct DevBoard {
   // Host copy of the obstacle bits (cost == kMax), one bit per cell, built on
   // the device with the masks.
   std::vector<uint64_t> obst_bits;
+  // Step price pairs for the lean wave visit (3 x n u32, in the block), or
+  // nullptr (a weight or halo-owner plane); rebuilt when the additive plane
+  // changes.
+  uint32_t* steps = nullptr;
   bool obstacle(size_t g) const { return (g >> 6) < obst_bits.size() && ((obst_bits[g >> 6] >> (g & 63)) & 1u); }
   // The static planes (bmask, raw, obstacle bits, edge prices, weights) share
   // one allocation: the board's own, or, for a board that lives for one call,
@@ -339,7 +348,7 @@ void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_a
 void board_set_legal(Ctx& c, DevBoard& b, const int32_t* owner, const int32_t* halo);
 void board_clear_legal(DevBoard& b);
 // Drop the board's additive price plane.
-void board_clear_additive(DevBoard& b);
+void board_clear_additive(Ctx& c, DevBoard& b);
 
 // Solve complete fields for `reqs` (FieldReq::board indexes `boards`) in one
 // launch.  Labels stay on the device in the workspace until the next solve.
@@ -403,6 +412,7 @@ struct Group {
   bool wave = false;    // the wave form covers every field (no weight plane, legal planes, overrides, pads)
   bool halo = false;    // some board of its fields has a halo-owner plane
   bool ex = false;      // some field has own pads or via-exempt pads
+  bool steps = false;   // every board of its fields has step price planes
   // Packed u32 labels (the wave form on boards whose every open step costs
   // at least 1 and below 0xFFFF); live groups all share one label width.
   bool pack = false;

@@ -166,6 +166,7 @@ BoardView DevBoard::view() const {
   v.pad_ = 0;
   v.owner = legal && legal_owner ? owner : nullptr;
   v.lhalo = legal && legal_halo ? lhalo : nullptr;
+  v.steps = steps;
   for (int i = 0; i < 16; ++i)
     if (via_allowed.size() > (size_t)i && via_allowed[i]) v.va |= 1u << i;
   return v;
@@ -1542,7 +1543,7 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
 // from the device queues like relax_queue.  Visits of one tile never overlap
 // (the queue protocol), so the write-back stays plain device-scope stores;
 // every label is read with a device-scope load.  Lean fields only (no
-// halo-owner planes or per-net extras).
+// halo-owner planes or per-net extras), on boards with step price planes.
 template <int TW, int TH, int LM, bool PK>
 __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>::waves))
     relax_wave_queue(RelaxArgs A, QueueArgs Q) {
@@ -1589,13 +1590,9 @@ __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>:
     const uint32_t f = const_load(A.item_field + item);
     const FieldDesc F = const_load(A.fields + f);
     const BoardView B = const_load(A.boards + F.board);
-    TileResult res;
-    if (B.add16)
-      res = wave_tile<TW, TH, LM, ADD_U16, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
-    else if (B.add)
-      res = wave_tile<TW, TH, LM, ADD_U32, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
-    else
-      res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
+    // the boards' step price planes (the host runs this kernel only on boards
+    // that carry them)
+    const TileResult res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, true, true>(A, item, f, !first, F, B, smem, 0u);
     if (threadIdx.x < 64) {
       // lanes 0..3: left, right, top, bottom neighbour; lane 4: the tile
       // itself when its visit hit the in-LDS round cap
@@ -1749,6 +1746,41 @@ __global__ void to_u16(const uint32_t* a, uint16_t* o, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) o[i] = (uint16_t)a[i];
+}
+
+// The lean wave visit's step prices of every cell, as the visit would form
+// them from the masks, the additive plane and the gap prices (wave_step):
+// x plane (from x-1 | from x+1), y plane (from y-1 | from y+1), via plane
+// (from the layer below | above).  Values past the narrow limit are never
+// read (such boards do not run the wave form).
+__global__ void __launch_bounds__(256) board_steps(const uint8_t* __restrict__ bm, const uint16_t* __restrict__ add16,
+                                                   const uint32_t* __restrict__ add, const uint32_t* __restrict__ xe,
+                                                   const uint32_t* __restrict__ ye, const uint32_t* __restrict__ ve,
+                                                   uint32_t w, uint32_t h, uint32_t layers, uint32_t* __restrict__ st) {
+  const size_t n = (size_t)w * h * layers, plane = (size_t)w * h;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t l = (uint32_t)(i / plane);
+    const uint32_t r = (uint32_t)(i - (size_t)l * plane), y = r / w, x = r - y * w;
+    const uint8_t m = bm[i];
+    const uint32_t a = add16 ? add16[i] : add ? add[i] : 0u;
+    const uint32_t xl = x >= 1 ? xe[x - 1] : 0u, xr = x + 1 < w ? xe[x] : 0u;
+    const uint32_t yl = y >= 1 ? ye[y - 1] : 0u, yr = y + 1 < h ? ye[y] : 0u;
+    const uint32_t vl = l >= 1 ? ve[l - 1] : 0u, vh = l + 1 < layers ? ve[l] : 0u;
+    st[i] = pack_steps(wave_step(m, M_NX, true, xl, a), wave_step(m, M_PX, true, xr, a));
+    st[n + i] = pack_steps(wave_step(m, M_NY, true, yl, a), wave_step(m, M_PY, true, yr, a));
+    st[2 * n + i] = pack_steps(wave_step(m, M_DN, l > 0, vl, a), wave_step(m, M_UP, l + 1 < layers, vh, a));
+  }
+}
+
+// (Re)build a board's step price planes on stream s, if it has them.
+static void build_steps(hipStream_t s, const DevBoard& b) {
+  const size_t n = (size_t)b.w * b.h * b.layers;
+  if (!b.steps || !n) return;
+  const uint32_t blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(board_steps, dim3(blocks), dim3(256), 0, s, b.bmask, b.add16, b.add, b.xe, b.ye, b.ve, b.w, b.h,
+                     b.layers, b.steps);
+  HIP_OK(hipGetLastError());
 }
 
 // Activate items idx[0..n) for the first round: flags and list.
@@ -2188,13 +2220,17 @@ struct KernelSet {
   const void* round;
   const void* list;
   const void* queue;
-  // relax_wave for this shape and layer bound (narrow sets only, else
-  // nullptr), by wave_index(packed labels, halo-owner planes, per-net
-  // extras); packed labels never run with halo-owner planes
+  // relax_wave forming its prices, for this shape and layer bound (narrow
+  // sets only, else nullptr), by wave_index(packed labels, halo-owner
+  // planes, per-net extras); packed labels never run with halo-owner planes,
+  // and lean launches (neither) run wave_sp
   const void* wave[8];
   // relax_wave_queue, u64 and packed labels (narrow sets up to 8 layers at
   // 16 x 16, else nullptr)
   const void* wave_queue[2];
+  // relax_wave on the boards' step price planes (lean launches), u64 and
+  // packed labels
+  const void* wave_sp[2];
 };
 
 static constexpr int wave_index(bool pk, bool ho, bool ex) { return (pk ? 4 : 0) + (ho ? 2 : 0) + (ex ? 1 : 0); }
@@ -2203,15 +2239,15 @@ template <int TW, int TH, int LM, bool NARROW, bool LEAN>
 static KernelSet make_set() {
   KernelSet k{TW, TH, LM, NARROW, LEAN, (const void*)relax_round<TW, TH, LM, NARROW, LEAN>,
               (const void*)relax_list<TW, TH, LM, NARROW, LEAN>, (const void*)relax_queue<TW, TH, LM, NARROW, LEAN>,
-              {}, {}};
+              {}, {}, {}};
   if constexpr (NARROW && TW == 16 && TH == 16 && LM <= 8) {
+    k.wave_sp[0] = (const void*)relax_wave<TW, TH, LM, false, false, false, true>;
+    k.wave_sp[1] = (const void*)relax_wave<TW, TH, LM, true, false, false, true>;
     k.wave_queue[0] = (const void*)relax_wave_queue<TW, TH, LM, false>;
     k.wave_queue[1] = (const void*)relax_wave_queue<TW, TH, LM, true>;
-    k.wave[wave_index(false, false, false)] = (const void*)relax_wave<TW, TH, LM, false, false, false>;
     k.wave[wave_index(false, false, true)] = (const void*)relax_wave<TW, TH, LM, false, false, true>;
     k.wave[wave_index(false, true, false)] = (const void*)relax_wave<TW, TH, LM, false, true, false>;
     k.wave[wave_index(false, true, true)] = (const void*)relax_wave<TW, TH, LM, false, true, true>;
-    k.wave[wave_index(true, false, false)] = (const void*)relax_wave<TW, TH, LM, true, false, false>;
     k.wave[wave_index(true, false, true)] = (const void*)relax_wave<TW, TH, LM, true, false, true>;
   }
   return k;
@@ -2293,9 +2329,11 @@ struct RoundLaunch {
   bool wave, packed;
 };
 static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool wave_ok, bool keyed,
-                                bool pack, bool halo, bool ex) {
+                                bool pack, bool halo, bool ex, bool sp) {
   if (halo) pack = false;
-  const void* w = K.wave[wave_index(pack, halo, ex)];
+  // lean launches run on the boards' step price planes (every board without
+  // a weight or halo-owner plane has them), the others form their prices
+  const void* w = (!halo && !ex) ? (sp ? K.wave_sp[pack ? 1 : 0] : nullptr) : K.wave[wave_index(pack, halo, ex)];
   if (wave_shape(S) && w && K.narrow && wave_ok && !keyed && Lmax <= K.lm) {
     const uint32_t nt = 64u * wave_count(K.lm, S.th);
     return RoundLaunch{w, nt, wave_lds_bytes(S.tw, S.th, K.lm, pack, ex), true, pack};
@@ -2384,6 +2422,8 @@ void ctx_init(Ctx& c, int device) {
       if (w) HIP_OK(hipFuncSetAttribute(w, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
     for (const void* w : k.wave_queue)
       if (w) HIP_OK(hipFuncSetAttribute(w, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 128));
+    for (const void* w : k.wave_sp)
+      if (w) HIP_OK(hipFuncSetAttribute(w, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64));
   }
 }
 
@@ -2501,6 +2541,10 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   const size_t o_bits = carve(boff, words * 8);
   const size_t o_edges = carve(boff, ((size_t)w + h + 16) * 4);
   const size_t o_cost = unit_w ? 0 : carve(boff, n * 4);
+  // step price planes for the lean wave visit: boards without a weight or a
+  // halo-owner plane
+  const bool with_steps = unit_w && !halo && n;
+  const size_t o_steps = with_steps ? carve(boff, n * 12) : 0;
   if (transient && !c.tblock_busy) {
     grow(c.tblock, c.tblock_cap, boff);
     b.block = c.tblock;
@@ -2533,7 +2577,7 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
     b.cost = (uint32_t*)(b.block + o_cost);
     HIP_OK(hipMemcpyAsync(b.cost, d_cost, n * 4, hipMemcpyDeviceToDevice, s));
   }
-  b.bytes = n * ((unit_w ? 0 : 4) + 1 + 1);
+  b.bytes = n * ((unit_w ? 0 : 4) + 1 + 1 + (with_steps ? 12 : 0));
   const uint32_t cblocks = (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 4096));
   if (add && max_plane <= 0xFFFFu) {
     // additive prices that fit 16 bits travel as u16 (2 B per cell visit)
@@ -2550,6 +2594,10 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
     b.bytes += n * 4;
   }
   b.halo_pen = halo_pen;
+  if (with_steps) {
+    b.steps = (uint32_t*)(b.block + o_steps);
+    build_steps(s, b);
+  }
   b.obst_bits.resize(words);
   if (words) HIP_OK(hipMemcpyAsync(b.obst_bits.data(), d_bits, words * 8, hipMemcpyDeviceToHost, s));
   HIP_OK(hipGetLastError());
@@ -2577,7 +2625,9 @@ void board_set_legal(Ctx& c, DevBoard& b, const int32_t* owner, const int32_t* h
 
 void board_clear_legal(DevBoard& b) { b.legal = false; }
 
-void board_clear_additive(DevBoard& b) {
+void board_clear_additive(Ctx& c, DevBoard& b) {
+  HIP_OK(hipSetDevice(c.device));
+  if (b.add || b.add16) HIP_OK(hipStreamSynchronize(c.stream));  // no solve reads the plane any more
   if (b.add) {
     (void)hipFree(b.add);
     b.add = nullptr;
@@ -2590,6 +2640,8 @@ void board_clear_additive(DevBoard& b) {
   }
   b.max_step = std::min<uint64_t>(b.max_ew + (b.halo ? b.halo_pen : 0u), kMax - 1);
   b.min_step = b.min_ew;
+  build_steps(c.stream, b);
+  HIP_OK(hipStreamSynchronize(c.stream));
 }
 
 void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_add) {
@@ -2612,6 +2664,7 @@ void board_set_additive(Ctx& c, DevBoard& b, const uint32_t* add, uint64_t max_a
   }
   b.max_step = std::min<uint64_t>(b.max_ew + max_a, kMax - 1);
   b.min_step = b.min_ew + *std::min_element(add, add + n);
+  build_steps(c.stream, b);
   HIP_OK(hipStreamSynchronize(c.stream));
 }
 
@@ -2765,9 +2818,10 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   }
   if ((size_t)Lmax * S.th * S.tw > 16384 || S.tw < 4 || S.th < 1)
     throw Error(1, "tile shape too large for LDS");
-  bool narrow = true, lean = true, wave_ok = true, halo = false, ex = false, ovr = false;
+  bool narrow = true, lean = true, wave_ok = true, halo = false, ex = false, ovr = false, sp = true;
   for (const FieldReq& q : reqs) {
     if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
+    sp = sp && c.views[q.board].steps != nullptr;
     narrow = narrow && field_narrow(*boards[q.board], q);
     lean = lean && field_lean(c.views[q.board], q);
     wave_ok = wave_ok && field_wave(c.views[q.board], q);
@@ -2897,7 +2951,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   // The queue form runs the wave form's visits (relax_wave_queue) for lean
   // batches of the wave shape; other batches run relax_queue.
   const bool wave_queue = c.mode == MODE_QUEUE && wave_shape(S) && K.wave_queue[0] && K.narrow && wave_ok &&
-                          !halo && !ex && Lmax <= K.lm;
+                          !halo && !ex && sp && Lmax <= K.lm;
   bool pack = !c.no_pack && (c.mode == MODE_PLAIN || wave_queue) && !halo;
   uint64_t pk_step = ovr ? kNarrowMaxStep : 0;
   for (const DevBoard* b : boards) {
@@ -2905,7 +2959,8 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
     pk_step = std::max<uint64_t>(pk_step, b->max_step);
   }
   RelaxArgs A = make_args(c, item);
-  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, c.mode == MODE_PLAIN && c.delta != kMax, pack, halo, ex);
+  const RoundLaunch RL =
+      round_launch(K, S, Lmax, wave_ok, c.mode == MODE_PLAIN && c.delta != kMax, pack, halo, ex, sp);
   // labels of this solve are packed u32
   const bool packed_run = c.mode == MODE_PLAIN ? RL.packed : (wave_queue && pack);
   if (packed_run) {
@@ -3508,6 +3563,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid, b
   G.lean = true;
   G.wave = true;
   G.halo = false;
+  G.steps = true;
   bool steps_ok = true;  // every open step of its boards costs 1 .. 0xFFFE
   for (const FieldReq& q : reqs) {
     if (q.board >= d.max_boards || !d.board_used[q.board]) throw Error(1, "field names an unknown board");
@@ -3524,6 +3580,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid, b
     G.lean = G.lean && field_lean(d.views[q.board], q);
     G.wave = G.wave && field_wave(d.views[q.board], q);
     G.halo = G.halo || d.views[q.board].halo != nullptr;
+    G.steps = G.steps && d.views[q.board].steps != nullptr;
     G.ex = G.ex || field_extras(q);
     steps_ok = steps_ok && b.min_step >= 1 && b.max_step < 0xFFFFu && !b.pack_failed;
     G.pk_step = std::max<uint64_t>(G.pk_step, b.max_step);
@@ -3688,9 +3745,10 @@ uint32_t driver_step(RoundDriver& d) {
   // The tile kernel and its LDS are sized for the most layers of any live
   // group's board, not for the server's maximum.
   uint32_t Lmax = 1;
-  bool narrow = true, lean = true, wave_ok = true, halo = false, pack = false, ex = false;
+  bool narrow = true, lean = true, wave_ok = true, halo = false, pack = false, ex = false, sp = true;
   for (uint32_t g = 0; g < d.max_groups; ++g)
     if (d.groups[g].live) {
+      sp = sp && d.groups[g].steps;
       Lmax = std::max(Lmax, d.groups[g].layers);
       narrow = narrow && d.groups[g].narrow;
       lean = lean && d.groups[g].lean;
@@ -3700,7 +3758,7 @@ uint32_t driver_step(RoundDriver& d) {
       pack = d.groups[g].pack;  // the same for every live group
     }
   const KernelSet& K = pick_kernels(S.tw, S.th, Lmax, narrow, lean);
-  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, pack, halo, ex);
+  const RoundLaunch RL = round_launch(K, S, Lmax, wave_ok, false, pack, halo, ex, sp);
   if (RL.packed != pack) throw Error(1, "packed groups without the packed kernel");
   A.item_max = d.imax;
   void* args[] = {&A};

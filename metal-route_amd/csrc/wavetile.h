@@ -199,9 +199,13 @@ __device__ __forceinline__ uint32_t add_k(const BoardView& B, uint32_t g) {
 // every label is read with a device-scope load (another XCD's L2 may hold a
 // stale copy) and every existing neighbour's border counts; `touched` is
 // false on the tile's first visit (seeds) only.
-template <int TW, int TH, int LM, int AK, bool PK, bool HO, bool EX, bool QU = false>
+// SP: the board's step price planes (B.steps) hold every cell's prices: the
+// visit loads them in the layout of each phase instead of forming them from
+// the masks and planes (lean fields only: no HO, no EX; AK unused).
+template <int TW, int TH, int LM, int AK, bool PK, bool HO, bool EX, bool QU = false, bool SP = false>
 __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t item, uint32_t f, bool touched, const FieldDesc& F,
                                 const BoardView& B, char* smem, uint32_t round) {
+  static_assert(!(SP && (HO || EX)), "step price planes are for lean fields");
   using WT = WaveTile<TW, TH, LM, PK>;
   using WL = WaveLab<PK>;
   using LT = typename WL::T;
@@ -243,7 +247,20 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   // window's edge, load cell by cell (clamped to the window) and pack.
   constexpr uint32_t AW = AK == ADD_U32 ? TW : AK == ADD_U16 ? TW / 2 : 1;  // additive words
   uint32_t mw[TW / 4], aw[AW];
-  if ((B.w % 4) == 0 && ((F.x0 + wx0) % 4) == 0 && wx0 + TW <= F.ww) {
+  uint32_t sx[TW];  // row layout: from x-1 | from x+1
+  // SP: the row's x price pairs, one word per cell (clamped to the window;
+  // cells past its edge are closed below)
+  const uint64_t nb = (uint64_t)B.plane * L;  // cells of the board
+  if constexpr (SP) {
+    const auto* PX = gbl(B.steps);
+    if (wx0 + TW <= F.ww) {
+#pragma unroll
+      for (int x = 0; x < TW; ++x) sx[x] = PX[rowg + wx0 + x];
+    } else {
+#pragma unroll
+      for (int x = 0; x < TW; ++x) sx[x] = PX[rowg + min(wx0 + x, F.ww - 1)];
+    }
+  } else if ((B.w % 4) == 0 && ((F.x0 + wx0) % 4) == 0 && wx0 + TW <= F.ww) {
     const uint32_t g0 = rowg + wx0;
     const auto* mp = (const __attribute__((address_space(1))) uint32_t*)(B.bmask + g0);
 #pragma unroll
@@ -367,7 +384,33 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   const bool el = row_in && has_l, er = row_in && has_r, et = col_in && has_t, eb = col_in && has_b;
   uint8_t hbm[4];
   uint32_t ha[4];
-  {
+  uint32_t hsw[4];  // SP: the halo cells' x (left, right) and y (top, bottom) price pairs
+  // SP: the column and via layouts' price pairs, issued with the rows'
+  uint32_t sy[TH];      // column layout: from y-1 | from y+1
+  uint32_t sv[NV][LM];  // via layout: from layer-1 | from layer+1
+  constexpr uint32_t RPJ = NT / TW;  // rows per NT cells
+  const uint32_t sx_ = tid % TW, sy_ = tid / TW;
+  if constexpr (SP) {
+    const uint32_t cgx = F.x0 + wx0 + cx;
+    const auto* PX = gbl(B.steps);
+    const auto* PY = gbl(B.steps + nb);
+    const auto* PV = gbl(B.steps + 2 * nb);
+    hsw[0] = PX[el ? rowg + wx0 - 1 : rowg];
+    hsw[1] = PX[er ? rowg + wx0 + TW : rowg];
+    hsw[2] = PY[et ? (cl * B.h + F.y0 + wy0 - 1) * B.w + cgx : rowg];
+    hsw[3] = PY[eb ? (cl * B.h + F.y0 + wy0 + TH) * B.w + cgx : rowg];
+    const uint32_t ccg = F.x0 + min(wx0 + cx, F.ww - 1);  // the column's board x, clamped
+    const uint32_t cbase = ((ccol ? cl : 0u) * B.h + F.y0) * B.w + ccg;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) sy[y] = PY[cbase + min(wy0 + y, F.wh - 1) * B.w];
+    const uint32_t vg = F.x0 + min(wx0 + sx_, F.ww - 1);
+#pragma unroll
+    for (uint32_t k = 0; k < NV; ++k) {
+      const uint32_t vy = F.y0 + min(wy0 + sy_ + RPJ * k, F.wh - 1);
+#pragma unroll
+      for (int l = 0; l < LM; ++l) sv[k][l] = PV[(min((uint32_t)l, L - 1) * B.h + vy) * B.w + vg];
+    }
+  } else {
     const uint32_t cgx = F.x0 + wx0 + cx;
     const uint32_t g[4] = {el ? rowg + wx0 - 1 : rowg, er ? rowg + wx0 + TW : rowg,
                            et ? (cl * B.h + F.y0 + wy0 - 1) * B.w + cgx : rowg,
@@ -396,19 +439,19 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   }
   // gap prices: x gaps uniform (lane i <= TW holds the price of entering
   // tile column i from column i-1), y gaps and via prices per lane
-  uint32_t xgv = 0;
-  {
+  uint32_t xgv = 0, yg0 = 0, yg1 = 0, vlo = 0, vhi = 0, ytop = 0, ybot = 0;
+  if constexpr (!SP) {
     const uint32_t gx = F.x0 + wx0 + lane;
     if (lane <= TW && gx >= 1 && gx < B.w) xgv = gbl(B.xe)[gx - 1];
+    const uint32_t gyy = F.y0 + wy0 + ry;
+    yg0 = (gyy >= 1 && gyy < B.h) ? gbl(B.ye)[gyy - 1] : 0u;
+    yg1 = (gyy + 1 < B.h) ? gbl(B.ye)[gyy] : 0u;
+    vlo = (rrow && rl > 0) ? gbl(B.ve)[rl - 1] : 0u;
+    vhi = (rrow && rl + 1 < L) ? gbl(B.ve)[rl] : 0u;
+    const uint32_t gyt = F.y0 + wy0, gyb = F.y0 + wy0 + TH;
+    ytop = (gyt >= 1 && gyt < B.h) ? gbl(B.ye)[gyt - 1] : 0u;
+    ybot = (gyb >= 1 && gyb < B.h) ? gbl(B.ye)[gyb - 1] : 0u;
   }
-  const uint32_t gyy = F.y0 + wy0 + ry;
-  const uint32_t yg0 = (gyy >= 1 && gyy < B.h) ? gbl(B.ye)[gyy - 1] : 0u;
-  const uint32_t yg1 = (gyy + 1 < B.h) ? gbl(B.ye)[gyy] : 0u;
-  const uint32_t vlo = (rrow && rl > 0) ? gbl(B.ve)[rl - 1] : 0u;
-  const uint32_t vhi = (rrow && rl + 1 < L) ? gbl(B.ve)[rl] : 0u;
-  const uint32_t gyt = F.y0 + wy0, gyb = F.y0 + wy0 + TH;
-  const uint32_t ytop = (gyt >= 1 && gyt < B.h) ? gbl(B.ye)[gyt - 1] : 0u;
-  const uint32_t ybot = (gyb >= 1 && gyb < B.h) ? gbl(B.ye)[gyb - 1] : 0u;
   // neighbour states (lanes 1..4): whether their border labels exist
   uint32_t stv = 0;
   if (!QU && lane >= 1 && lane < 5) {
@@ -442,8 +485,6 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   // j / NV -- the same x for every j, so each cell's LDS slot is a constant
   // offset from one per-thread base (`sbase`, also the via layout's base).
   // Layers beyond the board's re-read its last layer (dropped below).
-  constexpr uint32_t RPJ = NT / TW;  // rows per NT cells
-  const uint32_t sx_ = tid % TW, sy_ = tid / TW;
   const uint32_t sbase = sy_ * RS + sx_;
   LT sl[NS];
   if (touched || QU) {
@@ -466,9 +507,17 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   hb = (nbv & 8u) ? WL::halo(hb) : WL::unr;
   // prices of stepping out of the tile into the halo cells: the cell's edge
   // facing the tile must be open and the cell passable (narrow: at most 0x7FFE)
-  const uint32_t xleft = __builtin_amdgcn_readlane(xgv, 0), xright = __builtin_amdgcn_readlane(xgv, TW);
-  uint32_t p_lr = pack_steps(wave_step(hbm[0], M_PX, true, xleft, ha[0]), wave_step(hbm[1], M_NX, true, xright, ha[1]));
-  uint32_t p_tb = pack_steps(wave_step(hbm[2], M_PY, true, ytop, ha[2]), wave_step(hbm[3], M_NY, true, ybot, ha[3]));
+  uint32_t p_lr, p_tb;
+  if constexpr (SP) {
+    // entering the left / top halo cell from the tile is its step from x+1 /
+    // y+1 (hi half), the right / bottom one's from x-1 / y-1 (lo half)
+    p_lr = (el ? hsw[0] >> 16 : 0xFFFFu) | ((er ? hsw[1] & 0xFFFFu : 0xFFFFu) << 16);
+    p_tb = (et ? hsw[2] >> 16 : 0xFFFFu) | ((eb ? hsw[3] & 0xFFFFu : 0xFFFFu) << 16);
+  } else {
+    const uint32_t xleft = __builtin_amdgcn_readlane(xgv, 0), xright = __builtin_amdgcn_readlane(xgv, TW);
+    p_lr = pack_steps(wave_step(hbm[0], M_PX, true, xleft, ha[0]), wave_step(hbm[1], M_NX, true, xright, ha[1]));
+    p_tb = pack_steps(wave_step(hbm[2], M_PY, true, ytop, ha[2]), wave_step(hbm[3], M_NY, true, ybot, ha[3]));
+  }
   // formed here, so the halo cells' planes are not kept through the sweeps
   asm("" : "+v"(p_lr), "+v"(p_tb));
 
@@ -498,7 +547,23 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
   }
 
   // ---- step prices of every cell (row layout) ---------------------------------
-  uint32_t sx[TW];  // row layout: from x-1 | from x+1
+  if constexpr (SP) {
+    // loaded above; cells outside the window (edge tiles) and rows, columns
+    // and via cells beyond the board's layers are closed, as the formed
+    // prices close them
+    if (wx0 + TW > F.ww || wy0 + TH > F.wh || L < LM) {
+      const bool cin = ccol && wx0 + cx < F.ww, vin = wx0 + sx_ < F.ww;
+#pragma unroll
+      for (int x = 0; x < TW; ++x) sx[x] = (row_in && wx0 + x < F.ww) ? sx[x] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int y = 0; y < TH; ++y) sy[y] = (cin && wy0 + y < F.wh) ? sy[y] : 0xFFFFFFFFu;
+#pragma unroll
+      for (uint32_t k = 0; k < NV; ++k)
+#pragma unroll
+        for (int l = 0; l < LM; ++l)
+          sv[k][l] = ((uint32_t)l < L && vin && wy0 + sy_ + RPJ * k < F.wh) ? sv[k][l] : 0xFFFFFFFFu;
+    }
+  } else {
   uint32_t sv_st[TW];  // via prices of the row, staged after the y prices
   {
     uint32_t xg[TW + 1];
@@ -530,8 +595,6 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
     }
   }
   __syncthreads();
-  uint32_t sy[TH];        // column layout: from y-1 | from y+1
-  uint32_t sv[NV][LM];    // via layout: from layer-1 | from layer+1
 #pragma unroll
   for (int y = 0; y < TH; ++y) sy[y] = ccol ? lab32[(cl * TH + y) * RS + cx] : kMax;
   if (PK) {
@@ -546,6 +609,7 @@ __device__ __forceinline__ TileResult wave_tile(const RelaxArgs& A, uint32_t ite
     for (int l = 0; l < LM; ++l) sv[k][l] = (uint32_t)l < L ? lab32[VOFF + sbase + (l * TH + RPJ * k) * RS] : kMax;
   }
   __syncthreads();
+  }
   // ---- labels into the buffer -------------------------------------------------
   const bool sx_in = wx0 + sx_ < F.ww;
 #pragma unroll
@@ -800,7 +864,8 @@ __device__ __forceinline__ TileResult wave_visit(const RelaxArgs& A, uint32_t it
   return wave_tile<TW, TH, LM, AK, PK, false, EX>(A, item, f, touched, F, B, smem, A.round);
 }
 
-template <int TW, int TH, int LM, bool PK, bool HO, bool EX>
+// SP: every board of the launch carries step price planes (lean launches).
+template <int TW, int TH, int LM, bool PK, bool HO, bool EX, bool SP = false>
 __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>::waves)) relax_wave(RelaxArgs A) {
   extern __shared__ __align__(16) char smem[];
   const uint32_t item = round_item(A);
@@ -810,7 +875,9 @@ __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>:
   const FieldDesc F = const_load(A.fields + f);
   const BoardView B = const_load(A.boards + F.board);
   TileResult res;
-  if (B.add16)
+  if constexpr (SP)
+    res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, false, true>(A, item, f, touched, F, B, smem, A.round);
+  else if (B.add16)
     res = wave_visit<TW, TH, LM, PK, HO, EX, ADD_U16>(A, item, f, touched, F, B, smem);
   else if (B.add)
     res = wave_visit<TW, TH, LM, PK, HO, EX, ADD_U32>(A, item, f, touched, F, B, smem);
