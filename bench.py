@@ -732,7 +732,11 @@ def main():
             # visit; packed 4 B labels move 8 fewer (the survey's unit above
             # counts u64 labels either way)
             "label_bytes": int(max(lbytes)) if lbytes else 8,
-            "bytes_moved_per_cell_visit": bpc - 2 * (8 - (int(max(lbytes)) if lbytes else 8)),
+            # the wave form's lean visits load 12 B of step price pairs per
+            # cell from the board's step planes instead of its 1-7 B of planes
+            "bytes_moved_per_cell_visit": (2 * (int(max(lbytes)) if lbytes else 8)
+                                           + (12 if kernel.startswith("relax_wave") and a.config != 4
+                                              else bpc - 16)),
             "cell_visits_per_launch": int(cells_per_launch),
             "achieved_from_traffic": (round(traffic / (avg_launch_ms * 1e-3) / 1e9, 2) if traffic else None),
             "traffic_over_algorithmic": (round(traffic / (cells_per_launch * bpc), 3) if traffic else None),

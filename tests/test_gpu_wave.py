@@ -370,3 +370,41 @@ def test_wave_packed_certificate_falls_back_to_wide_labels(eng):
             assert (r is None) == (ref_p[i] is None), (price, i)
             if r is not None:
                 assert r[0] == ref_p[i] and r[1] == ref_c[i], (price, i)
+
+
+@pytest.mark.parametrize("L", [2, 4])
+def test_wave_resident_board_additive_updates(eng, L):
+    """A resident board's step price planes follow its additive plane: solve,
+    replace the plane (held as u32 from then on), solve, drop it, solve; every
+    field equals the oracle's under the plane in force, and the lean wave
+    kernel ran each time."""
+    rng = np.random.default_rng(90 + L)
+    w, h = 53, 41
+    g = unit_board(77 + L, w, h, L)
+    edges = mrcpu.Edges([int(v) for v in rng.integers(1, 30, size=w - 1)],
+                        [int(v) for v in rng.integers(1, 30, size=h - 1)],
+                        [int(rng.integers(1, 60)) for _ in range(L - 1)])
+    dedges = mrfield.Edges(list(edges.x), list(edges.y), list(edges.via))
+    free = free_cells(g)
+    sources = [[int(c) for c in rng.choice(free, size=1 + i % 2, replace=False)] for i in range(6)]
+    dsts = [int(rng.choice(free)) for _ in range(6)]
+    nets = [mrcpu.Net(s[0], d) for s, d in zip(sources, dsts)]
+    wins = [(0, 0, w - 1, h - 1)] * len(nets)
+    planes = [rng.integers(0, 40, size=g.n).astype(np.uint32),
+              rng.integers(7_000, 9_000, size=g.n).astype(np.uint32),
+              None]
+    board = mrfield.Board(eng, as_dev(g), dedges, mrfield.Pricing(additive=planes[0]))
+    try:
+        for k, add in enumerate(planes):
+            if k:
+                board.set_additive(add)
+            board.solve_fields(sources, dsts=dsts)
+            assert_wave_ran(eng)
+            spec = mrcpu.Spec(g, edges, additive=add) if add is not None else mrcpu.Spec(g, edges)
+            ref_d, ref_h = mrcpu.fields_batch(spec, nets, sources, wins)
+            for i in range(len(nets)):
+                got_d, got_h = board.read_field(i)
+                assert np.array_equal(got_d, ref_d[i]), (k, i, np.flatnonzero(got_d != ref_d[i])[:10])
+                assert np.array_equal(got_h, ref_h[i]), (k, i)
+    finally:
+        board.close()
