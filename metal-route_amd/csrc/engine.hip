@@ -1543,8 +1543,8 @@ __global__ void __launch_bounds__(256, (MinWaves<TW, LM>::round)) relax_queue(Re
 // from the device queues like relax_queue.  Visits of one tile never overlap
 // (the queue protocol), so the write-back stays plain device-scope stores;
 // every label is read with a device-scope load.  Lean fields only (no
-// halo-owner planes or per-net extras), on boards with step price planes.
-template <int TW, int TH, int LM, bool PK>
+// halo-owner planes or per-net extras); SP as in relax_wave.
+template <int TW, int TH, int LM, bool PK, bool SP>
 __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>::waves))
     relax_wave_queue(RelaxArgs A, QueueArgs Q) {
   extern __shared__ __align__(16) char smem[];
@@ -1590,9 +1590,17 @@ __global__ void __launch_bounds__(64 * wave_count(LM, TH), (WaveMin<TW, LM, PK>:
     const uint32_t f = const_load(A.item_field + item);
     const FieldDesc F = const_load(A.fields + f);
     const BoardView B = const_load(A.boards + F.board);
-    // the boards' step price planes (the host runs this kernel only on boards
-    // that carry them)
-    const TileResult res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, true, true>(A, item, f, !first, F, B, smem, 0u);
+    // SP: the boards' step price planes (the host picks it only for boards
+    // that carry them), else prices formed from the planes
+    TileResult res;
+    if constexpr (SP)
+      res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, true, true>(A, item, f, !first, F, B, smem, 0u);
+    else if (B.add16)
+      res = wave_tile<TW, TH, LM, ADD_U16, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
+    else if (B.add)
+      res = wave_tile<TW, TH, LM, ADD_U32, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
+    else
+      res = wave_tile<TW, TH, LM, ADD_NONE, PK, false, false, true>(A, item, f, !first, F, B, smem, 0u);
     if (threadIdx.x < 64) {
       // lanes 0..3: left, right, top, bottom neighbour; lane 4: the tile
       // itself when its visit hit the in-LDS round cap
@@ -2222,12 +2230,11 @@ struct KernelSet {
   const void* queue;
   // relax_wave forming its prices, for this shape and layer bound (narrow
   // sets only, else nullptr), by wave_index(packed labels, halo-owner
-  // planes, per-net extras); packed labels never run with halo-owner planes,
-  // and lean launches (neither) run wave_sp
+  // planes, per-net extras); packed labels never run with halo-owner planes
   const void* wave[8];
-  // relax_wave_queue, u64 and packed labels (narrow sets up to 8 layers at
-  // 16 x 16, else nullptr)
-  const void* wave_queue[2];
+  // relax_wave_queue, [2 * step planes + packed labels] (narrow sets up to
+  // 8 layers at 16 x 16, else nullptr)
+  const void* wave_queue[4];
   // relax_wave on the boards' step price planes (lean launches), u64 and
   // packed labels
   const void* wave_sp[2];
@@ -2243,11 +2250,15 @@ static KernelSet make_set() {
   if constexpr (NARROW && TW == 16 && TH == 16 && LM <= 8) {
     k.wave_sp[0] = (const void*)relax_wave<TW, TH, LM, false, false, false, true>;
     k.wave_sp[1] = (const void*)relax_wave<TW, TH, LM, true, false, false, true>;
-    k.wave_queue[0] = (const void*)relax_wave_queue<TW, TH, LM, false>;
-    k.wave_queue[1] = (const void*)relax_wave_queue<TW, TH, LM, true>;
+    k.wave_queue[0] = (const void*)relax_wave_queue<TW, TH, LM, false, false>;
+    k.wave_queue[1] = (const void*)relax_wave_queue<TW, TH, LM, true, false>;
+    k.wave_queue[2] = (const void*)relax_wave_queue<TW, TH, LM, false, true>;
+    k.wave_queue[3] = (const void*)relax_wave_queue<TW, TH, LM, true, true>;
+    k.wave[wave_index(false, false, false)] = (const void*)relax_wave<TW, TH, LM, false, false, false>;
     k.wave[wave_index(false, false, true)] = (const void*)relax_wave<TW, TH, LM, false, false, true>;
     k.wave[wave_index(false, true, false)] = (const void*)relax_wave<TW, TH, LM, false, true, false>;
     k.wave[wave_index(false, true, true)] = (const void*)relax_wave<TW, TH, LM, false, true, true>;
+    k.wave[wave_index(true, false, false)] = (const void*)relax_wave<TW, TH, LM, true, false, false>;
     k.wave[wave_index(true, false, true)] = (const void*)relax_wave<TW, TH, LM, true, false, true>;
   }
   return k;
@@ -2331,14 +2342,25 @@ struct RoundLaunch {
 static RoundLaunch round_launch(const KernelSet& K, const TileShape& S, uint32_t Lmax, bool wave_ok, bool keyed,
                                 bool pack, bool halo, bool ex, bool sp) {
   if (halo) pack = false;
-  // lean launches run on the boards' step price planes (every board without
-  // a weight or halo-owner plane has them), the others form their prices
-  const void* w = (!halo && !ex) ? (sp ? K.wave_sp[pack ? 1 : 0] : nullptr) : K.wave[wave_index(pack, halo, ex)];
+  // lean launches whose boards all carry step price planes load their
+  // prices, the others form them
+  const void* w = (sp && !halo && !ex) ? K.wave_sp[pack ? 1 : 0] : K.wave[wave_index(pack, halo, ex)];
   if (wave_shape(S) && w && K.narrow && wave_ok && !keyed && Lmax <= K.lm) {
     const uint32_t nt = 64u * wave_count(K.lm, S.th);
     return RoundLaunch{w, nt, wave_lds_bytes(S.tw, S.th, K.lm, pack, ex), true, pack};
   }
   return RoundLaunch{K.round, workgroup_threads(S), lds_bytes(S.tw, S.th, Lmax, K.narrow), false, false};
+}
+
+// Whether a lean wave visit on this board loads its step prices from the
+// board's step price planes rather than forming them.  Measured: loading wins
+// where the board has an additive plane (the formed visit loads it and adds
+// it into every price; C2, C3) or where the planes (12 B per cell) stay in an
+// XCD's L2 (C1: 13 % less kernel time); on a board without an additive plane
+// whose planes do not fit L2, forming from the 1 B mask is cheaper (C2 board
+// through the provider seam: 1.31 ms of relaxation formed, 1.51 ms loaded).
+static bool use_steps(const BoardView& v) {
+  return v.steps && (v.add || v.add16 || (uint64_t)v.plane * v.layers * 12 <= (2ull << 20));
 }
 
 // Whether a field and its board need none of what the lean kernels leave
@@ -2542,8 +2564,9 @@ void board_upload(Ctx& c, DevBoard& b, uint32_t w, uint32_t h, uint32_t layers, 
   const size_t o_edges = carve(boff, ((size_t)w + h + 16) * 4);
   const size_t o_cost = unit_w ? 0 : carve(boff, n * 4);
   // step price planes for the lean wave visit: boards without a weight or a
-  // halo-owner plane
-  const bool with_steps = unit_w && !halo && n;
+  // halo-owner plane, where the visit would load them (use_steps: an
+  // additive plane, or planes that fit an XCD's L2)
+  const bool with_steps = unit_w && !halo && n && (add || n * 12 <= (2ull << 20));
   const size_t o_steps = with_steps ? carve(boff, n * 12) : 0;
   if (transient && !c.tblock_busy) {
     grow(c.tblock, c.tblock_cap, boff);
@@ -2821,7 +2844,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   bool narrow = true, lean = true, wave_ok = true, halo = false, ex = false, ovr = false, sp = true;
   for (const FieldReq& q : reqs) {
     if (q.board >= boards.size()) throw Error(1, "field names a board outside the batch");
-    sp = sp && c.views[q.board].steps != nullptr;
+    sp = sp && use_steps(c.views[q.board]);
     narrow = narrow && field_narrow(*boards[q.board], q);
     lean = lean && field_lean(c.views[q.board], q);
     wave_ok = wave_ok && field_wave(c.views[q.board], q);
@@ -2951,7 +2974,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
   // The queue form runs the wave form's visits (relax_wave_queue) for lean
   // batches of the wave shape; other batches run relax_queue.
   const bool wave_queue = c.mode == MODE_QUEUE && wave_shape(S) && K.wave_queue[0] && K.narrow && wave_ok &&
-                          !halo && !ex && sp && Lmax <= K.lm;
+                          !halo && !ex && Lmax <= K.lm;
   bool pack = !c.no_pack && (c.mode == MODE_PLAIN || wave_queue) && !halo;
   uint64_t pk_step = ovr ? kNarrowMaxStep : 0;
   for (const DevBoard* b : boards) {
@@ -3143,7 +3166,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
       launch_fill32(s, (uint32_t*)W.labels, 0xFFFFFFFFu, (size_t)lab_off * 2);
       hipLaunchKernelGGL(seed_list, dim3(((uint32_t)seeds.size() + 255) / 256), dim3(256), 0, s, W.act, W.list,
                          d_seeds, (uint32_t)seeds.size(), (uint32_t*)nullptr);
-      const void* qfn = wave_queue ? K.wave_queue[pack ? 1 : 0] : K.queue;
+      const void* qfn = wave_queue ? K.wave_queue[(sp ? 2 : 0) + (pack ? 1 : 0)] : K.queue;
       const uint32_t qthreads = wave_queue ? 64u * wave_count(K.lm, S.th) : workgroup_threads(S);
       const size_t qlds = wave_queue ? wave_lds_bytes(S.tw, S.th, K.lm, pack, false) : lds;
       int occ = 0;
@@ -3580,7 +3603,7 @@ bool driver_admit(RoundDriver& d, const std::vector<FieldReq>& reqs, int& gid, b
     G.lean = G.lean && field_lean(d.views[q.board], q);
     G.wave = G.wave && field_wave(d.views[q.board], q);
     G.halo = G.halo || d.views[q.board].halo != nullptr;
-    G.steps = G.steps && d.views[q.board].steps != nullptr;
+    G.steps = G.steps && use_steps(d.views[q.board]);
     G.ex = G.ex || field_extras(q);
     steps_ok = steps_ok && b.min_step >= 1 && b.max_step < 0xFFFFu && !b.pack_failed;
     G.pk_step = std::max<uint64_t>(G.pk_step, b.max_step);

@@ -172,10 +172,14 @@ def test_queue_config3_windowed_batch(eng, shape):
             assert gp == (ref_p[k], ref_c[k]), i
 
 
+@pytest.mark.parametrize("history", [True, False])
 @pytest.mark.parametrize("shape", QSHAPES)
-def test_queue_config2_equals_plain(eng, shape):
+def test_queue_config2_equals_plain(eng, shape, history):
+    """The config-2 board with its history plane (the wave visits load their
+    step prices) and without (a 12.6 MB step plane would not fit L2: the
+    visits form their prices), queue form against plain form."""
     s = mrfield.synth_config(2, seed=1)
-    board = mrfield.Board(eng, s.grid, s.edges, s.pricing())
+    board = mrfield.Board(eng, s.grid, s.edges, s.pricing() if history else None)
     try:
         eng.set_tiles(*shape)
         got = board.solve_arrays(s.nets())

@@ -408,3 +408,24 @@ def test_wave_resident_board_additive_updates(eng, L):
                 assert np.array_equal(got_h, ref_h[i]), (k, i)
     finally:
         board.close()
+
+
+def test_wave_formed_prices_on_a_large_lean_board(eng):
+    """The config-2 board without its history plane through the provider
+    entry: its step planes would not fit an XCD's L2, so the lean wave visit
+    forms its prices from the masks instead of loading them.  32 nets' paths
+    and search costs equal the oracle's alone search."""
+    s = mrfield.synth_config(2, seed=1)
+    g = s.grid
+    og = mrcpu.Grid(g.w, g.h, g.layers, g.cost, board=g.board)
+    oe = mrcpu.Edges(s.edges.x, s.edges.y, s.edges.via)
+    idx = list(range(0, 256, 8))
+    nets = [mrcpu.Net(int(s.src[i]), int(s.dst[i])) for i in idx]
+    full = [(0, 0, g.w - 1, g.h - 1)] * len(nets)
+    got = eng.route_isolated_batch(g, [s.nets()[i] for i in idx], full, s.edges)
+    assert_wave_ran(eng)
+    ref_p, ref_c = mrcpu.alone_batch(mrcpu.Spec(og, oe), nets, full, threads=8)
+    for k, r in enumerate(got):
+        assert (r is None) == (ref_p[k] is None), idx[k]
+        if r is not None:
+            assert r == (ref_p[k], ref_c[k]), idx[k]
