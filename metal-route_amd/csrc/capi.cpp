@@ -1062,12 +1062,13 @@ int mrf_autotune(mrf_ctx* ctx, mrf_board* bd, const mrf_net* nets, size_t n_nets
     const uint32_t shapes[][2] = {{32, 16}, {16, 16}, {16, 8}, {8, 4}};
     for (auto& sh : shapes) {
       if (mrf::lds_bytes_for(sh[0], sh[1], dims.layers) > 160 * 1024 - 64) continue;
-      // 64 threads on 16 x 16 tiles is the wave form (plain form only; the
-      // other forms run that shape at 256 threads)
+      // 64 threads on 16 x 16 tiles is the wave form (plain form, and the
+      // queue form's wave visits; the graph form runs that shape at 256
+      // threads)
       for (uint32_t threads : {64u, 128u, 256u})
         for (uint32_t ml : {4u, 32u})
           for (int mode : {(int)MRF_MODE_PLAIN, (int)MRF_MODE_GRAPH, (int)MRF_MODE_QUEUE}) {
-            if (threads == 64 && (sh[0] != 16 || sh[1] != 16 || mode != MRF_MODE_PLAIN)) continue;
+            if (threads == 64 && (sh[0] != 16 || sh[1] != 16 || mode == MRF_MODE_GRAPH)) continue;
             std::vector<std::pair<uint32_t, uint32_t>> forms;  // (graph rounds, delta)
             if (mode == MRF_MODE_GRAPH) {
               forms = {{4, kMax}, {16, kMax}};
