@@ -72,6 +72,9 @@ def parse():
                     help="board sides of the serving mix (boards cycle through them)")
     ap.add_argument("--admit-small", type=int, default=0,
                     help="serving: tiles up to which a request may go ahead (0 = library default, 1 = arrival order)")
+    ap.add_argument("--time-every", type=int, default=4,
+                    help="plain form: HIP events around every round of every N-th timed step (1 = every step); "
+                         "the roofline's launch duration comes from those steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-seam", action="store_true", help="skip the provider-seam measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -631,9 +634,13 @@ def main():
 
     lat, relax_ms, path_ms, visits, cells_visited, rounds_l, host_rounds = [], [], [], [], [], [], []
     kern_ms, launches, changed, valid_ms, call_ms, fcells, wave, lbytes = [], [], [], [], [], [], [], []
+    timed_l = []
     got = None
     for i in range(a.warmup + a.steps):
         if i == a.warmup:
+            # per-round events (about 3 us of host turnaround a round) in the
+            # first of every --time-every steps of the timed region
+            eng.set_round_timing(max(1, a.time_every))
             t_start = time.perf_counter()
         t0 = time.perf_counter()
         got = step()
@@ -652,6 +659,7 @@ def main():
             valid_ms.append(st["validate_ms"])
             call_ms.append(st["total_ms"])
             launches.append(st["relax_launches"])
+            timed_l.append(st["timed_launches"])
             wave.append(st["wave_launches"])
             lbytes.append(st["label_bytes"])
             fcells.append(st["field_cells"])
@@ -660,9 +668,15 @@ def main():
     value = n_nets * a.steps / wall
     routed = int(np.count_nonzero(got.routed)) if got is not None else None
     bpc = BYTES_PER_CELL_VISIT[a.config]
-    # per launch of the relaxation kernel: algorithmic bytes / average duration
-    n_launch = float(np.mean(launches))
-    cells_per_launch = float(np.mean(cells_visited)) / max(n_launch, 1.0)
+    # per launch of the relaxation kernel: algorithmic bytes / average duration,
+    # over the steps whose every launch carried its HIP event pair (plain
+    # form; the other forms time their launch or replays in every step)
+    tsel = [k for k in range(len(kern_ms)) if launches[k] and timed_l[k] == launches[k]]
+    if form != "plain" or not tsel:
+        tsel = list(range(len(kern_ms)))
+    kern_ms = [kern_ms[k] for k in tsel]
+    n_launch = float(np.mean([launches[k] for k in tsel]))
+    cells_per_launch = float(np.mean([cells_visited[k] for k in tsel])) / max(n_launch, 1.0)
     avg_launch_ms = float(np.mean(kern_ms)) / max(n_launch, 1.0)
     achieved = cells_per_launch * bpc / (avg_launch_ms * 1e-3) / 1e9
     workload = f"c{a.config}-{g.w}x{g.h}x{g.layers}-{n_nets}nets-t{tw}x{th}"
@@ -710,6 +724,8 @@ def main():
         "routed": routed,
         "relax_phase_ms": round(float(np.mean(relax_ms)), 4),
         "relax_kernel_ms": round(float(np.mean(kern_ms)), 4),
+        "kernel_timing": (f"HIP events around every launch in {len(tsel)} of {a.steps} timed steps"
+                          + (f" (every {max(1, a.time_every)}th)" if form == "plain" else "")),
         "relax_launches": int(n_launch),
         "relax_avg_launch_ms": round(avg_launch_ms, 5),
         "path_ms": round(float(np.mean(path_ms)), 4),

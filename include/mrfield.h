@@ -154,7 +154,8 @@ typedef struct {
     uint32_t host_rounds;  /* launches of the round kernel (+1 for the persistent kernel) */
     double persistent_ms;  /* persistent form: the work-queue kernel alone */
   double relax_kernel_ms;  /* relaxation kernels alone: sum of per-launch HIP event times (plain
-                              form; the graph form reports its replay time) */
+                              form in calls timed by mrf_set_round_timing, else the relaxation
+                              phase; the graph form reports its replay time) */
   uint32_t relax_launches; /* relaxation kernel launches */
   uint64_t changed_visits; /* tile visits that improved at least one label */
   double validate_ms;      /* host re-validation of the device paths */
@@ -162,6 +163,8 @@ typedef struct {
                               64 threads; lean unit-weight narrow batches in the plain form) */
   uint32_t label_bytes;    /* label width of the last solve: 8, or 4 for packed labels (wave form on
                               boards whose every open step costs at least 1, certified per batch) */
+  uint32_t timed_launches; /* plain-form launches whose event times relax_kernel_ms sums (all of
+                              them in a timed call, none otherwise: see mrf_set_round_timing) */
 } mrf_stats;
 
 /* Solve form of a context.
@@ -206,6 +209,13 @@ int mrf_set_mode(mrf_ctx *ctx, int mode);
  * do not depend on it. */
 int mrf_set_delta(mrf_ctx *ctx, uint32_t delta);
 int mrf_last_stats(mrf_ctx *ctx, mrf_stats *out);
+/* Measurement control, no counterpart in the reference: which solving calls
+ * bracket each plain-form relaxation launch with a pair of HIP events (about
+ * 3 us of host turnaround per round).  0 (the default): none; 1: every call;
+ * k: the first of every k calls from now on.  An untimed call reports its
+ * relaxation phase as relax_kernel_ms and timed_launches 0.  Results do not
+ * depend on it. */
+int mrf_set_round_timing(mrf_ctx *ctx, uint32_t every);
 
 /* Label memory (bytes) one device batch may hold; a call whose fields need
  * more is split into batches that fit (mr-metal's MAX_BATCH_CELLS /

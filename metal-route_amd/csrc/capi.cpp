@@ -178,7 +178,10 @@ struct Lane {
   explicit Lane(mrf::Ctx& c, bool solve = true) : lk(c.lane, std::try_to_lock) {
     if (!lk.owns_lock()) throw Error(MRF_BACKEND_UNAVAILABLE, "field engine lane is busy");
     if (hipSetDevice(c.device) != hipSuccess) throw Error(MRF_BACKEND_UNAVAILABLE, "hipSetDevice failed");
-    if (solve) c.last = mrf::SolveStats{};
+    if (solve) {
+      c.last = mrf::SolveStats{};
+      c.time_rounds = c.timing_every && (c.timing_calls++ % c.timing_every) == 0;
+    }
   }
 };
 
@@ -412,6 +415,15 @@ int mrf_last_stats(mrf_ctx* ctx, mrf_stats* out) {
   out->validate_ms = s.validate_ms;
   out->wave_launches = s.wave_launches;
   out->label_bytes = s.label_bytes;
+  out->timed_launches = s.timed_launches;
+  return MRF_OK;
+}
+
+int mrf_set_round_timing(mrf_ctx* ctx, uint32_t every) {
+  if (!ctx) return MRF_BACKEND_UNAVAILABLE;
+  std::lock_guard<std::mutex> g(ctx->c.lane);
+  ctx->c.timing_every = every;
+  ctx->c.timing_calls = 0;
   return MRF_OK;
 }
 

@@ -170,6 +170,7 @@ struct SolveStats {
   double validate_ms = 0;      // host re-validation of device paths
   uint32_t launches = 0;       // relaxation kernel launches
   uint32_t wave_launches = 0;  // of which ran the one-wavefront-per-tile kernel
+  uint32_t timed_launches = 0; // of which kernel_ms holds the per-launch event times
   uint32_t label_bytes = 8;    // label width of the solve (4: packed u32 labels)
   void add(const SolveStats& o) {
     relax_ms += o.relax_ms;
@@ -187,6 +188,7 @@ struct SolveStats {
     validate_ms += o.validate_ms;
     launches += o.launches;
     wave_launches += o.wave_launches;
+    timed_launches += o.timed_launches;
     label_bytes = o.label_bytes;
   }
 };
@@ -298,6 +300,13 @@ struct Ctx {
   // Plain form: how long the host spins on a launch's stamp before it hands
   // over to a stream synchronisation (microseconds).
   uint32_t stamp_spin_us = 20000;
+  // Plain-form rounds bracketed by HIP events: in every `timing_every`-th
+  // solving call (1: every call, 0: none, the default: the event pairs cost
+  // host turnaround every round); `time_rounds` is this call's decision,
+  // `timing_calls` counts the calls.
+  uint32_t timing_every = 0;
+  uint64_t timing_calls = 0;
+  bool time_rounds = false;
   // Queue form: the longest a workgroup waits for work or a queue slot, and
   // the longest one runs, before the launch stops with an error (ms).
   uint32_t queue_wait_ms = 2000, queue_run_ms = 20000;

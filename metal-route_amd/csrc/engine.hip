@@ -3034,9 +3034,10 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         if (++W.stamp == 0) W.stamp = 1;
         A.stamp = W.stamp;
         A.round = rounds;
-        // per-round kernel times: not in allocation-free calls, which create
-        // no events
-        const bool per_round = !c.arena;
+        // per-round kernel times: in the calls the context times (every
+        // call unless set otherwise), never in allocation-free calls, which
+        // create no events
+        const bool per_round = !c.arena && c.time_rounds;
         if (per_round && pairs == 512) harvest(true);  // bounds the event pool
         const size_t ei = 2 * pairs;
         if (per_round) {
@@ -3061,6 +3062,7 @@ void solve_fields(Ctx& c, const std::vector<const DevBoard*>& boards, const std:
         std::atomic_thread_fence(std::memory_order_acquire);
         ++local.launches;
         if (RL.wave) ++local.wave_launches;
+        if (per_round) ++local.timed_launches;
         ++rounds;
         if (hc[1]) break;
         n_active = hc[0];

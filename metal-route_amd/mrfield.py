@@ -75,7 +75,8 @@ class StatsS(C.Structure):
                 ("tile_visits", C.c_uint64), ("local_rounds", C.c_uint64), ("cells_visited", C.c_uint64),
                 ("field_cells", C.c_uint64), ("host_rounds", C.c_uint32), ("persistent_ms", C.c_double),
                 ("relax_kernel_ms", C.c_double), ("relax_launches", C.c_uint32), ("changed_visits", C.c_uint64),
-                ("validate_ms", C.c_double), ("wave_launches", C.c_uint32), ("label_bytes", C.c_uint32)]
+                ("validate_ms", C.c_double), ("wave_launches", C.c_uint32), ("label_bytes", C.c_uint32),
+                ("timed_launches", C.c_uint32)]
 
 
 class TuningS(C.Structure):
@@ -176,6 +177,7 @@ def lib():
         "mrf_set_tiles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, E]),
         "mrf_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
         "mrf_set_delta": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "mrf_set_round_timing": (C.c_int, [C.c_void_p, C.c_uint32]),
         "mrf_last_stats": (C.c_int, [C.c_void_p, C.POINTER(StatsS)]),
         "mrf_route_isolated_batch": (C.c_int, [C.c_void_p, Dims, C.POINTER(GridS), C.POINTER(NetS), C.c_size_t,
                                                C.POINTER(WindowS), C.POINTER(EdgesS), C.POINTER(PathsS), E]),
@@ -493,6 +495,12 @@ class Engine:
         """Delta-stepping window of the plain form (MAX = off)."""
         if lib().mrf_set_delta(self._h, delta) != OK:
             raise MrfError(BACKEND_UNAVAILABLE, "bad delta")
+
+    def set_round_timing(self, every: int):
+        """Bracket each plain-form round with HIP events in every `every`-th
+        solving call (1 = every call; 0 = none, the library default)."""
+        if lib().mrf_set_round_timing(self._h, int(every)) != OK:
+            raise MrfError(BACKEND_UNAVAILABLE, "bad round timing")
 
     def set_label_budget(self, nbytes: int):
         """Label bytes one device batch may hold (0 = the library default)."""

@@ -429,3 +429,29 @@ def test_wave_formed_prices_on_a_large_lean_board(eng):
         assert (r is None) == (ref_p[k] is None), idx[k]
         if r is not None:
             assert r == (ref_p[k], ref_c[k]), idx[k]
+
+
+def test_round_timing_sampled_calls(eng):
+    """mrf_set_round_timing(2): the first of every two solving calls brackets
+    each round with HIP events (timed_launches = relax_launches), the other
+    runs without them; 0 restores the untimed default.  Results never
+    change."""
+    s = mrfield.synth_config(1, seed=5, scale_nets=8)
+    g = s.grid
+    full = [(0, 0, g.w - 1, g.h - 1)] * len(s.src)
+    try:
+        eng.set_round_timing(2)
+        seen = []
+        for _ in range(4):
+            got = eng.route_isolated_batch(g, s.nets(), full, s.edges)
+            st = eng.stats()
+            assert st["relax_launches"] > 0
+            seen.append((got, st["timed_launches"] == st["relax_launches"], st["timed_launches"] == 0))
+        assert [t for _, t, _ in seen] == [True, False, True, False]
+        assert [u for _, _, u in seen] == [False, True, False, True]
+        assert all(got == seen[0][0] for got, _, _ in seen)
+        eng.set_round_timing(0)
+        assert eng.route_isolated_batch(g, s.nets(), full, s.edges) == seen[0][0]
+        assert eng.stats()["timed_launches"] == 0
+    finally:
+        eng.set_round_timing(0)
