@@ -458,7 +458,8 @@ typedef struct {
 typedef struct {
     uint64_t submitted, completed, failed;
     uint64_t rounds, tile_visits; /* tile_visits: up to the last request completion */
-    double kernel_ms;         /* relaxation kernels, HIP events (up to the last request completion) */
+    double kernel_ms;         /* 0: the serving loop brackets no round with HIP events (an event
+                                 pair costs host turnaround every round) */
     size_t label_bytes;       /* admission budget */
     size_t label_bytes_peak;  /* most label bytes admitted at once */
     uint32_t inflight_peak;   /* most requests admitted at once */

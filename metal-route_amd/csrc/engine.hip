@@ -3798,25 +3798,31 @@ uint32_t driver_step(RoundDriver& d) {
   A.round = d.stamp_round++;
   if (++d.stamp == 0) d.stamp = 1;
   A.stamp = d.stamp;
-  // kernel time from an event pair per round, read once the pair has passed
+  // kernel time from an event pair per round, read once the pair has passed,
+  // when the context times rounds (a server's context does not)
+  const bool timed = c.timing_every != 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  for (hipEvent_t* e : {&e0, &e1}) {
-    if (!d.ev_free.empty()) {
-      *e = d.ev_free.back();
-      d.ev_free.pop_back();
-    } else {
-      HIP_OK(hipEventCreate(e));
+  if (timed) {
+    for (hipEvent_t* e : {&e0, &e1}) {
+      if (!d.ev_free.empty()) {
+        *e = d.ev_free.back();
+        d.ev_free.pop_back();
+      } else {
+        HIP_OK(hipEventCreate(e));
+      }
     }
+    HIP_OK(hipEventRecord(e0, s));
   }
-  HIP_OK(hipEventRecord(e0, s));
   A.lcap = lcap;
   A.lset_next = (uint32_t)(d.rounds % 3);
   A.lset_cur = (uint32_t)((d.rounds + 2) % 3);
   A.lset_clear = (uint32_t)((d.rounds + 1) % 3);
   HIP_OK(hipLaunchKernel(RL.fn, dim3(d.n_cur), dim3(RL.threads), args, RL.lds, s));
-  HIP_OK(hipEventRecord(e1, s));
   HIP_OK(hipGetLastError());
-  d.ev_pending.push_back({e0, e1});
+  if (timed) {
+    HIP_OK(hipEventRecord(e1, s));
+    d.ev_pending.push_back({e0, e1});
+  }
   volatile uint32_t* hc = d.hostctr;
   const auto spin0 = std::chrono::steady_clock::now();
   while (hc[2] != A.stamp) {
